@@ -1,0 +1,215 @@
+/*
+ * hbt_unbind.h -- C-ABI of the MI355X (gfx950) subhalo-unbinding library
+ * (libhbt_unbind.so).
+ *
+ * This is the drop-in boundary for HBT+'s subhalo unbinding hot path.  Every
+ * entry point below replaces one reference interface (cited file:line, paths
+ * relative to the HBT+ source tree):
+ *
+ *   hbt_unbind_one      Subhalo_t::Unbind                src/subhalo.h:111,
+ *                                                        src/subhalo_unbind.cpp:263-431
+ *   hbt_unbind_batch    SubhaloSnapshot_t::RefineParticles loop body
+ *                       (RecursiveUnbind + Unbind + TruncateSource)
+ *                                                        src/subhalo.h:112,114,245,
+ *                                                        src/subhalo_unbind.cpp:432-516
+ *   hbt_tree_potential  GravityTree_t::Build + EvaluatePotential
+ *                                                        src/oct_tree.h:62-63,
+ *                                                        src/gravity_tree.h:15
+ *   hbt_direct_potential  exact pair sum with the same softened kernel as
+ *                       GravityTree_t::EvaluatePotential (src/gravity_tree.cpp:144-160);
+ *                       used for small subhalos and as the precision-study path
+ *   hbt_id_to_index     ParticleSnapshot_t::GetIndices / MappedIndexTable_t::GetIndices
+ *                                                        src/hash_remote.tpp:58-109,
+ *                                                        src/hash.tpp:18-56
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Every function returns an int status:
+ *     HBT_OK (0) on success, a negative HBT_ERR_* code otherwise.  Nothing
+ *     throws or aborts across this boundary; hbt_last_error() gives a message.
+ *   - Particle arrays are structure-of-arrays (x[], y[], z[], ...), float32
+ *     positions/velocities/masses, int64 IDs, int32 types -- the fields of the
+ *     reference Particle_t (src/snapshot.h:51-72), split into columns.
+ *   - Pointers passed to hbt_unbind_batch / hbt_tree_potential /
+ *     hbt_direct_potential / hbt_id_to_index are DEVICE pointers (HBM); the
+ *     work is enqueued on the context's stream and the call returns after the
+ *     stream has finished, unless stated otherwise.
+ *   - hbt_unbind_one takes HOST pointers (it is the per-subhalo form a
+ *     reference caller uses in place of Subhalo_t::Unbind) and copies in/out.
+ *   - One context = one HIP stream + device work buffers.  Contexts are
+ *     independent, so distinct host threads may drive distinct contexts.
+ */
+#ifndef HBT_UNBIND_H
+#define HBT_UNBIND_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HBT_OK                  0
+#define HBT_ERR_INVALID_ARG    -1
+#define HBT_ERR_OUT_OF_MEMORY  -2
+#define HBT_ERR_DEVICE         -3   /* a HIP runtime call failed */
+#define HBT_ERR_UNSUPPORTED    -4   /* a configuration the library does not implement */
+#define HBT_ERR_CAPACITY       -5   /* an output buffer is too small */
+
+/* ---- precision policies -------------------------------------------------- */
+/* FAST: pair/tree arithmetic in FP32 with Kahan-compensated potential sums;
+ *       frame averages, kinetic energy and the E<0 cut in FP64.
+ * REPLAY: the reference's own arithmetic (FP64 walk over float node data,
+ *       float-rounded node moments, serial-order FP64 frame sums, no FMA
+ *       contraction) for bit-level certification. */
+#define HBT_PREC_FAST    0
+#define HBT_PREC_REPLAY  1
+
+/* direct-sum accumulation modes (hbt_direct_potential) */
+#define HBT_DIRECT_FP32        0   /* plain FP32 sum */
+#define HBT_DIRECT_FP32_KAHAN  1   /* FP32 terms, Kahan-compensated sum */
+#define HBT_DIRECT_FP64        2   /* FP64 throughout */
+
+/* status bits reported in hbt_subhalo.status */
+#define HBT_STATUS_CONVERGED   1   /* left the loop through the BoundMassPrecision test */
+#define HBT_STATUS_DISRUPTED   2   /* Nbound fell below MinNumPartOfSub */
+#define HBT_STATUS_EARLY       4   /* 0 or 1 source particles: no iteration */
+
+#define HBT_TYPE_MAX 6              /* ParticleType_t::TypeMax, src/datatypes.h:179-188 */
+
+/* Unbinding parameters: the HBTConfig fields Unbind reads
+ * (src/config_parser.h:22-110, derived values src/config_parser.cpp:95-103),
+ * PhysicalConst::G and the epoch's Cosmology_t (src/snapshot.h:17-39). */
+typedef struct hbt_params {
+  float   G;                        /* PhysicalConst::G, 43007.1 for kpc/h, km/s, 1e10 Msun/h */
+  float   softening;                /* SofteningHalo */
+  float   open_angle;               /* TreeNodeOpenAngle (theta) */
+  float   box_size;                 /* BoxSize */
+  int32_t periodic;                 /* PeriodicBoundaryOn */
+  float   bound_mass_precision;     /* BoundMassPrecision */
+  int32_t min_num_part_of_sub;      /* MinNumPartOfSub */
+  float   source_sub_relax_factor;  /* SourceSubRelaxFactor */
+  int32_t max_sample_size;          /* MaxSampleSizeOfPotentialEstimate; only 0 (no sampling) is supported */
+  float   scale_factor;             /* Cosmology.ScaleFactor */
+  float   hz;                       /* Cosmology.Hz */
+  int32_t snapshot_index;           /* epoch.GetSnapshotIndex() */
+  int32_t precision;                /* HBT_PREC_FAST or HBT_PREC_REPLAY */
+  int32_t reserved[3];
+} hbt_params;
+
+/* Particle columns (device or host pointers, see each entry point). */
+typedef struct hbt_particles {
+  int64_t        n;
+  const int64_t *id;                /* Particle_t::Id */
+  const float   *x, *y, *z;         /* ComovingPosition */
+  const float   *vx, *vy, *vz;      /* PhysicalVelocity */
+  const float   *mass;              /* Mass */
+  const int32_t *type;              /* Type (0..HBT_TYPE_MAX-1) */
+} hbt_particles;
+
+/* One subhalo record: the Subhalo_t fields Unbind reads and writes
+ * (src/subhalo.h:24-110), plus the layout of its particle list. */
+typedef struct hbt_subhalo {
+  /* -- inputs -- */
+  int64_t offset;                   /* first particle of the source list in the particle columns */
+  int64_t count;                    /* source list length (Particles.size()) */
+  int64_t parent;                   /* subhalo whose NestedSubhalos holds this one, -1 if none */
+  int64_t nbound_prev;              /* Nbound before this pass (orphan test, RecursiveUnbind) */
+  int64_t sink_track_id;            /* SinkTrackId (in/out) */
+  int32_t snapshot_index_of_death;  /* SnapshotIndexOfDeath (in/out) */
+  int32_t snapshot_index_of_sink;   /* SnapshotIndexOfSink (in/out) */
+  float   ref_pos[3];               /* ComovingAveragePosition: initial frame in, final frame out */
+  float   ref_vel[3];               /* PhysicalAverageVelocity */
+  float   mostbound_pos[3];         /* ComovingMostBoundPosition (in: previous, out: new) */
+  float   mostbound_vel[3];         /* PhysicalMostBoundVelocity */
+  /* -- outputs -- */
+  int64_t nbound;                   /* Nbound */
+  int64_t list_len;                 /* Particles.size() after unbinding (own list + appended tails) */
+  int64_t nsource;                  /* Particles.size() after TruncateSource */
+  int64_t nbound_type[HBT_TYPE_MAX];/* NboundType */
+  float   mbound_type[HBT_TYPE_MAX];/* MboundType */
+  float   mbound;                   /* Mbound */
+  float   specific_self_potential_energy;
+  float   specific_self_kinetic_energy;
+  float   specific_angular_momentum[3];
+  int32_t n_iter;                   /* unbinding passes run */
+  int32_t status;                   /* HBT_STATUS_* bits */
+  int64_t n_evals;                  /* binding-energy evaluations (full pass: Nlast, correction: Nbound) */
+  int64_t out_offset;               /* start of this subhalo's output list in out_index/out_energy */
+} hbt_subhalo;
+
+typedef struct hbt_ctx hbt_ctx;
+
+/* Library and context management. */
+const char *hbt_last_error(void);
+int hbt_version(int *major, int *minor);
+int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out);
+int hbt_ctx_destroy(hbt_ctx *ctx);
+int hbt_ctx_set_params(hbt_ctx *ctx, const hbt_params *params);
+/* The HIP stream (hipStream_t) the context enqueues on. */
+int hbt_ctx_stream(hbt_ctx *ctx, void **stream);
+
+/* Output capacity needed by hbt_unbind_batch for a given nest layout:
+ * for every subhalo, its own count plus the counts of everything nested
+ * below it (unbound tails are appended to the parent, src/subhalo_unbind.cpp:442).
+ * subs: HOST array of nsub records with offset/count/parent filled.
+ * Writes subs[i].out_offset and returns the total in *total. */
+int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total);
+
+/* Batched unbinding of nsub subhalos (RefineParticles loop body).
+ *   particles : device columns holding every source list
+ *   subs      : DEVICE array of nsub records (inputs filled, out_offset set by
+ *               hbt_plan_batch); outputs are written in place
+ *   out_index : DEVICE int64[total]; for each subhalo the final particle list
+ *               as indices into the particle columns: bound particles sorted by
+ *               binding energy, then the removed blocks (last removed first),
+ *               then appended tails, exactly the reference's Particles order
+ *   out_energy: DEVICE float[total]; the binding energy at each list position
+ *               (the reference's Elist E, SAVE_BINDING_ENERGY), NaN where none
+ *   subs_host : HOST copy of subs (same layout) used for planning; not modified
+ *   plain     : 0 = batched form; 1 = plain per-subhalo form (one subhalo at a
+ *               time through the same kernels, the loop RefineParticles runs) */
+int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles,
+                     hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
+                     int64_t *out_index, float *out_energy, int plain);
+
+/* Per-subhalo form with HOST buffers (Subhalo_t::Unbind).  `particles` are
+ * host columns of this subhalo's list (n = sub->count, sub->offset ignored);
+ * `sub` is a host record updated in place; out_index/out_energy are host
+ * arrays of length sub->count. */
+int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *sub,
+                   int64_t *out_index, float *out_energy);
+
+/* Tree potential (GravityTree_t::Build over the n_src sources in the given
+ * order, then EvaluatePotential at each target).  Returns the specific
+ * physical potential G*phi/a.  target_mass[i] != 0 removes the target's own
+ * softened self term (pass the target's mass when it is one of the sources,
+ * src/gravity_tree.cpp:79-98); target_mass may be NULL (all 0).
+ * Device pointers; out_phi is double[n_tgt]. */
+int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, const float *sy,
+                       const float *sz, const float *smass, int64_t n_tgt,
+                       const float *tx, const float *ty, const float *tz,
+                       const float *target_mass, double *out_phi);
+
+/* Direct-sum potential of each of n particles due to all others (self term
+ * excluded), softened with the same spline kernel.  Segmented: nseg groups
+ * given by seg_offset[nseg+1] (device int64); mode = HBT_DIRECT_*.
+ * out_phi is double[n] (device). */
+int hbt_direct_potential(hbt_ctx *ctx, int64_t nseg, const int64_t *seg_offset,
+                         const float *x, const float *y, const float *z,
+                         const float *mass, int mode, double *out_phi);
+
+/* ID -> index resolution: builds a device hash table of the n_table IDs
+ * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs
+ * and negative IDs give -1 (src/hash.tpp:52).  Device pointers. */
+int hbt_id_to_index(hbt_ctx *ctx, int64_t n_table, const int64_t *table_ids,
+                    int64_t n_query, const int64_t *query_ids, int64_t *out_index);
+
+/* Timing of the last hbt_unbind_batch call (device milliseconds from HIP
+ * events on the context stream) and the number of energy evaluations it did. */
+int hbt_last_batch_stats(hbt_ctx *ctx, double *device_ms, int64_t *n_evals,
+                         int64_t *n_passes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HBT_UNBIND_H */

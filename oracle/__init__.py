@@ -1,0 +1,172 @@
+"""ctypes wrapper of the CPU restatement oracle (liboracle.so) and a runner
+for the reference harness (oracle/_ref/hbt_ref_harness).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from hbtplus_amd.abi import PARAMS_DTYPE, SUB_DTYPE, particles_struct, plan_batch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "hbt_ref_harness")
+_lib = None
+
+
+def build(ref: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref and os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        L.oracle_unbind_batch.argtypes = [vp, vp, vp, i64, vp, vp, i32]
+        L.oracle_tree_potential.argtypes = [vp, i64, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]
+        L.oracle_tree_ncell.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.oracle_tree_ncell.restype = i64
+        L.oracle_direct_potential.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, i32]
+        L.oracle_id_to_index.argtypes = [i64, vp, i64, vp, vp]
+        L.oracle_abi_sizes.argtypes = [vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def unbind_batch(cat, nthreads: int = 1, params=None):
+    """Run the restatement on a catalogue.  Returns (subs_out, out_index, out_energy)."""
+    params = cat.params if params is None else params
+    cols = {k: np.ascontiguousarray(v) for k, v in cat.cols.items()}
+    subs = cat.subs.copy()
+    total = plan_batch(subs)
+    out_index = np.full(total, -1, np.int64)
+    out_energy = np.full(total, np.nan, np.float32)
+    ps = particles_struct(cols)
+    rc = lib().oracle_unbind_batch(_p(params), ctypes.addressof(ps), _p(subs), len(subs),
+                                   _p(out_index), _p(out_energy), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_unbind_batch failed: {rc}")
+    return subs, out_index, out_energy
+
+
+def tree_potential(params, src, tgt, tmass=None):
+    sx, sy, sz, sm = (_f32(a) for a in src)
+    tx, ty, tz = (_f32(a) for a in tgt)
+    tm = _f32(tmass) if tmass is not None else np.zeros(len(tx), np.float32)
+    out = np.zeros(len(tx), np.float64)
+    rc = lib().oracle_tree_potential(_p(params), len(sx), _p(sx), _p(sy), _p(sz), _p(sm), len(tx),
+                                     _p(tx), _p(ty), _p(tz), _p(tm), _p(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle_tree_potential failed: {rc}")
+    return out
+
+
+def tree_ncell(params, src):
+    sx, sy, sz, sm = (_f32(a) for a in src)
+    return lib().oracle_tree_ncell(_p(params), len(sx), _p(sx), _p(sy), _p(sz), _p(sm))
+
+
+def direct_potential(params, seg_offset, x, y, z, m, mode, nthreads=1):
+    seg = np.ascontiguousarray(seg_offset, np.int64)
+    x, y, z, m = (_f32(a) for a in (x, y, z, m))
+    out = np.zeros(len(x), np.float64)
+    rc = lib().oracle_direct_potential(_p(params), len(seg) - 1, _p(seg), _p(x), _p(y), _p(z),
+                                       _p(m), mode, _p(out), nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle_direct_potential failed")
+    return out
+
+
+def id_to_index(table_ids, query_ids):
+    t = np.ascontiguousarray(table_ids, np.int64)
+    q = np.ascontiguousarray(query_ids, np.int64)
+    out = np.zeros(len(q), np.int64)
+    lib().oracle_id_to_index(len(t), _p(t), len(q), _p(q), _p(out))
+    return out
+
+
+def abi_sizes():
+    a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    lib().oracle_abi_sizes(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+# ---- reference harness (development container only) ----------------------
+
+def have_reference() -> bool:
+    return os.path.exists(REF_HARNESS)
+
+
+def ref_unbind_batch(cat, params=None):
+    """Run the reference's own RecursiveUnbind/Unbind/TruncateSource on the
+    catalogue through oracle/_ref/hbt_ref_harness (single thread)."""
+    params = cat.params if params is None else params
+    cols = cat.cols
+    n = len(cols["x"])
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.ascontiguousarray(params, PARAMS_DTYPE).tobytes())
+            f.write(np.array([len(cat.subs), n], np.int64).tobytes())
+            f.write(np.ascontiguousarray(cols["id"], np.int64).tobytes())
+            for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):
+                f.write(_f32(cols[k]).tobytes())
+            f.write(np.ascontiguousarray(cols["type"], np.int32).tobytes())
+            f.write(np.ascontiguousarray(cat.subs, SUB_DTYPE).tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_HARNESS, "unbind", fin, fout], check=True, env=env)
+        raw = open(fout, "rb").read()
+    nsub = int(np.frombuffer(raw, np.int64, 1)[0])
+    pos = 16
+    subs = np.zeros(nsub, SUB_DTYPE)
+    plan = cat.subs.copy()
+    total = plan_batch(plan)
+    out_index = np.full(total, -1, np.int64)
+    out_energy = np.full(total, np.nan, np.float32)
+    for s in range(nsub):
+        r = np.frombuffer(raw, SUB_DTYPE, 1, pos)
+        pos += SUB_DTYPE.itemsize
+        subs[s] = r[0]
+        L = int(r["list_len"][0])
+        o = int(plan["out_offset"][s])
+        subs["out_offset"][s] = o
+        out_index[o:o + L] = np.frombuffer(raw, np.int64, L, pos)
+        pos += 8 * L
+        out_energy[o:o + L] = np.frombuffer(raw, np.float32, L, pos)
+        pos += 4 * L
+    return subs, out_index, out_energy
+
+
+def ref_tree_potential(params, src, tgt, tmass=None):
+    sx, sy, sz, sm = (_f32(a) for a in src)
+    tx, ty, tz = (_f32(a) for a in tgt)
+    tm = _f32(tmass) if tmass is not None else np.zeros(len(tx), np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.ascontiguousarray(params, PARAMS_DTYPE).tobytes())
+            f.write(np.array([len(sx), len(tx)], np.int64).tobytes())
+            for a in (sx, sy, sz, sm, tx, ty, tz, tm):
+                f.write(a.tobytes())
+        subprocess.run([REF_HARNESS, "tree", fin, fout], check=True)
+        return np.fromfile(fout, np.float64)

@@ -1,0 +1,704 @@
+// hbt_oracle.cpp -- CPU restatement of HBT+ subhalo unbinding.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle for the
+// MI355X library: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load liboracle.so, and only as the checker or as the
+// timed CPU baseline.  The product path (hbtplus_amd/, libhbt_unbind.so)
+// never links, loads or calls it.
+//
+// It restates the reference algorithm step by step, including the
+// float/double rounding of every expression, so that with one thread it is
+// bit-identical to the reference built with g++ on x86-64 (no FMA):
+//   serial-insertion octree          src/oct_tree.tpp:17-144
+//   node moments and threading       src/gravity_tree.cpp:18-77
+//   softened potential walk          src/gravity_tree.cpp:79-164
+//   binding energy                   src/gravity_tree.cpp:166-175
+//   relative velocity                src/snapshot.h:100-111
+//   two-pointer partition            src/subhalo_unbind.cpp:21-58
+//   frame averages                   src/subhalo_unbind.cpp:108-188
+//   kinematics                       src/subhalo_unbind.cpp:189-232
+//   Unbind iteration                 src/subhalo_unbind.cpp:263-431
+//   RecursiveUnbind / TruncateSource src/subhalo_unbind.cpp:432-458
+//   CountParticleTypes/CountParticles src/subhalo.cpp:400-444,479-499
+//   derived parameters               src/config_parser.cpp:95-103
+// Third-party behaviour restated: POSIX drand48 (48-bit LCG, a=0x5DEECE66D,
+// c=0xB, default state 0x1234ABCD330E) used at src/oct_tree.tpp:118, and
+// libstdc++ std::sort (called directly, same library as the reference build).
+//
+// Build: see oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp, no -march).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+#include <omp.h>
+
+#include "../include/hbt_unbind.h"
+
+namespace {
+
+typedef int64_t Int;
+
+// ---- parameters with the reference's derived values ------------------------
+struct Params {
+  float G, eps, theta2, res, res_half, box, box_half, bmp, relax, a, hz;
+  bool periodic;
+  int min_part, snap;
+  explicit Params(const hbt_params &p) {
+    G = p.G;
+    eps = p.softening;
+    theta2 = p.open_angle * p.open_angle;                  // float*float, config_parser.cpp:103
+    res = (float)((double)p.softening * 0.1);             // :101
+    res_half = (float)((double)res / 2.);                 // :102
+    box = p.box_size;
+    box_half = (float)((double)p.box_size / 2.);          // :100
+    bmp = p.bound_mass_precision;
+    relax = p.source_sub_relax_factor;
+    a = p.scale_factor;
+    hz = p.hz;
+    periodic = p.periodic != 0;
+    min_part = p.min_num_part_of_sub;
+    snap = p.snapshot_index;
+  }
+  // NEAREST macro, config_parser.h:142, in float and in double
+  inline float nearest(float x) const {
+    return (x > box_half) ? (x - box) : ((x < -box_half) ? (x + box) : x);
+  }
+  inline double nearest(double x) const {
+    return (x > (double)box_half) ? (x - (double)box)
+                                  : ((x < -(double)box_half) ? (x + (double)box) : x);
+  }
+};
+
+// ---- POSIX drand48 ------------------------------------------------------
+struct Drand48 {
+  uint64_t x = 0x1234ABCD330EULL;
+  double next() {
+    x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
+    return (double)x / 281474976710656.0;
+  }
+};
+
+struct Vec3f { float v[3]; };
+
+// ---- the reference octree, restated ----------------------------------------
+class Tree {
+ public:
+  Int np = 0;                          // particles are nodes [0,np), cells [np, np+ncell)
+  std::vector<Int> sons;               // 8 per cell, build phase
+  std::vector<float> cs, clen, cmass;  // 3 per cell, 1, 1
+  std::vector<Int> csib, cnext, pnext;
+  std::vector<float> px, py, pz, pm;   // gathered sources in insertion order
+
+  void build(const Params &P, Drand48 &rng) {
+    Int n = np;
+    sons.clear(); cs.clear(); clen.clear(); cmass.clear(); csib.clear(); cnext.clear();
+    pnext.assign(n, -1);
+    double xmin[3], xmax[3], Center[3], Len;
+    const float *pos[3] = {px.data(), py.data(), pz.data()};
+    for (int j = 0; j < 3; j++) xmin[j] = xmax[j] = pos[j][0];
+    for (Int i = 1; i < n; i++)
+      for (int j = 0; j < 3; j++) {
+        if (pos[j][i] > xmax[j]) xmax[j] = pos[j][i];
+        else if (pos[j][i] < xmin[j]) xmin[j] = pos[j][i];
+      }
+    Len = xmax[0] - xmin[0];
+    for (int j = 1; j < 3; j++)
+      if ((xmax[j] - xmin[j]) > Len) Len = xmax[j] - xmin[j];
+    for (int j = 0; j < 3; j++) Center[j] = 0.5 * (xmax[j] + xmin[j]);
+    double Lenhalf = 0.5 * Len;
+    append_cell();
+    const Int root = n;
+    for (Int i = 0; i < n; i++) {
+      Int node = root;
+      double lenhalf = Lenhalf, c[3] = {Center[0], Center[1], Center[2]};
+      while (true) {
+        lenhalf *= 0.5;
+        int sub = 0;
+        for (int j = 0; j < 3; j++) {
+          if (pos[j][i] > c[j]) { c[j] += lenhalf; sub += 1 << j; }
+          else c[j] -= lenhalf;
+        }
+        Int subid = son(node, sub);
+        if (subid < 0) { son(node, sub) = i; break; }
+        if (subid < n) {
+          Int nid = n + ncell();
+          append_cell();
+          son(node, sub) = nid;
+          node = nid;
+          if (lenhalf < (double)P.res_half) {
+            sub = (int)(Int)(8.0 * rng.next());
+            if (sub >= 8) sub = 7;
+          } else {
+            sub = 0;
+            for (int j = 0; j < 3; j++)
+              if (pos[j][subid] > c[j]) sub += 1 << j;
+          }
+          son(node, sub) = subid;
+        } else {
+          node = subid;
+        }
+      }
+    }
+    update_internal(P, root, -1, Len);
+  }
+
+  double potential(const Params &P, double tx, double ty, double tz, float target_mass) const {
+    const double h = 2.8 * (double)P.eps, h_inv = 1.0 / h;
+    double pot = (double)(float)(target_mass / P.eps);
+    Int no = np;
+    while (no >= 0) {
+      double dx, dy, dz, mass, r2;
+      if (no < np) {
+        dx = (double)px[no] - tx; dy = (double)py[no] - ty; dz = (double)pz[no] - tz;
+        if (P.periodic) { dx = P.nearest(dx); dy = P.nearest(dy); dz = P.nearest(dz); }
+        mass = pm[no];
+        no = pnext[no];
+        r2 = dx * dx + dy * dy + dz * dz;
+      } else {
+        Int c = no - np;
+        dx = (double)cs[3 * c] - tx; dy = (double)cs[3 * c + 1] - ty; dz = (double)cs[3 * c + 2] - tz;
+        if (P.periodic) { dx = P.nearest(dx); dy = P.nearest(dy); dz = P.nearest(dz); }
+        mass = cmass[c];
+        r2 = dx * dx + dy * dy + dz * dz;
+        float l2 = clen[c] * clen[c];
+        if ((double)l2 > r2 * (double)P.theta2) { no = cnext[c]; continue; }
+        no = csib[c];
+      }
+      double r = std::sqrt(r2);
+      if (r >= h) {
+        pot -= mass / r;
+      } else {
+        double u = r * h_inv, wp;
+        if (u < 0.5)
+          wp = -2.8 + u * u * (5.333333333333 + u * u * (6.4 * u - 9.6));
+        else
+          wp = -3.2 + 0.066666666667 / u +
+               u * u * (10.666666666667 + u * (-16.0 + u * (9.6 - 2.133333333333 * u)));
+        pot += mass * h_inv * wp;
+      }
+    }
+    return pot * (double)P.G / (double)P.a;
+  }
+
+  Int ncell() const { return (Int)clen.size(); }
+
+ private:
+  Int &son(Int node, int k) { return sons[8 * (node - np) + k]; }
+  void append_cell() {
+    for (int k = 0; k < 8; k++) sons.push_back(-1);
+    cs.push_back(0); cs.push_back(0); cs.push_back(0);
+    clen.push_back(0); cmass.push_back(0); csib.push_back(-1); cnext.push_back(-1);
+  }
+  void process(const Params &P, Int id, Int next, double &mass, double com[3], double len) {
+    if (id < np) {
+      double m = pm[id];
+      mass += m;
+      com[0] += px[id] * m; com[1] += py[id] * m; com[2] += pz[id] * m;
+      pnext[id] = next;
+    } else {
+      if (len >= (double)P.res) update_internal(P, id, next, len / 2.);
+      else update_internal(P, id, next, len);
+      Int c = id - np;
+      double m = cmass[c];
+      mass += m;
+      com[0] += cs[3 * c] * m; com[1] += cs[3 * c + 1] * m; com[2] += cs[3 * c + 2] * m;
+    }
+  }
+  void update_internal(const Params &P, Int no, Int sib, double len) {
+    Int s[8];
+    Int c = no - np;
+    for (int k = 0; k < 8; k++) s[k] = sons[8 * c + k];
+    clen[c] = (float)len;
+    csib[c] = sib;
+    int i = 0;
+    while (s[i] < 0) i++;
+    Int pp = s[i];
+    cnext[c] = pp;
+    double mass = 0., com[3] = {0., 0., 0.};
+    for (i++; i < 8; i++) {
+      if (s[i] >= 0) {
+        Int p = pp;
+        pp = s[i];
+        process(P, p, pp, mass, com, len);
+      }
+    }
+    process(P, pp, sib, mass, com, len);
+    cmass[c] = (float)mass;
+    for (int j = 0; j < 3; j++) cs[3 * c + j] = (float)(com[j] / mass);
+  }
+};
+
+// ---- the Unbind iteration --------------------------------------------------
+struct PE { Int pid; float E; };
+inline bool comp_energy(const PE &a, const PE &b) { return a.E < b.E; }
+
+// two-pointer partition, src/subhalo_unbind.cpp:21-58
+Int partition_energy(std::vector<PE> &El, Int len) {
+  if (len == 0) return 0;
+  if (len == 1) return El[0].E < 0;
+  PE tmp = El[0];
+  Int f = 0, b = len;
+  while (true) {
+    while (true) {
+      b--;
+      if (b == f) { El[f] = tmp; if (tmp.E < 0) b++; return b; }
+      if (El[b].E < 0) break;
+    }
+    El[f] = El[b];
+    while (true) {
+      f++;
+      if (f == b) { El[b] = tmp; if (tmp.E < 0) b++; return b; }
+      if (El[f].E > 0) break;
+    }
+    El[b] = El[f];
+  }
+}
+
+struct PartCols {  // host particle columns
+  const hbt_particles *p;
+  float x(Int g, int j) const { return j == 0 ? p->x[g] : (j == 1 ? p->y[g] : p->z[g]); }
+  float v(Int g, int j) const { return j == 0 ? p->vx[g] : (j == 1 ? p->vy[g] : p->vz[g]); }
+  float m(Int g) const { return p->mass[g]; }
+  int t(Int g) const { return p->type ? p->type[g] : 1; }
+};
+
+// RelativeVelocity, src/snapshot.h:100-111 (all float)
+inline void rel_vel(const Params &P, const float tp[3], const float tv[3], const float rp[3],
+                    const float rv[3], float dv[3]) {
+  for (int j = 0; j < 3; j++) {
+    float dx = tp[j] - rp[j];
+    if (P.periodic) dx = P.nearest(dx);
+    dv[j] = tv[j] - rv[j];
+    dv[j] += P.hz * P.a * dx;
+  }
+}
+inline float vnorm(const float v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+inline float vdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+struct Sub {
+  hbt_subhalo *rec;
+  std::vector<Int> list;        // Particles: global particle indices
+  std::vector<float> energy;    // Elist E aligned with `list` after convergence
+  Int nbound;
+};
+
+void count_types(const PartCols &C, Sub &S, bool skip_null) {
+  hbt_subhalo &r = *S.rec;
+  Int nb[HBT_TYPE_MAX] = {0};
+  float mb[HBT_TYPE_MAX] = {0};
+  for (Int i = 0; i < S.nbound; i++) {
+    Int g = S.list[i];
+    if (skip_null && C.p->id[g] == -1) continue;
+    int t = C.t(g);
+    nb[t]++;
+    mb[t] += C.m(g);
+  }
+  for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = nb[t]; r.mbound_type[t] = mb[t]; }
+}
+
+void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree) {
+  hbt_subhalo &r = *S.rec;
+  const Int N = (Int)S.list.size();
+  auto alive = [&]() { return r.snapshot_index_of_death == -1; };
+  S.energy.assign(N, std::numeric_limits<float>::quiet_NaN());
+  r.n_iter = 0; r.n_evals = 0; r.status = 0;
+  if (N < P.min_part) {
+    if (alive()) r.snapshot_index_of_death = P.snap;
+  }
+  if (N == 0 || N == 1) {  // early exits, CountParticles()
+    S.nbound = N;
+    count_types(C, S, false);
+    float m = 0.f;
+    for (int t = 0; t < HBT_TYPE_MAX; t++) m += r.mbound_type[t];
+    r.mbound = m;
+    r.status = HBT_STATUS_EARLY;
+    return;
+  }
+  float *RefPos = r.ref_pos, *RefVel = r.ref_vel;
+  float OldRefPos[3] = {0, 0, 0}, OldRefVel[3] = {0, 0, 0};
+  const Int old_mb_id = C.p->id[S.list[0]];
+  Int Nbound = N, Nlast = 0;
+  std::vector<PE> El(N);
+  for (Int i = 0; i < N; i++) { El[i].pid = i; El[i].E = 0.f; }
+  auto gpos = [&](Int i, float o[3]) { Int g = S.list[El[i].pid]; for (int j = 0; j < 3; j++) o[j] = C.x(g, j); };
+  auto gvel = [&](Int i, float o[3]) { Int g = S.list[El[i].pid]; for (int j = 0; j < 3; j++) o[j] = C.v(g, j); };
+  auto gmass = [&](Int i) { return C.m(S.list[El[i].pid]); };
+  auto load_tree = [&](Int b, Int e) {
+    Int n = e - b;
+    tree.np = n;
+    tree.px.resize(n); tree.py.resize(n); tree.pz.resize(n); tree.pm.resize(n);
+    for (Int i = 0; i < n; i++) {
+      float x[3]; gpos(b + i, x);
+      tree.px[i] = x[0]; tree.py[i] = x[1]; tree.pz[i] = x[2]; tree.pm[i] = gmass(b + i);
+    }
+    tree.build(P, rng);
+  };
+  bool correction = false;
+  while (true) {
+    r.n_iter++;
+    if (correction) {
+      float RefVelDiff[3];
+      rel_vel(P, OldRefPos, OldRefVel, RefPos, RefVel, RefVelDiff);
+      float dK = (float)(0.5 * (double)vnorm(RefVelDiff));
+      load_tree(Nbound, Nlast);
+      for (Int i = 0; i < Nbound; i++) {
+        float x[3], v[3], ov[3];
+        gpos(i, x); gvel(i, v);
+        rel_vel(P, x, v, OldRefPos, OldRefVel, ov);
+        float kin = vdot(ov, RefVelDiff) + dK;
+        double rhs = (double)kin - tree.potential(P, x[0], x[1], x[2], 0.f);
+        El[i].E = (float)((double)El[i].E + rhs);
+      }
+      r.n_evals += Nbound;
+      Nlast = Nbound;
+    } else {
+      Nlast = Nbound;
+      load_tree(0, Nlast);
+      for (Int i = 0; i < Nlast; i++) {
+        float x[3], v[3], dv[3];
+        gpos(i, x); gvel(i, v);
+        double pot = tree.potential(P, x[0], x[1], x[2], gmass(i));
+        rel_vel(P, x, v, RefPos, RefVel, dv);
+        El[i].E = (float)((double)vnorm(dv) * 0.5 + pot);
+      }
+      r.n_evals += Nlast;
+    }
+    Nbound = partition_energy(El, Nlast);
+    if (Nbound < P.min_part) {  // disruption
+      Nbound = 1;
+      Nlast = 1;
+      if (alive()) r.snapshot_index_of_death = P.snap;
+      for (Int i = 0; i < N; i++)
+        if (C.p->id[S.list[i]] == old_mb_id) { std::swap(S.list[i], S.list[0]); break; }
+      for (int j = 0; j < 3; j++) { RefPos[j] = r.mostbound_pos[j]; RefVel[j] = r.mostbound_vel[j]; }
+      r.mbound = C.m(S.list[0]);
+      r.status |= HBT_STATUS_DISRUPTED;
+      break;
+    }
+    std::sort(El.begin() + Nbound, El.begin() + Nlast, comp_energy);
+    Int Ndiff = Nlast - Nbound;
+    if (Ndiff < Nbound) {
+      correction = true;
+      for (int j = 0; j < 3; j++) { OldRefPos[j] = RefPos[j]; OldRefVel[j] = RefVel[j]; }
+    }
+    // AverageVelocity, src/subhalo_unbind.cpp:108-138
+    double msum;
+    if (Nbound == 1) {
+      gvel(0, RefVel);
+      msum = gmass(0);
+    } else {
+      double sv[3] = {0, 0, 0};
+      msum = 0.;
+      for (Int i = 0; i < Nbound; i++) {
+        float m = gmass(i), v[3];
+        gvel(i, v);
+        msum += m;
+        for (int j = 0; j < 3; j++) sv[j] += v[j] * m;  // float product, double sum
+      }
+      for (int j = 0; j < 3; j++) RefVel[j] = (float)(sv[j] / msum);
+    }
+    r.mbound = (float)msum;
+    // AveragePosition, src/subhalo_unbind.cpp:139-188
+    if (Nbound == 1) {
+      gpos(0, RefPos);
+    } else {
+      double origin[3] = {0, 0, 0}, sx[3] = {0, 0, 0}, ms = 0.;
+      if (P.periodic) { float o[3]; gpos(0, o); for (int j = 0; j < 3; j++) origin[j] = o[j]; }
+      for (Int i = 0; i < Nbound; i++) {
+        float m = gmass(i), x[3];
+        gpos(i, x);
+        ms += m;
+        if (P.periodic) {
+          for (int j = 0; j < 3; j++) sx[j] += P.nearest((double)x[j] - origin[j]) * (double)m;
+        } else {
+          for (int j = 0; j < 3; j++) sx[j] += x[j] * m;  // float product
+        }
+      }
+      for (int j = 0; j < 3; j++) {
+        sx[j] /= ms;
+        if (P.periodic) sx[j] += origin[j];
+        RefPos[j] = (float)sx[j];
+      }
+    }
+    if ((float)Nbound >= (float)Nlast * P.bmp) {  // converged
+      if (!alive()) r.snapshot_index_of_death = -1;
+      if (r.sink_track_id != -1) { r.snapshot_index_of_sink = -1; r.sink_track_id = -1; }
+      std::sort(El.begin(), El.begin() + Nbound, comp_energy);
+      std::vector<Int> nl(N);
+      for (Int i = 0; i < N; i++) { nl[i] = S.list[El[i].pid]; S.energy[i] = El[i].E; El[i].pid = i; }
+      S.list.swap(nl);
+      for (int j = 0; j < 3; j++) {
+        r.mostbound_pos[j] = C.x(S.list[0], j);
+        r.mostbound_vel[j] = C.v(S.list[0], j);
+      }
+      r.status |= HBT_STATUS_CONVERGED;
+      break;
+    }
+  }
+  S.nbound = Nbound;
+  // AverageKinematics, src/subhalo_unbind.cpp:189-232 (Elist order == list order here
+  // when converged; NumPart<=1 gives zeros)
+  if (Nbound <= 1) {
+    r.specific_self_potential_energy = 0.f;
+    r.specific_self_kinetic_energy = 0.f;
+    r.specific_angular_momentum[0] = r.specific_angular_momentum[1] = r.specific_angular_momentum[2] = 0.f;
+  } else {
+    double E = 0, K = 0, AM[3] = {0, 0, 0}, M = 0;
+    for (Int i = 0; i < Nbound; i++) {
+      float m = gmass(i), x[3], v[3];
+      gpos(i, x); gvel(i, v);
+      E += El[i].E * m;  // float product
+      double dx[3], dv[3];
+      for (int j = 0; j < 3; j++) {
+        dx[j] = x[j] - RefPos[j];  // float difference
+        if (P.periodic) dx[j] = P.nearest(dx[j]);
+        dx[j] *= P.a;
+        dv[j] = (double)(v[j] - RefVel[j]) + P.hz * dx[j];
+        K += dv[j] * dv[j] * m;
+      }
+      AM[0] += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
+      AM[1] += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
+      AM[2] += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
+      M += m;
+    }
+    E /= M;
+    K *= 0.5 / M;
+    r.specific_self_potential_energy = (float)(E - K);
+    r.specific_self_kinetic_energy = (float)K;
+    for (int j = 0; j < 3; j++) r.specific_angular_momentum[j] = (float)(AM[j] / M);
+  }
+  // CountParticleTypes: the Nbound>100 branch does not skip null IDs
+  count_types(C, S, Nbound <= 100);
+  if (!(r.status & HBT_STATUS_CONVERGED))
+    std::fill(S.energy.begin(), S.energy.end(), std::numeric_limits<float>::quiet_NaN());
+}
+
+struct Batch {
+  const Params &P;
+  const PartCols &C;
+  std::vector<Sub> &subs;
+  const std::vector<std::vector<Int>> &children;
+};
+
+void recursive_unbind(Batch &B, Int s, Drand48 &rng, Tree &tree) {
+  Sub &S = B.subs[s];
+  bool orphan = S.rec->nbound_prev <= 1;
+  std::vector<Int> backup;
+  if (orphan) backup = S.list;
+  for (Int c : B.children[s]) {
+    recursive_unbind(B, c, rng, tree);
+    Sub &K = B.subs[c];
+    S.list.insert(S.list.end(), K.list.begin() + K.nbound, K.list.end());
+  }
+  if (orphan) S.list.swap(backup);
+  unbind(B.P, B.C, S, rng, tree);
+  if (orphan) {
+    S.list.swap(backup);
+    S.energy.assign(S.list.size(), std::numeric_limits<float>::quiet_NaN());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
+  if (nsub < 0 || (nsub > 0 && !subs) || !total) return HBT_ERR_INVALID_ARG;
+  std::vector<Int> cap(nsub, 0);
+  for (Int s = 0; s < nsub; s++) {
+    if (subs[s].count < 0 || subs[s].parent >= nsub || subs[s].parent == s) return HBT_ERR_INVALID_ARG;
+    cap[s] = subs[s].count;
+  }
+  // a parent's list grows by its descendants' lists; parents must precede nothing in particular,
+  // so propagate with a depth-first accumulation (children may have any index)
+  std::vector<Int> depth(nsub, 0);
+  for (Int s = 0; s < nsub; s++) {
+    Int d = 0, p = subs[s].parent;
+    while (p >= 0) { d++; p = subs[p].parent; if (d > nsub) return HBT_ERR_INVALID_ARG; }
+    depth[s] = d;
+  }
+  std::vector<Int> ord(nsub);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](Int a, Int b) { return depth[a] > depth[b]; });
+  for (Int s : ord)
+    if (subs[s].parent >= 0) cap[subs[s].parent] += cap[s];
+  Int off = 0;
+  for (Int s = 0; s < nsub; s++) { subs[s].out_offset = off; off += cap[s]; }
+  *total = off;
+  return HBT_OK;
+}
+
+// Unbinds a batch of subhalos the way RefineParticles does: every top-level
+// subhalo (parent < 0) in index order through RecursiveUnbind, then
+// TruncateSource on all.  nthreads>1 runs top-level nests in parallel
+// (schedule(dynamic,1), src/subhalo_unbind.cpp:479) with one drand48 stream
+// per thread; nthreads==1 matches a single-threaded reference run.
+int oracle_unbind_batch(const hbt_params *hp, const hbt_particles *parts, hbt_subhalo *subs,
+                        int64_t nsub, int64_t *out_index, float *out_energy, int nthreads) {
+  if (!hp || !parts || (nsub > 0 && !subs)) return HBT_ERR_INVALID_ARG;
+  if (hp->max_sample_size > 0) return HBT_ERR_UNSUPPORTED;
+  Params P(*hp);
+  PartCols C{parts};
+  std::vector<Sub> S(nsub);
+  std::vector<std::vector<Int>> children(nsub);
+  std::vector<Int> roots;
+  for (Int s = 0; s < nsub; s++) {
+    S[s].rec = &subs[s];
+    S[s].list.resize(subs[s].count);
+    for (Int i = 0; i < subs[s].count; i++) S[s].list[i] = subs[s].offset + i;
+    S[s].nbound = subs[s].nbound_prev;
+    if (subs[s].parent >= 0) children[subs[s].parent].push_back(s);
+    else roots.push_back(s);
+  }
+  Batch B{P, C, S, children};
+  if (nthreads <= 1) {
+    Drand48 rng;
+    Tree tree;
+    for (Int s : roots) recursive_unbind(B, s, rng, tree);
+  } else {
+    // largest nests first keeps the dynamic schedule balanced
+    std::vector<Int> cost(nsub, 0);
+    for (Int s = 0; s < nsub; s++) { Int p = s; while (subs[p].parent >= 0) p = subs[p].parent; cost[p] += subs[s].count; }
+    std::vector<Int> order = roots;
+    std::stable_sort(order.begin(), order.end(), [&](Int a, Int b) { return cost[a] > cost[b]; });
+#pragma omp parallel num_threads(nthreads)
+    {
+      Drand48 rng;
+      Tree tree;
+#pragma omp for schedule(dynamic, 1)
+      for (Int k = 0; k < (Int)order.size(); k++) recursive_unbind(B, order[k], rng, tree);
+    }
+  }
+  for (Int s = 0; s < nsub; s++) {
+    hbt_subhalo &r = subs[s];
+    Sub &X = S[s];
+    r.nbound = X.nbound;
+    r.list_len = (Int)X.list.size();
+    Int ns = X.nbound <= 1 ? X.nbound : (Int)((float)X.nbound * P.relax);
+    if (ns > r.list_len) ns = r.list_len;
+    r.nsource = ns;
+    if (out_index)
+      for (Int i = 0; i < r.list_len; i++) out_index[r.out_offset + i] = X.list[i];
+    if (out_energy)
+      for (Int i = 0; i < r.list_len; i++)
+        out_energy[r.out_offset + i] = i < (Int)X.energy.size() ? X.energy[i] : std::numeric_limits<float>::quiet_NaN();
+  }
+  return HBT_OK;
+}
+
+// GravityTree_t::Build over sources in the given order + EvaluatePotential.
+int oracle_tree_potential(const hbt_params *hp, int64_t n_src, const float *sx, const float *sy,
+                          const float *sz, const float *sm, int64_t n_tgt, const float *tx,
+                          const float *ty, const float *tz, const float *tmass, double *out_phi) {
+  if (!hp || n_src < 1 || n_tgt < 0) return HBT_ERR_INVALID_ARG;
+  Params P(*hp);
+  Tree tree;
+  Drand48 rng;
+  tree.np = n_src;
+  tree.px.assign(sx, sx + n_src); tree.py.assign(sy, sy + n_src);
+  tree.pz.assign(sz, sz + n_src); tree.pm.assign(sm, sm + n_src);
+  tree.build(P, rng);
+  for (Int i = 0; i < n_tgt; i++)
+    out_phi[i] = tree.potential(P, tx[i], ty[i], tz[i], tmass ? tmass[i] : 0.f);
+  return HBT_OK;
+}
+
+// Number of cells of the reference tree over the given sources (test aid).
+int64_t oracle_tree_ncell(const hbt_params *hp, int64_t n_src, const float *sx, const float *sy,
+                          const float *sz, const float *sm) {
+  Params P(*hp);
+  Tree tree;
+  Drand48 rng;
+  tree.np = n_src;
+  tree.px.assign(sx, sx + n_src); tree.py.assign(sy, sy + n_src);
+  tree.pz.assign(sz, sz + n_src); tree.pm.assign(sm, sm + n_src);
+  tree.build(P, rng);
+  return tree.ncell();
+}
+
+// Exact pair sum with the walk's softened kernel (src/gravity_tree.cpp:144-160),
+// self term excluded; mode HBT_DIRECT_FP64 / FP32 / FP32_KAHAN.
+int oracle_direct_potential(const hbt_params *hp, int64_t nseg, const int64_t *seg, const float *x,
+                            const float *y, const float *z, const float *m, int mode,
+                            double *out_phi, int nthreads) {
+  if (!hp || nseg < 0) return HBT_ERR_INVALID_ARG;
+  Params P(*hp);
+  const double h = 2.8 * (double)P.eps, hinv = 1.0 / h;
+  const float hf = 2.8f * P.eps, hinvf = 1.0f / hf;
+  const double scale = (double)P.G / (double)P.a;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+  for (Int s = 0; s < nseg; s++) {
+    for (Int i = seg[s]; i < seg[s + 1]; i++) {
+      if (mode == HBT_DIRECT_FP64) {
+        double pot = 0;
+        for (Int j = seg[s]; j < seg[s + 1]; j++) {
+          if (j == i) continue;
+          double dx = (double)x[j] - x[i], dy = (double)y[j] - y[i], dz = (double)z[j] - z[i];
+          if (P.periodic) { dx = P.nearest(dx); dy = P.nearest(dy); dz = P.nearest(dz); }
+          double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+          if (r >= h) pot -= m[j] / r;
+          else {
+            double u = r * hinv, wp;
+            if (u < 0.5) wp = -2.8 + u * u * (5.333333333333 + u * u * (6.4 * u - 9.6));
+            else wp = -3.2 + 0.066666666667 / u + u * u * (10.666666666667 + u * (-16.0 + u * (9.6 - 2.133333333333 * u)));
+            pot += m[j] * hinv * wp;
+          }
+        }
+        out_phi[i] = pot * scale;
+      } else {
+        float pot = 0.f, comp = 0.f;
+        for (Int j = seg[s]; j < seg[s + 1]; j++) {
+          if (j == i) continue;
+          float dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+          if (P.periodic) { dx = P.nearest(dx); dy = P.nearest(dy); dz = P.nearest(dz); }
+          float r = std::sqrt(dx * dx + dy * dy + dz * dz), term;
+          if (r >= hf) term = -m[j] / r;
+          else {
+            float u = r * hinvf, wp;
+            if (u < 0.5f) wp = -2.8f + u * u * (5.333333333333f + u * u * (6.4f * u - 9.6f));
+            else wp = -3.2f + 0.066666666667f / u + u * u * (10.666666666667f + u * (-16.0f + u * (9.6f - 2.133333333333f * u)));
+            term = m[j] * hinvf * wp;
+          }
+          if (mode == HBT_DIRECT_FP32_KAHAN) {
+            float yk = term - comp, t = pot + yk;
+            comp = (t - pot) - yk;
+            pot = t;
+          } else {
+            pot += term;
+          }
+        }
+        out_phi[i] = (double)pot * scale;
+      }
+    }
+  }
+  return HBT_OK;
+}
+
+// ID -> index over a table (MappedIndexTable_t semantics, src/hash.tpp:18-56):
+// sorted (id, index) pairs and a binary search; unknown or negative IDs -> -1.
+int oracle_id_to_index(int64_t n_table, const int64_t *ids, int64_t n_query, const int64_t *q,
+                       int64_t *out) {
+  std::vector<std::pair<Int, Int>> t(n_table);
+  for (Int i = 0; i < n_table; i++) t[i] = {ids[i], i};
+  std::sort(t.begin(), t.end());
+  for (Int k = 0; k < n_query; k++) {
+    Int key = q[k];
+    if (key < 0) { out[k] = -1; continue; }
+    auto it = std::lower_bound(t.begin(), t.end(), std::make_pair(key, (Int)-1));
+    out[k] = (it != t.end() && it->first == key) ? it->second : -1;
+  }
+  return HBT_OK;
+}
+
+int oracle_abi_sizes(int64_t *sz_params, int64_t *sz_sub, int64_t *sz_parts) {
+  *sz_params = sizeof(hbt_params);
+  *sz_sub = sizeof(hbt_subhalo);
+  *sz_parts = sizeof(hbt_particles);
+  return 0;
+}
+
+}  // extern "C"

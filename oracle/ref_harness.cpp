@@ -1,0 +1,217 @@
+// ref_harness.cpp -- drives the HBT+ reference's own unbinding code.
+//
+// TEST INFRASTRUCTURE ONLY, built in the development container by
+// oracle/Makefile (target `ref`) against the reference sources where they lie
+// (subhalo_unbind.cpp, gravity_tree.cpp, config_parser.cpp, mymath.cpp,
+// subhalo.cpp, snapshot.cpp, mpi_wrapper.cpp); output goes to oracle/_ref/.
+// It pins the CPU restatement (oracle/hbt_oracle.cpp) and writes the golden
+// fixtures under tests/golden/.  No MPI call is made on this path.
+//
+// Usage:  hbt_ref_harness unbind <in.bin> <out.bin>
+//         hbt_ref_harness tree   <in.bin> <out.bin>
+// Binary formats are produced/consumed by oracle/refrun.py.
+//
+// `unbind` mirrors SubhaloSnapshot_t::RefineParticles (src/subhalo_unbind.cpp:478-515):
+// every top-level subhalo in index order through Subhalo_t::RecursiveUnbind,
+// then TruncateSource on every subhalo.  Run with OMP_NUM_THREADS=1.
+// `tree` builds GravityTree_t over the sources and calls EvaluatePotential.
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "subhalo.h"
+#include "gravity_tree.h"
+#include "config_parser.h"
+#include "../include/hbt_unbind.h"
+
+namespace {
+
+struct Epoch : public Snapshot_t {
+  HBTxyz dummy{{0.f, 0.f, 0.f}};
+  HBTInt size() const { return 0; }
+  const HBTxyz &GetComovingPosition(const HBTInt) const { return dummy; }
+  const HBTxyz &GetPhysicalVelocity(const HBTInt) const { return dummy; }
+  HBTReal GetMass(const HBTInt) const { return 0.f; }
+};
+
+struct Cols : public Snapshot_t {
+  std::vector<HBTxyz> pos;
+  std::vector<HBTReal> mass;
+  HBTInt size() const { return (HBTInt)pos.size(); }
+  const HBTxyz &GetComovingPosition(const HBTInt i) const { return pos[i]; }
+  const HBTxyz &GetPhysicalVelocity(const HBTInt i) const { return pos[i]; }
+  HBTReal GetMass(const HBTInt i) const { return mass[i]; }
+};
+
+void configure(const hbt_params &p, Snapshot_t &epoch) {
+  HBTConfig.SofteningHalo = p.softening;
+  HBTConfig.BoxSize = p.box_size;
+  HBTConfig.PeriodicBoundaryOn = p.periodic != 0;
+  HBTConfig.MinNumPartOfSub = p.min_num_part_of_sub;
+  HBTConfig.BoundMassPrecision = p.bound_mass_precision;
+  HBTConfig.SourceSubRelaxFactor = p.source_sub_relax_factor;
+  HBTConfig.TreeNodeOpenAngle = p.open_angle;
+  HBTConfig.MaxSampleSizeOfPotentialEstimate = p.max_sample_size;
+  HBTConfig.RefineMostboundParticle = true;
+  // derived parameters exactly as Parameter_t::ParseConfigFile computes them
+  HBTConfig.BoxHalf = HBTConfig.BoxSize / 2.;
+  HBTConfig.TreeNodeResolution = HBTConfig.SofteningHalo * 0.1;
+  HBTConfig.TreeNodeResolutionHalf = HBTConfig.TreeNodeResolution / 2.;
+  HBTConfig.TreeNodeOpenAngleSquare = HBTConfig.TreeNodeOpenAngle * HBTConfig.TreeNodeOpenAngle;
+  PhysicalConst::G = p.G;
+  PhysicalConst::H0 = 0.1f;
+  HBTConfig.MinSnapshotIndex = 0;
+  HBTConfig.MaxSnapshotIndex = 1 << 20;
+  epoch.SetSnapshotIndex(p.snapshot_index);
+  epoch.Cosmology.ScaleFactor = p.scale_factor;
+  epoch.Cosmology.Hz = p.hz;
+  epoch.Cosmology.OmegaM0 = 0.3f;
+  epoch.Cosmology.OmegaLambda0 = 0.7f;
+  epoch.Cosmology.OmegaZ = 0.3f;
+}
+
+template <class T>
+void rd(FILE *f, T *p, size_t n) {
+  if (n && fread(p, sizeof(T), n, f) != n) { fprintf(stderr, "short read\n"); exit(2); }
+}
+template <class T>
+void wr(FILE *f, const T *p, size_t n) {
+  if (n && fwrite(p, sizeof(T), n, f) != n) { fprintf(stderr, "short write\n"); exit(2); }
+}
+
+int run_unbind(FILE *in, FILE *out) {
+  hbt_params p;
+  int64_t nsub, npart;
+  rd(in, &p, 1); rd(in, &nsub, 1); rd(in, &npart, 1);
+  std::vector<int64_t> id(npart);
+  std::vector<float> x(npart), y(npart), z(npart), vx(npart), vy(npart), vz(npart), m(npart);
+  std::vector<int32_t> type(npart);
+  rd(in, id.data(), npart);
+  rd(in, x.data(), npart); rd(in, y.data(), npart); rd(in, z.data(), npart);
+  rd(in, vx.data(), npart); rd(in, vy.data(), npart); rd(in, vz.data(), npart);
+  rd(in, m.data(), npart); rd(in, type.data(), npart);
+  std::vector<hbt_subhalo> rec(nsub);
+  rd(in, rec.data(), nsub);
+
+  Epoch epoch;
+  configure(p, epoch);
+  std::unordered_map<int64_t, int64_t> where;
+  where.reserve(npart * 2);
+  for (int64_t i = 0; i < npart; i++) where[id[i]] = i;
+
+  SubhaloList_t subs(nsub);
+  std::vector<int64_t> roots;
+  for (int64_t s = 0; s < nsub; s++) {
+    Subhalo_t &S = subs[s];
+    const hbt_subhalo &r = rec[s];
+    S.Particles.resize(r.count);
+    for (int64_t i = 0; i < r.count; i++) {
+      int64_t g = r.offset + i;
+      Particle_t &q = S.Particles[i];
+      q.Id = id[g];
+      q.ComovingPosition = {{x[g], y[g], z[g]}};
+      q.PhysicalVelocity = {{vx[g], vy[g], vz[g]}};
+      q.Mass = m[g];
+      q.Type = (ParticleType_t)type[g];
+    }
+    S.Nbound = r.nbound_prev;
+    S.SinkTrackId = r.sink_track_id;
+    S.SnapshotIndexOfDeath = r.snapshot_index_of_death;
+    S.SnapshotIndexOfSink = r.snapshot_index_of_sink;
+    for (int j = 0; j < 3; j++) {
+      S.ComovingAveragePosition[j] = r.ref_pos[j];
+      S.PhysicalAverageVelocity[j] = r.ref_vel[j];
+      S.ComovingMostBoundPosition[j] = r.mostbound_pos[j];
+      S.PhysicalMostBoundVelocity[j] = r.mostbound_vel[j];
+    }
+    S.SpecificSelfPotentialEnergy = r.specific_self_potential_energy;
+    S.SpecificSelfKineticEnergy = r.specific_self_kinetic_energy;
+    for (int j = 0; j < 3; j++) S.SpecificAngularMomentum[j] = r.specific_angular_momentum[j];
+    if (r.parent >= 0) subs[r.parent].NestedSubhalos.push_back(s);
+    else roots.push_back(s);
+  }
+  for (int64_t s : roots) subs[s].RecursiveUnbind(subs, epoch);
+
+  std::vector<int64_t> list_len(nsub);
+  int64_t total = 0;
+  for (int64_t s = 0; s < nsub; s++) {
+    list_len[s] = (int64_t)subs[s].Particles.size();
+    total += list_len[s];
+  }
+  wr(out, &nsub, 1);
+  wr(out, &total, 1);
+  for (int64_t s = 0; s < nsub; s++) {
+    Subhalo_t &S = subs[s];
+    hbt_subhalo r = rec[s];
+    r.nbound = S.Nbound;
+    r.list_len = list_len[s];
+    r.mbound = S.Mbound;
+    for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = S.NboundType[t]; r.mbound_type[t] = S.MboundType[t]; }
+    for (int j = 0; j < 3; j++) {
+      r.ref_pos[j] = S.ComovingAveragePosition[j];
+      r.ref_vel[j] = S.PhysicalAverageVelocity[j];
+      r.mostbound_pos[j] = S.ComovingMostBoundPosition[j];
+      r.mostbound_vel[j] = S.PhysicalMostBoundVelocity[j];
+      r.specific_angular_momentum[j] = S.SpecificAngularMomentum[j];
+    }
+    r.specific_self_potential_energy = S.SpecificSelfPotentialEnergy;
+    r.specific_self_kinetic_energy = S.SpecificSelfKineticEnergy;
+    r.snapshot_index_of_death = S.SnapshotIndexOfDeath;
+    r.snapshot_index_of_sink = S.SnapshotIndexOfSink;
+    r.sink_track_id = S.SinkTrackId;
+    std::vector<int64_t> idx(list_len[s]);
+    for (int64_t i = 0; i < list_len[s]; i++) idx[i] = where[S.Particles[i].Id];
+    std::vector<float> en(list_len[s], __builtin_nanf(""));
+    for (size_t i = 0; i < S.Energies.size() && i < (size_t)list_len[s]; i++) en[i] = S.Energies[i];
+    S.TruncateSource();
+    r.nsource = (int64_t)S.Particles.size();
+    wr(out, &r, 1);
+    wr(out, idx.data(), idx.size());
+    wr(out, en.data(), en.size());
+  }
+  return 0;
+}
+
+int run_tree(FILE *in, FILE *out) {
+  hbt_params p;
+  int64_t ns, nt;
+  rd(in, &p, 1); rd(in, &ns, 1); rd(in, &nt, 1);
+  std::vector<float> sx(ns), sy(ns), sz(ns), sm(ns), tx(nt), ty(nt), tz(nt), tm(nt);
+  rd(in, sx.data(), ns); rd(in, sy.data(), ns); rd(in, sz.data(), ns); rd(in, sm.data(), ns);
+  rd(in, tx.data(), nt); rd(in, ty.data(), nt); rd(in, tz.data(), nt); rd(in, tm.data(), nt);
+  Epoch epoch;
+  configure(p, epoch);
+  Cols src;
+  src.Cosmology = epoch.Cosmology;
+  src.pos.resize(ns);
+  src.mass.assign(sm.begin(), sm.end());
+  for (int64_t i = 0; i < ns; i++) src.pos[i] = {{sx[i], sy[i], sz[i]}};
+  GravityTree_t tree;
+  tree.Reserve(ns);
+  tree.Build(src);
+  std::vector<double> phi(nt);
+  for (int64_t i = 0; i < nt; i++) {
+    HBTxyz t = {{tx[i], ty[i], tz[i]}};
+    phi[i] = tree.EvaluatePotential(t, tm[i]);
+  }
+  wr(out, phi.data(), nt);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc != 4) { fprintf(stderr, "usage: %s unbind|tree in.bin out.bin\n", argv[0]); return 1; }
+  FILE *in = fopen(argv[2], "rb");
+  FILE *out = fopen(argv[3], "wb");
+  if (!in || !out) { fprintf(stderr, "cannot open files\n"); return 1; }
+  int rc = 1;
+  if (!strcmp(argv[1], "unbind")) rc = run_unbind(in, out);
+  else if (!strcmp(argv[1], "tree")) rc = run_tree(in, out);
+  fclose(in);
+  fclose(out);
+  return rc;
+}
