@@ -1,0 +1,1010 @@
+// hbt_kernels.hpp -- the unbinding pass kernels (gfx950).
+//
+// One pass of the reference's Unbind loop (src/subhalo_unbind.cpp:310-422)
+// runs for every active segment (subhalo) of a level at once:
+//   pass_setup        full pass over Elist[0,Nlast) or correction pass over the
+//                     freshly removed block Elist[Nbound,Nlast) (:312-356)
+//   bbox / tree_params / keys
+//                     root cube and per-particle octant digits computed with the
+//                     reference's double-precision centre iteration
+//                     (src/oct_tree.tpp:118-224)
+//   (segmented sort of the keys)
+//   delta / emit / items / moments
+//                     compressed octree in pre-order with skip pointers; node
+//                     moments accumulated in double over float children in son
+//                     order (src/gravity_tree.cpp:18-77)
+//   walk              stackless walk per target (src/gravity_tree.cpp:79-175)
+//   partition_*       parallel replica of PartitionBindingEnergy (:21-58)
+//   frames            FP64 AverageVelocity/AveragePosition, convergence (:357-421)
+//   finalize          permuted list, kinematics, particle-type counts (:405-431)
+#pragma once
+#include "hbt_device.hpp"
+#include "../../include/hbt_unbind.h"
+
+namespace hbt {
+
+struct Work {
+  // per level
+  Seg *seg;
+  int nseg;
+  int64_t nslots;
+  const int *seg_of_slot;
+  const int64_t *ibase;
+  int *nbuf;                 // per-segment element counts for sort/scan calls
+  // particle columns of the caller (device)
+  const int64_t *g_id;
+  const float *g_x, *g_y, *g_z, *g_vx, *g_vy, *g_vz, *g_m;
+  const int32_t *g_type;
+  // working arrays [nslots]
+  int *gidx;
+  float *px, *py, *pz, *vx, *vy, *vz, *pm;
+  int *ptype;
+  float *E;
+  int *el, *el2, *sp;
+  u64 *k0, *k1;
+  u32 *v0, *v1;
+  signed char *dl;
+  int *cmin;                 // [nslots / WAVE]
+  u32 *bits;
+  int *pre, *nlo, *nhi, *npos, *nlist;
+  float4 *ipm;               // [2*nslots]
+  int2 *iax;                 // [2*nslots]: (float bits of len^2, skip)
+  u64 *scan64;
+  int *pk, *gk;
+  unsigned char *isb;
+  int *lvl;                  // [3*NLEVEL+1]: counts, offsets, cursors
+  int *active;               // [1]
+  // caller outputs
+  hbt_subhalo *subs;
+  int64_t *out_index;
+  float *out_energy;
+};
+
+__device__ __forceinline__ int seg_lane_local(const Work &W, int64_t s, int &k) {
+  k = W.seg_of_slot[s];
+  if (k < 0) return -1;
+  return (int)(s - W.ibase[k]);
+}
+
+#define SLOT_LOOP(s, W)                                                              \
+  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < (W).nslots;       \
+       s += (int64_t)gridDim.x * BLOCK)
+
+#define SEG_LOOP(k, W) \
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < (W).nseg; k += gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------------------
+// Level initialisation
+
+// Fills seg_of_slot for the level.
+__global__ void fill_seg_of_slot_kernel(int nseg, const int64_t *ibase, const int *cap_aligned,
+                                        int *seg_of_slot) {
+  for (int k = blockIdx.x; k < nseg; k += gridDim.x) {
+    const int64_t b = ibase[k];
+    const int c = cap_aligned[k];
+    for (int i = threadIdx.x; i < c; i += blockDim.x) seg_of_slot[b + i] = k;
+  }
+}
+
+// Builds the working list of every segment: own particles in source order,
+// then the unbound tails of nested subhalos (read from their output lists,
+// src/subhalo_unbind.cpp:437-443), and gathers the local particle columns.
+// child_ptr/child_list: CSR list of the subhalos nested in each segment.
+__global__ void build_lists_kernel(Work W, const int *child_ptr, const int *child_list) {
+  for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
+    Seg &S = W.seg[k];
+    const hbt_subhalo &r = W.subs[S.sub];
+    const int64_t b = S.ibase;
+    int pos = (int)r.count;
+    for (int i = threadIdx.x; i < (int)r.count; i += blockDim.x) W.gidx[b + i] = (int)(r.offset + i);
+    for (int c = child_ptr[k]; c < child_ptr[k + 1]; c++) {
+      const hbt_subhalo &q = W.subs[child_list[c]];
+      const int tail = (int)(q.list_len - q.nbound);
+      for (int i = threadIdx.x; i < tail; i += blockDim.x)
+        W.gidx[b + pos + i] = (int)W.out_index[q.out_offset + q.nbound + i];
+      pos += tail;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < pos; i += blockDim.x) {
+      const int g = W.gidx[b + i];
+      W.px[b + i] = W.g_x[g]; W.py[b + i] = W.g_y[g]; W.pz[b + i] = W.g_z[g];
+      W.vx[b + i] = W.g_vx[g]; W.vy[b + i] = W.g_vy[g]; W.vz[b + i] = W.g_vz[g];
+      W.pm[b + i] = W.g_m[g];
+      W.ptype[b + i] = W.g_type ? W.g_type[g] : 1;
+    }
+    if (threadIdx.x == 0) S.len = pos;
+    __syncthreads();
+  }
+}
+
+// Unbind entry (src/subhalo_unbind.cpp:263-309): early exits, initial Elist.
+__global__ void seg_init_kernel(Work W, DevParams P) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    hbt_subhalo &r = W.subs[S.sub];
+    const bool orphan = r.nbound_prev <= 1;
+    S.ulen = orphan ? (int)r.count : S.len;
+    S.flags = orphan ? F_ORPHAN : 0;
+    S.iter = 0;
+    S.evals = 0;
+    for (int j = 0; j < 3; j++) { S.ref_pos[j] = r.ref_pos[j]; S.ref_vel[j] = r.ref_vel[j]; }
+    if (S.ulen < P.min_part && r.snapshot_index_of_death == -1) r.snapshot_index_of_death = P.snap;
+    if (S.ulen <= 1) {
+      S.flags |= F_EARLY;
+      S.nbound = S.ulen;
+      S.nlast = S.ulen;
+    } else {
+      S.flags |= F_ACTIVE;
+      S.nbound = S.ulen;
+      S.nlast = S.ulen;
+    }
+  }
+}
+
+__global__ void el_init_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    if (l < W.seg[k].ulen) W.el[s] = l;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass: setup
+
+__global__ void pass_setup_kernel(Work W, DevParams P) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int i = 0; i < 3 * NLEVEL + 1; i++) W.lvl[i] = 0;
+    *W.active = 0;
+  }
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    S.flags &= ~F_FINAL;
+    S.n_src = 0; S.n_tgt = 0; S.src_lo = 0; S.nsort = 0; S.n_items = 0;
+    S.K = 0; S.nP = 0; S.nG = 0;
+    if (!(S.flags & F_ACTIVE)) continue;
+    S.iter++;
+    if (S.flags & F_CORR) {
+      // RelativeVelocity(OldRefPos, OldRefVel, RefPos, RefVel), dK (:314-316)
+      rel_vel(P, S.old_pos[0], S.old_pos[1], S.old_pos[2], S.old_vel[0], S.old_vel[1],
+              S.old_vel[2], S.ref_pos, S.ref_vel, S.rvd);
+      S.dK = (float)(0.5 * (double)vnorm_f(S.rvd));
+      S.src_lo = S.nbound;
+      S.n_src = S.nlast - S.nbound;
+      S.n_tgt = S.nbound;
+      S.nlast = S.nbound;  // Nlast=Nbound after the correction loop (:329)
+    } else {
+      S.nlast = S.nbound;
+      S.src_lo = 0;
+      S.n_src = S.nlast;
+      S.n_tgt = S.nlast;
+    }
+    S.evals += S.n_tgt;
+    S.nsort = S.n_src;
+    for (int j = 0; j < 3; j++) { S.bmin[j] = 0xffffffffu; S.bmax[j] = 0u; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tree: bounding box, root cube, keys
+
+__global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    bool live = false;
+    unsigned int e[3] = {0, 0, 0};
+    if (l >= 0) {
+      const Seg &S = W.seg[k];
+      if (l < S.n_src) {
+        const int64_t b = S.ibase;
+        const int li = W.el[b + S.src_lo + l];
+        e[0] = f2ord(W.px[b + li]); e[1] = f2ord(W.py[b + li]); e[2] = f2ord(W.pz[b + li]);
+        live = true;
+      }
+    }
+    // wave-level pre-reduction when the whole wave works for one segment
+    const int k0 = __shfl(k, 0, WAVE);
+    const bool uniform = __all(live && k == k0);
+    if (uniform) {
+      unsigned int mn[3] = {e[0], e[1], e[2]}, mx[3] = {e[0], e[1], e[2]};
+      for (int d = WAVE / 2; d > 0; d >>= 1)
+        for (int j = 0; j < 3; j++) {
+          mn[j] = min(mn[j], (unsigned int)__shfl_down((int)mn[j], d, WAVE));
+          mx[j] = max(mx[j], (unsigned int)__shfl_down((int)mx[j], d, WAVE));
+        }
+      if ((threadIdx.x & (WAVE - 1)) == 0) {
+        Seg &S = W.seg[k];
+        for (int j = 0; j < 3; j++) { atomicMin(&S.bmin[j], mn[j]); atomicMax(&S.bmax[j], mx[j]); }
+      }
+    } else if (live) {
+      Seg &S = W.seg[k];
+      for (int j = 0; j < 3; j++) { atomicMin(&S.bmin[j], e[j]); atomicMax(&S.bmax[j], e[j]); }
+    }
+  }
+}
+
+// Root cube exactly as OctTree_t::Build (src/oct_tree.tpp:118-139) and the
+// depth schedule: node length halves while >= TreeNodeResolution
+// (src/gravity_tree.cpp:37-40); a node created at depth >= max(1, dlen) is
+// below the resolution floor (src/oct_tree.tpp:200).
+__global__ void tree_params_kernel(Work W, DevParams P) {
+  SEG_LOOP(k, W) {
+#pragma clang fp contract(off)
+    Seg &S = W.seg[k];
+    if (S.n_src <= 0) continue;
+    double xmin[3], xmax[3];
+    for (int j = 0; j < 3; j++) { xmin[j] = ord2f(S.bmin[j]); xmax[j] = ord2f(S.bmax[j]); }
+    double Len = xmax[0] - xmin[0];
+    for (int j = 1; j < 3; j++)
+      if ((xmax[j] - xmin[j]) > Len) Len = xmax[j] - xmin[j];
+    for (int j = 0; j < 3; j++) S.center[j] = 0.5 * (xmax[j] + xmin[j]);
+    S.Len = Len;
+    int dlen = 0;
+    while (dlen < 1100 && ldexp(Len, -dlen) >= (double)P.res) dlen++;
+    S.dlen = dlen;
+    int D = dlen < 1 ? 1 : dlen;
+    if (D > MAX_DEPTH) { D = MAX_DEPTH; S.flags |= F_DEEP; }
+    S.D = D;
+    S.straddle = (P.periodic && Len > 0.5 * (double)P.box_half) ? 1 : 0;
+  }
+}
+
+// Octant digits of each source for depths 1..D: the reference's insertion
+// descends with lenhalf *= 0.5 and centre +/- lenhalf in double and compares
+// the float coordinate with strict '>' (src/oct_tree.tpp:151-184).  Digits
+// are packed most significant first in bits 62..0.
+__global__ void __launch_bounds__(BLOCK) keys_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+#pragma clang fp contract(off)
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.n_src) continue;
+    const int64_t b = S.ibase;
+    const int li = W.el[b + S.src_lo + l];
+    const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
+    double c0 = S.center[0], c1 = S.center[1], c2 = S.center[2];
+    double lh = 0.5 * S.Len;
+    u64 key = 0;
+    for (int d = 0; d < S.D; d++) {
+      lh *= 0.5;
+      int sub = 0;
+      if (x > c0) { c0 += lh; sub |= 1; } else c0 -= lh;
+      if (y > c1) { c1 += lh; sub |= 2; } else c1 -= lh;
+      if (z > c2) { c2 += lh; sub |= 4; } else c2 -= lh;
+      key = (key << 3) | (u64)sub;
+    }
+    key <<= 3 * (MAX_DEPTH - S.D);
+    W.k0[s] = key;
+    W.v0[s] = (u32)li;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tree: structure from sorted keys.  Pair l joins sorted sources l and l+1;
+// delta = number of leading digits they share (D when the keys are equal: the
+// two are in one cell at the depth of the resolution floor).  A node of the
+// compressed octree is a maximal run of sources whose pairs all have
+// delta >= v, split by pairs with delta == v; it sits at depth v.
+
+__device__ __forceinline__ int key_delta(u64 a, u64 b, int D) {
+  if (a == b) return D;
+  int d = (__clzll((long long)(a ^ b)) - 1) / 3;
+  return d < D ? d : D;
+}
+
+__global__ void __launch_bounds__(BLOCK) delta_kernel(Work W, int full_copy_sp) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    int v = 127;
+    if (l >= 0) {
+      const Seg &S = W.seg[k];
+      if (l < S.n_src) {
+        v = (l + 1 < S.n_src) ? key_delta(W.k0[s], W.k0[s + 1], S.D) : -1;
+        W.dl[s] = (signed char)v;
+        W.bits[s] = 0u;
+        if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
+      }
+    }
+    // chunk minimum over the wave (slots of one segment share a wave: bases are
+    // WAVE aligned)
+    int m = v;
+    for (int d = WAVE / 2; d > 0; d >>= 1) m = min(m, __shfl_down(m, d, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0) W.cmin[s / WAVE] = m;
+  }
+}
+
+// first j in [lo_l, l) scanning left from l-1 with dl[j] <= v (strict=false)
+// or dl[j] < v (strict=true); returns lo_l-1 if none.  Uses chunk minima.
+__device__ __forceinline__ int scan_left(const Work &W, int64_t b, int l, int v, bool strict) {
+  int j = l - 1;
+  auto hit = [&](int d) { return strict ? d < v : d <= v; };
+  // within the current chunk
+  while (j >= 0 && ((b + j + 1) % WAVE) != 0) {
+    if (hit(W.dl[b + j])) return j;
+    j--;
+  }
+  // whole chunks
+  while (j >= 0) {
+    const int m = W.cmin[(b + j) / WAVE];
+    if (hit(m)) break;
+    j -= WAVE;
+  }
+  while (j >= 0) {
+    if (hit(W.dl[b + j])) return j;
+    j--;
+  }
+  return -1;
+}
+
+// first j > l with dl[j] < v (the last pair of a segment holds -1)
+__device__ __forceinline__ int scan_right(const Work &W, int64_t b, int l, int v, int n) {
+  int j = l + 1;
+  while (j < n && ((b + j) % WAVE) != 0) {
+    if (W.dl[b + j] < v) return j;
+    j++;
+  }
+  while (j < n && W.cmin[(b + j) / WAVE] >= v) j += WAVE;
+  while (j < n && W.dl[b + j] >= v) j++;
+  return j < n ? j : n - 1;
+}
+
+// Emits the node of which pair l is the leftmost split.
+__global__ void __launch_bounds__(BLOCK) emit_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.n_src) continue;
+    W.nlo[s] = -1;
+    if (l + 1 >= S.n_src) continue;
+    const int v = W.dl[s];
+    const int64_t b = S.ibase;
+    const int L = scan_left(W, b, l, v, false);
+    if (L >= 0 && W.dl[b + L] == v) continue;  // not the leftmost split of its node
+    const int R = scan_right(W, b, l, v, S.n_src);
+    W.nlo[s] = L + 1;
+    W.nhi[s] = R;
+    atomicOr(&W.bits[b + L + 1], 1u << v);
+  }
+}
+
+// Node length at depth v: Len halved while >= resolution, stored as float.
+__device__ __forceinline__ float node_len(const Seg &S, int v) {
+  const int e = v < S.dlen ? v : S.dlen;
+  return (float)ldexp(S.Len, -e);
+}
+
+// Writes leaves and node headers in pre-order.  pre = exclusive prefix of
+// popcount(bits) over the segment's sorted sources.
+__global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.n_src) continue;
+    const int64_t b = S.ibase, ib = 2 * b;
+    const u32 bl = W.bits[s];
+    const int pos = l + W.pre[s] + __popc(bl);
+    const int li = (int)W.v0[s];
+    W.ipm[ib + pos] = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
+    W.iax[ib + pos] = make_int2(0, (int)(ib + pos + 1));
+    if (l + 1 == S.n_src) W.seg[k].n_items = pos + 1;
+    const int lo = W.nlo[s];
+    if (lo >= 0) {
+      const int v = W.dl[s], hi = W.nhi[s];
+      const u32 blo = W.bits[b + lo];
+      const int npos = lo + W.pre[b + lo] + __popc(blo & ((1u << v) - 1u));
+      const int hpos = hi + W.pre[b + hi] + __popc(W.bits[b + hi]);
+      const float len = node_len(S, v);
+      const float lsq = len * len;
+      W.ipm[ib + npos] = make_float4(0.f, 0.f, 0.f, 0.f);
+      W.iax[ib + npos] = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
+      W.npos[s] = (int)(ib + npos);
+      atomicAdd(&W.lvl[v], 1);
+    }
+  }
+}
+
+__global__ void level_offsets_kernel(int *lvl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int o = 0;
+    for (int v = 0; v < NLEVEL; v++) {
+      lvl[NLEVEL + v] = o;
+      o += lvl[v];
+      lvl[2 * NLEVEL + v] = 0;
+    }
+    lvl[3 * NLEVEL] = o;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) node_list_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l + 1 >= S.n_src) continue;
+    if (W.nlo[s] < 0) continue;
+    const int v = W.dl[s];
+    const int at = W.lvl[NLEVEL + v] + atomicAdd(&W.lvl[2 * NLEVEL + v], 1);
+    W.nlist[at] = W.npos[s];
+  }
+}
+
+// Node moments for depth v: mass and centre of mass accumulated in double
+// over the float values of the children in son order, stored as float
+// (UpdateInternalNodes/ProcessNode/FillNodeCenter, src/gravity_tree.cpp:18-77).
+__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v) {
+#pragma clang fp contract(off)
+  const int lo = W.lvl[NLEVEL + v], hi = lo + W.lvl[v];
+  for (int i = lo + blockIdx.x * BLOCK + threadIdx.x; i < hi; i += gridDim.x * BLOCK) {
+    const int g = W.nlist[i];
+    const int end = W.iax[g].y;
+    double mass = 0., cx = 0., cy = 0., cz = 0.;
+    int c = g + 1;
+    while (c < end) {
+      const float4 q = W.ipm[c];
+      const double m = q.w;
+      mass += m;
+      cx += (double)q.x * m;
+      cy += (double)q.y * m;
+      cz += (double)q.z * m;
+      const int nx = W.iax[c].y;
+      c = nx > c ? nx : end;  // skips always advance
+    }
+    W.ipm[g] = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Walk.  Pre-order items: an internal node is opened when
+// len^2 > r^2 * theta^2 (then the next item is its first child), otherwise its
+// monopole is used and the walk continues at its skip pointer; a leaf has
+// len^2 = 0 and skip = next item.
+
+// Reference arithmetic (src/gravity_tree.cpp:79-164): FP64 walk over float data.
+__device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int n_items, float tx,
+                              float ty, float tz, float tmass) {
+#pragma clang fp contract(off)
+  const double h = 2.8 * (double)P.eps, hinv = 1.0 / h;
+  double pot = (double)(float)(tmass / P.eps);
+  const double x = tx, y = ty, z = tz;
+  const int end = (int)(ib + n_items);
+  int j = (int)ib;
+  while (j < end) {
+    const float4 q = W.ipm[j];
+    const int2 a = W.iax[j];
+    double dx = (double)q.x - x, dy = (double)q.y - y, dz = (double)q.z - z;
+    if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const float lsq = __int_as_float(a.x);
+    if ((double)lsq > r2 * (double)P.theta2) { j++; continue; }
+    j = a.y > j ? a.y : end;  // skips always advance
+    pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
+  }
+  return pot * (double)P.G / (double)P.a;
+}
+
+// FP32 walk with Kahan-compensated accumulation.  Straddling segments (a
+// periodic subhalo wider than a quarter box) form the separation in double.
+template <bool STRADDLE>
+__device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n_items, float tx,
+                            float ty, float tz, float tmass) {
+#pragma clang fp contract(fast)
+  const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
+  Kahan acc;
+  acc.init(tmass / P.eps);
+  const int end = (int)(ib + n_items);
+  int j = (int)ib;
+  while (j < end) {
+    const float4 q = W.ipm[j];
+    const int2 a = W.iax[j];
+    float dx, dy, dz;
+    if (STRADDLE) {
+      dx = (float)nearest_d((double)q.x - (double)tx, P);
+      dy = (float)nearest_d((double)q.y - (double)ty, P);
+      dz = (float)nearest_d((double)q.z - (double)tz, P);
+    } else {
+      dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
+    }
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (__int_as_float(a.x) > r2 * P.theta2) { j++; continue; }
+    j = a.y > j ? a.y : end;  // skips always advance
+    acc.add(kernel_term_f(q.w, r2, h2, hinv));
+  }
+  return (double)acc.s * (double)P.G / (double)P.a;
+}
+
+__device__ __forceinline__ double walk_dispatch(const Work &W, const DevParams &P, const Seg &S,
+                                                float tx, float ty, float tz, float tmass) {
+  const int64_t ib = 2 * S.ibase;
+  if (P.replay) return walk_replay(W, P, ib, S.n_items, tx, ty, tz, tmass);
+  if (S.straddle) return walk_fast<true>(W, P, ib, S.n_items, tx, ty, tz, tmass);
+  return walk_fast<false>(W, P, ib, S.n_items, tx, ty, tz, tmass);
+}
+
+// Binding energies of the pass's targets, in spatial order.
+// Full pass: E = 0.5|dv|^2 + phi (BindingEnergy, :341-354).
+// Correction pass: E += OldVel.RefVelDiff + dK - phi_removed (:319-328).
+__global__ void __launch_bounds__(BLOCK) walk_kernel(Work W, DevParams P) {
+  SLOT_LOOP(s, W) {
+#pragma clang fp contract(off)
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.n_tgt) continue;
+    const int64_t b = S.ibase;
+    const int li = W.sp[s];
+    const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
+    const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
+    if (!(S.flags & F_CORR)) {
+      const double pot = walk_dispatch(W, P, S, x, y, z, W.pm[b + li]);
+      float dv[3];
+      rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
+      if (P.replay) {
+        W.E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
+      } else {
+        const double kin = 0.5 * ((double)dv[0] * dv[0] + (double)dv[1] * dv[1] + (double)dv[2] * dv[2]);
+        W.E[b + li] = (float)(kin + pot);
+      }
+    } else {
+      const double pot = walk_dispatch(W, P, S, x, y, z, 0.f);
+      float ov[3];
+      rel_vel(P, x, y, z, vx, vy, vz, S.old_pos, S.old_vel, ov);
+      const float kin = vdot_f(ov, S.rvd) + S.dK;
+      W.E[b + li] = (float)((double)W.E[b + li] + ((double)kin - pot));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partition (PartitionBindingEnergy, src/subhalo_unbind.cpp:21-58), in
+// parallel.  Over positions 1..len-1, p_k is the k-th position from the front
+// holding E>0 and g_k the k-th from the back holding E<0.  The two-pointer
+// loop moves g_k to p_{k-1} (p_0 = 0) and p_k to g_k for k = 1..K, where K is
+// the largest k with p_k < g_k; then g_{K+1} moves to p_K if g_{K+1} > p_K.
+// Element 0 fills the last hole h, and Nbound = h + (E_0 < 0).
+
+__global__ void __launch_bounds__(BLOCK) partition_rank_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l < 1 || l >= S.nlast) continue;
+    const int64_t b = S.ibase;
+    const float e = W.E[b + W.el[s]];
+    const u64 x = W.scan64[s];
+    if (e > 0.f) W.pk[b + (int)(x & 0xffffffffu) + 1] = l;
+    if (e < 0.f) W.gk[b + S.nG - (int)(x >> 32)] = l;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) partition_k_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    Seg &S = W.seg[k];
+    const int m = min(S.nP, S.nG);
+    if (l < 1 || l > m) continue;
+    const int64_t b = S.ibase;
+    if (W.pk[b + l] < W.gk[b + l] && (l == m || W.pk[b + l + 1] >= W.gk[b + l + 1])) S.K = l;
+  }
+}
+
+__global__ void partition_hole_kernel(Work W) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE)) continue;
+    const int64_t b = S.ibase;
+    const int K = S.K;
+    const int pK = K ? W.pk[b + K] : 0;
+    const bool gextra = (K + 1 <= S.nG) && (W.gk[b + K + 1] > pK);
+    S.hole = gextra ? W.gk[b + K + 1] : pK;
+    S.nP = gextra ? K + 1 : K;  // number of g moves (reuse nP)
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) partition_move_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.nlast) continue;
+    const int64_t b = S.ibase;
+    int dest = l;
+    if (l == 0) {
+      dest = S.hole;
+    } else {
+      const float e = W.E[b + W.el[s]];
+      const u64 x = W.scan64[s];
+      if (e > 0.f) {
+        const int r = (int)(x & 0xffffffffu) + 1;
+        if (r <= S.K) dest = W.gk[b + r];
+      } else if (e < 0.f) {
+        const int r = S.nG - (int)(x >> 32);
+        if (r <= S.nP) dest = r == 1 ? 0 : W.pk[b + r - 1];
+      }
+    }
+    W.el2[b + dest] = W.el[s];
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) partition_copy_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    if (l < W.seg[k].nlast) W.el[s] = W.el2[s];
+  }
+}
+
+// Nbound after the partition; disruption test (:357-379).
+__global__ void partition_done_kernel(Work W, DevParams P) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE)) continue;
+    const int64_t b = S.ibase;
+    // the original element 0 ends in the last hole (Etmp, :37-38 and :50-51)
+    int nb = S.hole + (W.E[b + W.el[b + S.hole]] < 0.f ? 1 : 0);
+    if (S.nlast == 0) nb = 0;
+    S.nbound = nb;
+    if (nb < P.min_part) {
+      S.flags = (S.flags & ~F_ACTIVE) | F_DISR | F_FINAL;
+      S.nbound = 1;
+      S.nlast = 1;
+      S.nsort = 0;
+    } else {
+      S.nsort = S.nlast - S.nbound;  // removed block to sort
+    }
+  }
+}
+
+// spatial list maintenance: isb[l] = particle l is still bound
+__global__ void __launch_bounds__(BLOCK) bound_flags_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE) || l >= S.nlast) continue;
+    W.isb[S.ibase + W.el[s]] = l < S.nbound ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) spatial_compact_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE) || l >= S.nlast) continue;
+    const int li = W.sp[s];
+    if (W.isb[S.ibase + li]) W.el2[S.ibase + W.pre[s]] = li;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) spatial_copy_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE) || l >= S.nbound) continue;
+    W.sp[s] = W.el2[s];
+  }
+}
+
+// sort keys for a block of Elist by energy: (ordered E, position) -> local index
+__global__ void __launch_bounds__(BLOCK) energy_keys_kernel(Work W, int bound_block) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.nsort) continue;
+    const int64_t b = S.ibase;
+    const int at = bound_block ? l : S.nbound + l;
+    const int li = W.el[b + at];
+    W.k0[s] = ((u64)f2ord(W.E[b + li]) << 32) | (u64)(unsigned)l;
+    W.v0[s] = (u32)li;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) energy_writeback_kernel(Work W, int bound_block) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.nsort) continue;
+    const int at = bound_block ? l : S.nbound + l;
+    W.el[S.ibase + at] = (int)W.v0[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Frame update (:380-421): correction switch, FP64 mass-weighted averages of
+// the bound block in Elist order, convergence.  One block per segment.
+__global__ void __launch_bounds__(BLOCK) frames_kernel(Work W, DevParams P) {
+#pragma clang fp contract(off)
+  for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE)) continue;  // uniform
+    const int64_t b = S.ibase;
+    const int nb = S.nbound;
+    const int l0 = W.el[b];
+    double msum = 0, s1 = 0, s2 = 0, s3 = 0, q1 = 0, q2 = 0, q3 = 0;
+    double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
+    if (nb > 1) {
+      if (P.replay) {
+        if (threadIdx.x == 0) {
+          for (int i = 0; i < nb; i++) {
+            const int li = W.el[b + i];
+            const float m = W.pm[b + li];
+            msum += m;
+            s1 += W.vx[b + li] * m; s2 += W.vy[b + li] * m; s3 += W.vz[b + li] * m;
+            if (P.periodic) {
+              q1 += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
+              q2 += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
+              q3 += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
+            } else {
+              q1 += W.px[b + li] * m; q2 += W.py[b + li] * m; q3 += W.pz[b + li] * m;
+            }
+          }
+        }
+      } else {
+        for (int i = threadIdx.x; i < nb; i += BLOCK) {
+          const int li = W.el[b + i];
+          const float m = W.pm[b + li];
+          msum += m;
+          s1 += W.vx[b + li] * m; s2 += W.vy[b + li] * m; s3 += W.vz[b + li] * m;
+          if (P.periodic) {
+            q1 += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
+            q2 += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
+            q3 += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
+          } else {
+            q1 += W.px[b + li] * m; q2 += W.py[b + li] * m; q3 += W.pz[b + li] * m;
+          }
+        }
+        double v[7] = {msum, s1, s2, s3, q1, q2, q3};
+        __shared__ double r7[7][BLOCK / WAVE];
+        for (int c = 0; c < 7; c++) {
+          double t = wave_reduce_sum(v[c]);
+          if ((threadIdx.x & (WAVE - 1)) == 0) r7[c][threadIdx.x / WAVE] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          for (int c = 0; c < 7; c++) {
+            double t = 0;
+            for (int w = 0; w < BLOCK / WAVE; w++) t += r7[c][w];
+            v[c] = t;
+          }
+          msum = v[0]; s1 = v[1]; s2 = v[2]; s3 = v[3]; q1 = v[4]; q2 = v[5]; q3 = v[6];
+        }
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) {
+      // correction switch uses the frame of the pass just done (:383-392)
+      const int ndiff = S.nlast - nb;
+      if (ndiff < nb) {
+        S.flags |= F_CORR;
+        for (int j = 0; j < 3; j++) { S.old_pos[j] = S.ref_pos[j]; S.old_vel[j] = S.ref_vel[j]; }
+      }
+      if (nb == 0) {
+        msum = 0.;
+      } else if (nb == 1) {
+        S.ref_vel[0] = W.vx[b + l0]; S.ref_vel[1] = W.vy[b + l0]; S.ref_vel[2] = W.vz[b + l0];
+        msum = W.pm[b + l0];
+        S.ref_pos[0] = W.px[b + l0]; S.ref_pos[1] = W.py[b + l0]; S.ref_pos[2] = W.pz[b + l0];
+      } else {
+        S.ref_vel[0] = (float)(s1 / msum); S.ref_vel[1] = (float)(s2 / msum); S.ref_vel[2] = (float)(s3 / msum);
+        double q[3] = {q1 / msum, q2 / msum, q3 / msum};
+        for (int j = 0; j < 3; j++) {
+          if (P.periodic) q[j] += o[j];
+          S.ref_pos[j] = (float)q[j];
+        }
+      }
+      S.mbound = (float)msum;
+      if ((float)nb >= (float)S.nlast * P.bmp) {
+        S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_FINAL;
+        S.nsort = nb;  // bound block to sort
+      } else {
+        S.nsort = 0;
+        atomicAdd(W.active, 1);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Outputs for segments that finished in this pass, early exits and orphans.
+
+// per-slot: output list and energies
+__global__ void __launch_bounds__(BLOCK) output_list_kernel(Work W, int early) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    const bool fin = early ? (S.flags & F_EARLY) != 0 : (S.flags & F_FINAL) != 0;
+    if (!fin || l >= S.len) continue;
+    const int64_t o = S.out_off + l;
+    if ((S.flags & F_CONV) && !(S.flags & F_ORPHAN)) {
+      const int li = l < S.ulen ? W.el[s] : l;
+      W.out_index[o] = W.gidx[S.ibase + li];
+      W.out_energy[o] = l < S.ulen ? W.E[S.ibase + li] : __builtin_nanf("");
+    } else {
+      // disrupted, early, orphan: the list keeps its order
+      W.out_index[o] = W.gidx[s];
+      W.out_energy[o] = __builtin_nanf("");
+    }
+  }
+}
+
+// per-segment (block): kinematics, type counts, record fields
+__global__ void __launch_bounds__(BLOCK) output_record_kernel(Work W, DevParams P, int early) {
+#pragma clang fp contract(off)
+  __shared__ double r7[7][BLOCK / WAVE];
+  __shared__ double rt[HBT_TYPE_MAX][BLOCK / WAVE];
+  __shared__ int ct[HBT_TYPE_MAX][BLOCK / WAVE];
+  for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
+    Seg &S = W.seg[k];
+    const bool fin = early ? (S.flags & F_EARLY) != 0 : (S.flags & F_FINAL) != 0;
+    if (!fin) continue;  // uniform
+    hbt_subhalo &r = W.subs[S.sub];
+    const int64_t b = S.ibase;
+    const bool conv = (S.flags & F_CONV) != 0;
+    const int nb = S.nbound;
+    // particle at list position i (Elist order for converged, list order otherwise)
+    auto at = [&](int i) { return conv ? W.el[b + i] : i; };
+    // AverageKinematics (:189-232) over [0, nb) in the frame RefPos/RefVel
+    double E = 0, K = 0, A0 = 0, A1 = 0, A2 = 0, M = 0;
+    double mt[HBT_TYPE_MAX];
+    int nt[HBT_TYPE_MAX];
+    for (int t = 0; t < HBT_TYPE_MAX; t++) { mt[t] = 0; nt[t] = 0; }
+    float mts[HBT_TYPE_MAX];
+    const bool serial = P.replay != 0;
+    // CountParticleTypes skips null IDs only for Nbound <= 100; CountParticles never
+    const bool skip_null = !(S.flags & F_EARLY) && nb <= 100;
+    auto accum = [&](int i) {
+      const int li = at(i);
+      const float m = W.pm[b + li];
+      if (nb > 1) {
+        E += W.E[b + li] * m;
+        const float x[3] = {W.px[b + li], W.py[b + li], W.pz[b + li]};
+        const float v[3] = {W.vx[b + li], W.vy[b + li], W.vz[b + li]};
+        double dx[3], dv[3];
+        for (int j = 0; j < 3; j++) {
+          dx[j] = x[j] - S.ref_pos[j];
+          if (P.periodic) dx[j] = nearest_d(dx[j], P);
+          dx[j] *= P.a;
+          dv[j] = (double)(v[j] - S.ref_vel[j]) + P.hz * dx[j];
+          K += dv[j] * dv[j] * m;
+        }
+        A0 += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
+        A1 += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
+        A2 += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
+        M += m;
+      }
+    };
+    if (serial) {
+      if (threadIdx.x == 0) {
+        for (int i = 0; i < nb; i++) accum(i);
+        for (int t = 0; t < HBT_TYPE_MAX; t++) mts[t] = 0.f;
+        // CountParticleTypes (src/subhalo.cpp:400-444): float sums in list order;
+        // the Nbound<=100 branch skips null IDs
+        for (int i = 0; i < nb; i++) {
+          const int li = at(i);
+          if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
+          const int t = W.ptype[b + li];
+          nt[t]++;
+          mts[t] += W.pm[b + li];
+        }
+      }
+    } else {
+      for (int i = threadIdx.x; i < nb; i += BLOCK) {
+        accum(i);
+        const int li = at(i);
+        if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
+        const int t = W.ptype[b + li];
+        nt[t]++;
+        mt[t] += W.pm[b + li];
+      }
+      double v[7] = {E, K, A0, A1, A2, M, 0};
+      for (int c = 0; c < 6; c++) {
+        double t = wave_reduce_sum(v[c]);
+        if ((threadIdx.x & (WAVE - 1)) == 0) r7[c][threadIdx.x / WAVE] = t;
+      }
+      for (int t = 0; t < HBT_TYPE_MAX; t++) {
+        double a = wave_reduce_sum(mt[t]);
+        int c = wave_reduce_sum(nt[t]);
+        if ((threadIdx.x & (WAVE - 1)) == 0) { rt[t][threadIdx.x / WAVE] = a; ct[t][threadIdx.x / WAVE] = c; }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int c = 0; c < 6; c++) {
+          double t = 0;
+          for (int w = 0; w < BLOCK / WAVE; w++) t += r7[c][w];
+          v[c] = t;
+        }
+        E = v[0]; K = v[1]; A0 = v[2]; A1 = v[3]; A2 = v[4]; M = v[5];
+        for (int t = 0; t < HBT_TYPE_MAX; t++) {
+          double a = 0;
+          int c = 0;
+          for (int w = 0; w < BLOCK / WAVE; w++) { a += rt[t][w]; c += ct[t][w]; }
+          mts[t] = (float)a;
+          nt[t] = c;
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const bool early_exit = (S.flags & F_EARLY) != 0;
+      if (early_exit) {
+        // CountParticles (src/subhalo.cpp:479-499): Mbound = float sum of MboundType
+        float mb = 0.f;
+        for (int t = 0; t < HBT_TYPE_MAX; t++) mb += mts[t];
+        r.mbound = mb;
+        r.status = HBT_STATUS_EARLY;
+      } else {
+        if (nb <= 1) {
+          r.specific_self_potential_energy = 0.f;
+          r.specific_self_kinetic_energy = 0.f;
+          r.specific_angular_momentum[0] = r.specific_angular_momentum[1] = r.specific_angular_momentum[2] = 0.f;
+        } else {
+          E /= M;
+          K *= 0.5 / M;
+          r.specific_self_potential_energy = (float)(E - K);
+          r.specific_self_kinetic_energy = (float)K;
+          r.specific_angular_momentum[0] = (float)(A0 / M);
+          r.specific_angular_momentum[1] = (float)(A1 / M);
+          r.specific_angular_momentum[2] = (float)(A2 / M);
+        }
+        if (conv) {
+          r.status = HBT_STATUS_CONVERGED;
+          if (r.snapshot_index_of_death != -1) r.snapshot_index_of_death = -1;
+          if (r.sink_track_id != -1) { r.snapshot_index_of_sink = -1; r.sink_track_id = -1; }
+          for (int j = 0; j < 3; j++) { r.ref_pos[j] = S.ref_pos[j]; r.ref_vel[j] = S.ref_vel[j]; }
+          const int l0 = W.el[b];
+          r.mostbound_pos[0] = W.px[b + l0]; r.mostbound_pos[1] = W.py[b + l0]; r.mostbound_pos[2] = W.pz[b + l0];
+          r.mostbound_vel[0] = W.vx[b + l0]; r.mostbound_vel[1] = W.vy[b + l0]; r.mostbound_vel[2] = W.vz[b + l0];
+          r.mbound = S.mbound;
+        } else {
+          // disruption (:361-379): frame <- previous most-bound coordinates
+          r.status = HBT_STATUS_DISRUPTED;
+          if (r.snapshot_index_of_death == -1) r.snapshot_index_of_death = P.snap;
+          for (int j = 0; j < 3; j++) { r.ref_pos[j] = r.mostbound_pos[j]; r.ref_vel[j] = r.mostbound_vel[j]; }
+          r.mbound = W.pm[b];
+        }
+      }
+      for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = nt[t]; r.mbound_type[t] = mts[t]; }
+      r.nbound = nb;
+      r.list_len = S.len;
+      long long ns = nb <= 1 ? nb : (long long)((float)nb * P.relax);
+      if (ns > S.len) ns = S.len;
+      r.nsource = ns;
+      r.n_iter = S.iter;
+      r.n_evals = S.evals;
+      if (S.flags & F_ORPHAN) {
+        // RecursiveUnbind keeps the orphan's extended list in its original order
+      }
+      S.flags &= ~F_FINAL;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace hbt

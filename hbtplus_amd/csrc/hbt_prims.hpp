@@ -1,0 +1,229 @@
+// hbt_prims.hpp -- segmented device primitives for gfx950 (wave64).
+//
+// Every batch is a set of segments (one per subhalo).  Segment k owns the
+// slot range [base[k], base[k] + cap[k]) of every per-particle work array;
+// the number of live elements of the current operation is a per-segment
+// device value (n[k]), so launches have static shapes (graph friendly) and
+// work inside them is data dependent.  Segments are cut into TILE-element
+// tiles (Tile table, built once per batch) for block-level work.
+//
+// Primitives:
+//   segmented sort of (u64 key, u32 value) pairs by (key, value):
+//     tile_sort_kernel    : LDS bitonic sort of each tile
+//     merge_pass_kernel   : one bottom-up merge level (co-rank by binary search)
+//   segmented exclusive scan (load/store functors):
+//     scan_tiles_kernel / scan_tile_offsets_kernel / scan_add_kernel
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hbt {
+
+typedef unsigned long long u64;
+typedef unsigned int u32;
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int TILE = 1024;           // elements per tile (4 per thread)
+constexpr int ITEMS = TILE / BLOCK;
+
+struct Tile {
+  int seg;     // segment index
+  int start;   // local offset of the tile inside the segment
+};
+
+__device__ __forceinline__ bool pair_less(u64 ka, u32 va, u64 kb, u32 vb) {
+  return ka < kb || (ka == kb && va < vb);
+}
+
+// ---- wave / block scans ----------------------------------------------------
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
+  const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    T y = __shfl_up(x, d, WAVE);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_reduce_sum(T x) {
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_down(x, d, WAVE);
+  return x;  // valid in lane 0
+}
+
+// Exclusive scan over the block (BLOCK threads, one value each); returns the
+// exclusive prefix and writes the block total to *total.
+template <class T>
+__device__ __forceinline__ T block_exclusive_scan(T x, T *total) {
+  __shared__ T wsum[BLOCK / WAVE];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  T inc = wave_inclusive_scan(x);
+  if (lane == WAVE - 1) wsum[w] = inc;
+  __syncthreads();
+  T off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / WAVE; i++) {
+    if (i < w) off += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - x;
+}
+
+// ---- segmented sort ----------------------------------------------------------
+
+// Sorts each tile's live elements [start, min(start+TILE, n[seg])) in place.
+__global__ void __launch_bounds__(BLOCK)
+tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__restrict__ base,
+                 const int *__restrict__ n, u64 *__restrict__ keys, u32 *__restrict__ vals) {
+  __shared__ u64 sk[TILE];
+  __shared__ u32 sv[TILE];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const int cnt = min(TILE, n[tl.seg] - tl.start);
+    if (cnt <= 1) continue;  // uniform across the block
+    int P = 1;
+    while (P < cnt) P <<= 1;
+    const int64_t off = base[tl.seg] + tl.start;
+    for (int i = threadIdx.x; i < P; i += BLOCK) {
+      if (i < cnt) { sk[i] = keys[off + i]; sv[i] = vals[off + i]; }
+      else { sk[i] = ~0ull; sv[i] = 0xffffffffu; }
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int p = threadIdx.x; p < (P >> 1); p += BLOCK) {
+          const int i = 2 * j * (p / j) + (p % j);
+          const int q = i + j;
+          const bool up = (i & k) == 0;
+          u64 ka = sk[i], kb = sk[q];
+          u32 va = sv[i], vb = sv[q];
+          if (pair_less(kb, vb, ka, va) == up) {
+            sk[i] = kb; sk[q] = ka; sv[i] = vb; sv[q] = va;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) { keys[off + i] = sk[i]; vals[off + i] = sv[i]; }
+    __syncthreads();
+  }
+}
+
+// One merge level: sorted runs of `width` elements are merged pairwise from
+// (kin, vin) into (kout, vout).  Elements of segments with n <= width are
+// copied.  seg_of_slot maps a slot to its segment.
+__global__ void __launch_bounds__(BLOCK)
+merge_pass_kernel(int64_t nslots, const int *__restrict__ seg_of_slot,
+                  const int64_t *__restrict__ base, const int *__restrict__ n, int width,
+                  const u64 *__restrict__ kin, const u32 *__restrict__ vin, u64 *__restrict__ kout,
+                  u32 *__restrict__ vout) {
+  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < nslots;
+       s += (int64_t)gridDim.x * BLOCK) {
+    const int k = seg_of_slot[s];
+    if (k < 0) continue;
+    const int64_t b = base[k];
+    const int l = (int)(s - b), cnt = n[k];
+    if (l >= cnt) continue;
+    const u64 key = kin[s];
+    const u32 val = vin[s];
+    if (cnt <= width) { kout[s] = key; vout[s] = val; continue; }
+    const int run = l / width;
+    const int a0 = (run & ~1) * width;  // start of the merged pair
+    int pos;
+    if ((run & 1) == 0) {
+      // count elements of the right run that precede (key,val)
+      int lo = a0 + width, hi = min(a0 + 2 * width, cnt);
+      const int b0 = lo;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
+      }
+      pos = (l - a0) + (lo - b0);
+    } else {
+      int lo = a0, hi = a0 + width;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
+      }
+      pos = (l - a0 - width) + (lo - a0);
+    }
+    kout[b + a0 + pos] = key;
+    vout[b + a0 + pos] = val;
+  }
+}
+
+// ---- segmented exclusive scan ----------------------------------------------
+// Load(seg, local) -> T for local < n[seg]; Store(seg, local, exclusive prefix).
+// Pass 1 scans each tile and stores tile-local prefixes and the tile total;
+// pass 2 scans the tile totals of each segment (tile_off, seg_total);
+// pass 3 adds the tile offsets.
+
+template <class T, class Load, class Store>
+__global__ void __launch_bounds__(BLOCK)
+scan_tiles_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restrict__ n,
+                  T *__restrict__ tile_sum, Load load, Store store) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const int cnt = min(TILE, n[tl.seg] - tl.start);
+    if (cnt <= 0) {
+      if (threadIdx.x == 0) tile_sum[t] = T(0);
+      continue;
+    }
+    T v[ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+      const int l = threadIdx.x * ITEMS + i;
+      v[i] = l < cnt ? load(tl.seg, tl.start + l) : T(0);
+      s += v[i];
+    }
+    T total;
+    T off = block_exclusive_scan(s, &total);
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+      const int l = threadIdx.x * ITEMS + i;
+      if (l < cnt) store(tl.seg, tl.start + l, off);
+      off += v[i];
+    }
+    if (threadIdx.x == 0) tile_sum[t] = total;
+  }
+}
+
+template <class T>
+__global__ void scan_tile_offsets_kernel(int nseg, const int *__restrict__ first_tile,
+                                         const int *__restrict__ ntile_of, const int *__restrict__ n,
+                                         T *__restrict__ tile_sum, T *__restrict__ seg_total) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nseg; k += gridDim.x * blockDim.x) {
+    const int f = first_tile[k];
+    const int nt = min(ntile_of[k], (n[k] + TILE - 1) / TILE);
+    T run = 0;
+    for (int t = f; t < f + nt; t++) {
+      T x = tile_sum[t];
+      tile_sum[t] = run;
+      run += x;
+    }
+    if (seg_total) seg_total[k] = run;
+  }
+}
+
+template <class T, class Update>
+__global__ void __launch_bounds__(BLOCK)
+scan_add_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restrict__ n,
+                const T *__restrict__ tile_off, Update update) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    if (tl.start == 0) continue;  // first tile of a segment has offset 0
+    const int cnt = min(TILE, n[tl.seg] - tl.start);
+    const T o = tile_off[t];
+    for (int l = threadIdx.x; l < cnt; l += BLOCK) update(tl.seg, tl.start + l, o);
+  }
+}
+
+}  // namespace hbt

@@ -1,0 +1,733 @@
+// hbt_unbind.hip -- host orchestration and C-ABI of libhbt_unbind.so (gfx950).
+//
+// A batch of subhalos is processed level by level: a level is a set of
+// subhalos whose nested children are all done (RecursiveUnbind order,
+// src/subhalo_unbind.cpp:432-447).  Batched form: one level per nest height,
+// all its subhalos advance through the unbinding passes together.  Plain form:
+// one subhalo per level, in the reference's depth-first order -- the same
+// kernels driven one subhalo at a time, as the reference's RefineParticles loop
+// calls Subhalo_t::Unbind per subhalo.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hbt_unbind.h"
+#include "hbt_kernels.hpp"
+
+using namespace hbt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return fail(HBT_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+#define RETCHK(x)              \
+  do {                         \
+    int r_ = (x);              \
+    if (r_ != HBT_OK) return r_; \
+  } while (0)
+
+DevParams make_dev_params(const hbt_params &p) {
+  DevParams d;
+  d.G = p.G;
+  d.eps = p.softening;
+  d.theta2 = p.open_angle * p.open_angle;
+  d.res = (float)((double)p.softening * 0.1);
+  d.box = p.box_size;
+  d.box_half = (float)((double)p.box_size / 2.);
+  d.bmp = p.bound_mass_precision;
+  d.relax = p.source_sub_relax_factor;
+  d.a = p.scale_factor;
+  d.hz = p.hz;
+  d.periodic = p.periodic != 0;
+  d.min_part = p.min_num_part_of_sub;
+  d.snap = p.snapshot_index;
+  d.replay = p.precision == HBT_PREC_REPLAY;
+  return d;
+}
+
+// Grow-only device allocation.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return HBT_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (want == 0) return HBT_OK;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) return fail(HBT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+    bytes = want;
+    return HBT_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return (T *)p; }
+};
+
+}  // namespace
+
+struct hbt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hbt_params params{};
+  int cus = 256;
+  // per-slot work arrays
+  DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, cmin, bits, pre, nlo, nhi,
+      npos, nlist, ipm, iax, scan64, pk, gk, isb, seg_of_slot;
+  // per-level tables
+  DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal;
+  DBuf lvl, active;
+  // staging for hbt_unbind_one / hbt_tree_potential
+  DBuf st_cols, st_subs, st_out_i, st_out_e, st_phi;
+  int *h_active = nullptr;  // pinned
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  int64_t last_evals = 0, last_passes = 0;
+  ~hbt_ctx() {
+    for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
+                    &cmin, &bits, &pre, &nlo, &nhi, &npos, &nlist, &ipm, &iax, &scan64, &pk, &gk,
+                    &isb, &seg_of_slot, &seg, &ibase, &nbuf, &tiles, &first_tile, &ntile, &tsum,
+                    &stot, &child_ptr, &child_list, &capal, &lvl, &active, &st_cols, &st_subs,
+                    &st_out_i, &st_out_e, &st_phi})
+      b->release();
+    if (h_active) (void)hipHostFree(h_active);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Level description built on the host.
+struct Level {
+  std::vector<int> subs;        // subhalo indices
+  std::vector<Seg> seg;
+  std::vector<int64_t> ibase;
+  std::vector<int> capal, first_tile, ntile, child_ptr, child_list;
+  std::vector<Tile> tiles;
+  int64_t nslots = 0;
+  int maxcap = 0;
+};
+
+void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
+                 const std::vector<std::vector<int>> &children, const std::vector<int> &members,
+                 Level &L) {
+  const int n = (int)members.size();
+  L.subs = members;
+  L.seg.assign(n, Seg{});
+  L.ibase.resize(n);
+  L.capal.resize(n);
+  L.first_tile.resize(n);
+  L.ntile.resize(n);
+  L.child_ptr.assign(n + 1, 0);
+  L.child_list.clear();
+  L.tiles.clear();
+  int64_t off = 0;
+  L.maxcap = 0;
+  for (int k = 0; k < n; k++) {
+    const int s = members[k];
+    const int c = (int)cap[s];
+    const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
+    Seg &S = L.seg[k];
+    S.ibase = off;
+    S.out_off = subs[s].out_offset;
+    S.sub = s;
+    S.cap = c;
+    S.count = (int)subs[s].count;
+    L.ibase[k] = off;
+    L.capal[k] = ca;
+    L.first_tile[k] = (int)L.tiles.size();
+    const int nt = (c + TILE - 1) / TILE;
+    L.ntile[k] = nt;
+    for (int t = 0; t < nt; t++) L.tiles.push_back(Tile{k, t * TILE});
+    off += ca;
+    L.maxcap = std::max(L.maxcap, c);
+    for (int ch : children[s]) L.child_list.push_back(ch);
+    L.child_ptr[k + 1] = (int)L.child_list.size();
+  }
+  L.nslots = off;
+}
+
+template <class T>
+int upload(DBuf &b, const std::vector<T> &v, hipStream_t st) {
+  RETCHK(b.ensure(std::max<size_t>(v.size() * sizeof(T), 16)));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return HBT_OK;
+}
+
+int grid_for(int64_t n, int cus) {
+  int64_t g = (n + BLOCK - 1) / BLOCK;
+  int64_t cap = (int64_t)cus * 16;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// ---- functors for the segmented scans ---------------------------------------
+struct PopLoad {
+  const int64_t *ibase; const u32 *bits;
+  __device__ u64 operator()(int k, int l) const { return (u64)__popc(bits[ibase[k] + l]); }
+};
+struct PreStore {
+  const int64_t *ibase; int *pre;
+  __device__ void operator()(int k, int l, u64 x) const { pre[ibase[k] + l] = (int)x; }
+};
+struct PreAdd {
+  const int64_t *ibase; int *pre;
+  __device__ void operator()(int k, int l, u64 x) const { pre[ibase[k] + l] += (int)x; }
+};
+struct PartLoad {
+  const int64_t *ibase; const int *el; const float *E;
+  __device__ u64 operator()(int k, int l) const {
+    if (l < 1) return 0;
+    const float e = E[ibase[k] + el[ibase[k] + l]];
+    return (u64)(e > 0.f ? 1u : 0u) | ((u64)(e < 0.f ? 1u : 0u) << 32);
+  }
+};
+struct S64Store {
+  const int64_t *ibase; u64 *out;
+  __device__ void operator()(int k, int l, u64 x) const { out[ibase[k] + l] = x; }
+};
+struct S64Add {
+  const int64_t *ibase; u64 *out;
+  __device__ void operator()(int k, int l, u64 x) const { out[ibase[k] + l] += x; }
+};
+struct BoundLoad {
+  const int64_t *ibase; const int *sp; const unsigned char *isb;
+  __device__ u64 operator()(int k, int l) const { return (u64)isb[ibase[k] + sp[ibase[k] + l]]; }
+};
+
+enum NMode { N_SORT = 0, N_SRC = 1, N_NLAST_ACTIVE = 2 };
+
+__global__ void set_n_kernel(Work W, int mode) {
+  SEG_LOOP(k, W) {
+    const Seg &S = W.seg[k];
+    int v = 0;
+    if (mode == N_SORT) v = S.nsort;
+    else if (mode == N_SRC) v = S.n_src;
+    else v = (S.flags & F_ACTIVE) ? S.nlast : 0;
+    W.nbuf[k] = v;
+  }
+}
+
+__global__ void partition_counts_kernel(Work W, const u64 *stot) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE)) { S.nP = S.nG = 0; continue; }
+    const u64 t = stot[k];
+    S.nP = (int)(t & 0xffffffffu);
+    S.nG = (int)(t >> 32);
+    S.K = 0;
+  }
+}
+
+__global__ void copy_back_kernel(int64_t n, const u64 *k1, const u32 *v1, u64 *k0, u32 *v0) {
+  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < n; s += (int64_t)gridDim.x * BLOCK) {
+    k0[s] = k1[s];
+    v0[s] = v1[s];
+  }
+}
+
+struct Runner {
+  hbt_ctx *c;
+  Work W;
+  DevParams P;
+  Level *L;
+  int gs, gseg, gtile;
+
+  int sort_pairs() {  // sorts (k0,v0) by nbuf, result in (k0,v0)
+    tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), (int)L->tiles.size(),
+                                                     W.ibase, W.nbuf, W.k0, W.v0);
+    u64 *ki = W.k0, *ko = W.k1;
+    u32 *vi = W.v0, *vo = W.v1;
+    int passes = 0;
+    for (int w = TILE; w < L->maxcap; w *= 2) {
+      merge_pass_kernel<<<gs, BLOCK, 0, c->stream>>>(W.nslots, W.seg_of_slot, W.ibase, W.nbuf, w, ki,
+                                                     vi, ko, vo);
+      std::swap(ki, ko);
+      std::swap(vi, vo);
+      passes++;
+    }
+    if (passes & 1)
+      copy_back_kernel<<<gs, BLOCK, 0, c->stream>>>(W.nslots, W.k1, W.v1, W.k0, W.v0);
+    return HBT_OK;
+  }
+
+  template <class Load, class Store, class Add>
+  int seg_scan(Load ld, Store st, Add ad, u64 *seg_total) {
+    const int nt = (int)L->tiles.size();
+    u64 *ts = c->tsum.as<u64>();
+    scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, W.nbuf, ts, ld, st);
+    scan_tile_offsets_kernel<u64><<<gseg, BLOCK, 0, c->stream>>>(
+        W.nseg, c->first_tile.as<int>(), c->ntile.as<int>(), W.nbuf, ts, seg_total);
+    scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, W.nbuf, ts, ad);
+    return HBT_OK;
+  }
+
+  // tree over the pass's sources: keys, sort, structure, moments
+  int build_tree(bool copy_sp) {
+    bbox_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SRC);
+    RETCHK(sort_pairs());
+    delta_kernel<<<gs, BLOCK, 0, c->stream>>>(W, copy_sp ? 1 : 0);
+    emit_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
+                    (u64 *)nullptr));
+    items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    level_offsets_kernel<<<1, 1, 0, c->stream>>>(W.lvl);
+    node_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    for (int v = MAX_DEPTH; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
+    return HBT_OK;
+  }
+
+  int pass() {
+    pass_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    RETCHK(build_tree(true));
+    walk_kernel<<<gs, BLOCK, 0, c->stream>>>(W, P);
+    // partition
+    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_NLAST_ACTIVE);
+    u64 *stot = c->stot.as<u64>();
+    RETCHK(seg_scan(PartLoad{W.ibase, W.el, W.E}, S64Store{W.ibase, W.scan64},
+                    S64Add{W.ibase, W.scan64}, stot));
+    partition_counts_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, stot);
+    partition_rank_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    partition_k_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    partition_hole_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
+    partition_move_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    partition_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    partition_done_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    // spatial target list keeps the bound particles
+    bound_flags_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_NLAST_ACTIVE);
+    RETCHK(seg_scan(BoundLoad{W.ibase, W.sp, W.isb}, PreStore{W.ibase, W.pre},
+                    PreAdd{W.ibase, W.pre}, (u64 *)nullptr));
+    spatial_compact_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    spatial_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    // sort the removed block by energy
+    energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
+    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
+    RETCHK(sort_pairs());
+    energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
+    // frames, convergence
+    frames_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    // converged: sort the bound block by energy
+    energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
+    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
+    RETCHK(sort_pairs());
+    energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
+    output_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
+    output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
+    HIPCHK(hipGetLastError());
+    return HBT_OK;
+  }
+};
+
+int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
+  const size_t s = (size_t)S;
+  RETCHK(c->gidx.ensure(s * 4));
+  RETCHK(c->pxyz.ensure(s * 12));
+  RETCHK(c->vxyz.ensure(s * 12));
+  RETCHK(c->pm.ensure(s * 4));
+  RETCHK(c->ptype.ensure(s * 4));
+  RETCHK(c->E.ensure(s * 4));
+  RETCHK(c->el.ensure(s * 4));
+  RETCHK(c->el2.ensure(s * 4));
+  RETCHK(c->sp.ensure(s * 4));
+  RETCHK(c->k0.ensure(s * 8));
+  RETCHK(c->k1.ensure(s * 8));
+  RETCHK(c->v0.ensure(s * 4));
+  RETCHK(c->v1.ensure(s * 4));
+  RETCHK(c->dl.ensure(s));
+  RETCHK(c->cmin.ensure((s / WAVE + 1) * 4));
+  RETCHK(c->bits.ensure(s * 4));
+  RETCHK(c->pre.ensure(s * 4));
+  RETCHK(c->nlo.ensure(s * 4));
+  RETCHK(c->nhi.ensure(s * 4));
+  RETCHK(c->npos.ensure(s * 4));
+  RETCHK(c->nlist.ensure(s * 4));
+  RETCHK(c->ipm.ensure(s * 2 * 16));
+  RETCHK(c->iax.ensure(s * 2 * 8));
+  RETCHK(c->scan64.ensure(s * 8));
+  RETCHK(c->pk.ensure((s + 1) * 4));
+  RETCHK(c->gk.ensure((s + 1) * 4));
+  RETCHK(c->isb.ensure(s));
+  RETCHK(c->seg_of_slot.ensure(s * 4));
+  RETCHK(c->nbuf.ensure((size_t)(nseg + 1) * 4));
+  RETCHK(c->tsum.ensure((size_t)(ntiles + 1) * 8));
+  RETCHK(c->stot.ensure((size_t)(nseg + 1) * 8));
+  RETCHK(c->lvl.ensure((3 * NLEVEL + 1) * 4));
+  RETCHK(c->active.ensure(16));
+  return HBT_OK;
+}
+
+Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subhalo *subs,
+               int64_t *out_index, float *out_energy) {
+  Work W{};
+  W.seg = c->seg.as<Seg>();
+  W.nseg = (int)L.seg.size();
+  W.nslots = L.nslots;
+  W.seg_of_slot = c->seg_of_slot.as<int>();
+  W.ibase = c->ibase.as<int64_t>();
+  W.nbuf = c->nbuf.as<int>();
+  if (parts) {
+    W.g_id = parts->id; W.g_x = parts->x; W.g_y = parts->y; W.g_z = parts->z;
+    W.g_vx = parts->vx; W.g_vy = parts->vy; W.g_vz = parts->vz; W.g_m = parts->mass;
+    W.g_type = parts->type;
+  }
+  const size_t S = (size_t)L.nslots;
+  W.gidx = c->gidx.as<int>();
+  W.px = c->pxyz.as<float>(); W.py = W.px + S; W.pz = W.py + S;
+  W.vx = c->vxyz.as<float>(); W.vy = W.vx + S; W.vz = W.vy + S;
+  W.pm = c->pm.as<float>();
+  W.ptype = c->ptype.as<int>();
+  W.E = c->E.as<float>();
+  W.el = c->el.as<int>(); W.el2 = c->el2.as<int>(); W.sp = c->sp.as<int>();
+  W.k0 = c->k0.as<u64>(); W.k1 = c->k1.as<u64>(); W.v0 = c->v0.as<u32>(); W.v1 = c->v1.as<u32>();
+  W.dl = c->dl.as<signed char>();
+  W.cmin = c->cmin.as<int>();
+  W.bits = c->bits.as<u32>();
+  W.pre = c->pre.as<int>(); W.nlo = c->nlo.as<int>(); W.nhi = c->nhi.as<int>();
+  W.npos = c->npos.as<int>(); W.nlist = c->nlist.as<int>();
+  W.ipm = c->ipm.as<float4>(); W.iax = c->iax.as<int2>();
+  W.scan64 = c->scan64.as<u64>();
+  W.pk = c->pk.as<int>(); W.gk = c->gk.as<int>();
+  W.isb = c->isb.as<unsigned char>();
+  W.lvl = c->lvl.as<int>();
+  W.active = c->active.as<int>();
+  W.subs = subs;
+  W.out_index = out_index;
+  W.out_energy = out_energy;
+  return W;
+}
+
+// Runs one level: builds the working lists, iterates passes until every
+// segment has converged or been disrupted, writes outputs.
+int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *subs,
+              int64_t *out_index, float *out_energy, int64_t *passes) {
+  if (L.seg.empty()) return HBT_OK;
+  const int64_t nseg = (int64_t)L.seg.size();
+  RETCHK(ensure_slots(c, L.nslots, nseg, (int64_t)L.tiles.size()));
+  RETCHK(upload(c->seg, L.seg, c->stream));
+  RETCHK(upload(c->ibase, L.ibase, c->stream));
+  RETCHK(upload(c->capal, L.capal, c->stream));
+  RETCHK(upload(c->tiles, L.tiles, c->stream));
+  RETCHK(upload(c->first_tile, L.first_tile, c->stream));
+  RETCHK(upload(c->ntile, L.ntile, c->stream));
+  RETCHK(upload(c->child_ptr, L.child_ptr, c->stream));
+  RETCHK(upload(c->child_list, L.child_list, c->stream));
+  Runner R;
+  R.c = c;
+  R.L = &L;
+  R.P = make_dev_params(c->params);
+  R.W = make_work(c, L, parts, subs, out_index, out_energy);
+  R.gs = grid_for(L.nslots, c->cus);
+  R.gseg = grid_for(nseg, c->cus);
+  R.gtile = (int)std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)c->cus * 8);
+  if (R.gtile < 1) R.gtile = 1;
+  const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
+  fill_seg_of_slot_kernel<<<gsb, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, c->capal.as<int>(),
+                                                        c->seg_of_slot.as<int>());
+  build_lists_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, c->child_ptr.as<int>(), c->child_list.as<int>());
+  seg_init_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P);
+  el_init_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
+  output_list_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 1);
+  output_record_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, R.P, 1);
+  HIPCHK(hipGetLastError());
+  // every pass that does not converge removes at least one particle
+  static const bool debug = getenv("HBT_DEBUG") != nullptr;
+  for (int it = 0; it < L.maxcap + 2; it++) {
+    RETCHK(R.pass());
+    (*passes)++;
+    if (debug) {
+      std::vector<Seg> hs(nseg);
+      HIPCHK(hipMemcpyAsync(hs.data(), R.W.seg, nseg * sizeof(Seg), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      const Seg &S = hs[0];
+      std::vector<float> e(std::min(8, S.len));
+      std::vector<int> el(std::min(8, S.len));
+      if (!e.empty()) {
+        HIPCHK(hipMemcpy(e.data(), R.W.E + S.ibase, e.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(el.data(), R.W.el + S.ibase, el.size() * 4, hipMemcpyDeviceToHost));
+      }
+      fprintf(stderr,
+              "[hbt] pass %d seg0: flags %d len %d nbound %d nlast %d n_src %d n_tgt %d items %d "
+              "nP %d nG %d K %d hole %d D %d dlen %d Len %g iter %d\n",
+              it, S.flags, S.len, S.nbound, S.nlast, S.n_src, S.n_tgt, S.n_items, S.nP, S.nG, S.K,
+              S.hole, S.D, S.dlen, S.Len, S.iter);
+      fprintf(stderr, "[hbt]   E:");
+      for (size_t i = 0; i < e.size(); i++) fprintf(stderr, " %g", e[i]);
+      fprintf(stderr, "  el:");
+      for (size_t i = 0; i < el.size(); i++) fprintf(stderr, " %d", el[i]);
+      fprintf(stderr, "\n");
+    }
+    HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (*c->h_active == 0) break;
+  }
+  return HBT_OK;
+}
+
+// Nest structure on the host: capacities, heights, children in index order.
+int plan_nests(const hbt_subhalo *subs, int64_t nsub, std::vector<int64_t> &cap,
+               std::vector<int> &height, std::vector<std::vector<int>> &children,
+               std::vector<int> &roots) {
+  cap.assign(nsub, 0);
+  height.assign(nsub, 0);
+  children.assign(nsub, {});
+  roots.clear();
+  for (int64_t s = 0; s < nsub; s++) {
+    if (subs[s].count < 0 || subs[s].parent >= nsub || subs[s].parent == s)
+      return fail(HBT_ERR_INVALID_ARG, "bad subhalo record (count/parent)");
+    cap[s] = subs[s].count;
+    if (subs[s].parent >= 0) children[subs[s].parent].push_back((int)s);
+    else roots.push_back((int)s);
+  }
+  // post-order from roots gives heights and capacities; detects cycles
+  std::vector<int> order;
+  order.reserve(nsub);
+  std::vector<std::pair<int, size_t>> st;
+  for (int r : roots) {
+    st.push_back({r, 0});
+    while (!st.empty()) {
+      auto &t = st.back();
+      if (t.second < children[t.first].size()) {
+        int ch = children[t.first][t.second++];
+        st.push_back({ch, 0});
+        if ((int64_t)st.size() > nsub + 1) return fail(HBT_ERR_INVALID_ARG, "nest too deep");
+      } else {
+        int s = t.first;
+        order.push_back(s);
+        int h = 0;
+        for (int ch : children[s]) { h = std::max(h, height[ch] + 1); cap[s] += cap[ch]; }
+        height[s] = h;
+        st.pop_back();
+      }
+    }
+  }
+  if ((int64_t)order.size() != nsub) return fail(HBT_ERR_INVALID_ARG, "cycle in parent links");
+  roots = order;  // post-order (RecursiveUnbind order)
+  return HBT_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C-ABI
+// ============================================================================
+extern "C" {
+
+const char *hbt_last_error(void) { return g_err.c_str(); }
+
+int hbt_version(int *major, int *minor) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  return HBT_OK;
+}
+
+int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
+  if (!params || !out) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (params->max_sample_size > 0)
+    return fail(HBT_ERR_UNSUPPORTED, "MaxSampleSizeOfPotentialEstimate > 0 is not supported");
+  hbt_ctx *c = new (std::nothrow) hbt_ctx();
+  if (!c) return fail(HBT_ERR_OUT_OF_MEMORY, "context allocation");
+  c->device = device;
+  c->params = *params;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  }
+  if (e != hipSuccess) {
+    delete c;
+    return fail(HBT_ERR_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return HBT_OK;
+}
+
+int hbt_ctx_destroy(hbt_ctx *ctx) {
+  if (!ctx) return HBT_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return HBT_OK;
+}
+
+int hbt_ctx_set_params(hbt_ctx *ctx, const hbt_params *params) {
+  if (!ctx || !params) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (params->max_sample_size > 0)
+    return fail(HBT_ERR_UNSUPPORTED, "MaxSampleSizeOfPotentialEstimate > 0 is not supported");
+  ctx->params = *params;
+  return HBT_OK;
+}
+
+int hbt_ctx_stream(hbt_ctx *ctx, void **stream) {
+  if (!ctx || !stream) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  *stream = (void *)ctx->stream;
+  return HBT_OK;
+}
+
+int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
+  if (nsub < 0 || (nsub > 0 && !subs) || !total) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  std::vector<int64_t> cap;
+  std::vector<int> height, order;
+  std::vector<std::vector<int>> children;
+  RETCHK(plan_nests(subs, nsub, cap, height, children, order));
+  int64_t off = 0;
+  for (int64_t s = 0; s < nsub; s++) {
+    subs[s].out_offset = off;
+    off += cap[s];
+  }
+  *total = off;
+  return HBT_OK;
+}
+
+int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs,
+                     const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
+                     float *out_energy, int plain) {
+  if (!ctx || !particles || (nsub > 0 && (!subs || !subs_host || !out_index || !out_energy)))
+    return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (nsub == 0) return HBT_OK;
+  if (particles->n >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 particles");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<int64_t> cap;
+  std::vector<int> height, order;
+  std::vector<std::vector<int>> children;
+  RETCHK(plan_nests(subs_host, nsub, cap, height, children, order));
+  for (int64_t s = 0; s < nsub; s++) {
+    if (subs_host[s].offset < 0 || subs_host[s].offset + subs_host[s].count > particles->n)
+      return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the particle columns");
+    if (cap[s] >= (int64_t)INT32_MAX / 2) return fail(HBT_ERR_UNSUPPORTED, "subhalo list too long");
+  }
+  hbt_particles P = *particles;
+  int64_t passes = 0;
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  if (plain) {
+    for (int s : order) {
+      Level L;
+      build_level(subs_host, cap, children, std::vector<int>{s}, L);
+      RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
+    }
+  } else {
+    int hmax = 0;
+    for (int64_t s = 0; s < nsub; s++) hmax = std::max(hmax, height[s]);
+    std::vector<std::vector<int>> by_h(hmax + 1);
+    // largest first within a level keeps the heavy segments' blocks early
+    for (int s : order) by_h[height[s]].push_back(s);
+    for (int h = 0; h <= hmax; h++) {
+      std::stable_sort(by_h[h].begin(), by_h[h].end(), [&](int a, int b) { return cap[a] > cap[b]; });
+      Level L;
+      build_level(subs_host, cap, children, by_h[h], L);
+      RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
+    }
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  ctx->last_passes = passes;
+  // evaluation count
+  std::vector<hbt_subhalo> back(nsub);
+  HIPCHK(hipMemcpy(back.data(), subs, nsub * sizeof(hbt_subhalo), hipMemcpyDeviceToHost));
+  int64_t ev = 0;
+  for (auto &r : back) ev += r.n_evals;
+  ctx->last_evals = ev;
+  return HBT_OK;
+}
+
+int hbt_last_batch_stats(hbt_ctx *ctx, double *device_ms, int64_t *n_evals, int64_t *n_passes) {
+  if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (device_ms) *device_ms = ctx->last_ms;
+  if (n_evals) *n_evals = ctx->last_evals;
+  if (n_passes) *n_passes = ctx->last_passes;
+  return HBT_OK;
+}
+
+int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *sub,
+                   int64_t *out_index, float *out_energy) {
+  if (!ctx || !particles || !sub || !out_index || !out_energy)
+    return fail(HBT_ERR_INVALID_ARG, "null argument");
+  const int64_t n = sub->count;
+  if (n < 0 || n > particles->n) return fail(HBT_ERR_INVALID_ARG, "bad count");
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  RETCHK(ctx->st_cols.ensure(nn * (8 + 7 * 4 + 4)));
+  RETCHK(ctx->st_subs.ensure(sizeof(hbt_subhalo)));
+  RETCHK(ctx->st_out_i.ensure(nn * 8));
+  RETCHK(ctx->st_out_e.ensure(nn * 4));
+  char *base = ctx->st_cols.as<char>();
+  hbt_particles D;
+  D.n = n;
+  int64_t *did = (int64_t *)base;
+  float *f = (float *)(base + nn * 8);
+  D.id = did;
+  D.x = f; D.y = f + nn; D.z = f + 2 * nn; D.vx = f + 3 * nn; D.vy = f + 4 * nn; D.vz = f + 5 * nn;
+  D.mass = f + 6 * nn;
+  D.type = (int32_t *)(f + 7 * nn);
+  hipStream_t st = ctx->stream;
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync((void *)D.id, particles->id, n * 8, hipMemcpyHostToDevice, st));
+    const float *src[7] = {particles->x, particles->y, particles->z, particles->vx, particles->vy,
+                           particles->vz, particles->mass};
+    const float *dst[7] = {D.x, D.y, D.z, D.vx, D.vy, D.vz, D.mass};
+    for (int i = 0; i < 7; i++)
+      HIPCHK(hipMemcpyAsync((void *)dst[i], src[i], n * 4, hipMemcpyHostToDevice, st));
+    if (particles->type)
+      HIPCHK(hipMemcpyAsync((void *)D.type, particles->type, n * 4, hipMemcpyHostToDevice, st));
+    else
+      D.type = nullptr;
+  }
+  const hbt_subhalo keep = *sub;
+  hbt_subhalo h = *sub;
+  h.offset = 0;
+  h.parent = -1;
+  h.out_offset = 0;
+  HIPCHK(hipMemcpyAsync(ctx->st_subs.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  RETCHK(hbt_unbind_batch(ctx, &D, ctx->st_subs.as<hbt_subhalo>(), &h, 1, ctx->st_out_i.as<int64_t>(),
+                          ctx->st_out_e.as<float>(), 1));
+  HIPCHK(hipMemcpyAsync(sub, ctx->st_subs.p, sizeof(hbt_subhalo), hipMemcpyDeviceToHost, st));
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(out_index, ctx->st_out_i.p, n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(out_energy, ctx->st_out_e.p, n * 4, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  sub->offset = keep.offset;
+  sub->parent = keep.parent;
+  sub->out_offset = keep.out_offset;
+  return HBT_OK;
+}
+
+}  // extern "C"
+
+#include "hbt_tree_api.hpp"
+#include "hbt_direct.hpp"
+#include "hbt_idhash.hpp"

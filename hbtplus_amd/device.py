@@ -1,0 +1,150 @@
+"""Host-side driver of libhbt_unbind.so through its C-ABI.
+
+torch is used only to hold device memory; every computation is a call into
+the HIP library.  Mirrors the reference's entry points:
+  GpuUnbinder.unbind_batch   SubhaloSnapshot_t::RefineParticles (batched or plain)
+  GpuUnbinder.unbind_one     Subhalo_t::Unbind (host buffers)
+  GpuUnbinder.tree_potential GravityTree_t::Build + EvaluatePotential
+  GpuUnbinder.direct_potential, GpuUnbinder.id_to_index
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .abi import (PARAMS_DTYPE, PARTICLE_COLUMNS, SUB_DTYPE, check, load_library,
+                  particles_struct, plan_batch)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the HIP library needs a device")
+    return torch
+
+
+class GpuUnbinder:
+    def __init__(self, params, device: int = 0):
+        self.torch = _torch()
+        self.lib = load_library()
+        self.device = device
+        self.params = np.ascontiguousarray(params, PARAMS_DTYPE).copy()
+        self.ctx = ctypes.c_void_p()
+        check(self.lib.hbt_ctx_create(self.params.ctypes.data, device, ctypes.byref(self.ctx)),
+              "hbt_ctx_create")
+
+    # context management -------------------------------------------------
+    def close(self):
+        if self.ctx:
+            self.lib.hbt_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        self.params = np.ascontiguousarray(params, PARAMS_DTYPE).copy()
+        check(self.lib.hbt_ctx_set_params(self.ctx, self.params.ctypes.data), "hbt_ctx_set_params")
+
+    def _dev(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to(f"cuda:{self.device}")
+
+    # device-resident inputs ----------------------------------------------
+    def upload(self, cat):
+        """Copy a catalogue to the device once; returns a handle for unbind_device."""
+        cols = {k: self._dev(cat.cols[k]) for k in PARTICLE_COLUMNS}
+        subs = cat.subs.copy()
+        total = plan_batch(subs)
+        return dict(cols=cols, subs_host=subs, total=total,
+                    subs0=self._dev(subs.view(np.uint8)),
+                    ps=particles_struct(cols, ptr=lambda t: t.data_ptr()))
+
+    def unbind_device(self, h, plain=False):
+        """Run on device-resident inputs (inputs are not modified; the record
+        array is re-initialised from a device copy).  Returns device tensors."""
+        t = self.torch
+        subs_dev = h["subs0"].clone()
+        out_i = t.empty(max(h["total"], 1), dtype=t.int64, device=f"cuda:{self.device}")
+        out_e = t.empty(max(h["total"], 1), dtype=t.float32, device=f"cuda:{self.device}")
+        t.cuda.synchronize(self.device)
+        rc = self.lib.hbt_unbind_batch(self.ctx, ctypes.byref(h["ps"]), subs_dev.data_ptr(),
+                                       h["subs_host"].ctypes.data, len(h["subs_host"]),
+                                       out_i.data_ptr(), out_e.data_ptr(), 1 if plain else 0)
+        check(rc, "hbt_unbind_batch")
+        return subs_dev, out_i, out_e
+
+    def unbind_batch(self, cat, plain=False):
+        """RefineParticles on a catalogue.  Returns (subs, out_index, out_energy)
+        as numpy arrays in the layout of oracle.unbind_batch."""
+        h = self.upload(cat)
+        subs_dev, out_i, out_e = self.unbind_device(h, plain=plain)
+        subs = subs_dev.cpu().numpy().view(SUB_DTYPE).copy()
+        total = h["total"]
+        return subs, out_i.cpu().numpy()[:total].copy(), out_e.cpu().numpy()[:total].copy()
+
+    def last_stats(self):
+        ms = ctypes.c_double()
+        ev = ctypes.c_int64()
+        ps = ctypes.c_int64()
+        check(self.lib.hbt_last_batch_stats(self.ctx, ctypes.byref(ms), ctypes.byref(ev),
+                                            ctypes.byref(ps)), "hbt_last_batch_stats")
+        return ms.value, ev.value, ps.value
+
+    # per-subhalo form with host buffers --------------------------------------
+    def unbind_one(self, cols: dict, sub: np.ndarray):
+        cols = {k: np.ascontiguousarray(v) for k, v in cols.items()}
+        rec = np.ascontiguousarray(sub, SUB_DTYPE).copy()
+        n = int(rec["count"][0])
+        out_i = np.zeros(max(n, 1), np.int64)
+        out_e = np.zeros(max(n, 1), np.float32)
+        ps = particles_struct(cols)
+        check(self.lib.hbt_unbind_one(self.ctx, ctypes.byref(ps), rec.ctypes.data,
+                                      out_i.ctypes.data, out_e.ctypes.data), "hbt_unbind_one")
+        return rec, out_i[:n], out_e[:n]
+
+    # tree / direct / hash ---------------------------------------------------
+    def tree_potential(self, src, tgt, tmass=None):
+        f = lambda a: self._dev(np.asarray(a, np.float32))  # noqa: E731
+        sx, sy, sz, sm = (f(a) for a in src)
+        tx, ty, tz = (f(a) for a in tgt)
+        tm = f(tmass) if tmass is not None else None
+        out = self.torch.empty(len(tgt[0]), dtype=self.torch.float64, device=f"cuda:{self.device}")
+        self.torch.cuda.synchronize(self.device)
+        check(self.lib.hbt_tree_potential(self.ctx, len(src[0]), sx.data_ptr(), sy.data_ptr(),
+                                          sz.data_ptr(), sm.data_ptr(), len(tgt[0]), tx.data_ptr(),
+                                          ty.data_ptr(), tz.data_ptr(),
+                                          tm.data_ptr() if tm is not None else None,
+                                          out.data_ptr()), "hbt_tree_potential")
+        return out.cpu().numpy()
+
+    def direct_potential(self, seg_offset, x, y, z, m, mode):
+        f = lambda a: self._dev(np.asarray(a, np.float32))  # noqa: E731
+        seg = self._dev(np.asarray(seg_offset, np.int64))
+        xs, ys, zs, ms = f(x), f(y), f(z), f(m)
+        out = self.torch.empty(max(len(x), 1), dtype=self.torch.float64, device=f"cuda:{self.device}")
+        self.torch.cuda.synchronize(self.device)
+        check(self.lib.hbt_direct_potential(self.ctx, len(seg_offset) - 1, seg.data_ptr(),
+                                            xs.data_ptr(), ys.data_ptr(), zs.data_ptr(),
+                                            ms.data_ptr(), mode, out.data_ptr()),
+              "hbt_direct_potential")
+        return out.cpu().numpy()[:len(x)]
+
+    def id_to_index(self, table_ids, query_ids):
+        t = self._dev(np.asarray(table_ids, np.int64))
+        q = self._dev(np.asarray(query_ids, np.int64))
+        out = self.torch.empty(max(len(query_ids), 1), dtype=self.torch.int64,
+                               device=f"cuda:{self.device}")
+        self.torch.cuda.synchronize(self.device)
+        check(self.lib.hbt_id_to_index(self.ctx, len(table_ids), t.data_ptr(), len(query_ids),
+                                       q.data_ptr(), out.data_ptr()), "hbt_id_to_index")
+        return out.cpu().numpy()[:len(query_ids)]

@@ -22,7 +22,8 @@
 //   CountParticleTypes/CountParticles src/subhalo.cpp:400-444,479-499
 //   derived parameters               src/config_parser.cpp:95-103
 // Third-party behaviour restated: POSIX drand48 (48-bit LCG, a=0x5DEECE66D,
-// c=0xB, default state 0x1234ABCD330E) used at src/oct_tree.tpp:118, and
+// c=0xB; glibc starts an unseeded generator at state 0) used at
+// src/oct_tree.tpp:118, and
 // libstdc++ std::sort (called directly, same library as the reference build).
 //
 // Build: see oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp, no -march).
@@ -74,8 +75,10 @@ struct Params {
 };
 
 // ---- POSIX drand48 ------------------------------------------------------
+// glibc leaves an unseeded generator at state 0 (the reference never seeds
+// it); X <- (0x5DEECE66D X + 0xB) mod 2^48, result X / 2^48.
 struct Drand48 {
-  uint64_t x = 0x1234ABCD330EULL;
+  uint64_t x = 0;
   double next() {
     x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
     return (double)x / 281474976710656.0;

@@ -1,0 +1,139 @@
+"""Writes the golden fixtures under tests/golden/ from the reference itself.
+
+Run in the development container only (needs oracle/_ref/hbt_ref_harness,
+built by `make -C oracle ref` from the reference sources):
+
+    python tests/golden/make_golden.py
+
+Each fixture (<name>.npz) holds the inputs (particle columns, hbt_subhalo
+records, hbt_params) and what the reference's RecursiveUnbind/Unbind/
+TruncateSource produced for them (records, particle lists, binding
+energies), or, for the tree fixtures, GravityTree_t::EvaluatePotential at
+the given targets.  Runs are single-threaded with sampling off.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from hbtplus_amd import synth  # noqa: E402
+from hbtplus_amd.abi import SUB_DTYPE, make_params  # noqa: E402
+
+
+def edge_catalogue():
+    """Degenerate and special cases of Unbind/RecursiveUnbind: empty list,
+    a single particle, fewer than MinNumPartOfSub, an all-unbound list, an
+    orphan (Nbound<=1) with a nested child, a trapped subhalo (SinkTrackId set)
+    and a previously dead one."""
+    rng = np.random.Generator(np.random.Philox(key=4242))
+    groups, parents, nbprev, sink, death = [], [], [], [], []
+
+    def nfw(n, c=10.0, r200=60.0, bulk=(0, 0, 0), centre=(5e4, 5e4, 5e4), sig_scale=1.0):
+        x = synth.nfw_unit_radii(rng, np.full(n, c))
+        pos = synth._isotropic(rng, n) * (x * r200 / c)[:, None]
+        vel = rng.normal(size=(n, 3)) * 8.0 * sig_scale
+        return (np.array(centre, float), np.array(bulk, float), pos, vel)
+
+    groups.append(nfw(0)); parents.append(-1); nbprev.append(0); sink.append(-1); death.append(-1)
+    groups.append(nfw(1)); parents.append(-1); nbprev.append(1); sink.append(-1); death.append(-1)
+    groups.append(nfw(12)); parents.append(-1); nbprev.append(12); sink.append(-1); death.append(-1)
+    groups.append(nfw(300, sig_scale=60.0)); parents.append(-1); nbprev.append(300); sink.append(-1); death.append(-1)
+    # orphan host with one nested child; the child's unbound tail is appended
+    groups.append(nfw(1)); parents.append(-1); nbprev.append(1); sink.append(-1); death.append(-1)
+    groups.append(nfw(400, centre=(5e4 + 10, 5e4, 5e4), sig_scale=0.4)); parents.append(4); nbprev.append(400); sink.append(-1); death.append(-1)
+    # a trapped subhalo that converges (sink cleared) and a dead one revived
+    groups.append(nfw(500, sig_scale=0.4)); parents.append(-1); nbprev.append(500); sink.append(77); death.append(-1)
+    groups.append(nfw(500, sig_scale=0.4)); parents.append(-1); nbprev.append(500); sink.append(-1); death.append(3)
+    params = make_params(softening=0.5, open_angle=0.45, periodic=1, box_size=1e5, snapshot_index=9)
+    cat = synth._assemble(rng, groups, 1e5, True, params, "edge cases", parent=np.array(parents))
+    cat.subs["nbound_prev"] = nbprev
+    cat.subs["sink_track_id"] = sink
+    cat.subs["snapshot_index_of_sink"] = np.where(np.array(sink) >= 0, 4, -1)
+    cat.subs["snapshot_index_of_death"] = death
+    return cat
+
+
+def floor_catalogue():
+    """Pairs of particles closer than the tree resolution (0.1 softening):
+    the reference places them with drand48 below the resolution floor."""
+    rng = np.random.Generator(np.random.Philox(key=4343))
+    n = 800
+    x = synth.nfw_unit_radii(rng, np.full(n, 10.0))
+    pos = synth._isotropic(rng, n) * (x * 8.0)[:, None]
+    pos = np.concatenate([pos, pos[:200] + rng.normal(size=(200, 3)) * 0.004])
+    vel = rng.normal(size=(len(pos), 3)) * 5.0
+    params = make_params(softening=0.5, open_angle=0.45, periodic=0)
+    return synth._assemble(rng, [(np.zeros(3), np.zeros(3), pos, vel)], 1e5, False, params, "floor")
+
+
+def hubble_catalogue():
+    cat = synth.config2(seed=33, nsub=12, hi=1500)
+    cat.params["scale_factor"] = 0.5
+    cat.params["hz"] = 0.1 * np.sqrt(0.3 / 0.125 + 0.7)
+    cat.name = "C2 sample at a=0.5 with Hubble flow"
+    return cat
+
+
+CASES = {
+    "c1_small": lambda: synth.config1(n_nfw=4000, n_cont=200, bound_mass_precision=0.995),
+    "c1_small_bmp05": lambda: synth.config1(n_nfw=4000, n_cont=200, bound_mass_precision=0.5),
+    "c2_small": lambda: synth.config2(seed=31, nsub=40, hi=1500),
+    "c3_small": lambda: synth.config3(seed=32, nsub=40),
+    "c4_small": lambda: synth.config4(n_host=8000, n_sat=6, sat_lo=200, sat_hi=1500),
+    "hubble": hubble_catalogue,
+    "edge": edge_catalogue,
+    "floor": floor_catalogue,
+}
+
+
+def save_unbind(name, cat):
+    subs, idx, en = oracle.ref_unbind_batch(cat)
+    np.savez_compressed(
+        os.path.join(HERE, name + ".npz"),
+        kind="unbind", name=cat.name, params=cat.params.view(np.uint8), subs_in=cat.subs.view(np.uint8),
+        **{"col_" + k: v for k, v in cat.cols.items()},
+        ref_subs=subs.view(np.uint8), ref_index=idx, ref_energy=en)
+    print(f"{name}: {len(subs)} subhalos, {cat.n_particles} particles, nbound {subs['nbound'].sum()}")
+
+
+def straddle_catalogue():
+    """An isolated NFW halo moved across the periodic boundary at x=0."""
+    cat = synth.config1(n_nfw=3000, n_cont=0)
+    c = cat.cols
+    c["x"] = np.mod(c["x"].astype(np.float64) + 5.0, 1e5).astype(np.float32)
+    c["y"] = (c["y"] + 5e4).astype(np.float32)
+    c["z"] = (c["z"] + 5e4).astype(np.float32)
+    cat.params["periodic"] = 1
+    cat.params["box_size"] = 1e5
+    return cat
+
+
+def save_tree(name, cat):
+    c = cat.cols
+    x = c["x"]
+    src = (x, c["y"], c["z"], c["mass"])
+    phi = oracle.ref_tree_potential(cat.params, src, src[:3], c["mass"])
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), kind="tree", params=cat.params.view(np.uint8),
+                        x=x, y=c["y"], z=c["z"], mass=c["mass"], ref_phi=phi)
+    print(f"{name}: {len(x)} targets")
+
+
+def main():
+    if not oracle.have_reference():
+        raise SystemExit("oracle/_ref/hbt_ref_harness is missing (make -C oracle ref)")
+    for name, make in CASES.items():
+        save_unbind(name, make())
+    save_tree("tree_c1", synth.config1(n_nfw=3000, n_cont=600))
+    save_tree("tree_straddle", straddle_catalogue())
+    save_tree("tree_floor", floor_catalogue())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,60 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point the
+header declares, plans batches like the Python mirror, rejects bad input, and
+its records match the header layout."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+import oracle
+from hbtplus_amd import abi, synth
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "hbt_unbind.h")
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load_library()
+    names = set(re.findall(r"\b(hbt_[a-z_0-9]+)\s*\(", open(HEADER).read()))
+    assert len(names) >= 13
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing
+
+
+def test_record_layout_matches_header():
+    sp, ss, sparts = oracle.abi_sizes()
+    assert sp == abi.PARAMS_DTYPE.itemsize == 64
+    assert ss == abi.SUB_DTYPE.itemsize == 240
+    assert sparts == ctypes.sizeof(abi.Particles) == 80
+
+
+def test_plan_batch_matches_python():
+    lib = abi.load_library()
+    cat = synth.config4(n_host=3000, n_sat=7, sat_lo=100, sat_hi=400)
+    a = cat.subs.copy()
+    t = ctypes.c_int64()
+    assert lib.hbt_plan_batch(a.ctypes.data, len(a), ctypes.byref(t)) == 0
+    b = cat.subs.copy()
+    assert t.value == abi.plan_batch(b)
+    assert np.array_equal(a["out_offset"], b["out_offset"])
+
+
+def test_plan_batch_rejects_cycles():
+    lib = abi.load_library()
+    s = np.zeros(2, abi.SUB_DTYPE)
+    s["count"] = 3
+    s["parent"] = [1, 0]
+    t = ctypes.c_int64()
+    assert lib.hbt_plan_batch(s.ctypes.data, 2, ctypes.byref(t)) == -1
+    assert b"cycle" in lib.hbt_last_error()
+
+
+def test_null_arguments_are_errors_not_crashes():
+    lib = abi.load_library()
+    assert lib.hbt_unbind_batch(None, None, None, None, 1, None, None, 0) == -1
+    assert lib.hbt_tree_potential(None, 0, None, None, None, None, 0, None, None, None, None, None) == -1
+    assert lib.hbt_ctx_create(None, 0, None) == -1
+    p = abi.make_params(max_sample_size=1000)
+    ctx = ctypes.c_void_p()
+    assert lib.hbt_ctx_create(p.ctypes.data, 0, ctypes.byref(ctx)) == -4  # sampling unsupported

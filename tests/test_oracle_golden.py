@@ -1,0 +1,41 @@
+"""The CPU restatement against the reference's own outputs (golden fixtures
+written by tests/golden/make_golden.py from the reference compiled from its
+sources).  Bit-identical: records, particle lists (up to the order of exactly
+tied energies, which std::sort leaves unspecified), energies, potentials."""
+import numpy as np
+import pytest
+
+import oracle
+from fixtures import TREE_CASES, UNBIND_CASES, load_tree, load_unbind
+from helpers import assert_exact
+
+
+@pytest.mark.parametrize("name", UNBIND_CASES)
+def test_oracle_matches_reference_unbinding(name):
+    cat, ref = load_unbind(name)
+    got = oracle.unbind_batch(cat)
+    assert_exact(ref, got)
+
+
+@pytest.mark.parametrize("name", TREE_CASES)
+def test_oracle_matches_reference_tree(name):
+    params, (x, y, z, m), ref = load_tree(name)
+    got = oracle.tree_potential(params, (x, y, z, m), (x, y, z), m)
+    assert np.array_equal(got, ref)
+
+
+def test_fixture_outcomes_are_nontrivial():
+    """The fixtures exercise convergence, disruption, early exits, nesting,
+    orphans and correction passes."""
+    seen = set()
+    for name in UNBIND_CASES:
+        cat, (rs, ri, re_) = load_unbind(name)
+        got = oracle.unbind_batch(cat)[0]
+        seen |= {int(s) for s in got["status"]}
+        if (got["n_iter"] > 2).any():
+            seen.add("multi-pass")
+        if (cat.subs["parent"] >= 0).any():
+            seen.add("nested")
+        if ((cat.subs["nbound_prev"] <= 1) & (rs["list_len"] > cat.subs["count"])).any():
+            seen.add("orphan-with-tail")
+    assert {1, 2, 4, "multi-pass", "nested", "orphan-with-tail"} <= seen

@@ -1,0 +1,135 @@
+"""Checks that pin the oracle by mathematics rather than by the reference:
+the softened kernel's closed-form properties, the tree walk against an exact
+pair sum at opening angle 0, frame averages and kinematics against numpy,
+and structural invariants of the unbinding output."""
+import numpy as np
+import pytest
+
+import oracle
+from hbtplus_amd import abi, synth
+from hbtplus_amd.abi import HBT_DIRECT_FP64, HBT_STATUS_CONVERGED
+from helpers import list_of
+
+
+def _two_body(r, eps=0.5):
+    """potential at the origin of a unit mass at distance r, via the oracle's
+    direct sum (self term excluded), in units where G/a = 1"""
+    p = abi.make_params(G=1.0, softening=eps)
+    x = np.array([0.0, r], np.float32)
+    z = np.zeros(2, np.float32)
+    m = np.array([1.0, 1.0], np.float32)
+    return oracle.direct_potential(p, [0, 2], x, z, z, m, HBT_DIRECT_FP64)[0]
+
+
+def test_kernel_newtonian_beyond_h():
+    h = 2.8 * 0.5
+    for r in (1.001 * h, 1.5 * h, 10.0):
+        assert _two_body(r) == pytest.approx(-1.0 / float(np.float32(r)), rel=1e-12)
+
+
+def test_kernel_continuity_and_self_term():
+    eps = 0.5
+    h = 2.8 * eps
+    # the spline potential is continuous with a continuous derivative at u=1/2 and u=1
+    for u0 in (0.5, 1.0):
+        r0 = u0 * h
+        d = 1e-4
+        lo, hi = _two_body(r0 * (1 - d)), _two_body(r0 * (1 + d))
+        assert abs(hi - lo) < 5 * d * r0 / r0 ** 2
+        lo2, hi2 = _two_body(r0 * (1 - 2 * d)), _two_body(r0 * (1 + 2 * d))
+        slope_l = (lo - lo2) / (r0 * d)
+        slope_r = (hi2 - hi) / (r0 * d)
+        assert slope_l == pytest.approx(slope_r, rel=2e-2)
+    # at r -> 0 the potential of a point mass is -m/eps: the walk's +m/eps cancels it
+    assert _two_body(1e-6) == pytest.approx(-1.0 / eps, rel=1e-5)
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_tree_at_zero_opening_angle_is_the_pair_sum(periodic):
+    rng = np.random.default_rng(5)
+    n = 600
+    pos = rng.normal(size=(n, 3)) * 20.0
+    if periodic:
+        pos[:, 0] = np.mod(pos[:, 0] + 3.0, 1e5)
+        pos[:, 1:] += 5e4
+    x, y, z = (pos[:, i].astype(np.float32) for i in range(3))
+    m = rng.uniform(0.5, 1.5, n).astype(np.float32) * 1e-4
+    p = abi.make_params(open_angle=0.0, periodic=periodic, box_size=1e5)
+    tree = oracle.tree_potential(p, (x, y, z, m), (x, y, z), m)
+    direct = oracle.direct_potential(p, [0, n], x, y, z, m, HBT_DIRECT_FP64)
+    assert np.allclose(tree, direct, rtol=1e-10, atol=0)
+
+
+def test_tree_error_at_default_opening_angle_is_small():
+    cat = synth.config1(n_nfw=3000, n_cont=0)
+    c = cat.cols
+    src = (c["x"], c["y"], c["z"], c["mass"])
+    tree = oracle.tree_potential(cat.params, src, src[:3], c["mass"])
+    direct = oracle.direct_potential(cat.params, [0, len(c["x"])], *src, HBT_DIRECT_FP64)
+    rms = np.sqrt(np.mean(((tree - direct) / direct) ** 2))
+    assert 1e-7 < rms < 2e-3  # a monopole tree at theta=0.45 is approximate but close
+
+
+def _bound_arrays(cat, subs, idx, s):
+    lst = list_of(subs, idx, s)[: int(subs["nbound"][s])]
+    c = cat.cols
+    return (np.stack([c["x"][lst], c["y"][lst], c["z"][lst]], 1).astype(np.float64),
+            np.stack([c["vx"][lst], c["vy"][lst], c["vz"][lst]], 1).astype(np.float64),
+            c["mass"][lst].astype(np.float64))
+
+
+def test_frames_and_kinematics_match_numpy():
+    cat = synth.config2(seed=41, nsub=30, hi=2000)
+    subs, idx, en = oracle.unbind_batch(cat)
+    box = float(cat.params["box_size"][0])
+    checked = 0
+    for s in range(len(subs)):
+        if not subs["status"][s] & HBT_STATUS_CONVERGED:
+            continue
+        x, v, m = _bound_arrays(cat, subs, idx, s)
+        # the final frame is the mass-weighted mean of the bound set of the last
+        # pass; after convergence that set is unchanged
+        V = (v * m[:, None]).sum(0) / m.sum()
+        o = x[0]
+        dx = x - o
+        dx = np.where(dx > box / 2, dx - box, np.where(dx < -box / 2, dx + box, dx))
+        X = o + (dx * m[:, None]).sum(0) / m.sum()
+        assert np.allclose(subs["ref_vel"][s], V, rtol=1e-5, atol=1e-3)
+        assert np.allclose(subs["ref_pos"][s], X, rtol=1e-6, atol=1e-3)
+        dv = v - subs["ref_vel"][s]
+        K = 0.5 * (dv ** 2 * m[:, None]).sum() / m.sum()
+        assert subs["specific_self_kinetic_energy"][s] == pytest.approx(K, rel=1e-4)
+        dxr = x - subs["ref_pos"][s]
+        dxr = np.where(dxr > box / 2, dxr - box, np.where(dxr < -box / 2, dxr + box, dxr))
+        L = (np.cross(dxr, dv) * m[:, None]).sum(0) / m.sum()
+        assert np.allclose(subs["specific_angular_momentum"][s], L, rtol=1e-3, atol=1e-3 * np.abs(L).max())
+        assert subs["mbound"][s] == pytest.approx(m.sum(), rel=1e-6)
+        checked += 1
+    assert checked > 10
+
+
+def test_unbinding_output_invariants():
+    cat = synth.config2(seed=42, nsub=40, hi=3000)
+    subs, idx, en = oracle.unbind_batch(cat)
+    for s in range(len(subs)):
+        nb, L = int(subs["nbound"][s]), int(subs["list_len"][s])
+        lst = list_of(subs, idx, s)
+        e = list_of(subs, en, s)
+        # the list is a permutation of the source list
+        o, cnt = int(cat.subs["offset"][s]), int(cat.subs["count"][s])
+        assert np.array_equal(np.sort(lst), np.arange(o, o + cnt))
+        assert subs["nsource"][s] == (nb if nb <= 1 else min(int(nb * np.float32(3.0)), L))
+        if subs["status"][s] & HBT_STATUS_CONVERGED:
+            assert np.all(e[:nb] < 0) and np.all(np.diff(e[:nb]) >= 0)
+            assert np.all(e[nb:] >= 0)
+            assert nb >= np.float32(subs["n_evals"][s] * 0)  # present
+            assert subs["nbound_type"][s].sum() == nb
+        else:
+            assert nb <= 1
+
+
+def test_plan_batch_capacities():
+    cat = synth.config4(n_host=5000, n_sat=5, sat_lo=100, sat_hi=500)
+    subs = cat.subs.copy()
+    total = abi.plan_batch(subs)
+    assert total == 2 * cat.subs["count"].sum() - cat.subs["count"][0]
