@@ -1,0 +1,185 @@
+"""Benchmark: subhalo-particle binding-energy evaluations per second.
+
+One step = one RefineParticles pass over the whole batch: every subhalo is
+unbound to convergence (batched mixed-precision form, inputs resident in
+HBM).  Default workload: 5,000 power-law subhalos (N_p 20..2e5, 20%
+contaminants) in a periodic 100 Mpc/h box.
+
+Prints one JSON line.  Besides the headline (batched) throughput it reports
+the plain per-subhalo GPU form on the same inputs (same kernels, one subhalo
+at a time; the bound sets must be identical), the roofline of the dominant
+kernel (the potential walk: HIP-event timed launches, FLOPs from an
+instrumented pass) and the CPU baseline (the oracle restatement of the
+reference, OpenMP over subhalos, on a bounded sample).
+
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "subhalo-particle energy evals/s: batched mixed-prec vs plain GPU, same bound sets"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def make_workload(name):
+    from hbtplus_amd import synth
+    if name == "c2":
+        cat = synth.config2()
+        desc = ("5,000 power-law subhalos (dN/dN_p ~ N_p^-1.9 on [20, 2e5], log-normal c, "
+                "20% contaminants at 3 v_esc), periodic 100 Mpc/h box, eps=0.5, theta=0.45")
+    elif name == "c1":
+        cat = synth.config1(bound_mass_precision=0.995)
+        desc = "single 24k-particle NFW subhalo with contaminants, eps=0.5, theta=0.45"
+    elif name == "c3":
+        cat = synth.config3()
+        desc = "50,000 subhalos, N_p log-uniform on [20, 4096], 15% contaminants"
+    elif name == "c4":
+        cat = synth.config4()
+        desc = "4M-particle NFW host with 200 nested satellites (N_p 1e3..1e5), eps=1"
+    elif name == "c5":
+        _, _, cat = synth.config5()
+        desc = "200,000 power-law subhalos (config-2 law) of one rank's share"
+    else:
+        raise SystemExit(f"unknown config {name}")
+    return cat, desc
+
+
+def cpu_baseline(cat, budget_s=20.0):
+    """The oracle (CPU restatement of the reference, OpenMP over subhalos) on
+    a bounded sample: subhalos taken in a fixed random order until the
+    estimated work fills about budget_s seconds."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(cat.subs))
+    top = cat.subs["parent"] < 0
+    order = order[top[order]]
+    # calibrate on a small sample, then grow to the budget
+    take, elapsed, evals = 16, 0.0, 0
+    while True:
+        sub = cat.subset(order[:take])
+        t0 = time.perf_counter()
+        subs, _, _ = oracle.unbind_batch(sub, nthreads=threads)
+        elapsed = time.perf_counter() - t0
+        evals = int(subs["n_evals"].sum())
+        if elapsed > budget_s / 4 or take >= len(order):
+            break
+        take = min(len(order), int(take * max(2.0, budget_s / 4 / max(elapsed, 1e-3))))
+    return dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
+                sample=f"{take} of {len(order)} top-level subhalos ({sub.n_particles} particles, "
+                       f"{evals} evals) in {elapsed:.1f} s, OpenMP schedule(dynamic,1)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--replay", action="store_true", help="REPLAY precision instead of FAST")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the unbinding of one rank's share runs on one MI355X; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from hbtplus_amd import abi
+    from hbtplus_amd.abi import SUB_DTYPE
+    from hbtplus_amd.device import GpuUnbinder
+
+    cat, desc = make_workload(args.config)
+    params = cat.params.copy()
+    params["precision"] = abi.HBT_PREC_REPLAY if args.replay else abi.HBT_PREC_FAST
+    U = GpuUnbinder(params, device=0)
+    h = U.upload(cat)
+    torch.cuda.synchronize()
+    print(f"[bench] {args.config}: {len(cat.subs)} subhalos, {cat.n_particles} particles", file=sys.stderr,
+          flush=True)
+
+    for _ in range(args.warmup):
+        U.unbind_device(h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for _ in range(args.steps):
+        out = U.unbind_device(h)
+        dev_ms += U.last_stats()[0]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    subs = out[0].cpu().numpy().view(SUB_DTYPE)
+    evals = int(subs["n_evals"].sum())
+    passes = U.last_stats()[2]
+    value = evals / (ms_per_step / 1e3)
+    print(f"[bench] batched: {ms_per_step:.2f} ms/step, {evals} evals/step, {value:.4g} evals/s",
+          file=sys.stderr, flush=True)
+
+    # roofline of the walk kernel: timed launches, then an instrumented pass
+    U.profile(1)
+    U.unbind_device(h)
+    prof_t = U.profile_read()
+    U.profile(2)
+    U.unbind_device(h)
+    prof_c = U.profile_read()
+    U.profile(0)
+    I, O, T = prof_c["interactions"], prof_c["opened"], prof_c["targets"]
+    flops = 11.0 * I + 10.0 * O + 20.0 * T
+    walk_s = prof_t["walk_ms"] / 1e3
+    launches = max(prof_t["walk_launches"], 1)
+    achieved = flops / walk_s / 1e12 if walk_s > 0 else 0.0
+    roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
+                    kernel="walk_kernel", launches_per_step=launches,
+                    avg_launch_ms=round(prof_t["walk_ms"] / launches, 4),
+                    flops_per_step=flops, interactions_per_eval=round(I / max(T, 1), 1),
+                    opened_per_eval=round(O / max(T, 1), 1),
+                    walk_share_of_step=round(prof_t["walk_ms"] / ms_per_step, 3))
+
+    result = {"metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+              "config": {"workload": desc, "subhalos": int(len(cat.subs)),
+                         "particles": int(cat.n_particles), "evals_per_step": evals,
+                         "passes_per_step": int(passes),
+                         "precision": "REPLAY (fp64 walk)" if args.replay else
+                         "FAST (fp32+Kahan walk, fp64 frames)",
+                         "device_ms_per_step": round(dev_ms / args.steps, 3)},
+              "roofline": roofline}
+
+    if not args.no_plain:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pout = U.unbind_device(h, plain=True)
+        torch.cuda.synchronize()
+        pt = time.perf_counter() - t0
+        same = all(np.array_equal(a.cpu().numpy().view(np.uint8), b.cpu().numpy().view(np.uint8))
+                   for a, b in zip(out, pout))
+        result["plain"] = {"value": evals / pt, "unit": "evals/s", "ms_per_step": round(pt * 1e3, 1),
+                           "identical_outputs": bool(same)}
+        result["batched_over_plain"] = round(value / (evals / pt), 2)
+        print(f"[bench] plain: {pt * 1e3:.1f} ms, identical outputs {same}", file=sys.stderr, flush=True)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(cat)
+        print(f"[bench] cpu: {result['cpu_baseline']}", file=sys.stderr, flush=True)
+    U.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -117,6 +117,8 @@ def load_library() -> ctypes.CDLL:
         "hbt_direct_potential": ([vp, i64, vp, vp, vp, vp, vp, i32, vp], i32),
         "hbt_id_to_index": ([vp, i64, vp, i64, vp, vp], i32),
         "hbt_last_batch_stats": ([vp, vp, vp, vp], i32),
+        "hbt_profile": ([vp, i32], i32),
+        "hbt_profile_read": ([vp, vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -54,6 +54,7 @@ struct Work {
   unsigned char *isb;
   int *lvl;                  // [3*NLEVEL+1]: counts, offsets, cursors
   int *active;               // [1]
+  unsigned long long *counters;  // [4]: interactions, opened nodes, targets (instrumented passes)
   // caller outputs
   hbt_subhalo *subs;
   int64_t *out_index;
@@ -470,8 +471,9 @@ __global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v) {
 // len^2 = 0 and skip = next item.
 
 // Reference arithmetic (src/gravity_tree.cpp:79-164): FP64 walk over float data.
+template <bool COUNT>
 __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int n_items, float tx,
-                              float ty, float tz, float tmass) {
+                              float ty, float tz, float tmass, unsigned &ni, unsigned &no) {
 #pragma clang fp contract(off)
   const double h = 2.8 * (double)P.eps, hinv = 1.0 / h;
   double pot = (double)(float)(tmass / P.eps);
@@ -485,8 +487,9 @@ __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int
     if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
     const double r2 = dx * dx + dy * dy + dz * dz;
     const float lsq = __int_as_float(a.x);
-    if ((double)lsq > r2 * (double)P.theta2) { j++; continue; }
+    if ((double)lsq > r2 * (double)P.theta2) { j++; if (COUNT) no++; continue; }
     j = a.y > j ? a.y : end;  // skips always advance
+    if (COUNT) ni++;
     pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
   }
   return pot * (double)P.G / (double)P.a;
@@ -494,9 +497,9 @@ __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int
 
 // FP32 walk with Kahan-compensated accumulation.  Straddling segments (a
 // periodic subhalo wider than a quarter box) form the separation in double.
-template <bool STRADDLE>
+template <bool STRADDLE, bool COUNT>
 __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n_items, float tx,
-                            float ty, float tz, float tmass) {
+                            float ty, float tz, float tmass, unsigned &ni, unsigned &no) {
 #pragma clang fp contract(fast)
   const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
   Kahan acc;
@@ -515,25 +518,38 @@ __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n
       dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
     }
     const float r2 = dx * dx + dy * dy + dz * dz;
-    if (__int_as_float(a.x) > r2 * P.theta2) { j++; continue; }
+    if (__int_as_float(a.x) > r2 * P.theta2) { j++; if (COUNT) no++; continue; }
     j = a.y > j ? a.y : end;  // skips always advance
+    if (COUNT) ni++;
     acc.add(kernel_term_f(q.w, r2, h2, hinv));
   }
   return (double)acc.s * (double)P.G / (double)P.a;
 }
 
+template <bool COUNT>
 __device__ __forceinline__ double walk_dispatch(const Work &W, const DevParams &P, const Seg &S,
-                                                float tx, float ty, float tz, float tmass) {
+                                                float tx, float ty, float tz, float tmass,
+                                                unsigned &ni, unsigned &no) {
   const int64_t ib = 2 * S.ibase;
-  if (P.replay) return walk_replay(W, P, ib, S.n_items, tx, ty, tz, tmass);
-  if (S.straddle) return walk_fast<true>(W, P, ib, S.n_items, tx, ty, tz, tmass);
-  return walk_fast<false>(W, P, ib, S.n_items, tx, ty, tz, tmass);
+  if (P.replay) return walk_replay<COUNT>(W, P, ib, S.n_items, tx, ty, tz, tmass, ni, no);
+  if (S.straddle) return walk_fast<true, COUNT>(W, P, ib, S.n_items, tx, ty, tz, tmass, ni, no);
+  return walk_fast<false, COUNT>(W, P, ib, S.n_items, tx, ty, tz, tmass, ni, no);
+}
+
+__device__ __forceinline__ void flush_counts(const Work &W, unsigned ni, unsigned no, unsigned nt) {
+  unsigned long long a = ni, b = no, c = nt;
+  a = wave_reduce_sum(a); b = wave_reduce_sum(b); c = wave_reduce_sum(c);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    atomicAdd(&W.counters[0], a); atomicAdd(&W.counters[1], b); atomicAdd(&W.counters[2], c);
+  }
 }
 
 // Binding energies of the pass's targets, in spatial order.
 // Full pass: E = 0.5|dv|^2 + phi (BindingEnergy, :341-354).
 // Correction pass: E += OldVel.RefVelDiff + dK - phi_removed (:319-328).
+template <bool COUNT>
 __global__ void __launch_bounds__(BLOCK) walk_kernel(Work W, DevParams P) {
+  unsigned ni = 0, no = 0, nt = 0;
   SLOT_LOOP(s, W) {
 #pragma clang fp contract(off)
     int k;
@@ -541,12 +557,13 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(Work W, DevParams P) {
     if (l < 0) continue;
     const Seg &S = W.seg[k];
     if (l >= S.n_tgt) continue;
+    if (COUNT) nt++;
     const int64_t b = S.ibase;
     const int li = W.sp[s];
     const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
     const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
     if (!(S.flags & F_CORR)) {
-      const double pot = walk_dispatch(W, P, S, x, y, z, W.pm[b + li]);
+      const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, W.pm[b + li], ni, no);
       float dv[3];
       rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
       if (P.replay) {
@@ -556,13 +573,14 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(Work W, DevParams P) {
         W.E[b + li] = (float)(kin + pot);
       }
     } else {
-      const double pot = walk_dispatch(W, P, S, x, y, z, 0.f);
+      const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, 0.f, ni, no);
       float ov[3];
       rel_vel(P, x, y, z, vx, vy, vz, S.old_pos, S.old_vel, ov);
       const float kin = vdot_f(ov, S.rvd) + S.dK;
       W.E[b + li] = (float)((double)W.E[b + li] + ((double)kin - pot));
     }
   }
+  if (COUNT) flush_counts(W, ni, no, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -579,7 +597,8 @@ __global__ void __launch_bounds__(BLOCK) partition_rank_kernel(Work W) {
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
-    if (l < 1 || l >= S.nlast) continue;
+    // only segments partitioned in this pass: the scan covers active segments only
+    if (!(S.flags & F_ACTIVE) || l < 1 || l >= S.nlast) continue;
     const int64_t b = S.ibase;
     const float e = W.E[b + W.el[s]];
     const u64 x = W.scan64[s];
@@ -595,7 +614,7 @@ __global__ void __launch_bounds__(BLOCK) partition_k_kernel(Work W) {
     if (l < 0) continue;
     Seg &S = W.seg[k];
     const int m = min(S.nP, S.nG);
-    if (l < 1 || l > m) continue;
+    if (!(S.flags & F_ACTIVE) || l < 1 || l > m) continue;
     const int64_t b = S.ibase;
     if (W.pk[b + l] < W.gk[b + l] && (l == m || W.pk[b + l + 1] >= W.gk[b + l + 1])) S.K = l;
   }
@@ -620,7 +639,7 @@ __global__ void __launch_bounds__(BLOCK) partition_move_kernel(Work W) {
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
-    if (l >= S.nlast) continue;
+    if (!(S.flags & F_ACTIVE) || l >= S.nlast) continue;
     const int64_t b = S.ibase;
     int dest = l;
     if (l == 0) {
@@ -645,7 +664,8 @@ __global__ void __launch_bounds__(BLOCK) partition_copy_kernel(Work W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
-    if (l < W.seg[k].nlast) W.el[s] = W.el2[s];
+    const Seg &S = W.seg[k];
+    if ((S.flags & F_ACTIVE) && l < S.nlast) W.el[s] = W.el2[s];
   }
 }
 
@@ -740,7 +760,9 @@ __global__ void __launch_bounds__(BLOCK) frames_kernel(Work W, DevParams P) {
 #pragma clang fp contract(off)
   for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
     Seg &S = W.seg[k];
-    if (!(S.flags & F_ACTIVE)) continue;  // uniform
+    const bool active = (S.flags & F_ACTIVE) != 0;
+    __syncthreads();  // every thread has read the flags before thread 0 updates them
+    if (!active) continue;  // uniform
     const int64_t b = S.ibase;
     const int nb = S.nbound;
     const int l0 = W.el[b];
@@ -863,6 +885,7 @@ __global__ void __launch_bounds__(BLOCK) output_record_kernel(Work W, DevParams 
   for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
     Seg &S = W.seg[k];
     const bool fin = early ? (S.flags & F_EARLY) != 0 : (S.flags & F_FINAL) != 0;
+    __syncthreads();  // every thread has read the flags before thread 0 updates them
     if (!fin) continue;  // uniform
     hbt_subhalo &r = W.subs[S.sub];
     const int64_t b = S.ibase;

@@ -97,7 +97,12 @@ struct hbt_ctx {
       npos, nlist, ipm, iax, scan64, pk, gk, isb, seg_of_slot;
   // per-level tables
   DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal;
-  DBuf lvl, active;
+  DBuf lvl, active, counters;
+  // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
+  int prof_mode = 0;
+  hipEvent_t evw0 = nullptr, evw1 = nullptr;
+  double walk_ms = 0;
+  int64_t walk_launches = 0;
   // staging for hbt_unbind_one / hbt_tree_potential
   DBuf st_cols, st_subs, st_out_i, st_out_e, st_phi;
   int *h_active = nullptr;  // pinned
@@ -108,12 +113,14 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &cmin, &bits, &pre, &nlo, &nhi, &npos, &nlist, &ipm, &iax, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &seg, &ibase, &nbuf, &tiles, &first_tile, &ntile, &tsum,
-                    &stot, &child_ptr, &child_list, &capal, &lvl, &active, &st_cols, &st_subs,
+                    &stot, &child_ptr, &child_list, &capal, &lvl, &active, &counters, &st_cols, &st_subs,
                     &st_out_i, &st_out_e, &st_phi})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (evw0) (void)hipEventDestroy(evw0);
+    if (evw1) (void)hipEventDestroy(evw1);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -307,7 +314,15 @@ struct Runner {
   int pass() {
     pass_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     RETCHK(build_tree(true));
-    walk_kernel<<<gs, BLOCK, 0, c->stream>>>(W, P);
+    if (c->prof_mode == 2) {
+      walk_kernel<true><<<gs, BLOCK, 0, c->stream>>>(W, P);
+    } else if (c->prof_mode == 1) {
+      HIPCHK(hipEventRecord(c->evw0, c->stream));
+      walk_kernel<false><<<gs, BLOCK, 0, c->stream>>>(W, P);
+      HIPCHK(hipEventRecord(c->evw1, c->stream));
+    } else {
+      walk_kernel<false><<<gs, BLOCK, 0, c->stream>>>(W, P);
+    }
     // partition
     set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_NLAST_ACTIVE);
     u64 *stot = c->stot.as<u64>();
@@ -381,6 +396,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->stot.ensure((size_t)(nseg + 1) * 8));
   RETCHK(c->lvl.ensure((3 * NLEVEL + 1) * 4));
   RETCHK(c->active.ensure(16));
+  RETCHK(c->counters.ensure(64));
   return HBT_OK;
 }
 
@@ -418,6 +434,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.isb = c->isb.as<unsigned char>();
   W.lvl = c->lvl.as<int>();
   W.active = c->active.as<int>();
+  W.counters = c->counters.as<unsigned long long>();
   W.subs = subs;
   W.out_index = out_index;
   W.out_energy = out_energy;
@@ -486,6 +503,12 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     }
     HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->prof_mode == 1) {
+      float wms = 0;
+      HIPCHK(hipEventElapsedTime(&wms, c->evw0, c->evw1));
+      c->walk_ms += wms;
+      c->walk_launches++;
+    }
     if (*c->h_active == 0) break;
   }
   return HBT_OK;
@@ -561,6 +584,8 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 4, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipEventCreate(&c->evw0);
+  if (e == hipSuccess) e = hipEventCreate(&c->evw1);
   if (e == hipSuccess) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
@@ -672,6 +697,32 @@ int hbt_last_batch_stats(hbt_ctx *ctx, double *device_ms, int64_t *n_evals, int6
   return HBT_OK;
 }
 
+int hbt_profile(hbt_ctx *ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return fail(HBT_ERR_INVALID_ARG, "bad argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->prof_mode = mode;
+  ctx->walk_ms = 0;
+  ctx->walk_launches = 0;
+  RETCHK(ctx->counters.ensure(64));
+  HIPCHK(hipMemsetAsync(ctx->counters.p, 0, 64, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return HBT_OK;
+}
+
+int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int64_t *interactions,
+                     int64_t *opened, int64_t *targets) {
+  if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned long long h[4] = {0, 0, 0, 0};
+  if (ctx->counters.p) HIPCHK(hipMemcpy(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost));
+  if (walk_ms) *walk_ms = ctx->walk_ms;
+  if (walk_launches) *walk_launches = ctx->walk_launches;
+  if (interactions) *interactions = (int64_t)h[0];
+  if (opened) *opened = (int64_t)h[1];
+  if (targets) *targets = (int64_t)h[2];
+  return HBT_OK;
+}
+
 int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *sub,
                    int64_t *out_index, float *out_energy) {
   if (!ctx || !particles || !sub || !out_index || !out_energy)
@@ -711,6 +762,8 @@ int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *su
   h.offset = 0;
   h.parent = -1;
   h.out_offset = 0;
+  // Subhalo_t::Unbind has no orphan rule (that belongs to RecursiveUnbind)
+  h.nbound_prev = INT64_MAX;
   HIPCHK(hipMemcpyAsync(ctx->st_subs.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
   RETCHK(hbt_unbind_batch(ctx, &D, ctx->st_subs.as<hbt_subhalo>(), &h, 1, ctx->st_out_i.as<int64_t>(),
                           ctx->st_out_e.as<float>(), 1));
@@ -723,6 +776,7 @@ int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *su
   sub->offset = keep.offset;
   sub->parent = keep.parent;
   sub->out_offset = keep.out_offset;
+  sub->nbound_prev = keep.nbound_prev;
   return HBT_OK;
 }
 

@@ -100,6 +100,17 @@ class GpuUnbinder:
                                             ctypes.byref(ps)), "hbt_last_batch_stats")
         return ms.value, ev.value, ps.value
 
+    def profile(self, mode):
+        check(self.lib.hbt_profile(self.ctx, mode), "hbt_profile")
+
+    def profile_read(self):
+        ms = ctypes.c_double()
+        vals = [ctypes.c_int64() for _ in range(4)]
+        check(self.lib.hbt_profile_read(self.ctx, ctypes.byref(ms), *[ctypes.byref(v) for v in vals]),
+              "hbt_profile_read")
+        return dict(walk_ms=ms.value, walk_launches=vals[0].value, interactions=vals[1].value,
+                    opened=vals[2].value, targets=vals[3].value)
+
     # per-subhalo form with host buffers --------------------------------------
     def unbind_one(self, cols: dict, sub: np.ndarray):
         cols = {k: np.ascontiguousarray(v) for k, v in cols.items()}

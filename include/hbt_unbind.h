@@ -113,7 +113,8 @@ typedef struct hbt_subhalo {
   int64_t offset;                   /* first particle of the source list in the particle columns */
   int64_t count;                    /* source list length (Particles.size()) */
   int64_t parent;                   /* subhalo whose NestedSubhalos holds this one, -1 if none */
-  int64_t nbound_prev;              /* Nbound before this pass (orphan test, RecursiveUnbind) */
+  int64_t nbound_prev;              /* Nbound before this pass: RecursiveUnbind's orphan test
+                                       (Nbound <= 1); pass a value >= 2 for plain Unbind semantics */
   int64_t sink_track_id;            /* SinkTrackId (in/out) */
   int32_t snapshot_index_of_death;  /* SnapshotIndexOfDeath (in/out) */
   int32_t snapshot_index_of_sink;   /* SnapshotIndexOfSink (in/out) */
@@ -208,6 +209,15 @@ int hbt_id_to_index(hbt_ctx *ctx, int64_t n_table, const int64_t *table_ids,
  * events on the context stream) and the number of energy evaluations it did. */
 int hbt_last_batch_stats(hbt_ctx *ctx, double *device_ms, int64_t *n_evals,
                          int64_t *n_passes);
+
+/* Instrumentation of the potential-walk kernel (no counterpart in the
+ * reference; used by the benchmark for the roofline).  mode 0: off;
+ * 1: time every walk launch with HIP events on the context stream;
+ * 2: count, in every walk launch, the summed sources (leaves and accepted
+ * nodes), the opened nodes and the targets.  Resets the accumulators. */
+int hbt_profile(hbt_ctx *ctx, int mode);
+int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches,
+                     int64_t *interactions, int64_t *opened, int64_t *targets);
 
 #ifdef __cplusplus
 }
