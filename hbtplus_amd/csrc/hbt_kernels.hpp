@@ -105,16 +105,22 @@ __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *chil
         W.gidx[b + pos + i] = (int)W.out_index[q.out_offset + q.nbound + i];
       pos += tail;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < pos; i += blockDim.x) {
-      const int g = W.gidx[b + i];
-      W.px[b + i] = W.g_x[g]; W.py[b + i] = W.g_y[g]; W.pz[b + i] = W.g_z[g];
-      W.vx[b + i] = W.g_vx[g]; W.vy[b + i] = W.g_vy[g]; W.vz[b + i] = W.g_vz[g];
-      W.pm[b + i] = W.g_m[g];
-      W.ptype[b + i] = W.g_type ? W.g_type[g] : 1;
-    }
     if (threadIdx.x == 0) S.len = pos;
     __syncthreads();
+  }
+}
+
+// Local copies of the listed particles' columns (slot-parallel).
+__global__ void __launch_bounds__(BLOCK) gather_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0 || l >= W.seg[k].len) continue;
+    const int g = W.gidx[s];
+    W.px[s] = W.g_x[g]; W.py[s] = W.g_y[g]; W.pz[s] = W.g_z[g];
+    W.vx[s] = W.g_vx[g]; W.vy[s] = W.g_vy[g]; W.vz[s] = W.g_vz[g];
+    W.pm[s] = W.g_m[g];
+    W.ptype[s] = W.g_type ? W.g_type[g] : 1;
   }
 }
 
@@ -383,33 +389,48 @@ __device__ __forceinline__ float node_len(const Seg &S, int v) {
 
 // Writes leaves and node headers in pre-order.  pre = exclusive prefix of
 // popcount(bits) over the segment's sorted sources.
+// Adds the wave's nodes to the per-depth counters: one atomic per depth per
+// wave instead of one per node (the counters are few and heavily shared).
+__device__ __forceinline__ void wave_count_levels(int *cnt, int v) {
+  if (__ballot(v >= 0) == 0ull) return;  // wave-uniform
+  const int lane = threadIdx.x & (WAVE - 1);
+  for (int lev = 0; lev < NLEVEL; lev++) {
+    const unsigned long long m = __ballot(v == lev);
+    if (m && lane == 0) atomicAdd(&cnt[lev], __popcll(m));
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
-    if (l < 0) continue;
-    const Seg &S = W.seg[k];
-    if (l >= S.n_src) continue;
-    const int64_t b = S.ibase, ib = 2 * b;
-    const u32 bl = W.bits[s];
-    const int pos = l + W.pre[s] + __popc(bl);
-    const int li = (int)W.v0[s];
-    W.ipm[ib + pos] = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
-    W.iax[ib + pos] = make_int2(0, (int)(ib + pos + 1));
-    if (l + 1 == S.n_src) W.seg[k].n_items = pos + 1;
-    const int lo = W.nlo[s];
-    if (lo >= 0) {
-      const int v = W.dl[s], hi = W.nhi[s];
-      const u32 blo = W.bits[b + lo];
-      const int npos = lo + W.pre[b + lo] + __popc(blo & ((1u << v) - 1u));
-      const int hpos = hi + W.pre[b + hi] + __popc(W.bits[b + hi]);
-      const float len = node_len(S, v);
-      const float lsq = len * len;
-      W.ipm[ib + npos] = make_float4(0.f, 0.f, 0.f, 0.f);
-      W.iax[ib + npos] = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
-      W.npos[s] = (int)(ib + npos);
-      atomicAdd(&W.lvl[v], 1);
+    int node_v = -1;
+    if (l >= 0) {
+      const Seg &S = W.seg[k];
+      if (l < S.n_src) {
+        const int64_t b = S.ibase, ib = 2 * b;
+        const u32 bl = W.bits[s];
+        const int pos = l + W.pre[s] + __popc(bl);
+        const int li = (int)W.v0[s];
+        W.ipm[ib + pos] = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
+        W.iax[ib + pos] = make_int2(0, (int)(ib + pos + 1));
+        if (l + 1 == S.n_src) W.seg[k].n_items = pos + 1;
+        const int lo = W.nlo[s];
+        if (lo >= 0) {
+          const int v = W.dl[s], hi = W.nhi[s];
+          const u32 blo = W.bits[b + lo];
+          const int npos = lo + W.pre[b + lo] + __popc(blo & ((1u << v) - 1u));
+          const int hpos = hi + W.pre[b + hi] + __popc(W.bits[b + hi]);
+          const float len = node_len(S, v);
+          const float lsq = len * len;
+          W.ipm[ib + npos] = make_float4(0.f, 0.f, 0.f, 0.f);
+          W.iax[ib + npos] = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
+          W.npos[s] = (int)(ib + npos);
+          node_v = v;
+        }
+      }
     }
+    wave_count_levels(W.lvl, node_v);
   }
 }
 
@@ -425,17 +446,31 @@ __global__ void level_offsets_kernel(int *lvl) {
   }
 }
 
+// Lists the nodes of each depth; positions within a depth come from one
+// atomic per depth per wave plus the lane's rank among the wave's nodes.
 __global__ void __launch_bounds__(BLOCK) node_list_kernel(Work W) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const unsigned long long below = (1ull << lane) - 1ull;
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
-    if (l < 0) continue;
-    const Seg &S = W.seg[k];
-    if (l + 1 >= S.n_src) continue;
-    if (W.nlo[s] < 0) continue;
-    const int v = W.dl[s];
-    const int at = W.lvl[NLEVEL + v] + atomicAdd(&W.lvl[2 * NLEVEL + v], 1);
-    W.nlist[at] = W.npos[s];
+    int v = -1, npos = 0;
+    if (l >= 0) {
+      const Seg &S = W.seg[k];
+      if (l + 1 < S.n_src && W.nlo[s] >= 0) {
+        v = W.dl[s];
+        npos = W.npos[s];
+      }
+    }
+    if (__ballot(v >= 0) == 0ull) continue;  // wave-uniform
+    for (int lev = 0; lev < NLEVEL; lev++) {
+      const unsigned long long m = __ballot(v == lev);
+      if (!m) continue;  // wave-uniform
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&W.lvl[2 * NLEVEL + lev], __popcll(m));
+      base = __shfl(base, 0, WAVE);
+      if (v == lev) W.nlist[W.lvl[NLEVEL + lev] + base + __popcll(m & below)] = npos;
+    }
   }
 }
 
@@ -542,6 +577,135 @@ __device__ __forceinline__ void flush_counts(const Work &W, unsigned ni, unsigne
   if ((threadIdx.x & (WAVE - 1)) == 0) {
     atomicAdd(&W.counters[0], a); atomicAdd(&W.counters[1], b); atomicAdd(&W.counters[2], c);
   }
+}
+
+// Wave-shared traversal.  A wave's 64 targets (consecutive in spatial order,
+// one segment) walk the pre-order items together: the wave stands on item j,
+// every lane whose own path reaches j decides for itself (open, or use the
+// monopole and skip the subtree), and the wave moves to j+1 if any lane opened
+// j, else to skip(j) -- lanes still skipping an accepted ancestor A have
+// skip(A) >= skip(j), so no lane's next item is passed over.  Each lane sees
+// exactly the items, decisions and summation order of its own walk; the item
+// index is wave-uniform, so items are read with scalar loads.
+template <bool STRADDLE, bool COUNT>
+__device__ float walk_shared_fast(const Work &W, const DevParams &P, int ib, int end, bool live,
+                                  float tx, float ty, float tz, float tmass, unsigned &ni,
+                                  unsigned &no) {
+#pragma clang fp contract(fast)
+  const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
+  Kahan acc;
+  acc.init(live ? tmass / P.eps : 0.f);
+  int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
+  int j = ib;
+  while (j < end) {
+    j = __builtin_amdgcn_readfirstlane(j);
+    const float4 q = W.ipm[j];
+    const int2 a = W.iax[j];
+    bool opened = false;
+    if (until <= j) {
+      float dx, dy, dz;
+      if (STRADDLE) {
+        dx = (float)nearest_d((double)q.x - (double)tx, P);
+        dy = (float)nearest_d((double)q.y - (double)ty, P);
+        dz = (float)nearest_d((double)q.z - (double)tz, P);
+      } else {
+        dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
+      }
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (__int_as_float(a.x) > r2 * P.theta2) {
+        opened = true;
+        if (COUNT) no++;
+      } else {
+        until = a.y;
+        if (COUNT) ni++;
+        acc.add(kernel_term_f(q.w, r2, h2, hinv));
+      }
+    }
+    j = __any(opened) ? j + 1 : (a.y > j ? a.y : end);  // skips always advance
+  }
+  return acc.s;
+}
+
+template <bool COUNT>
+__device__ double walk_shared_replay(const Work &W, const DevParams &P, int ib, int end, bool live,
+                                     float tx, float ty, float tz, float tmass, unsigned &ni,
+                                     unsigned &no) {
+#pragma clang fp contract(off)
+  const double h = 2.8 * (double)P.eps, hinv = 1.0 / h;
+  double pot = live ? (double)(float)(tmass / P.eps) : 0.0;
+  const double x = tx, y = ty, z = tz;
+  int until = live ? ib : INT_MAX;
+  int j = ib;
+  while (j < end) {
+    j = __builtin_amdgcn_readfirstlane(j);
+    const float4 q = W.ipm[j];
+    const int2 a = W.iax[j];
+    bool opened = false;
+    if (until <= j) {
+      double dx = (double)q.x - x, dy = (double)q.y - y, dz = (double)q.z - z;
+      if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if ((double)__int_as_float(a.x) > r2 * (double)P.theta2) {
+        opened = true;
+        if (COUNT) no++;
+      } else {
+        until = a.y;
+        if (COUNT) ni++;
+        pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
+      }
+    }
+    j = __any(opened) ? j + 1 : (a.y > j ? a.y : end);  // skips always advance
+  }
+  return pot;
+}
+
+// Binding energies with the wave-shared walk (same results as walk_kernel).
+template <bool COUNT, bool REPLAY>
+__global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P) {
+  unsigned ni = 0, no = 0, nt = 0;
+  SLOT_LOOP(s, W) {
+#pragma clang fp contract(off)
+    // segments are wave-aligned: the whole wave belongs to one segment
+    const int k = __builtin_amdgcn_readfirstlane(W.seg_of_slot[s]);
+    const Seg &S = W.seg[k];
+    const int64_t b = S.ibase;
+    const int l = (int)(s - b);
+    const bool live = l < S.n_tgt;
+    if (!__any(live)) continue;
+    if (COUNT && live) nt++;
+    const int li = live ? W.sp[s] : 0;
+    const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
+    const bool corr = (S.flags & F_CORR) != 0;
+    const float tm = corr ? 0.f : W.pm[b + li];
+    const int ib = (int)(2 * b), end = ib + S.n_items;
+    double pot;
+    if (REPLAY) {
+      pot = walk_shared_replay<COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
+    } else if (S.straddle) {
+      pot = (double)walk_shared_fast<true, COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
+    } else {
+      pot = (double)walk_shared_fast<false, COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
+    }
+    pot = pot * (double)P.G / (double)P.a;
+    if (!live) continue;
+    const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
+    if (!corr) {
+      float dv[3];
+      rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
+      if (REPLAY) {
+        W.E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
+      } else {
+        const double kin = 0.5 * ((double)dv[0] * dv[0] + (double)dv[1] * dv[1] + (double)dv[2] * dv[2]);
+        W.E[b + li] = (float)(kin + pot);
+      }
+    } else {
+      float ov[3];
+      rel_vel(P, x, y, z, vx, vy, vz, S.old_pos, S.old_vel, ov);
+      const float kin = vdot_f(ov, S.rvd) + S.dK;
+      W.E[b + li] = (float)((double)W.E[b + li] + ((double)kin - pot));
+    }
+  }
+  if (COUNT) flush_counts(W, ni, no, nt);
 }
 
 // Binding energies of the pass's targets, in spatial order.

@@ -100,6 +100,9 @@ struct hbt_ctx {
   DBuf lvl, active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
+  int walk_mode = 0;  // 0: wave-shared traversal, 1: independent per-lane walks
+  int use_graph = 1;  // capture the pass sequence in a hipGraph and replay it per pass
+  int graph_passes = 2;  // passes per graph launch (extra passes on a converged level are no-ops)
   hipEvent_t evw0 = nullptr, evw1 = nullptr;
   double walk_ms = 0;
   int64_t walk_launches = 0;
@@ -262,7 +265,7 @@ struct Runner {
   Work W;
   DevParams P;
   Level *L;
-  int gs, gseg, gtile;
+  int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
 
   int sort_pairs() {  // sorts (k0,v0) by nbuf, result in (k0,v0)
     tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), (int)L->tiles.size(),
@@ -311,18 +314,29 @@ struct Runner {
     return HBT_OK;
   }
 
+  // walk variants: wave-shared traversal (default) or one independent walk per lane
+  void launch_walk(bool count) {
+    const bool shared = c->walk_mode == 0;
+    if (shared) {
+      if (P.replay) {
+        if (count) walk_shared_kernel<true, true><<<gs, BLOCK, 0, c->stream>>>(W, P);
+        else walk_shared_kernel<false, true><<<gs, BLOCK, 0, c->stream>>>(W, P);
+      } else {
+        if (count) walk_shared_kernel<true, false><<<gs, BLOCK, 0, c->stream>>>(W, P);
+        else walk_shared_kernel<false, false><<<gs, BLOCK, 0, c->stream>>>(W, P);
+      }
+    } else {
+      if (count) walk_kernel<true><<<gs, BLOCK, 0, c->stream>>>(W, P);
+      else walk_kernel<false><<<gs, BLOCK, 0, c->stream>>>(W, P);
+    }
+  }
+
   int pass() {
     pass_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     RETCHK(build_tree(true));
-    if (c->prof_mode == 2) {
-      walk_kernel<true><<<gs, BLOCK, 0, c->stream>>>(W, P);
-    } else if (c->prof_mode == 1) {
-      HIPCHK(hipEventRecord(c->evw0, c->stream));
-      walk_kernel<false><<<gs, BLOCK, 0, c->stream>>>(W, P);
-      HIPCHK(hipEventRecord(c->evw1, c->stream));
-    } else {
-      walk_kernel<false><<<gs, BLOCK, 0, c->stream>>>(W, P);
-    }
+    if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evw0, c->stream));
+    launch_walk(c->prof_mode == 2);
+    if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evw1, c->stream));
     // partition
     set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_NLAST_ACTIVE);
     u64 *stot = c->stot.as<u64>();
@@ -348,14 +362,14 @@ struct Runner {
     RETCHK(sort_pairs());
     energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
     // frames, convergence
-    frames_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    frames_kernel<<<gblk, BLOCK, 0, c->stream>>>(W, P);
     // converged: sort the bound block by energy
     energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
     set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
     RETCHK(sort_pairs());
     energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
     output_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
-    output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
+    output_record_kernel<<<gblk, BLOCK, 0, c->stream>>>(W, P, 0);
     HIPCHK(hipGetLastError());
     return HBT_OK;
   }
@@ -466,9 +480,11 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   R.gtile = (int)std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)c->cus * 8);
   if (R.gtile < 1) R.gtile = 1;
   const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
+  R.gblk = gsb;
   fill_seg_of_slot_kernel<<<gsb, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, c->capal.as<int>(),
                                                         c->seg_of_slot.as<int>());
   build_lists_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, c->child_ptr.as<int>(), c->child_list.as<int>());
+  gather_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
   seg_init_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P);
   el_init_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
   output_list_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 1);
@@ -476,6 +492,47 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   HIPCHK(hipGetLastError());
   // every pass that does not converge removes at least one particle
   static const bool debug = getenv("HBT_DEBUG") != nullptr;
+  // The pass sequence has static launch shapes and arguments for the whole
+  // level, so it is captured once as a hipGraph (graph_passes passes plus the
+  // read-back of the active count) and replayed until no segment is active.
+  const bool graph = c->use_graph && c->prof_mode == 0 && !debug;
+  if (graph) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t gx = nullptr;
+    auto release = [&]() {
+      if (gx) (void)hipGraphExecDestroy(gx);
+      if (g) (void)hipGraphDestroy(g);
+    };
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = HBT_OK;
+    for (int p = 0; p < c->graph_passes && rc == HBT_OK; p++) rc = R.pass();
+    hipError_t e1 = hipMemcpyAsync(c->h_active, R.W.active, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc != HBT_OK || e1 != hipSuccess || e2 != hipSuccess) {
+      release();
+      return rc != HBT_OK ? rc : fail(HBT_ERR_DEVICE, "pass graph capture failed");
+    }
+    hipError_t e3 = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+      release();
+      return fail(HBT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    // every pass that does not converge removes at least one particle
+    const int max_launch = (L.maxcap + 2 + c->graph_passes - 1) / c->graph_passes;
+    for (int it = 0; it < max_launch; it++) {
+      hipError_t e = hipGraphLaunch(gx, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        release();
+        return fail(HBT_ERR_DEVICE, std::string("pass graph: ") + hipGetErrorString(e));
+      }
+      *passes += c->graph_passes;
+      if (*c->h_active == 0) break;
+    }
+    release();
+    return HBT_OK;
+  }
+  // every pass that does not converge removes at least one particle
   for (int it = 0; it < L.maxcap + 2; it++) {
     RETCHK(R.pass());
     (*passes)++;
@@ -579,6 +636,9 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   if (!c) return fail(HBT_ERR_OUT_OF_MEMORY, "context allocation");
   c->device = device;
   c->params = *params;
+  if (const char *wm = getenv("HBT_WALK_MODE")) c->walk_mode = atoi(wm) ? 1 : 0;
+  if (const char *g = getenv("HBT_GRAPH")) c->use_graph = atoi(g) ? 1 : 0;
+  if (const char *g = getenv("HBT_GRAPH_PASSES")) c->graph_passes = std::max(1, atoi(g));
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 4, hipHostMallocDefault);

@@ -69,10 +69,12 @@ def assert_tolerant(ref, got, band_rel=1e-5):
         a_e = list_of(rs, re_, s)
         na, nb = int(rs["nbound"][s]), int(gs["nbound"][s])
         stats["n"] += 1
-        if not (rs["status"][s] & HBT_STATUS_CONVERGED) or not (gs["status"][s] & HBT_STATUS_CONVERGED):
-            assert rs["status"][s] == gs["status"][s] or _band_count(a_e) > 0, \
-                f"subhalo {s}: status {rs['status'][s]} vs {gs['status'][s]}"
+        if not (gs["status"][s] & HBT_STATUS_CONVERGED):
+            # disrupted or early exit: one (or no) particle, the list keeps its order
+            assert na == nb and np.array_equal(a_i, b_i), f"subhalo {s}: non-converged outcome differs"
             continue
+        if na <= 1:
+            raise AssertionError(f"subhalo {s}: converged on the GPU, disrupted in the reference")
         emin = np.nanmin(a_e)
         band = band_rel * abs(emin)
         e_of = dict(zip(a_i.tolist(), a_e.tolist()))
@@ -93,7 +95,3 @@ def assert_tolerant(ref, got, band_rel=1e-5):
             run = np.maximum.accumulate(eg)
             assert np.all(run - eg <= band), f"subhalo {s}: bound order differs beyond the band"
     return stats
-
-
-def _band_count(e):
-    return int(np.sum(np.isfinite(e)))

@@ -169,3 +169,39 @@ def test_id_to_index(gpu):
     ref = oracle.id_to_index(table, q)
     assert np.array_equal(got, ref)
     assert U.id_to_index(table, np.zeros(0, np.int64)).size == 0
+
+
+# ---- execution variants give identical results ---------------------------------
+
+def _unbinder_with_env(params, **env):
+    import os
+    from hbtplus_amd.device import GpuUnbinder
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return GpuUnbinder(params, device=0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("prec", [abi.HBT_PREC_FAST, abi.HBT_PREC_REPLAY])
+def test_walk_modes_and_graph_are_identical(prec):
+    """wave-shared vs per-lane walk, hipGraph replay vs direct launches,
+    1 vs 3 passes per graph launch: byte-identical outputs"""
+    cat = synth.config2(seed=61, nsub=200, hi=20000)
+    p = _with(cat.params, precision=prec)
+    runs = []
+    for env in (dict(HBT_WALK_MODE=0, HBT_GRAPH=1, HBT_GRAPH_PASSES=2),
+                dict(HBT_WALK_MODE=1, HBT_GRAPH=1, HBT_GRAPH_PASSES=2),
+                dict(HBT_WALK_MODE=0, HBT_GRAPH=0),
+                dict(HBT_WALK_MODE=0, HBT_GRAPH=1, HBT_GRAPH_PASSES=3)):
+        U = _unbinder_with_env(p, **env)
+        runs.append(U.unbind_batch(cat))
+        U.close()
+    for other in runs[1:]:
+        for x, y in zip(runs[0], other):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
