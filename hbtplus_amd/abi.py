@@ -14,6 +14,7 @@ import numpy as np
 HBT_OK = 0
 HBT_PREC_FAST = 0
 HBT_PREC_REPLAY = 1
+HBT_PREC_FP32 = 2
 HBT_DIRECT_FP32 = 0
 HBT_DIRECT_FP32_KAHAN = 1
 HBT_DIRECT_FP64 = 2
@@ -112,6 +113,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_ctx_stream": ([vp, vp], i32),
         "hbt_plan_batch": ([vp, i64, vp], i32),
         "hbt_unbind_batch": ([vp, vp, vp, vp, i64, vp, vp, i32], i32),
+        "hbt_unbind_batch_indexed": ([vp, vp, vp, i64, vp, vp, i64, vp, vp, i32], i32),
         "hbt_unbind_one": ([vp, vp, vp, vp, vp], i32),
         "hbt_tree_potential": ([vp, i64, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp], i32),
         "hbt_direct_potential": ([vp, i64, vp, vp, vp, vp, vp, i32, vp], i32),

@@ -27,7 +27,7 @@ enum : int {
 // (src/config_parser.cpp:95-103).
 struct DevParams {
   float G, eps, theta2, res, box, box_half, bmp, relax, a, hz;
-  int periodic, min_part, snap, replay;
+  int periodic, min_part, snap, replay, kahan;
 };
 
 // Per-subhalo state for one level of the batch.
@@ -109,26 +109,39 @@ __device__ __forceinline__ double kernel_term_d(double m, double r, double h, do
   return m * hinv * wp;
 }
 
+// FP32 form with explicit fused multiply-adds (the walk variants share this
+// code, so their results agree bit for bit): rsq for 1/r, r = r2 * (1/r);
+// r2 == 0 is the self term (u = 0).
+__device__ __forceinline__ float pair_r2(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
 __device__ __forceinline__ float kernel_term_f(float m, float r2, float h2, float hinv) {
-  // FP32 form: rsq for 1/r, r = r2 * (1/r); r2 == 0 is the self term (u = 0).
+#pragma clang fp contract(off)
   const float rinv = __builtin_amdgcn_rsqf(r2);
   if (r2 >= h2) return -m * rinv;
   const float r = r2 > 0.f ? r2 * rinv : 0.f;
-  const float u = r * hinv;
+  const float u = r * hinv, u2 = u * u;
   float wp;
-  if (u < 0.5f)
-    wp = -2.8f + u * u * (5.333333333333f + u * u * (6.4f * u - 9.6f));
-  else
-    wp = -3.2f + 0.066666666667f * __builtin_amdgcn_rcpf(u) +
-         u * u * (10.666666666667f + u * (-16.0f + u * (9.6f - 2.133333333333f * u)));
+  if (u < 0.5f) {
+    wp = __builtin_fmaf(u2, __builtin_fmaf(u2, __builtin_fmaf(6.4f, u, -9.6f), 5.333333333333f), -2.8f);
+  } else {
+    float t = __builtin_fmaf(-2.133333333333f, u, 9.6f);
+    t = __builtin_fmaf(t, u, -16.0f);
+    t = __builtin_fmaf(t, u, 10.666666666667f);
+    wp = __builtin_fmaf(u2, t, __builtin_fmaf(0.066666666667f, __builtin_amdgcn_rcpf(u), -3.2f));
+  }
   return m * hinv * wp;
 }
 
+template <bool COMPENSATED = true>
 struct Kahan {
   float s, c;
   __device__ __forceinline__ void init(float x) { s = x; c = 0.f; }
   __device__ __forceinline__ void add(float x) {
 #pragma clang fp contract(off)
+    if (!COMPENSATED) { s += x; return; }
     float y = x - c;
     float t = s + y;
     c = (t - s) - y;

@@ -23,7 +23,7 @@ direct_kernel(const DTile *tiles, int ntiles, const float *x, const float *y, co
     const double h = 2.8 * (double)P.eps, hinv = 1.0 / h;
     const float hf = 2.8f * P.eps, h2f = hf * hf, hinvf = 1.0f / hf;
     double accd = 0.0;
-    Kahan acc;
+    Kahan<true> acc;
     acc.init(0.f);
     for (int64_t j0 = T.seg_lo; j0 < T.seg_hi; j0 += BLOCK) {
       const int64_t j = j0 + threadIdx.x;
@@ -42,7 +42,7 @@ direct_kernel(const DTile *tiles, int ntiles, const float *x, const float *y, co
           } else {
             float dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
             if (P.periodic) { dx = nearest_f(dx, P); dy = nearest_f(dy, P); dz = nearest_f(dz, P); }
-            const float term = kernel_term_f(s.w, dx * dx + dy * dy + dz * dz, h2f, hinvf);
+            const float term = kernel_term_f(s.w, pair_r2(dx, dy, dz), h2f, hinvf);
             if (MODE == HBT_DIRECT_FP32_KAHAN) acc.add(term);
             else acc.s += term;
           }

@@ -35,6 +35,7 @@ struct Work {
   const int64_t *g_id;
   const float *g_x, *g_y, *g_z, *g_vx, *g_vy, *g_vz, *g_m;
   const int32_t *g_type;
+  const int64_t *g_index;    // optional: list entry -> particle column position
   // working arrays [nslots]
   int *gidx;
   float *px, *py, *pz, *vx, *vy, *vz, *pm;
@@ -60,6 +61,14 @@ struct Work {
   int64_t *out_index;
   float *out_energy;
 };
+
+// counts list entries that are not positions in [0, n)
+__global__ void check_index_kernel(const int64_t *idx, int64_t n_list, int64_t n, int *bad) {
+  int cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_list; i += (int64_t)gridDim.x * blockDim.x)
+    cnt += (idx[i] < 0 || idx[i] >= n) ? 1 : 0;
+  if (cnt) atomicAdd(bad, cnt);
+}
 
 __device__ __forceinline__ int seg_lane_local(const Work &W, int64_t s, int &k) {
   k = W.seg_of_slot[s];
@@ -97,7 +106,8 @@ __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *chil
     const hbt_subhalo &r = W.subs[S.sub];
     const int64_t b = S.ibase;
     int pos = (int)r.count;
-    for (int i = threadIdx.x; i < (int)r.count; i += blockDim.x) W.gidx[b + i] = (int)(r.offset + i);
+    for (int i = threadIdx.x; i < (int)r.count; i += blockDim.x)
+      W.gidx[b + i] = W.g_index ? (int)W.g_index[r.offset + i] : (int)(r.offset + i);
     for (int c = child_ptr[k]; c < child_ptr[k + 1]; c++) {
       const hbt_subhalo &q = W.subs[child_list[c]];
       const int tail = (int)(q.list_len - q.nbound);
@@ -196,39 +206,39 @@ __global__ void pass_setup_kernel(Work W, DevParams P) {
 // ---------------------------------------------------------------------------
 // Tree: bounding box, root cube, keys
 
-__global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W) {
-  SLOT_LOOP(s, W) {
-    int k;
-    int l = seg_lane_local(W, s, k);
-    bool live = false;
-    unsigned int e[3] = {0, 0, 0};
-    if (l >= 0) {
-      const Seg &S = W.seg[k];
-      if (l < S.n_src) {
-        const int64_t b = S.ibase;
-        const int li = W.el[b + S.src_lo + l];
-        e[0] = f2ord(W.px[b + li]); e[1] = f2ord(W.py[b + li]); e[2] = f2ord(W.pz[b + li]);
-        live = true;
+// Bounding box of each segment's sources: a block reduces one tile, then one
+// set of atomics per tile (ordered-integer encoding of the floats).
+__global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W, const Tile *tiles, int ntiles) {
+  __shared__ unsigned int sh[6][BLOCK / WAVE];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    Seg &S = W.seg[tl.seg];
+    const int cnt = min(TILE, S.n_src - tl.start);
+    if (cnt <= 0) continue;  // uniform
+    const int64_t b = S.ibase;
+    unsigned int mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int li = W.el[b + S.src_lo + tl.start + i];
+      const unsigned int e[3] = {f2ord(W.px[b + li]), f2ord(W.py[b + li]), f2ord(W.pz[b + li])};
+      for (int j = 0; j < 3; j++) { mn[j] = min(mn[j], e[j]); mx[j] = max(mx[j], e[j]); }
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1)
+      for (int j = 0; j < 3; j++) {
+        mn[j] = min(mn[j], (unsigned int)__shfl_down((int)mn[j], d, WAVE));
+        mx[j] = max(mx[j], (unsigned int)__shfl_down((int)mx[j], d, WAVE));
+      }
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+      for (int j = 0; j < 3; j++) { sh[j][threadIdx.x / WAVE] = mn[j]; sh[3 + j][threadIdx.x / WAVE] = mx[j]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < 3; j++) {
+        unsigned int a = sh[j][0], c = sh[3 + j][0];
+        for (int w = 1; w < BLOCK / WAVE; w++) { a = min(a, sh[j][w]); c = max(c, sh[3 + j][w]); }
+        atomicMin(&S.bmin[j], a);
+        atomicMax(&S.bmax[j], c);
       }
     }
-    // wave-level pre-reduction when the whole wave works for one segment
-    const int k0 = __shfl(k, 0, WAVE);
-    const bool uniform = __all(live && k == k0);
-    if (uniform) {
-      unsigned int mn[3] = {e[0], e[1], e[2]}, mx[3] = {e[0], e[1], e[2]};
-      for (int d = WAVE / 2; d > 0; d >>= 1)
-        for (int j = 0; j < 3; j++) {
-          mn[j] = min(mn[j], (unsigned int)__shfl_down((int)mn[j], d, WAVE));
-          mx[j] = max(mx[j], (unsigned int)__shfl_down((int)mx[j], d, WAVE));
-        }
-      if ((threadIdx.x & (WAVE - 1)) == 0) {
-        Seg &S = W.seg[k];
-        for (int j = 0; j < 3; j++) { atomicMin(&S.bmin[j], mn[j]); atomicMax(&S.bmax[j], mx[j]); }
-      }
-    } else if (live) {
-      Seg &S = W.seg[k];
-      for (int j = 0; j < 3; j++) { atomicMin(&S.bmin[j], e[j]); atomicMax(&S.bmax[j], e[j]); }
-    }
+    __syncthreads();
   }
 }
 
@@ -389,22 +399,10 @@ __device__ __forceinline__ float node_len(const Seg &S, int v) {
 
 // Writes leaves and node headers in pre-order.  pre = exclusive prefix of
 // popcount(bits) over the segment's sorted sources.
-// Adds the wave's nodes to the per-depth counters: one atomic per depth per
-// wave instead of one per node (the counters are few and heavily shared).
-__device__ __forceinline__ void wave_count_levels(int *cnt, int v) {
-  if (__ballot(v >= 0) == 0ull) return;  // wave-uniform
-  const int lane = threadIdx.x & (WAVE - 1);
-  for (int lev = 0; lev < NLEVEL; lev++) {
-    const unsigned long long m = __ballot(v == lev);
-    if (m && lane == 0) atomicAdd(&cnt[lev], __popcll(m));
-  }
-}
-
 __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
-    int node_v = -1;
     if (l >= 0) {
       const Seg &S = W.seg[k];
       if (l < S.n_src) {
@@ -426,50 +424,8 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
           W.ipm[ib + npos] = make_float4(0.f, 0.f, 0.f, 0.f);
           W.iax[ib + npos] = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
           W.npos[s] = (int)(ib + npos);
-          node_v = v;
         }
       }
-    }
-    wave_count_levels(W.lvl, node_v);
-  }
-}
-
-__global__ void level_offsets_kernel(int *lvl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int o = 0;
-    for (int v = 0; v < NLEVEL; v++) {
-      lvl[NLEVEL + v] = o;
-      o += lvl[v];
-      lvl[2 * NLEVEL + v] = 0;
-    }
-    lvl[3 * NLEVEL] = o;
-  }
-}
-
-// Lists the nodes of each depth; positions within a depth come from one
-// atomic per depth per wave plus the lane's rank among the wave's nodes.
-__global__ void __launch_bounds__(BLOCK) node_list_kernel(Work W) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const unsigned long long below = (1ull << lane) - 1ull;
-  SLOT_LOOP(s, W) {
-    int k;
-    int l = seg_lane_local(W, s, k);
-    int v = -1, npos = 0;
-    if (l >= 0) {
-      const Seg &S = W.seg[k];
-      if (l + 1 < S.n_src && W.nlo[s] >= 0) {
-        v = W.dl[s];
-        npos = W.npos[s];
-      }
-    }
-    if (__ballot(v >= 0) == 0ull) continue;  // wave-uniform
-    for (int lev = 0; lev < NLEVEL; lev++) {
-      const unsigned long long m = __ballot(v == lev);
-      if (!m) continue;  // wave-uniform
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&W.lvl[2 * NLEVEL + lev], __popcll(m));
-      base = __shfl(base, 0, WAVE);
-      if (v == lev) W.nlist[W.lvl[NLEVEL + lev] + base + __popcll(m & below)] = npos;
     }
   }
 }
@@ -479,9 +435,13 @@ __global__ void __launch_bounds__(BLOCK) node_list_kernel(Work W) {
 // (UpdateInternalNodes/ProcessNode/FillNodeCenter, src/gravity_tree.cpp:18-77).
 __global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v) {
 #pragma clang fp contract(off)
-  const int lo = W.lvl[NLEVEL + v], hi = lo + W.lvl[v];
-  for (int i = lo + blockIdx.x * BLOCK + threadIdx.x; i < hi; i += gridDim.x * BLOCK) {
-    const int g = W.nlist[i];
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l + 1 >= S.n_src || W.dl[s] != v || W.nlo[s] < 0) continue;
+    const int g = W.npos[s];
     const int end = W.iax[g].y;
     double mass = 0., cx = 0., cy = 0., cz = 0.;
     int c = g + 1;
@@ -535,9 +495,9 @@ __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int
 template <bool STRADDLE, bool COUNT>
 __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n_items, float tx,
                             float ty, float tz, float tmass, unsigned &ni, unsigned &no) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
   const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
-  Kahan acc;
+  Kahan<true> acc;
   acc.init(tmass / P.eps);
   const int end = (int)(ib + n_items);
   int j = (int)ib;
@@ -552,7 +512,7 @@ __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n
     } else {
       dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
     }
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r2 = pair_r2(dx, dy, dz);
     if (__int_as_float(a.x) > r2 * P.theta2) { j++; if (COUNT) no++; continue; }
     j = a.y > j ? a.y : end;  // skips always advance
     if (COUNT) ni++;
@@ -587,13 +547,13 @@ __device__ __forceinline__ void flush_counts(const Work &W, unsigned ni, unsigne
 // skip(A) >= skip(j), so no lane's next item is passed over.  Each lane sees
 // exactly the items, decisions and summation order of its own walk; the item
 // index is wave-uniform, so items are read with scalar loads.
-template <bool STRADDLE, bool COUNT>
+template <bool STRADDLE, bool COUNT, bool KAHAN = true>
 __device__ float walk_shared_fast(const Work &W, const DevParams &P, int ib, int end, bool live,
                                   float tx, float ty, float tz, float tmass, unsigned &ni,
                                   unsigned &no) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
   const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
-  Kahan acc;
+  Kahan<KAHAN> acc;
   acc.init(live ? tmass / P.eps : 0.f);
   int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
   int j = ib;
@@ -611,7 +571,7 @@ __device__ float walk_shared_fast(const Work &W, const DevParams &P, int ib, int
       } else {
         dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
       }
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r2 = pair_r2(dx, dy, dz);
       if (__int_as_float(a.x) > r2 * P.theta2) {
         opened = true;
         if (COUNT) no++;
@@ -681,6 +641,9 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P)
     double pot;
     if (REPLAY) {
       pot = walk_shared_replay<COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
+    } else if (!P.kahan) {
+      pot = S.straddle ? (double)walk_shared_fast<true, COUNT, false>(W, P, ib, end, live, x, y, z, tm, ni, no)
+                       : (double)walk_shared_fast<false, COUNT, false>(W, P, ib, end, live, x, y, z, tm, ni, no);
     } else if (S.straddle) {
       pot = (double)walk_shared_fast<true, COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
     } else {
@@ -918,100 +881,131 @@ __global__ void __launch_bounds__(BLOCK) energy_writeback_kernel(Work W, int bou
 }
 
 // ---------------------------------------------------------------------------
-// Frame update (:380-421): correction switch, FP64 mass-weighted averages of
-// the bound block in Elist order, convergence.  One block per segment.
-__global__ void __launch_bounds__(BLOCK) frames_kernel(Work W, DevParams P) {
+// FAST frame sums, tile-parallel: every tile of an active segment sums its part
+// of the bound block in FP64 (mass, m*v, m*x relative to the periodic origin);
+// frames_combine_kernel adds the tiles of each segment in tile order, so the
+// result does not depend on scheduling.
+constexpr int FRAME_SUMS = 7;
+
+__global__ void __launch_bounds__(BLOCK)
+frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double *part) {
 #pragma clang fp contract(off)
-  for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
+  __shared__ double sh[FRAME_SUMS][BLOCK / WAVE];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const int cnt = (S.flags & F_ACTIVE) ? min(TILE, S.nbound - tl.start) : 0;
+    if (cnt <= 0) continue;  // uniform
+    const int64_t b = S.ibase;
+    const int l0 = W.el[b];
+    const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
+    double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int li = W.el[b + tl.start + i];
+      const float m = W.pm[b + li];
+      v[0] += m;
+      v[1] += W.vx[b + li] * m; v[2] += W.vy[b + li] * m; v[3] += W.vz[b + li] * m;
+      if (P.periodic) {
+        v[4] += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
+        v[5] += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
+        v[6] += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
+      } else {
+        v[4] += W.px[b + li] * m; v[5] += W.py[b + li] * m; v[6] += W.pz[b + li] * m;
+      }
+    }
+    for (int c = 0; c < FRAME_SUMS; c++) {
+      const double r = wave_reduce_sum(v[c]);
+      if ((threadIdx.x & (WAVE - 1)) == 0) sh[c][threadIdx.x / WAVE] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < FRAME_SUMS) {
+      double r = 0;
+      for (int w = 0; w < BLOCK / WAVE; w++) r += sh[threadIdx.x][w];
+      part[(int64_t)t * FRAME_SUMS + threadIdx.x] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// Correction switch, new frame, convergence (:380-421) for one segment, given
+// the FP64 sums over its bound block.
+__device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, double s1, double s2,
+                             double s3, double q1, double q2, double q3) {
+#pragma clang fp contract(off)
+  const int64_t b = S.ibase;
+  const int nb = S.nbound;
+  const int l0 = W.el[b];
+  const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
+  if (S.nlast - nb < nb) {  // correction switch uses the frame of the pass just done
+    S.flags |= F_CORR;
+    for (int j = 0; j < 3; j++) { S.old_pos[j] = S.ref_pos[j]; S.old_vel[j] = S.ref_vel[j]; }
+  }
+  if (nb == 0) {
+    msum = 0.;
+  } else if (nb == 1) {
+    S.ref_vel[0] = W.vx[b + l0]; S.ref_vel[1] = W.vy[b + l0]; S.ref_vel[2] = W.vz[b + l0];
+    msum = W.pm[b + l0];
+    S.ref_pos[0] = W.px[b + l0]; S.ref_pos[1] = W.py[b + l0]; S.ref_pos[2] = W.pz[b + l0];
+  } else {
+    S.ref_vel[0] = (float)(s1 / msum); S.ref_vel[1] = (float)(s2 / msum); S.ref_vel[2] = (float)(s3 / msum);
+    double q[3] = {q1 / msum, q2 / msum, q3 / msum};
+    for (int j = 0; j < 3; j++) {
+      if (P.periodic) q[j] += o[j];
+      S.ref_pos[j] = (float)q[j];
+    }
+  }
+  S.mbound = (float)msum;
+  if ((float)nb >= (float)S.nlast * P.bmp) {
+    S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_FINAL;
+    S.nsort = nb;  // bound block to sort
+  } else {
+    S.nsort = 0;
+    atomicAdd(W.active, 1);
+  }
+}
+
+__global__ void frames_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part) {
+#pragma clang fp contract(off)
+  SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
-    const bool active = (S.flags & F_ACTIVE) != 0;
-    __syncthreads();  // every thread has read the flags before thread 0 updates them
-    if (!active) continue;  // uniform
+    if (!(S.flags & F_ACTIVE)) continue;
+    double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
+    const int nt = (S.nbound + TILE - 1) / TILE;
+    for (int t = first_tile[k]; t < first_tile[k] + nt; t++)
+      for (int c = 0; c < FRAME_SUMS; c++) v[c] += part[(int64_t)t * FRAME_SUMS + c];
+    frame_update(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Frame update (:380-421) with the reference's summation order (REPLAY): one
+// thread per segment sums the bound block serially in Elist order.
+__global__ void frames_serial_kernel(Work W, DevParams P) {
+#pragma clang fp contract(off)
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_ACTIVE)) continue;
     const int64_t b = S.ibase;
     const int nb = S.nbound;
     const int l0 = W.el[b];
+    const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
     double msum = 0, s1 = 0, s2 = 0, s3 = 0, q1 = 0, q2 = 0, q3 = 0;
-    double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
     if (nb > 1) {
-      if (P.replay) {
-        if (threadIdx.x == 0) {
-          for (int i = 0; i < nb; i++) {
-            const int li = W.el[b + i];
-            const float m = W.pm[b + li];
-            msum += m;
-            s1 += W.vx[b + li] * m; s2 += W.vy[b + li] * m; s3 += W.vz[b + li] * m;
-            if (P.periodic) {
-              q1 += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
-              q2 += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
-              q3 += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
-            } else {
-              q1 += W.px[b + li] * m; q2 += W.py[b + li] * m; q3 += W.pz[b + li] * m;
-            }
-          }
+      for (int i = 0; i < nb; i++) {
+        const int li = W.el[b + i];
+        const float m = W.pm[b + li];
+        msum += m;
+        s1 += W.vx[b + li] * m; s2 += W.vy[b + li] * m; s3 += W.vz[b + li] * m;
+        if (P.periodic) {
+          q1 += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
+          q2 += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
+          q3 += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
+        } else {
+          q1 += W.px[b + li] * m; q2 += W.py[b + li] * m; q3 += W.pz[b + li] * m;
         }
-      } else {
-        for (int i = threadIdx.x; i < nb; i += BLOCK) {
-          const int li = W.el[b + i];
-          const float m = W.pm[b + li];
-          msum += m;
-          s1 += W.vx[b + li] * m; s2 += W.vy[b + li] * m; s3 += W.vz[b + li] * m;
-          if (P.periodic) {
-            q1 += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
-            q2 += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
-            q3 += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
-          } else {
-            q1 += W.px[b + li] * m; q2 += W.py[b + li] * m; q3 += W.pz[b + li] * m;
-          }
-        }
-        double v[7] = {msum, s1, s2, s3, q1, q2, q3};
-        __shared__ double r7[7][BLOCK / WAVE];
-        for (int c = 0; c < 7; c++) {
-          double t = wave_reduce_sum(v[c]);
-          if ((threadIdx.x & (WAVE - 1)) == 0) r7[c][threadIdx.x / WAVE] = t;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          for (int c = 0; c < 7; c++) {
-            double t = 0;
-            for (int w = 0; w < BLOCK / WAVE; w++) t += r7[c][w];
-            v[c] = t;
-          }
-          msum = v[0]; s1 = v[1]; s2 = v[2]; s3 = v[3]; q1 = v[4]; q2 = v[5]; q3 = v[6];
-        }
-        __syncthreads();
       }
     }
-    if (threadIdx.x == 0) {
-      // correction switch uses the frame of the pass just done (:383-392)
-      const int ndiff = S.nlast - nb;
-      if (ndiff < nb) {
-        S.flags |= F_CORR;
-        for (int j = 0; j < 3; j++) { S.old_pos[j] = S.ref_pos[j]; S.old_vel[j] = S.ref_vel[j]; }
-      }
-      if (nb == 0) {
-        msum = 0.;
-      } else if (nb == 1) {
-        S.ref_vel[0] = W.vx[b + l0]; S.ref_vel[1] = W.vy[b + l0]; S.ref_vel[2] = W.vz[b + l0];
-        msum = W.pm[b + l0];
-        S.ref_pos[0] = W.px[b + l0]; S.ref_pos[1] = W.py[b + l0]; S.ref_pos[2] = W.pz[b + l0];
-      } else {
-        S.ref_vel[0] = (float)(s1 / msum); S.ref_vel[1] = (float)(s2 / msum); S.ref_vel[2] = (float)(s3 / msum);
-        double q[3] = {q1 / msum, q2 / msum, q3 / msum};
-        for (int j = 0; j < 3; j++) {
-          if (P.periodic) q[j] += o[j];
-          S.ref_pos[j] = (float)q[j];
-        }
-      }
-      S.mbound = (float)msum;
-      if ((float)nb >= (float)S.nlast * P.bmp) {
-        S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_FINAL;
-        S.nsort = nb;  // bound block to sort
-      } else {
-        S.nsort = 0;
-        atomicAdd(W.active, 1);
-      }
-    }
-    __syncthreads();
+    frame_update(W, P, S, msum, s1, s2, s3, q1, q2, q3);
   }
 }
 
@@ -1040,157 +1034,174 @@ __global__ void __launch_bounds__(BLOCK) output_list_kernel(Work W, int early) {
   }
 }
 
-// per-segment (block): kinematics, type counts, record fields
-__global__ void __launch_bounds__(BLOCK) output_record_kernel(Work W, DevParams P, int early) {
+// Writes the record of a segment that finished, from its sums over the bound
+// block: kinematics (AverageKinematics, :189-232), particle types
+// (CountParticleTypes / CountParticles, src/subhalo.cpp:400-499) and the
+// state markers of Unbind (:269-421).
+__device__ void write_record(Work &W, const DevParams &P, Seg &S, double E, double K, double A0,
+                             double A1, double A2, double M, const int nt[HBT_TYPE_MAX],
+                             const float mts[HBT_TYPE_MAX]) {
 #pragma clang fp contract(off)
-  __shared__ double r7[7][BLOCK / WAVE];
-  __shared__ double rt[HBT_TYPE_MAX][BLOCK / WAVE];
-  __shared__ int ct[HBT_TYPE_MAX][BLOCK / WAVE];
-  for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
+  hbt_subhalo &r = W.subs[S.sub];
+  const int64_t b = S.ibase;
+  const bool conv = (S.flags & F_CONV) != 0;
+  const int nb = S.nbound;
+  if (S.flags & F_EARLY) {
+    float mb = 0.f;  // CountParticles: Mbound = float sum of MboundType
+    for (int t = 0; t < HBT_TYPE_MAX; t++) mb += mts[t];
+    r.mbound = mb;
+    r.status = HBT_STATUS_EARLY;
+  } else {
+    if (nb <= 1) {
+      r.specific_self_potential_energy = 0.f;
+      r.specific_self_kinetic_energy = 0.f;
+      r.specific_angular_momentum[0] = r.specific_angular_momentum[1] = r.specific_angular_momentum[2] = 0.f;
+    } else {
+      E /= M;
+      K *= 0.5 / M;
+      r.specific_self_potential_energy = (float)(E - K);
+      r.specific_self_kinetic_energy = (float)K;
+      r.specific_angular_momentum[0] = (float)(A0 / M);
+      r.specific_angular_momentum[1] = (float)(A1 / M);
+      r.specific_angular_momentum[2] = (float)(A2 / M);
+    }
+    if (conv) {
+      r.status = HBT_STATUS_CONVERGED;
+      if (r.snapshot_index_of_death != -1) r.snapshot_index_of_death = -1;
+      if (r.sink_track_id != -1) { r.snapshot_index_of_sink = -1; r.sink_track_id = -1; }
+      for (int j = 0; j < 3; j++) { r.ref_pos[j] = S.ref_pos[j]; r.ref_vel[j] = S.ref_vel[j]; }
+      const int l0 = W.el[b];
+      r.mostbound_pos[0] = W.px[b + l0]; r.mostbound_pos[1] = W.py[b + l0]; r.mostbound_pos[2] = W.pz[b + l0];
+      r.mostbound_vel[0] = W.vx[b + l0]; r.mostbound_vel[1] = W.vy[b + l0]; r.mostbound_vel[2] = W.vz[b + l0];
+      r.mbound = S.mbound;
+    } else {
+      // disruption (:361-379): frame <- previous most-bound coordinates
+      r.status = HBT_STATUS_DISRUPTED;
+      if (r.snapshot_index_of_death == -1) r.snapshot_index_of_death = P.snap;
+      for (int j = 0; j < 3; j++) { r.ref_pos[j] = r.mostbound_pos[j]; r.ref_vel[j] = r.mostbound_vel[j]; }
+      r.mbound = W.pm[b];
+    }
+  }
+  for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = nt[t]; r.mbound_type[t] = mts[t]; }
+  r.nbound = nb;
+  r.list_len = S.len;
+  long long ns = nb <= 1 ? nb : (long long)((float)nb * P.relax);
+  if (ns > S.len) ns = S.len;
+  r.nsource = ns;
+  r.n_iter = S.iter;
+  r.n_evals = S.evals;
+  S.flags &= ~F_FINAL;
+}
+
+// Kinematics of one bound particle (AverageKinematics, :205-224).
+__device__ __forceinline__ void kin_accum(const Work &W, const DevParams &P, const Seg &S, int li,
+                                          double v[6]) {
+#pragma clang fp contract(off)
+  const int64_t b = S.ibase;
+  const float m = W.pm[b + li];
+  v[0] += W.E[b + li] * m;
+  const float x[3] = {W.px[b + li], W.py[b + li], W.pz[b + li]};
+  const float vel[3] = {W.vx[b + li], W.vy[b + li], W.vz[b + li]};
+  double dx[3], dv[3];
+  for (int j = 0; j < 3; j++) {
+    dx[j] = x[j] - S.ref_pos[j];
+    if (P.periodic) dx[j] = nearest_d(dx[j], P);
+    dx[j] *= P.a;
+    dv[j] = (double)(vel[j] - S.ref_vel[j]) + P.hz * dx[j];
+    v[1] += dv[j] * dv[j] * m;
+  }
+  v[2] += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
+  v[3] += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
+  v[4] += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
+  v[5] += m;
+}
+
+// Serial form (REPLAY and early exits): one thread per finished segment,
+// sums in list order as the reference does with one thread.
+__global__ void output_record_kernel(Work W, DevParams P, int early) {
+#pragma clang fp contract(off)
+  SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     const bool fin = early ? (S.flags & F_EARLY) != 0 : (S.flags & F_FINAL) != 0;
-    __syncthreads();  // every thread has read the flags before thread 0 updates them
-    if (!fin) continue;  // uniform
-    hbt_subhalo &r = W.subs[S.sub];
+    if (!fin) continue;
     const int64_t b = S.ibase;
     const bool conv = (S.flags & F_CONV) != 0;
     const int nb = S.nbound;
-    // particle at list position i (Elist order for converged, list order otherwise)
-    auto at = [&](int i) { return conv ? W.el[b + i] : i; };
-    // AverageKinematics (:189-232) over [0, nb) in the frame RefPos/RefVel
-    double E = 0, K = 0, A0 = 0, A1 = 0, A2 = 0, M = 0;
-    double mt[HBT_TYPE_MAX];
-    int nt[HBT_TYPE_MAX];
-    for (int t = 0; t < HBT_TYPE_MAX; t++) { mt[t] = 0; nt[t] = 0; }
-    float mts[HBT_TYPE_MAX];
-    const bool serial = P.replay != 0;
-    // CountParticleTypes skips null IDs only for Nbound <= 100; CountParticles never
     const bool skip_null = !(S.flags & F_EARLY) && nb <= 100;
-    auto accum = [&](int i) {
-      const int li = at(i);
-      const float m = W.pm[b + li];
-      if (nb > 1) {
-        E += W.E[b + li] * m;
-        const float x[3] = {W.px[b + li], W.py[b + li], W.pz[b + li]};
-        const float v[3] = {W.vx[b + li], W.vy[b + li], W.vz[b + li]};
-        double dx[3], dv[3];
-        for (int j = 0; j < 3; j++) {
-          dx[j] = x[j] - S.ref_pos[j];
-          if (P.periodic) dx[j] = nearest_d(dx[j], P);
-          dx[j] *= P.a;
-          dv[j] = (double)(v[j] - S.ref_vel[j]) + P.hz * dx[j];
-          K += dv[j] * dv[j] * m;
-        }
-        A0 += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
-        A1 += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
-        A2 += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
-        M += m;
-      }
-    };
-    if (serial) {
-      if (threadIdx.x == 0) {
-        for (int i = 0; i < nb; i++) accum(i);
-        for (int t = 0; t < HBT_TYPE_MAX; t++) mts[t] = 0.f;
-        // CountParticleTypes (src/subhalo.cpp:400-444): float sums in list order;
-        // the Nbound<=100 branch skips null IDs
-        for (int i = 0; i < nb; i++) {
-          const int li = at(i);
-          if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
-          const int t = W.ptype[b + li];
-          nt[t]++;
-          mts[t] += W.pm[b + li];
-        }
-      }
-    } else {
-      for (int i = threadIdx.x; i < nb; i += BLOCK) {
-        accum(i);
-        const int li = at(i);
-        if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
-        const int t = W.ptype[b + li];
-        nt[t]++;
-        mt[t] += W.pm[b + li];
-      }
-      double v[7] = {E, K, A0, A1, A2, M, 0};
-      for (int c = 0; c < 6; c++) {
-        double t = wave_reduce_sum(v[c]);
-        if ((threadIdx.x & (WAVE - 1)) == 0) r7[c][threadIdx.x / WAVE] = t;
-      }
-      for (int t = 0; t < HBT_TYPE_MAX; t++) {
-        double a = wave_reduce_sum(mt[t]);
-        int c = wave_reduce_sum(nt[t]);
-        if ((threadIdx.x & (WAVE - 1)) == 0) { rt[t][threadIdx.x / WAVE] = a; ct[t][threadIdx.x / WAVE] = c; }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        for (int c = 0; c < 6; c++) {
-          double t = 0;
-          for (int w = 0; w < BLOCK / WAVE; w++) t += r7[c][w];
-          v[c] = t;
-        }
-        E = v[0]; K = v[1]; A0 = v[2]; A1 = v[3]; A2 = v[4]; M = v[5];
-        for (int t = 0; t < HBT_TYPE_MAX; t++) {
-          double a = 0;
-          int c = 0;
-          for (int w = 0; w < BLOCK / WAVE; w++) { a += rt[t][w]; c += ct[t][w]; }
-          mts[t] = (float)a;
-          nt[t] = c;
-        }
-      }
-      __syncthreads();
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    int nt[HBT_TYPE_MAX] = {0, 0, 0, 0, 0, 0};
+    float mts[HBT_TYPE_MAX] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < nb; i++) {
+      const int li = conv ? W.el[b + i] : i;
+      if (nb > 1) kin_accum(W, P, S, li, v);
     }
-    if (threadIdx.x == 0) {
-      const bool early_exit = (S.flags & F_EARLY) != 0;
-      if (early_exit) {
-        // CountParticles (src/subhalo.cpp:479-499): Mbound = float sum of MboundType
-        float mb = 0.f;
-        for (int t = 0; t < HBT_TYPE_MAX; t++) mb += mts[t];
-        r.mbound = mb;
-        r.status = HBT_STATUS_EARLY;
-      } else {
-        if (nb <= 1) {
-          r.specific_self_potential_energy = 0.f;
-          r.specific_self_kinetic_energy = 0.f;
-          r.specific_angular_momentum[0] = r.specific_angular_momentum[1] = r.specific_angular_momentum[2] = 0.f;
-        } else {
-          E /= M;
-          K *= 0.5 / M;
-          r.specific_self_potential_energy = (float)(E - K);
-          r.specific_self_kinetic_energy = (float)K;
-          r.specific_angular_momentum[0] = (float)(A0 / M);
-          r.specific_angular_momentum[1] = (float)(A1 / M);
-          r.specific_angular_momentum[2] = (float)(A2 / M);
-        }
-        if (conv) {
-          r.status = HBT_STATUS_CONVERGED;
-          if (r.snapshot_index_of_death != -1) r.snapshot_index_of_death = -1;
-          if (r.sink_track_id != -1) { r.snapshot_index_of_sink = -1; r.sink_track_id = -1; }
-          for (int j = 0; j < 3; j++) { r.ref_pos[j] = S.ref_pos[j]; r.ref_vel[j] = S.ref_vel[j]; }
-          const int l0 = W.el[b];
-          r.mostbound_pos[0] = W.px[b + l0]; r.mostbound_pos[1] = W.py[b + l0]; r.mostbound_pos[2] = W.pz[b + l0];
-          r.mostbound_vel[0] = W.vx[b + l0]; r.mostbound_vel[1] = W.vy[b + l0]; r.mostbound_vel[2] = W.vz[b + l0];
-          r.mbound = S.mbound;
-        } else {
-          // disruption (:361-379): frame <- previous most-bound coordinates
-          r.status = HBT_STATUS_DISRUPTED;
-          if (r.snapshot_index_of_death == -1) r.snapshot_index_of_death = P.snap;
-          for (int j = 0; j < 3; j++) { r.ref_pos[j] = r.mostbound_pos[j]; r.ref_vel[j] = r.mostbound_vel[j]; }
-          r.mbound = W.pm[b];
-        }
-      }
-      for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = nt[t]; r.mbound_type[t] = mts[t]; }
-      r.nbound = nb;
-      r.list_len = S.len;
-      long long ns = nb <= 1 ? nb : (long long)((float)nb * P.relax);
-      if (ns > S.len) ns = S.len;
-      r.nsource = ns;
-      r.n_iter = S.iter;
-      r.n_evals = S.evals;
-      if (S.flags & F_ORPHAN) {
-        // RecursiveUnbind keeps the orphan's extended list in its original order
-      }
-      S.flags &= ~F_FINAL;
+    for (int i = 0; i < nb; i++) {
+      const int li = conv ? W.el[b + i] : i;
+      if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
+      const int t = W.ptype[b + li];
+      nt[t]++;
+      mts[t] += W.pm[b + li];
+    }
+    write_record(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], nt, mts);
+  }
+}
+
+// FAST form, tile-parallel: partial sums per tile (6 kinematic sums, 6 type
+// counts, 6 type masses in FP64), combined per segment in tile order.
+constexpr int REC_SUMS = 18;
+
+__global__ void __launch_bounds__(BLOCK)
+output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double *part) {
+#pragma clang fp contract(off)
+  __shared__ double sh[REC_SUMS][BLOCK / WAVE];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const bool fin = (S.flags & F_FINAL) != 0;
+    const int cnt = fin ? min(TILE, S.nbound - tl.start) : 0;
+    if (cnt <= 0) continue;  // uniform
+    const int64_t b = S.ibase;
+    const bool conv = (S.flags & F_CONV) != 0;
+    const int nb = S.nbound;
+    const bool skip_null = nb <= 100;
+    double v[REC_SUMS];
+    for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int li = conv ? W.el[b + tl.start + i] : tl.start + i;
+      if (nb > 1) kin_accum(W, P, S, li, v);
+      if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
+      const int ty = W.ptype[b + li];
+      for (int q = 0; q < HBT_TYPE_MAX; q++)
+        if (q == ty) { v[6 + q] += 1.0; v[12 + q] += W.pm[b + li]; }
+    }
+    for (int c = 0; c < REC_SUMS; c++) {
+      const double r = wave_reduce_sum(v[c]);
+      if ((threadIdx.x & (WAVE - 1)) == 0) sh[c][threadIdx.x / WAVE] = r;
     }
     __syncthreads();
+    if (threadIdx.x < REC_SUMS) {
+      double r = 0;
+      for (int w = 0; w < BLOCK / WAVE; w++) r += sh[threadIdx.x][w];
+      part[(int64_t)t * REC_SUMS + threadIdx.x] = r;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void output_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part) {
+#pragma clang fp contract(off)
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_FINAL)) continue;
+    double v[REC_SUMS];
+    for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
+    const int nt_ = (S.nbound + TILE - 1) / TILE;
+    for (int t = first_tile[k]; t < first_tile[k] + nt_; t++)
+      for (int c = 0; c < REC_SUMS; c++) v[c] += part[(int64_t)t * REC_SUMS + c];
+    int nt[HBT_TYPE_MAX];
+    float mts[HBT_TYPE_MAX];
+    for (int q = 0; q < HBT_TYPE_MAX; q++) { nt[q] = (int)v[6 + q]; mts[q] = (float)v[12 + q]; }
+    write_record(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], nt, mts);
   }
 }
 

@@ -59,6 +59,7 @@ DevParams make_dev_params(const hbt_params &p) {
   d.min_part = p.min_num_part_of_sub;
   d.snap = p.snapshot_index;
   d.replay = p.precision == HBT_PREC_REPLAY;
+  d.kahan = p.precision != HBT_PREC_FP32;
   return d;
 }
 
@@ -96,7 +97,7 @@ struct hbt_ctx {
   DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, cmin, bits, pre, nlo, nhi,
       npos, nlist, ipm, iax, scan64, pk, gk, isb, seg_of_slot;
   // per-level tables
-  DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal;
+  DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal, tpart;
   DBuf lvl, active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
@@ -109,6 +110,7 @@ struct hbt_ctx {
   // staging for hbt_unbind_one / hbt_tree_potential
   DBuf st_cols, st_subs, st_out_i, st_out_e, st_phi;
   int *h_active = nullptr;  // pinned
+  const int64_t *list_index = nullptr;  // set for the duration of hbt_unbind_batch_indexed
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   int64_t last_evals = 0, last_passes = 0;
@@ -116,7 +118,7 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &cmin, &bits, &pre, &nlo, &nhi, &npos, &nlist, &ipm, &iax, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &seg, &ibase, &nbuf, &tiles, &first_tile, &ntile, &tsum,
-                    &stot, &child_ptr, &child_list, &capal, &lvl, &active, &counters, &st_cols, &st_subs,
+                    &stot, &child_ptr, &child_list, &capal, &tpart, &lvl, &active, &counters, &st_cols, &st_subs,
                     &st_out_i, &st_out_e, &st_phi})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
@@ -298,7 +300,7 @@ struct Runner {
 
   // tree over the pass's sources: keys, sort, structure, moments
   int build_tree(bool copy_sp) {
-    bbox_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, c->tiles.as<Tile>(), (int)L->tiles.size());
     tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SRC);
@@ -308,8 +310,6 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     (u64 *)nullptr));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    level_offsets_kernel<<<1, 1, 0, c->stream>>>(W.lvl);
-    node_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     for (int v = MAX_DEPTH; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
     return HBT_OK;
   }
@@ -362,14 +362,26 @@ struct Runner {
     RETCHK(sort_pairs());
     energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
     // frames, convergence
-    frames_kernel<<<gblk, BLOCK, 0, c->stream>>>(W, P);
+    if (P.replay) {
+      frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    } else {
+      frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, c->tiles.as<Tile>(), (int)L->tiles.size(),
+                                                            c->tpart.as<double>());
+      frames_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, c->first_tile.as<int>(), c->tpart.as<double>());
+    }
     // converged: sort the bound block by energy
     energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
     set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
     RETCHK(sort_pairs());
     energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
     output_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
-    output_record_kernel<<<gblk, BLOCK, 0, c->stream>>>(W, P, 0);
+    if (P.replay) {
+      output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
+    } else {
+      output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, c->tiles.as<Tile>(), (int)L->tiles.size(),
+                                                            c->tpart.as<double>());
+      output_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, c->first_tile.as<int>(), c->tpart.as<double>());
+    }
     HIPCHK(hipGetLastError());
     return HBT_OK;
   }
@@ -407,6 +419,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->seg_of_slot.ensure(s * 4));
   RETCHK(c->nbuf.ensure((size_t)(nseg + 1) * 4));
   RETCHK(c->tsum.ensure((size_t)(ntiles + 1) * 8));
+  RETCHK(c->tpart.ensure((size_t)(ntiles + 1) * 8 * REC_SUMS));
   RETCHK(c->stot.ensure((size_t)(nseg + 1) * 8));
   RETCHK(c->lvl.ensure((3 * NLEVEL + 1) * 4));
   RETCHK(c->active.ensure(16));
@@ -423,6 +436,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.seg_of_slot = c->seg_of_slot.as<int>();
   W.ibase = c->ibase.as<int64_t>();
   W.nbuf = c->nbuf.as<int>();
+  W.g_index = c->list_index;
   if (parts) {
     W.g_id = parts->id; W.g_x = parts->x; W.g_y = parts->y; W.g_z = parts->z;
     W.g_vx = parts->vx; W.g_vy = parts->vy; W.g_vz = parts->vz; W.g_m = parts->mass;
@@ -488,14 +502,16 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   seg_init_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P);
   el_init_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
   output_list_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 1);
-  output_record_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, R.P, 1);
+  output_record_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P, 1);
   HIPCHK(hipGetLastError());
   // every pass that does not converge removes at least one particle
   static const bool debug = getenv("HBT_DEBUG") != nullptr;
   // The pass sequence has static launch shapes and arguments for the whole
   // level, so it is captured once as a hipGraph (graph_passes passes plus the
   // read-back of the active count) and replayed until no segment is active.
-  const bool graph = c->use_graph && c->prof_mode == 0 && !debug;
+  // the plain form (one subhalo per level) launches directly, as a per-subhalo
+  // Unbind loop does; graphs serve the batched form
+  const bool graph = c->use_graph && c->prof_mode == 0 && !debug && L.seg.size() > 1;
   if (graph) {
     hipGraph_t g = nullptr;
     hipGraphExec_t gx = nullptr;
@@ -695,10 +711,47 @@ int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
   return HBT_OK;
 }
 
+static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64_t n_list,
+                             hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
+                             int64_t *out_index, float *out_energy, int plain);
+
 int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs,
                      const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
                      float *out_energy, int plain) {
-  if (!ctx || !particles || (nsub > 0 && (!subs || !subs_host || !out_index || !out_energy)))
+  if (!ctx || !particles) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  ctx->list_index = nullptr;
+  return unbind_batch_impl(ctx, particles, particles->n, subs, subs_host, nsub, out_index, out_energy,
+                           plain);
+}
+
+int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
+                             int64_t n_list, hbt_subhalo *subs, const hbt_subhalo *subs_host,
+                             int64_t nsub, int64_t *out_index, float *out_energy, int plain) {
+  if (!ctx || !snapshot || (n_list > 0 && !list_index) || n_list < 0)
+    return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (n_list >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 list entries");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (n_list > 0) {
+    RETCHK(ctx->active.ensure(16));
+    int zero = 0;
+    HIPCHK(hipMemcpyAsync(ctx->active.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    check_index_kernel<<<grid_for(n_list, ctx->cus), BLOCK, 0, ctx->stream>>>(list_index, n_list, snapshot->n,
+                                                                               ctx->active.as<int>());
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, ctx->active.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (bad) return fail(HBT_ERR_INVALID_ARG, std::to_string(bad) + " list entries are not snapshot positions");
+  }
+  ctx->list_index = list_index;
+  int rc = unbind_batch_impl(ctx, snapshot, n_list, subs, subs_host, nsub, out_index, out_energy, plain);
+  ctx->list_index = nullptr;
+  return rc;
+}
+
+static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64_t n_list,
+                             hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
+                             int64_t *out_index, float *out_energy, int plain) {
+  if (nsub > 0 && (!subs || !subs_host || !out_index || !out_energy))
     return fail(HBT_ERR_INVALID_ARG, "null argument");
   if (nsub == 0) return HBT_OK;
   if (particles->n >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 particles");
@@ -708,7 +761,7 @@ int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *
   std::vector<std::vector<int>> children;
   RETCHK(plan_nests(subs_host, nsub, cap, height, children, order));
   for (int64_t s = 0; s < nsub; s++) {
-    if (subs_host[s].offset < 0 || subs_host[s].offset + subs_host[s].count > particles->n)
+    if (subs_host[s].offset < 0 || subs_host[s].offset + subs_host[s].count > n_list)
       return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the particle columns");
     if (cap[s] >= (int64_t)INT32_MAX / 2) return fail(HBT_ERR_UNSUPPORTED, "subhalo list too long");
   }

@@ -92,6 +92,40 @@ class GpuUnbinder:
         total = h["total"]
         return subs, out_i.cpu().numpy()[:total].copy(), out_e.cpu().numpy()[:total].copy()
 
+    # snapshot + ID lists (lists resolved on the device) ---------------------
+    def upload_snapshot(self, snap_cols, list_ids, subs):
+        """Snapshot columns, the concatenated source-ID lists and the records
+        (offset/count index the ID list) to the device."""
+        cols = {k: self._dev(snap_cols[k]) for k in PARTICLE_COLUMNS}
+        subs = subs.copy()
+        total = plan_batch(subs)
+        return dict(cols=cols, ids=self._dev(np.asarray(list_ids, np.int64)), subs_host=subs,
+                    total=total, subs0=self._dev(subs.view(np.uint8)),
+                    ps=particles_struct(cols, ptr=lambda t: t.data_ptr()))
+
+    def resolve_ids(self, h):
+        """ID -> snapshot position for every list entry (hbt_id_to_index)."""
+        t = self.torch
+        n = len(h["ids"])
+        out = t.empty(max(n, 1), dtype=t.int64, device=f"cuda:{self.device}")
+        t.cuda.synchronize(self.device)
+        check(self.lib.hbt_id_to_index(self.ctx, len(h["cols"]["id"]), h["cols"]["id"].data_ptr(), n,
+                                       h["ids"].data_ptr(), out.data_ptr()), "hbt_id_to_index")
+        return out
+
+    def unbind_indexed(self, h, list_index, plain=False):
+        t = self.torch
+        subs_dev = h["subs0"].clone()
+        out_i = t.empty(max(h["total"], 1), dtype=t.int64, device=f"cuda:{self.device}")
+        out_e = t.empty(max(h["total"], 1), dtype=t.float32, device=f"cuda:{self.device}")
+        t.cuda.synchronize(self.device)
+        check(self.lib.hbt_unbind_batch_indexed(self.ctx, ctypes.byref(h["ps"]), list_index.data_ptr(),
+                                                len(h["ids"]), subs_dev.data_ptr(),
+                                                h["subs_host"].ctypes.data, len(h["subs_host"]),
+                                                out_i.data_ptr(), out_e.data_ptr(), 1 if plain else 0),
+              "hbt_unbind_batch_indexed")
+        return subs_dev, out_i, out_e
+
     def last_stats(self):
         ms = ctypes.c_double()
         ev = ctypes.c_int64()

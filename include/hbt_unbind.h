@@ -63,6 +63,7 @@ extern "C" {
  *       contraction) for bit-level certification. */
 #define HBT_PREC_FAST    0
 #define HBT_PREC_REPLAY  1
+#define HBT_PREC_FP32    2   /* FAST without the Kahan compensation (precision study) */
 
 /* direct-sum accumulation modes (hbt_direct_potential) */
 #define HBT_DIRECT_FP32        0   /* plain FP32 sum */
@@ -172,6 +173,21 @@ int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total);
 int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles,
                      hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
                      int64_t *out_index, float *out_energy, int plain);
+
+/* Batched unbinding of subhalos whose source lists are positions in a
+ * snapshot's particle columns -- what RefineParticles sees after the source
+ * IDs were resolved (ParticleExchanger_t::QueryParticles,
+ * src/particle_exchanger.h:193-208; on the device: hbt_id_to_index).
+ *   snapshot  : device particle columns of the snapshot
+ *   list_index: DEVICE int64[n_list]; subs[].offset/count are ranges of it and
+ *               its entries are snapshot positions (negative or out-of-range
+ *               positions are rejected with HBT_ERR_INVALID_ARG)
+ *   out_index : snapshot positions, layout as in hbt_unbind_batch
+ * Other arguments as in hbt_unbind_batch. */
+int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot,
+                             const int64_t *list_index, int64_t n_list, hbt_subhalo *subs,
+                             const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
+                             float *out_energy, int plain);
 
 /* Per-subhalo form with HOST buffers (Subhalo_t::Unbind).  `particles` are
  * host columns of this subhalo's list (n = sub->count, sub->offset ignored);

@@ -205,3 +205,33 @@ def test_walk_modes_and_graph_are_identical(prec):
     for other in runs[1:]:
         for x, y in zip(runs[0], other):
             assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+
+
+def test_indexed_batch_matches_direct(gpu):
+    """Lists given as IDs, resolved on the device, unbound from a shuffled
+    snapshot with background particles: same results as the direct batch."""
+    snap, ids, cat = synth.config5(seed=5, nsub=300)
+    U = gpu(cat.params)
+    h = U.upload_snapshot(snap, ids, cat.subs)
+    idx = U.resolve_ids(h)
+    pos = idx.cpu().numpy()
+    assert np.array_equal(snap["id"][pos], ids)
+    s1, i1, e1 = U.unbind_indexed(h, idx)
+    s2, i2, e2 = U.unbind_batch(cat)
+    s1 = s1.cpu().numpy().view(abi.SUB_DTYPE)
+    i1 = i1.cpu().numpy()[: len(i2)]
+    for f in abi.OUTPUT_FIELDS:
+        assert np.array_equal(s1[f], s2[f]), f
+    # out_index: snapshot positions vs catalogue positions of the same particles
+    assert np.array_equal(snap["id"][i1], cat.cols["id"][i2])
+    assert np.array_equal(e1.cpu().numpy()[: len(e2)].view(np.uint32), e2.view(np.uint32))
+
+
+def test_indexed_batch_rejects_unresolved(gpu):
+    snap, ids, cat = synth.config5(seed=6, nsub=20)
+    U = gpu(cat.params)
+    h = U.upload_snapshot(snap, ids, cat.subs)
+    idx = U.resolve_ids(h)
+    idx[3] = -1
+    with pytest.raises(RuntimeError, match="not snapshot positions"):
+        U.unbind_indexed(h, idx)
