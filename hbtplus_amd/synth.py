@@ -66,17 +66,30 @@ def _mu(x):
 
 
 def nfw_unit_radii(rng, c: np.ndarray) -> np.ndarray:
-    """x = r/r_s with M(<x) uniform, by bisection on mu(x) = u mu(c)."""
+    """x = r/r_s with M(<x) uniform: solve mu(x) = u mu(c) by inverting a table
+    of mu on a log grid (per 1% bin of c), then three Newton steps."""
     c = np.asarray(c, np.float64)
     target = rng.random(c.shape) * _mu(c)
-    lo = np.zeros_like(c)
-    hi = c.copy()
-    for _ in range(55):
-        mid = 0.5 * (lo + hi)
-        below = _mu(mid) < target
-        lo = np.where(below, mid, lo)
-        hi = np.where(below, hi, mid)
-    return 0.5 * (lo + hi)
+    if c.size == 0:
+        return c.copy()
+    lc = np.log(c)
+    lc0 = lc.min()
+    bins = np.floor((lc - lc0) / 0.01).astype(np.int64)
+    x = np.empty_like(c)
+    ly = np.linspace(np.log(1e-7), 0.0, 2048)
+    order = np.argsort(bins, kind="stable")
+    edges = np.searchsorted(bins[order], np.arange(bins.max() + 2))
+    for b in range(len(edges) - 1):
+        sel = order[edges[b]:edges[b + 1]]
+        if sel.size == 0:
+            continue
+        cb = float(np.exp(lc0 + 0.01 * (b + 0.5)))
+        ms = _mu(cb * np.exp(ly)) / _mu(cb)
+        x[sel] = c[sel] * np.exp(np.interp(target[sel] / _mu(c[sel]), ms, ly))
+    for _ in range(3):  # Newton on mu(x) - target, mu'(x) = x / (1+x)^2
+        f = _mu(x) - target
+        x = np.clip(x - f * (1.0 + x) ** 2 / np.maximum(x, 1e-12), 1e-9, c)
+    return x
 
 
 class _JeansTable:
@@ -200,6 +213,9 @@ def _assemble(rng, groups, box, periodic, params, name, parent=None, shuffle_lis
 
 def _power_law_population(rng, nsub, lo, hi, cont_frac, cont_mode, c_median, c_sigma_ln, G,
                           box, centre_fn, log_uniform=False):
+    """Arrays of a power-law subhalo population, grouped by subhalo with a
+    random order inside each group: (owner, relative positions, relative
+    velocities, centres, bulk velocities)."""
     if log_uniform:
         npart = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), nsub))).astype(np.int64)
     else:
@@ -224,14 +240,49 @@ def _power_law_population(rng, nsub, lo, hi, cont_frac, cont_mode, c_median, c_s
         cv[:, 0] += 300.0
     centres = centre_fn(nsub)
     bulk = rng.normal(size=(nsub, 3)) * 300.0
-    pstart = np.concatenate([[0], np.cumsum(npart)])
-    cstart = np.concatenate([[0], np.cumsum(ncont)])
-    groups = []
-    for s in range(nsub):
-        rp = np.concatenate([rel_p[pstart[s]:pstart[s + 1]], cp[cstart[s]:cstart[s + 1]]])
-        rv = np.concatenate([rel_v[pstart[s]:pstart[s + 1]], cv[cstart[s]:cstart[s + 1]]])
-        groups.append((centres[s], bulk[s], rp, rv))
-    return groups
+    own = np.concatenate([owner, cowner])
+    order = np.argsort(own + rng.random(len(own)))  # group by subhalo, random order inside
+    return (own[order], np.concatenate([rel_p, cp])[order], np.concatenate([rel_v, cv])[order],
+            centres, bulk)
+
+
+def _assemble_arrays(rng, own, rel_p, rel_v, centres, bulk, box, periodic, params, name):
+    """Vectorised form of _assemble for particles already grouped by owner."""
+    nsub = len(centres)
+    counts = np.bincount(own, minlength=nsub).astype(np.int64)
+    off = np.zeros(nsub, np.int64)
+    off[1:] = np.cumsum(counts)[:-1]
+    pos = centres[own] + rel_p
+    vel = bulk[own] + rel_v
+    safe = np.maximum(counts, 1)[:, None]
+    frame_p = centres + np.stack([np.bincount(own, rel_p[:, j], nsub) for j in range(3)], 1) / safe
+    frame_v = bulk + np.stack([np.bincount(own, rel_v[:, j], nsub) for j in range(3)], 1) / safe
+    if periodic:
+        pos = np.mod(pos, box)
+        frame_p = np.mod(frame_p, box)
+    n = len(own)
+    cols = {
+        "id": unique_ids(rng, n),
+        "x": pos[:, 0].astype(np.float32), "y": pos[:, 1].astype(np.float32),
+        "z": pos[:, 2].astype(np.float32),
+        "vx": vel[:, 0].astype(np.float32), "vy": vel[:, 1].astype(np.float32),
+        "vz": vel[:, 2].astype(np.float32),
+        "mass": np.full(n, PARTICLE_MASS, np.float32),
+        "type": np.ones(n, np.int32),
+    }
+    subs = np.zeros(nsub, SUB_DTYPE)
+    subs["offset"] = off
+    subs["count"] = counts
+    subs["parent"] = -1
+    subs["nbound_prev"] = counts
+    subs["sink_track_id"] = -1
+    subs["snapshot_index_of_death"] = -1
+    subs["snapshot_index_of_sink"] = -1
+    subs["ref_pos"] = frame_p.astype(np.float32)
+    subs["ref_vel"] = frame_v.astype(np.float32)
+    subs["mostbound_pos"] = subs["ref_pos"]
+    subs["mostbound_vel"] = subs["ref_vel"]
+    return Catalogue(cols, subs, params, name)
 
 
 def config1(seed: int = 1001, bound_mass_precision: float = 0.5, n_nfw=20000, n_cont=4000) -> Catalogue:
@@ -259,10 +310,10 @@ def config2(seed: int = 1002, nsub: int = 5000, hi: float = 2.0e5) -> Catalogue:
     20% uniform contaminants at 3x the local escape speed."""
     rng = _rng(seed)
     box = 1.0e5
-    groups = _power_law_population(rng, nsub, 20, hi, 0.2, "escape", 10.0, 0.3, G_DEFAULT, box,
-                                   lambda n: rng.random((n, 3)) * box)
+    pop = _power_law_population(rng, nsub, 20, hi, 0.2, "escape", 10.0, 0.3, G_DEFAULT, box,
+                                lambda n: rng.random((n, 3)) * box)
     params = make_params(softening=0.5, open_angle=0.45, periodic=1, box_size=box)
-    return _assemble(rng, groups, box, True, params, f"C2 {nsub} power-law subhalos")
+    return _assemble_arrays(rng, *pop, box, True, params, f"C2 {nsub} power-law subhalos")
 
 
 def config3(seed: int = 1003, nsub: int = 50000) -> Catalogue:
@@ -270,10 +321,10 @@ def config3(seed: int = 1003, nsub: int = 50000) -> Catalogue:
     15% contaminants as in config 1."""
     rng = _rng(seed)
     box = 1.0e5
-    groups = _power_law_population(rng, nsub, 20, 4096, 0.15, "gauss", 10.0, 0.0, G_DEFAULT, box,
-                                   lambda n: rng.random((n, 3)) * box, log_uniform=True)
+    pop = _power_law_population(rng, nsub, 20, 4096, 0.15, "gauss", 10.0, 0.0, G_DEFAULT, box,
+                                lambda n: rng.random((n, 3)) * box, log_uniform=True)
     params = make_params(softening=0.5, open_angle=0.45, periodic=1, box_size=box)
-    return _assemble(rng, groups, box, True, params, f"C3 {nsub} log-uniform subhalos")
+    return _assemble_arrays(rng, *pop, box, True, params, f"C3 {nsub} log-uniform subhalos")
 
 
 def config4(seed: int = 1004, n_host: int = 4_000_000, n_sat: int = 200,
@@ -317,10 +368,10 @@ def config5(seed: int = 1005, nsub: int = 200_000, background_frac: float = 0.1)
     device.  Returns (snapshot columns, id lists, catalogue-of-lists)."""
     rng = _rng(seed)
     box = 1.0e5
-    groups = _power_law_population(rng, nsub, 20, 2.0e5, 0.2, "escape", 10.0, 0.3, G_DEFAULT, box,
-                                   lambda n: rng.random((n, 3)) * box)
+    pop = _power_law_population(rng, nsub, 20, 2.0e5, 0.2, "escape", 10.0, 0.3, G_DEFAULT, box,
+                                lambda n: rng.random((n, 3)) * box)
     params = make_params(softening=0.5, open_angle=0.45, periodic=1, box_size=box)
-    cat = _assemble(rng, groups, box, True, params, f"C5 {nsub} subhalos")
+    cat = _assemble_arrays(rng, *pop, box, True, params, f"C5 {nsub} subhalos")
     nlist = cat.n_particles
     nbg = int(background_frac * nlist)
     bg = {
