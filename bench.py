@@ -172,6 +172,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--replay", action="store_true", help="REPLAY precision instead of FAST")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default execution tuning")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device three times (a kernel of known "
+                         "traffic that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the unbinding of one rank's share runs on one MI355X; "
@@ -184,6 +188,16 @@ def main():
     from hbtplus_amd.abi import SUB_DTYPE
     from hbtplus_amd.device import GpuUnbinder
 
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.arange(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: 3 x {args.copy_probe} MiB read + written", file=sys.stderr, flush=True)
+        del a, b
     if args.config == "c5":
         return bench_c5(args)
     cat, desc = make_workload(args.config)
@@ -195,6 +209,14 @@ def main():
     print(f"[bench] {args.config}: {len(cat.subs)} subhalos, {cat.n_particles} particles", file=sys.stderr,
           flush=True)
 
+    tuning = None
+    if not args.no_autotune:
+        # execution tuning (walk group sizes per size bin, passes per graph launch),
+        # chosen by timing before the timed region; results are identical for every choice
+        t = U.autotune(h, reps=1)
+        tuning = {"walk_group": [int(v) for v in t["walk_group"]],
+                  "bin_edge": [int(v) for v in t["bin_edge"]], "graph_passes": int(t["graph_passes"])}
+        print(f"[bench] autotuned: {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         U.unbind_device(h)
     torch.cuda.synchronize()
@@ -222,6 +244,7 @@ def main():
     prof_c = U.profile_read()
     U.profile(0)
     I, O, T = prof_c["interactions"], prof_c["opened"], prof_c["targets"]
+    lane_eff = (I + O) / max(prof_c["lane_slots"], 1)
     flops = 11.0 * I + 10.0 * O + 20.0 * T
     walk_s = prof_t["walk_ms"] / 1e3
     launches = max(prof_t["walk_launches"], 1)
@@ -232,6 +255,7 @@ def main():
                     avg_launch_ms=round(prof_t["walk_ms"] / launches, 4),
                     flops_per_step=flops, interactions_per_eval=round(I / max(T, 1), 1),
                     opened_per_eval=round(O / max(T, 1), 1),
+                    lane_efficiency=round(lane_eff, 3),
                     walk_share_of_step=round(prof_t["walk_ms"] / ms_per_step, 3))
 
     result = {"metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
@@ -242,7 +266,8 @@ def main():
                          "passes_per_step": int(passes),
                          "precision": "REPLAY (fp64 walk)" if args.replay else
                          "FAST (fp32+Kahan walk, fp64 frames)",
-                         "device_ms_per_step": round(dev_ms / args.steps, 3)},
+                         "device_ms_per_step": round(dev_ms / args.steps, 3),
+                         "tuning": tuning},
               "roofline": roofline}
 
     if not args.no_plain:

@@ -44,6 +44,11 @@ SUB_DTYPE = np.dtype([
 assert PARAMS_DTYPE.itemsize == 64, PARAMS_DTYPE.itemsize
 assert SUB_DTYPE.itemsize == 240, SUB_DTYPE.itemsize
 
+# hbt_tuning (32 bytes): walk group sizes per size bin, bin edges, graph settings
+TUNING_DTYPE = np.dtype([("walk_group", "<i4", (3,)), ("bin_edge", "<i4", (2,)),
+                         ("graph_passes", "<i4"), ("use_graph", "<i4"), ("reserved", "<i4")])
+assert TUNING_DTYPE.itemsize == 32
+
 OUTPUT_FIELDS = ("nbound", "list_len", "nsource", "nbound_type", "mbound_type", "mbound",
                  "specific_self_potential_energy", "specific_self_kinetic_energy",
                  "specific_angular_momentum", "ref_pos", "ref_vel", "mostbound_pos",
@@ -120,7 +125,10 @@ def load_library() -> ctypes.CDLL:
         "hbt_id_to_index": ([vp, i64, vp, i64, vp, vp], i32),
         "hbt_last_batch_stats": ([vp, vp, vp, vp], i32),
         "hbt_profile": ([vp, i32], i32),
-        "hbt_profile_read": ([vp, vp, vp, vp, vp, vp], i32),
+        "hbt_profile_read": ([vp, vp, vp, vp, vp, vp, vp], i32),
+        "hbt_get_tuning": ([vp, vp], i32),
+        "hbt_set_tuning": ([vp, vp], i32),
+        "hbt_autotune": ([vp, vp, vp, vp, i64, vp, vp, i32, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

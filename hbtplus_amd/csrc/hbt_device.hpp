@@ -44,10 +44,11 @@ struct Seg {
   int n_src, src_lo, n_tgt;   // current pass
   int n_items;
   int nP, nG, K, hole;        // partition
-  int nsort;                  // elements of the current sort
+  int nsort, sort_lo;         // energy sort: Elist[sort_lo, sort_lo + nsort)
   int iter;
   long long evals;
-  int D, dlen, straddle, nchunk_base;
+  int D, dlen, straddle;
+  int wgroup;                 // lanes per traversal group in the walk
   float ref_pos[3], ref_vel[3], old_pos[3], old_vel[3];
   float rvd[3], dK;
   float mbound;

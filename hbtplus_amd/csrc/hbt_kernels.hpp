@@ -45,9 +45,9 @@ struct Work {
   u64 *k0, *k1;
   u32 *v0, *v1;
   signed char *dl;
-  int *cmin;                 // [nslots / WAVE]
   u32 *bits;
-  int *pre, *nlo, *nhi, *npos, *nlist;
+  int *pre, *nlo, *nhi, *npos;
+  int *npar, *nint, *ncnt;   // [2*nslots] per item: parent item, internal children, arrivals
   float4 *ipm;               // [2*nslots]
   int2 *iax;                 // [2*nslots]: (float bits of len^2, skip)
   u64 *scan64;
@@ -178,7 +178,7 @@ __global__ void pass_setup_kernel(Work W, DevParams P) {
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     S.flags &= ~F_FINAL;
-    S.n_src = 0; S.n_tgt = 0; S.src_lo = 0; S.nsort = 0; S.n_items = 0;
+    S.n_src = 0; S.n_tgt = 0; S.src_lo = 0; S.nsort = 0; S.sort_lo = 0; S.n_items = 0;
     S.K = 0; S.nP = 0; S.nG = 0;
     if (!(S.flags & F_ACTIVE)) continue;
     S.iter++;
@@ -317,57 +317,52 @@ __global__ void __launch_bounds__(BLOCK) delta_kernel(Work W, int full_copy_sp) 
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
-    int v = 127;
-    if (l >= 0) {
-      const Seg &S = W.seg[k];
-      if (l < S.n_src) {
-        v = (l + 1 < S.n_src) ? key_delta(W.k0[s], W.k0[s + 1], S.D) : -1;
-        W.dl[s] = (signed char)v;
-        W.bits[s] = 0u;
-        if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
-      }
-    }
-    // chunk minimum over the wave (slots of one segment share a wave: bases are
-    // WAVE aligned)
-    int m = v;
-    for (int d = WAVE / 2; d > 0; d >>= 1) m = min(m, __shfl_down(m, d, WAVE));
-    if ((threadIdx.x & (WAVE - 1)) == 0) W.cmin[s / WAVE] = m;
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (l >= S.n_src) continue;
+    W.dl[s] = (signed char)((l + 1 < S.n_src) ? key_delta(W.k0[s], W.k0[s + 1], S.D) : -1);
+    W.bits[s] = 0u;
+    if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
+    // item positions 2l and 2l+1 of the segment (items < 2*n_src)
+    const int64_t it = 2 * S.ibase + 2 * l;
+    W.nint[it] = 0; W.nint[it + 1] = 0;
+    W.ncnt[it] = 0; W.ncnt[it + 1] = 0;
   }
 }
 
-// first j in [lo_l, l) scanning left from l-1 with dl[j] <= v (strict=false)
-// or dl[j] < v (strict=true); returns lo_l-1 if none.  Uses chunk minima.
-__device__ __forceinline__ int scan_left(const Work &W, int64_t b, int l, int v, bool strict) {
-  int j = l - 1;
-  auto hit = [&](int d) { return strict ? d < v : d <= v; };
-  // within the current chunk
-  while (j >= 0 && ((b + j + 1) % WAVE) != 0) {
-    if (hit(W.dl[b + j])) return j;
-    j--;
+// First v octant digits of a key (digit 0 in bits 62..60).
+__device__ __forceinline__ u64 key_prefix(u64 k, int v) { return v == 0 ? 0ull : k >> (63 - 3 * v); }
+
+// First index in [lo, hi+1] whose v-digit prefix is >= p (keys ascending).
+__device__ __forceinline__ int prefix_lower(const u64 *key, int lo, int hi, int v, u64 p) {
+  while (lo <= hi) {
+    const int m = (lo + hi) >> 1;
+    if (key_prefix(key[m], v) < p) lo = m + 1;
+    else hi = m - 1;
   }
-  // whole chunks
-  while (j >= 0) {
-    const int m = W.cmin[(b + j) / WAVE];
-    if (hit(m)) break;
-    j -= WAVE;
-  }
-  while (j >= 0) {
-    if (hit(W.dl[b + j])) return j;
-    j--;
-  }
-  return -1;
+  return lo;
 }
 
-// first j > l with dl[j] < v (the last pair of a segment holds -1)
-__device__ __forceinline__ int scan_right(const Work &W, int64_t b, int l, int v, int n) {
-  int j = l + 1;
-  while (j < n && ((b + j) % WAVE) != 0) {
-    if (W.dl[b + j] < v) return j;
-    j++;
+// The node split by pair l: the sources sharing the first v = dl[l] digits
+// with source l, found by binary search over the sorted keys.  Returns false
+// when pair l is not the node's leftmost split (only the leftmost split emits
+// the node).
+__device__ __forceinline__ bool node_of_pair(const Work &W, int64_t b, int l, int n, int v, int D,
+                                             int &lo, int &hi) {
+  const u64 *key = W.k0 + b;
+  const int vl = l > 0 ? W.dl[b + l - 1] : -1;
+  if (vl == v) return false;               // source l-1 is in the node, in another child
+  if (vl < v) {
+    lo = l;                                // the node starts at source l
+  } else {
+    lo = prefix_lower(key, 0, l - 1, v, key_prefix(key[l], v));
+    // leftmost split iff sources lo and l are in the same child
+    if (key_delta(key[lo], key[l], D) <= v) return false;
   }
-  while (j < n && W.cmin[(b + j) / WAVE] >= v) j += WAVE;
-  while (j < n && W.dl[b + j] >= v) j++;
-  return j < n ? j : n - 1;
+  const int vr = l + 2 < n ? W.dl[b + l + 1] : -1;
+  if (vr < v) hi = l + 1;
+  else hi = prefix_lower(key, l + 2, n - 1, v, key_prefix(key[l], v) + 1) - 1;
+  return true;
 }
 
 // Emits the node of which pair l is the leftmost split.
@@ -381,13 +376,11 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Work W) {
     W.nlo[s] = -1;
     if (l + 1 >= S.n_src) continue;
     const int v = W.dl[s];
-    const int64_t b = S.ibase;
-    const int L = scan_left(W, b, l, v, false);
-    if (L >= 0 && W.dl[b + L] == v) continue;  // not the leftmost split of its node
-    const int R = scan_right(W, b, l, v, S.n_src);
-    W.nlo[s] = L + 1;
-    W.nhi[s] = R;
-    atomicOr(&W.bits[b + L + 1], 1u << v);
+    int lo, hi;
+    if (!node_of_pair(W, S.ibase, l, S.n_src, v, S.D, lo, hi)) continue;
+    W.nlo[s] = lo;
+    W.nhi[s] = hi;
+    atomicOr(&W.bits[S.ibase + lo], 1u << v);
   }
 }
 
@@ -424,38 +417,70 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
           W.ipm[ib + npos] = make_float4(0.f, 0.f, 0.f, 0.f);
           W.iax[ib + npos] = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
           W.npos[s] = (int)(ib + npos);
+          // parent: the enclosing node at the deeper of the two boundary splits
+          const int vl = lo > 0 ? W.dl[b + lo - 1] : -1;
+          const int vr = W.dl[b + hi];  // -1 for the last source of the segment
+          const int pv = vl > vr ? vl : vr;
+          int ppos = -1;
+          if (pv >= 0) {
+            const int plo = (vl == pv) ? prefix_lower(W.k0 + b, 0, lo - 1, pv, key_prefix(W.k0[b + lo], pv)) : lo;
+            ppos = (int)ib + plo + W.pre[b + plo] + __popc(W.bits[b + plo] & ((1u << pv) - 1u));
+            atomicAdd(&W.nint[ppos], 1);
+          }
+          W.npar[ib + npos] = ppos;
         }
       }
     }
   }
 }
 
-// Node moments for depth v: mass and centre of mass accumulated in double
-// over the float values of the children in son order, stored as float
+// Node moments: mass and centre of mass accumulated in double over the float
+// values of the children in son order, stored as float
 // (UpdateInternalNodes/ProcessNode/FillNodeCenter, src/gravity_tree.cpp:18-77).
-__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v) {
+__device__ __forceinline__ void node_moments(const Work &W, int g) {
 #pragma clang fp contract(off)
+  const int end = W.iax[g].y;
+  double mass = 0., cx = 0., cy = 0., cz = 0.;
+  int c = g + 1;
+  while (c < end) {
+    const float4 q = W.ipm[c];
+    const double m = q.w;
+    mass += m;
+    cx += (double)q.x * m;
+    cy += (double)q.y * m;
+    cz += (double)q.z * m;
+    const int nx = W.iax[c].y;
+    c = nx > c ? nx : end;  // skips always advance
+  }
+  W.ipm[g] = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
+}
+
+// All nodes in one launch, bottom up: a thread starts at each node whose
+// children are all leaves; after finishing a node it counts itself in at the
+// parent, and the thread that completes the parent's count goes on with the
+// parent (device-scope fences order the children's moments before the count,
+// and the count before the parent reads them).  No thread waits on another.
+__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
-    if (l + 1 >= S.n_src || W.dl[s] != v || W.nlo[s] < 0) continue;
-    const int g = W.npos[s];
-    const int end = W.iax[g].y;
-    double mass = 0., cx = 0., cy = 0., cz = 0.;
-    int c = g + 1;
-    while (c < end) {
-      const float4 q = W.ipm[c];
-      const double m = q.w;
-      mass += m;
-      cx += (double)q.x * m;
-      cy += (double)q.y * m;
-      cz += (double)q.z * m;
-      const int nx = W.iax[c].y;
-      c = nx > c ? nx : end;  // skips always advance
+    if (l + 1 >= S.n_src || W.nlo[s] < 0) continue;
+    int g = W.npos[s];
+    if (W.nint[g] != 0) continue;
+    // a parent precedes its child in pre-order, so g decreases on every
+    // step and the climb is at most one node per depth
+    for (int step = 0; step <= NLEVEL; step++) {
+      node_moments(W, g);
+      const int p = W.npar[g];
+      if (p < 0 || p >= g) break;
+      __threadfence();
+      const int arrived = atomicAdd(&W.ncnt[p], 1) + 1;
+      if (arrived < W.nint[p]) break;
+      __threadfence();
+      g = p;
     }
-    W.ipm[g] = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
   }
 }
 
@@ -531,98 +556,204 @@ __device__ __forceinline__ double walk_dispatch(const Work &W, const DevParams &
   return walk_fast<false, COUNT>(W, P, ib, S.n_items, tx, ty, tz, tmass, ni, no);
 }
 
-__device__ __forceinline__ void flush_counts(const Work &W, unsigned ni, unsigned no, unsigned nt) {
-  unsigned long long a = ni, b = no, c = nt;
-  a = wave_reduce_sum(a); b = wave_reduce_sum(b); c = wave_reduce_sum(c);
+__device__ __forceinline__ void flush_counts(const Work &W, unsigned ni, unsigned no, unsigned nt,
+                                             unsigned ns = 0) {
+  unsigned long long a = ni, b = no, c = nt, d = ns;
+  a = wave_reduce_sum(a); b = wave_reduce_sum(b); c = wave_reduce_sum(c); d = wave_reduce_sum(d);
   if ((threadIdx.x & (WAVE - 1)) == 0) {
     atomicAdd(&W.counters[0], a); atomicAdd(&W.counters[1], b); atomicAdd(&W.counters[2], c);
+    atomicAdd(&W.counters[3], d);
   }
 }
 
-// Wave-shared traversal.  A wave's 64 targets (consecutive in spatial order,
-// one segment) walk the pre-order items together: the wave stands on item j,
-// every lane whose own path reaches j decides for itself (open, or use the
-// monopole and skip the subtree), and the wave moves to j+1 if any lane opened
-// j, else to skip(j) -- lanes still skipping an accepted ancestor A have
+// Group-shared traversal.  The wave's 64 targets (consecutive in spatial
+// order, one segment) are split into groups of GROUP lanes; each group walks
+// the pre-order items together: the group stands on item j, every lane whose
+// own path reaches j decides for itself (open, or use the monopole and skip
+// the subtree), and the group moves to j+1 if any of its lanes opened j, else
+// to skip(j) -- lanes still skipping an accepted ancestor A have
 // skip(A) >= skip(j), so no lane's next item is passed over.  Each lane sees
-// exactly the items, decisions and summation order of its own walk; the item
-// index is wave-uniform, so items are read with scalar loads.
-template <bool STRADDLE, bool COUNT, bool KAHAN = true>
-__device__ float walk_shared_fast(const Work &W, const DevParams &P, int ib, int end, bool live,
-                                  float tx, float ty, float tz, float tmass, unsigned &ni,
-                                  unsigned &no) {
-#pragma clang fp contract(off)
-  const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
+// exactly the items, decisions and summation order of its own walk, whatever
+// GROUP is.  GROUP = 64: the item index is wave-uniform and items are read
+// with scalar loads; smaller groups visit fewer items that none of their lanes
+// needs, at the price of vector loads and of the wave running until its
+// slowest group is done.
+template <int GROUP>
+__device__ __forceinline__ u64 group_lanes() {
+  if (GROUP == WAVE) return ~0ull;
+  const int base = (int)(threadIdx.x & (WAVE - 1)) & ~(GROUP - 1);
+  return ((1ull << GROUP) - 1ull) << base;
+}
+
+template <int GROUP>
+__device__ __forceinline__ bool group_any(bool p, u64 lanes) {
+  if (GROUP == WAVE) return __any(p);
+  return (__ballot(p) & lanes) != 0ull;
+}
+
+// Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN).
+// Straddling segments (a periodic subhalo wider than a quarter box) form the
+// separation in double.
+template <bool STRADDLE, bool KAHAN>
+struct FastTerm {
+  float h2, hinv;
   Kahan<KAHAN> acc;
-  acc.init(live ? tmass / P.eps : 0.f);
-  int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
-  int j = ib;
-  while (j < end) {
-    j = __builtin_amdgcn_readfirstlane(j);
-    const float4 q = W.ipm[j];
-    const int2 a = W.iax[j];
-    bool opened = false;
-    if (until <= j) {
-      float dx, dy, dz;
-      if (STRADDLE) {
-        dx = (float)nearest_d((double)q.x - (double)tx, P);
-        dy = (float)nearest_d((double)q.y - (double)ty, P);
-        dz = (float)nearest_d((double)q.z - (double)tz, P);
-      } else {
-        dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
-      }
-      const float r2 = pair_r2(dx, dy, dz);
-      if (__int_as_float(a.x) > r2 * P.theta2) {
-        opened = true;
-        if (COUNT) no++;
-      } else {
-        until = a.y;
-        if (COUNT) ni++;
-        acc.add(kernel_term_f(q.w, r2, h2, hinv));
-      }
-    }
-    j = __any(opened) ? j + 1 : (a.y > j ? a.y : end);  // skips always advance
-  }
-  return acc.s;
-}
-
-template <bool COUNT>
-__device__ double walk_shared_replay(const Work &W, const DevParams &P, int ib, int end, bool live,
-                                     float tx, float ty, float tz, float tmass, unsigned &ni,
-                                     unsigned &no) {
+  __device__ __forceinline__ void init(const DevParams &P, bool live, float tmass) {
 #pragma clang fp contract(off)
-  const double h = 2.8 * (double)P.eps, hinv = 1.0 / h;
-  double pot = live ? (double)(float)(tmass / P.eps) : 0.0;
-  const double x = tx, y = ty, z = tz;
-  int until = live ? ib : INT_MAX;
-  int j = ib;
-  while (j < end) {
-    j = __builtin_amdgcn_readfirstlane(j);
-    const float4 q = W.ipm[j];
-    const int2 a = W.iax[j];
+    const float h = 2.8f * P.eps;
+    h2 = h * h;
+    hinv = 1.0f / h;
+    acc.init(live ? tmass / P.eps : 0.f);
+  }
+  // returns true when the item is opened
+  __device__ __forceinline__ bool visit(const DevParams &P, const float4 &q, float lsq, float tx,
+                                        float ty, float tz) {
+#pragma clang fp contract(off)
+    float dx, dy, dz;
+    if (STRADDLE) {
+      dx = (float)nearest_d((double)q.x - (double)tx, P);
+      dy = (float)nearest_d((double)q.y - (double)ty, P);
+      dz = (float)nearest_d((double)q.z - (double)tz, P);
+    } else {
+      dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
+    }
+    const float r2 = pair_r2(dx, dy, dz);
+    if (lsq > r2 * P.theta2) return true;
+    acc.add(kernel_term_f(q.w, r2, h2, hinv));
+    return false;
+  }
+  __device__ __forceinline__ double result() const { return (double)acc.s; }
+};
+
+// Reference arithmetic (src/gravity_tree.cpp:79-164): FP64 over float data.
+struct ReplayTerm {
+  double h, hinv, pot;
+  __device__ __forceinline__ void init(const DevParams &P, bool live, float tmass) {
+#pragma clang fp contract(off)
+    h = 2.8 * (double)P.eps;
+    hinv = 1.0 / h;
+    pot = live ? (double)(float)(tmass / P.eps) : 0.0;
+  }
+  __device__ __forceinline__ bool visit(const DevParams &P, const float4 &q, float lsq, float tx,
+                                        float ty, float tz) {
+#pragma clang fp contract(off)
+    double dx = (double)q.x - (double)tx, dy = (double)q.y - (double)ty, dz = (double)q.z - (double)tz;
+    if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if ((double)lsq > r2 * (double)P.theta2) return true;
+    pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
+    return false;
+  }
+  __device__ __forceinline__ double result() const { return pot; }
+};
+
+// The item arrays come in as restrict-qualified kernel arguments: nothing in
+// the walk kernel writes them, so reads at a group-uniform index compile to
+// scalar loads when GROUP = 64.  Both possible successors of item j (j+1 and
+// skip(j)) are fetched while j is evaluated, so one load latency is exposed per
+// step instead of one per dependent read.  Fetch indices past the end are
+// clamped to ib (a valid item whose value is then not used).
+template <int GROUP, bool COUNT, class Term>
+__device__ double walk_group(const float4 *__restrict__ ipm, const int2 *__restrict__ iax,
+                             const DevParams &P, int ib, int end, bool live, float tx, float ty,
+                             float tz, float tmass, unsigned &ni, unsigned &no, unsigned &ns) {
+  Term t;
+  t.init(P, live, tmass);
+  const u64 lanes = group_lanes<GROUP>();
+  const bool leader = (threadIdx.x & (GROUP - 1)) == 0;
+  int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
+  int j = group_any<GROUP>(live, lanes) ? ib : end;
+  if (GROUP == WAVE) j = __builtin_amdgcn_readfirstlane(j);
+  float4 q = ipm[j < end ? j : ib];
+  int2 a = iax[j < end ? j : ib];
+  while (true) {
+    if (GROUP == WAVE) {
+      if (j >= end) break;
+    } else if (!__any(j < end)) {
+      break;
+    }
+    const bool on = GROUP == WAVE || j < end;
+    const int j1 = j + 1;
+    const int js = a.y > j ? a.y : end;  // skips always advance
+    const int c1 = j1 < end ? j1 : ib, cs = js < end ? js : ib;
+    const float4 q1 = ipm[c1], qs = ipm[cs];
+    const int2 a1 = iax[c1], as = iax[cs];
     bool opened = false;
-    if (until <= j) {
-      double dx = (double)q.x - x, dy = (double)q.y - y, dz = (double)q.z - z;
-      if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if ((double)__int_as_float(a.x) > r2 * (double)P.theta2) {
-        opened = true;
-        if (COUNT) no++;
-      } else {
-        until = a.y;
-        if (COUNT) ni++;
-        pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
+    if (on) {
+      if (COUNT && leader) ns += GROUP;
+      if (until <= j) {
+        opened = t.visit(P, q, __int_as_float(a.x), tx, ty, tz);
+        if (opened) {
+          if (COUNT) no++;
+        } else {
+          until = a.y;
+          if (COUNT) ni++;
+        }
       }
     }
-    j = __any(opened) ? j + 1 : (a.y > j ? a.y : end);  // skips always advance
+    const bool go = group_any<GROUP>(opened, lanes);
+    if (on) {
+      j = go ? j1 : js;
+      q = go ? q1 : qs;
+      a = go ? a1 : as;
+    }
   }
-  return pot;
+  return t.result();
 }
 
-// Binding energies with the wave-shared walk (same results as walk_kernel).
+template <int GROUP, bool COUNT, bool REPLAY>
+__device__ __forceinline__ double walk_group_dispatch(const float4 *__restrict__ ipm,
+                                                      const int2 *__restrict__ iax,
+                                                      const DevParams &P, const Seg &S, int ib,
+                                                      int end, bool live, float x, float y, float z,
+                                                      float tm, unsigned &ni, unsigned &no,
+                                                      unsigned &ns) {
+  if (REPLAY)
+    return walk_group<GROUP, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+  if (!P.kahan) {
+    if (S.straddle)
+      return walk_group<GROUP, COUNT, FastTerm<true, false>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    return walk_group<GROUP, COUNT, FastTerm<false, false>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+  }
+  if (S.straddle)
+    return walk_group<GROUP, COUNT, FastTerm<true, true>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+  return walk_group<GROUP, COUNT, FastTerm<false, true>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+}
+
+// Lanes per group chosen per segment (S.wgroup, set from its size bin).  The
+// straddling and uncompensated forms (rare; precision study) keep 64.
 template <bool COUNT, bool REPLAY>
-__global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P) {
-  unsigned ni = 0, no = 0, nt = 0;
+__device__ __forceinline__ double walk_binned(const float4 *__restrict__ ipm,
+                                              const int2 *__restrict__ iax, const DevParams &P,
+                                              const Seg &S, int ib, int end, bool live, float x,
+                                              float y, float z, float tm, unsigned &ni,
+                                              unsigned &no, unsigned &ns) {
+  if (REPLAY) {
+    switch (S.wgroup) {
+      case 8: return walk_group<8, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      case 16: return walk_group<16, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      case 32: return walk_group<32, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      default: return walk_group<64, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    }
+  }
+  if (!P.kahan || S.straddle)
+    return walk_group_dispatch<64, COUNT, false>(ipm, iax, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
+  using T = FastTerm<false, true>;
+  switch (S.wgroup) {
+    case 8: return walk_group<8, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    case 16: return walk_group<16, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    case 32: return walk_group<32, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    default: return walk_group<64, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+  }
+}
+
+// Binding energies with the group-shared walk (same results as walk_kernel
+// for every group size).  GROUP = 0: lanes per group from each segment's bin.
+template <int GROUP, bool COUNT, bool REPLAY>
+__global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
+                                                            const float4 *__restrict__ ipm,
+                                                            const int2 *__restrict__ iax) {
+  unsigned ni = 0, no = 0, nt = 0, ns = 0;
   SLOT_LOOP(s, W) {
 #pragma clang fp contract(off)
     // segments are wave-aligned: the whole wave belongs to one segment
@@ -639,16 +770,11 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P)
     const float tm = corr ? 0.f : W.pm[b + li];
     const int ib = (int)(2 * b), end = ib + S.n_items;
     double pot;
-    if (REPLAY) {
-      pot = walk_shared_replay<COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
-    } else if (!P.kahan) {
-      pot = S.straddle ? (double)walk_shared_fast<true, COUNT, false>(W, P, ib, end, live, x, y, z, tm, ni, no)
-                       : (double)walk_shared_fast<false, COUNT, false>(W, P, ib, end, live, x, y, z, tm, ni, no);
-    } else if (S.straddle) {
-      pot = (double)walk_shared_fast<true, COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
-    } else {
-      pot = (double)walk_shared_fast<false, COUNT>(W, P, ib, end, live, x, y, z, tm, ni, no);
-    }
+    if (GROUP == 0)
+      pot = walk_binned<COUNT, REPLAY>(ipm, iax, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
+    else
+      pot = walk_group_dispatch<GROUP == 0 ? 64 : GROUP, COUNT, REPLAY>(ipm, iax, P, S, ib, end, live, x, y,
+                                                                       z, tm, ni, no, ns);
     pot = pot * (double)P.G / (double)P.a;
     if (!live) continue;
     const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
@@ -668,7 +794,7 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P)
       W.E[b + li] = (float)((double)W.E[b + li] + ((double)kin - pot));
     }
   }
-  if (COUNT) flush_counts(W, ni, no, nt);
+  if (COUNT) flush_counts(W, ni, no, nt, ns);
 }
 
 // Binding energies of the pass's targets, in spatial order.
@@ -810,10 +936,8 @@ __global__ void partition_done_kernel(Work W, DevParams P) {
       S.flags = (S.flags & ~F_ACTIVE) | F_DISR | F_FINAL;
       S.nbound = 1;
       S.nlast = 1;
-      S.nsort = 0;
-    } else {
-      S.nsort = S.nlast - S.nbound;  // removed block to sort
     }
+    S.nsort = 0;  // the energy sort follows the frames
   }
 }
 
@@ -852,8 +976,11 @@ __global__ void __launch_bounds__(BLOCK) spatial_copy_kernel(Work W) {
   }
 }
 
-// sort keys for a block of Elist by energy: (ordered E, position) -> local index
-__global__ void __launch_bounds__(BLOCK) energy_keys_kernel(Work W, int bound_block) {
+// Sort keys for Elist[sort_lo, sort_lo + nsort) by energy: (block, ordered E,
+// position) -> local index.  The block bit keeps the bound block Elist[0, nbound)
+// and the removed block Elist[nbound, nlast) apart, so one segmented sort
+// orders each block by energy in place; position breaks ties (a stable order).
+__global__ void __launch_bounds__(BLOCK) energy_keys_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
@@ -861,22 +988,22 @@ __global__ void __launch_bounds__(BLOCK) energy_keys_kernel(Work W, int bound_bl
     const Seg &S = W.seg[k];
     if (l >= S.nsort) continue;
     const int64_t b = S.ibase;
-    const int at = bound_block ? l : S.nbound + l;
+    const int at = S.sort_lo + l;
     const int li = W.el[b + at];
-    W.k0[s] = ((u64)f2ord(W.E[b + li]) << 32) | (u64)(unsigned)l;
+    const u64 block = at >= S.nbound ? 1ull : 0ull;
+    W.k0[s] = (block << 63) | ((u64)f2ord(W.E[b + li]) << 31) | (u64)(unsigned)at;
     W.v0[s] = (u32)li;
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) energy_writeback_kernel(Work W, int bound_block) {
+__global__ void __launch_bounds__(BLOCK) energy_writeback_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
     if (l >= S.nsort) continue;
-    const int at = bound_block ? l : S.nbound + l;
-    W.el[S.ibase + at] = (int)W.v0[s];
+    W.el[S.ibase + S.sort_lo + l] = (int)W.v0[s];
   }
 }
 
@@ -955,11 +1082,15 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
     }
   }
   S.mbound = (float)msum;
+  // energy sort of the removed block Elist[nb, nlast) (:359-360) and, on
+  // convergence, of the bound block Elist[0, nb) too (:401)
   if ((float)nb >= (float)S.nlast * P.bmp) {
     S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_FINAL;
-    S.nsort = nb;  // bound block to sort
+    S.sort_lo = 0;
+    S.nsort = S.nlast;
   } else {
-    S.nsort = 0;
+    S.sort_lo = nb;
+    S.nsort = S.nlast - nb;
     atomicAdd(W.active, 1);
   }
 }

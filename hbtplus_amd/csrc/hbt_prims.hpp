@@ -78,21 +78,26 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T *total) {
 
 // ---- segmented sort ----------------------------------------------------------
 
-// Sorts each tile's live elements [start, min(start+TILE, n[seg])) in place.
+// Sorts each tile's live elements [start, min(start+TILE, n[seg])) from
+// (kin, vin) into (kout, vout) (the two may be the same arrays).
 __global__ void __launch_bounds__(BLOCK)
 tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__restrict__ base,
-                 const int *__restrict__ n, u64 *__restrict__ keys, u32 *__restrict__ vals) {
+                 const int *__restrict__ n, const u64 *kin, const u32 *vin, u64 *kout, u32 *vout) {
   __shared__ u64 sk[TILE];
   __shared__ u32 sv[TILE];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const int cnt = min(TILE, n[tl.seg] - tl.start);
-    if (cnt <= 1) continue;  // uniform across the block
+    if (cnt <= 0) continue;  // uniform across the block
+    const int64_t off = base[tl.seg] + tl.start;
+    if (cnt == 1) {
+      if (threadIdx.x == 0) { kout[off] = kin[off]; vout[off] = vin[off]; }
+      continue;
+    }
     int P = 1;
     while (P < cnt) P <<= 1;
-    const int64_t off = base[tl.seg] + tl.start;
     for (int i = threadIdx.x; i < P; i += BLOCK) {
-      if (i < cnt) { sk[i] = keys[off + i]; sv[i] = vals[off + i]; }
+      if (i < cnt) { sk[i] = kin[off + i]; sv[i] = vin[off + i]; }
       else { sk[i] = ~0ull; sv[i] = 0xffffffffu; }
     }
     __syncthreads();
@@ -111,7 +116,7 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
         __syncthreads();
       }
     }
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) { keys[off + i] = sk[i]; vals[off + i] = sv[i]; }
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) { kout[off + i] = sk[i]; vout[off + i] = sv[i]; }
     __syncthreads();
   }
 }

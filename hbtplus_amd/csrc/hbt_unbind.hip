@@ -94,16 +94,17 @@ struct hbt_ctx {
   hbt_params params{};
   int cus = 256;
   // per-slot work arrays
-  DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, cmin, bits, pre, nlo, nhi,
-      npos, nlist, ipm, iax, scan64, pk, gk, isb, seg_of_slot;
+  DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
+      npos, npar, nint, ncnt, ipm, iax, scan64, pk, gk, isb, seg_of_slot;
   // per-level tables
   DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal, tpart;
   DBuf lvl, active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
-  int walk_mode = 0;  // 0: wave-shared traversal, 1: independent per-lane walks
-  int use_graph = 1;  // capture the pass sequence in a hipGraph and replay it per pass
-  int graph_passes = 2;  // passes per graph launch (extra passes on a converged level are no-ops)
+  int walk_mode = 0;  // 0: group-shared traversal, 1: independent per-lane walks
+  // lanes per traversal group by size bin, graph capture of the pass sequence
+  // (extra passes of a graph launch on a converged level are no-ops)
+  hbt_tuning tune = {{64, 16, 16}, {1024, 1 << 30}, 2, 1, 0};
   hipEvent_t evw0 = nullptr, evw1 = nullptr;
   double walk_ms = 0;
   int64_t walk_launches = 0;
@@ -116,7 +117,7 @@ struct hbt_ctx {
   int64_t last_evals = 0, last_passes = 0;
   ~hbt_ctx() {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
-                    &cmin, &bits, &pre, &nlo, &nhi, &npos, &nlist, &ipm, &iax, &scan64, &pk, &gk,
+                    &bits, &pre, &nlo, &nhi, &npos, &npar, &nint, &ncnt, &ipm, &iax, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &seg, &ibase, &nbuf, &tiles, &first_tile, &ntile, &tsum,
                     &stot, &child_ptr, &child_list, &capal, &tpart, &lvl, &active, &counters, &st_cols, &st_subs,
                     &st_out_i, &st_out_e, &st_phi})
@@ -143,9 +144,13 @@ struct Level {
   int maxcap = 0;
 };
 
+int walk_group_for(const hbt_tuning &t, int64_t len) {
+  return len < t.bin_edge[0] ? t.walk_group[0] : (len < t.bin_edge[1] ? t.walk_group[1] : t.walk_group[2]);
+}
+
 void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
                  const std::vector<std::vector<int>> &children, const std::vector<int> &members,
-                 Level &L) {
+                 const hbt_tuning &tune, Level &L) {
   const int n = (int)members.size();
   L.subs = members;
   L.seg.assign(n, Seg{});
@@ -168,6 +173,7 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
     S.sub = s;
     S.cap = c;
     S.count = (int)subs[s].count;
+    S.wgroup = walk_group_for(tune, c);
     L.ibase[k] = off;
     L.capal[k] = ca;
     L.first_tile[k] = (int)L.tiles.size();
@@ -255,13 +261,6 @@ __global__ void partition_counts_kernel(Work W, const u64 *stot) {
   }
 }
 
-__global__ void copy_back_kernel(int64_t n, const u64 *k1, const u32 *v1, u64 *k0, u32 *v0) {
-  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < n; s += (int64_t)gridDim.x * BLOCK) {
-    k0[s] = k1[s];
-    v0[s] = v1[s];
-  }
-}
-
 struct Runner {
   hbt_ctx *c;
   Work W;
@@ -270,20 +269,20 @@ struct Runner {
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
 
   int sort_pairs() {  // sorts (k0,v0) by nbuf, result in (k0,v0)
+    // merge levels for the largest segment of the level; the tile sort writes
+    // to the buffer from which the ping-pong of the merges ends in (k0,v0)
+    int merges = 0;
+    for (int w = TILE; w < L->maxcap; w *= 2) merges++;
+    u64 *ki = (merges & 1) ? W.k1 : W.k0, *ko = (merges & 1) ? W.k0 : W.k1;
+    u32 *vi = (merges & 1) ? W.v1 : W.v0, *vo = (merges & 1) ? W.v0 : W.v1;
     tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), (int)L->tiles.size(),
-                                                     W.ibase, W.nbuf, W.k0, W.v0);
-    u64 *ki = W.k0, *ko = W.k1;
-    u32 *vi = W.v0, *vo = W.v1;
-    int passes = 0;
+                                                     W.ibase, W.nbuf, W.k0, W.v0, ki, vi);
     for (int w = TILE; w < L->maxcap; w *= 2) {
       merge_pass_kernel<<<gs, BLOCK, 0, c->stream>>>(W.nslots, W.seg_of_slot, W.ibase, W.nbuf, w, ki,
                                                      vi, ko, vo);
       std::swap(ki, ko);
       std::swap(vi, vo);
-      passes++;
     }
-    if (passes & 1)
-      copy_back_kernel<<<gs, BLOCK, 0, c->stream>>>(W.nslots, W.k1, W.v1, W.k0, W.v0);
     return HBT_OK;
   }
 
@@ -310,20 +309,32 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     (u64 *)nullptr));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    for (int v = MAX_DEPTH; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
+    moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     return HBT_OK;
   }
 
   // walk variants: wave-shared traversal (default) or one independent walk per lane
+  template <int GROUP>
+  void launch_shared_walk(bool count) {
+    if (P.replay) {
+      if (count) walk_shared_kernel<GROUP, true, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
+      else walk_shared_kernel<GROUP, false, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
+    } else {
+      if (count) walk_shared_kernel<GROUP, true, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
+      else walk_shared_kernel<GROUP, false, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
+    }
+  }
+
   void launch_walk(bool count) {
-    const bool shared = c->walk_mode == 0;
-    if (shared) {
-      if (P.replay) {
-        if (count) walk_shared_kernel<true, true><<<gs, BLOCK, 0, c->stream>>>(W, P);
-        else walk_shared_kernel<false, true><<<gs, BLOCK, 0, c->stream>>>(W, P);
-      } else {
-        if (count) walk_shared_kernel<true, false><<<gs, BLOCK, 0, c->stream>>>(W, P);
-        else walk_shared_kernel<false, false><<<gs, BLOCK, 0, c->stream>>>(W, P);
+    if (c->walk_mode == 0) {
+      const int *g = c->tune.walk_group;
+      const int uniform = (g[0] == g[1] && g[1] == g[2]) ? g[0] : 0;
+      switch (uniform) {
+        case 0: launch_shared_walk<0>(count); break;  // per-segment bins
+        case 8: launch_shared_walk<8>(count); break;
+        case 16: launch_shared_walk<16>(count); break;
+        case 32: launch_shared_walk<32>(count); break;
+        default: launch_shared_walk<64>(count); break;
       }
     } else {
       if (count) walk_kernel<true><<<gs, BLOCK, 0, c->stream>>>(W, P);
@@ -356,12 +367,7 @@ struct Runner {
                     PreAdd{W.ibase, W.pre}, (u64 *)nullptr));
     spatial_compact_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     spatial_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    // sort the removed block by energy
-    energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
-    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
-    RETCHK(sort_pairs());
-    energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
-    // frames, convergence
+    // frames, convergence (the frame sums read the bound block only, in Elist order)
     if (P.replay) {
       frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     } else {
@@ -369,11 +375,11 @@ struct Runner {
                                                             c->tpart.as<double>());
       frames_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, c->first_tile.as<int>(), c->tpart.as<double>());
     }
-    // converged: sort the bound block by energy
-    energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
+    // energy sort of the removed block, and of the bound block on convergence
+    energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
     RETCHK(sort_pairs());
-    energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 1);
+    energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     output_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
     if (P.replay) {
       output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
@@ -403,13 +409,14 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->v0.ensure(s * 4));
   RETCHK(c->v1.ensure(s * 4));
   RETCHK(c->dl.ensure(s));
-  RETCHK(c->cmin.ensure((s / WAVE + 1) * 4));
   RETCHK(c->bits.ensure(s * 4));
   RETCHK(c->pre.ensure(s * 4));
   RETCHK(c->nlo.ensure(s * 4));
   RETCHK(c->nhi.ensure(s * 4));
   RETCHK(c->npos.ensure(s * 4));
-  RETCHK(c->nlist.ensure(s * 4));
+  RETCHK(c->npar.ensure(s * 2 * 4));
+  RETCHK(c->nint.ensure(s * 2 * 4));
+  RETCHK(c->ncnt.ensure(s * 2 * 4));
   RETCHK(c->ipm.ensure(s * 2 * 16));
   RETCHK(c->iax.ensure(s * 2 * 8));
   RETCHK(c->scan64.ensure(s * 8));
@@ -452,10 +459,10 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.el = c->el.as<int>(); W.el2 = c->el2.as<int>(); W.sp = c->sp.as<int>();
   W.k0 = c->k0.as<u64>(); W.k1 = c->k1.as<u64>(); W.v0 = c->v0.as<u32>(); W.v1 = c->v1.as<u32>();
   W.dl = c->dl.as<signed char>();
-  W.cmin = c->cmin.as<int>();
   W.bits = c->bits.as<u32>();
   W.pre = c->pre.as<int>(); W.nlo = c->nlo.as<int>(); W.nhi = c->nhi.as<int>();
-  W.npos = c->npos.as<int>(); W.nlist = c->nlist.as<int>();
+  W.npos = c->npos.as<int>();
+  W.npar = c->npar.as<int>(); W.nint = c->nint.as<int>(); W.ncnt = c->ncnt.as<int>();
   W.ipm = c->ipm.as<float4>(); W.iax = c->iax.as<int2>();
   W.scan64 = c->scan64.as<u64>();
   W.pk = c->pk.as<int>(); W.gk = c->gk.as<int>();
@@ -511,7 +518,8 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   // read-back of the active count) and replayed until no segment is active.
   // the plain form (one subhalo per level) launches directly, as a per-subhalo
   // Unbind loop does; graphs serve the batched form
-  const bool graph = c->use_graph && c->prof_mode == 0 && !debug && L.seg.size() > 1;
+  const bool graph = c->tune.use_graph && c->prof_mode == 0 && !debug && L.seg.size() > 1;
+  const int graph_passes = c->tune.graph_passes;
   if (graph) {
     hipGraph_t g = nullptr;
     hipGraphExec_t gx = nullptr;
@@ -521,7 +529,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     };
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = HBT_OK;
-    for (int p = 0; p < c->graph_passes && rc == HBT_OK; p++) rc = R.pass();
+    for (int p = 0; p < graph_passes && rc == HBT_OK; p++) rc = R.pass();
     hipError_t e1 = hipMemcpyAsync(c->h_active, R.W.active, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (rc != HBT_OK || e1 != hipSuccess || e2 != hipSuccess) {
@@ -534,7 +542,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
       return fail(HBT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
     }
     // every pass that does not converge removes at least one particle
-    const int max_launch = (L.maxcap + 2 + c->graph_passes - 1) / c->graph_passes;
+    const int max_launch = (L.maxcap + 2 + graph_passes - 1) / graph_passes;
     for (int it = 0; it < max_launch; it++) {
       hipError_t e = hipGraphLaunch(gx, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -542,7 +550,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
         release();
         return fail(HBT_ERR_DEVICE, std::string("pass graph: ") + hipGetErrorString(e));
       }
-      *passes += c->graph_passes;
+      *passes += graph_passes;
       if (*c->h_active == 0) break;
     }
     release();
@@ -653,8 +661,12 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   c->device = device;
   c->params = *params;
   if (const char *wm = getenv("HBT_WALK_MODE")) c->walk_mode = atoi(wm) ? 1 : 0;
-  if (const char *g = getenv("HBT_GRAPH")) c->use_graph = atoi(g) ? 1 : 0;
-  if (const char *g = getenv("HBT_GRAPH_PASSES")) c->graph_passes = std::max(1, atoi(g));
+  if (const char *wg = getenv("HBT_WALK_GROUP")) {
+    const int g = atoi(wg);
+    for (int b = 0; b < 3; b++) c->tune.walk_group[b] = (g == 8 || g == 16 || g == 32) ? g : 64;
+  }
+  if (const char *g = getenv("HBT_GRAPH")) c->tune.use_graph = atoi(g) ? 1 : 0;
+  if (const char *g = getenv("HBT_GRAPH_PASSES")) c->tune.graph_passes = std::max(1, atoi(g));
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 4, hipHostMallocDefault);
@@ -724,6 +736,94 @@ int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *
                            plain);
 }
 
+static bool tuning_valid(const hbt_tuning &t) {
+  for (int b = 0; b < 3; b++) {
+    const int g = t.walk_group[b];
+    if (g != 8 && g != 16 && g != 32 && g != 64) return false;
+  }
+  return t.bin_edge[0] >= 0 && t.bin_edge[1] >= t.bin_edge[0] && t.graph_passes >= 1 &&
+         t.graph_passes <= 16 && (t.use_graph == 0 || t.use_graph == 1);
+}
+
+int hbt_get_tuning(hbt_ctx *ctx, hbt_tuning *out) {
+  if (!ctx || !out) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  *out = ctx->tune;
+  return HBT_OK;
+}
+
+int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *tuning) {
+  if (!ctx || !tuning) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (!tuning_valid(*tuning))
+    return fail(HBT_ERR_INVALID_ARG, "tuning: walk groups must be 8/16/32/64, bin edges ascending, "
+                                     "graph_passes in 1..16, use_graph 0 or 1");
+  ctx->tune = *tuning;
+  ctx->tune.reserved = 0;
+  return HBT_OK;
+}
+
+int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs,
+                 const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index, float *out_energy,
+                 int reps, hbt_tuning *chosen) {
+  if (!ctx || !particles || !chosen || reps < 1) return fail(HBT_ERR_INVALID_ARG, "bad argument");
+  const hbt_tuning start = ctx->tune;
+  hbt_tuning best = start;
+  double best_ms = 0;
+  // device time of the batch under t, best of reps; t replaces best when it is
+  // faster by more than 1% (earlier, simpler settings win ties)
+  auto trial = [&](const hbt_tuning &t, bool first) -> int {
+    ctx->tune = t;
+    double ms = 1e300;
+    for (int r = 0; r < reps; r++) {
+      const int rc = hbt_unbind_batch(ctx, particles, subs, subs_host, nsub, out_index, out_energy, 0);
+      if (rc != HBT_OK) return rc;
+      ms = std::min(ms, ctx->last_ms);
+    }
+    if (first || ms < 0.99 * best_ms) {
+      best = t;
+      best_ms = ms;
+    }
+    return HBT_OK;
+  };
+  int rc = HBT_OK;
+  // 1. one group size for every bin
+  const int groups[4] = {64, 32, 16, 8};
+  for (int i = 0; i < 4 && rc == HBT_OK; i++) {
+    hbt_tuning t = best;
+    for (int b = 0; b < 3; b++) t.walk_group[b] = groups[i];
+    t.bin_edge[0] = 0;
+    t.bin_edge[1] = 1 << 30;
+    rc = trial(t, i == 0);
+  }
+  // 2. a separate group size for small subhalos
+  const int edges[2] = {256, 2048};
+  for (int e = 0; e < 2 && rc == HBT_OK; e++) {
+    for (int i = 0; i < 2 && rc == HBT_OK; i++) {
+      const int g0 = groups[i];
+      if (g0 == best.walk_group[2]) continue;
+      hbt_tuning t = best;
+      t.walk_group[0] = g0;
+      t.walk_group[1] = t.walk_group[2];
+      t.bin_edge[0] = edges[e];
+      t.bin_edge[1] = 1 << 30;
+      rc = trial(t, false);
+    }
+  }
+  // 3. passes per graph launch
+  for (int gp = 1; gp <= 4 && rc == HBT_OK; gp++) {
+    if (gp == best.graph_passes) continue;
+    hbt_tuning t = best;
+    t.graph_passes = gp;
+    rc = trial(t, false);
+  }
+  if (rc != HBT_OK) {
+    ctx->tune = start;
+    return rc;
+  }
+  ctx->tune = best;
+  *chosen = best;
+  return HBT_OK;
+}
+
 int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
                              int64_t n_list, hbt_subhalo *subs, const hbt_subhalo *subs_host,
                              int64_t nsub, int64_t *out_index, float *out_energy, int plain) {
@@ -771,7 +871,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   if (plain) {
     for (int s : order) {
       Level L;
-      build_level(subs_host, cap, children, std::vector<int>{s}, L);
+      build_level(subs_host, cap, children, std::vector<int>{s}, ctx->tune, L);
       RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
     }
   } else {
@@ -783,7 +883,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     for (int h = 0; h <= hmax; h++) {
       std::stable_sort(by_h[h].begin(), by_h[h].end(), [&](int a, int b) { return cap[a] > cap[b]; });
       Level L;
-      build_level(subs_host, cap, children, by_h[h], L);
+      build_level(subs_host, cap, children, by_h[h], ctx->tune, L);
       RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
     }
   }
@@ -823,7 +923,7 @@ int hbt_profile(hbt_ctx *ctx, int mode) {
 }
 
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int64_t *interactions,
-                     int64_t *opened, int64_t *targets) {
+                     int64_t *opened, int64_t *targets, int64_t *lane_slots) {
   if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(ctx->device));
   unsigned long long h[4] = {0, 0, 0, 0};
@@ -833,6 +933,7 @@ int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int6
   if (interactions) *interactions = (int64_t)h[0];
   if (opened) *opened = (int64_t)h[1];
   if (targets) *targets = (int64_t)h[2];
+  if (lane_slots) *lane_slots = (int64_t)h[3];
   return HBT_OK;
 }
 

@@ -13,6 +13,7 @@ import ctypes
 
 import numpy as np
 
+from . import abi
 from .abi import (PARAMS_DTYPE, PARTICLE_COLUMNS, SUB_DTYPE, check, load_library,
                   particles_struct, plan_batch)
 
@@ -92,6 +93,31 @@ class GpuUnbinder:
         total = h["total"]
         return subs, out_i.cpu().numpy()[:total].copy(), out_e.cpu().numpy()[:total].copy()
 
+    # execution tuning (results do not depend on it) -----------------------
+    def get_tuning(self):
+        t = np.zeros(1, abi.TUNING_DTYPE)
+        check(self.lib.hbt_get_tuning(self.ctx, t.ctypes.data), "hbt_get_tuning")
+        return t[0]
+
+    def set_tuning(self, t):
+        t = np.asarray(t, abi.TUNING_DTYPE).reshape(1)
+        check(self.lib.hbt_set_tuning(self.ctx, t.ctypes.data), "hbt_set_tuning")
+
+    def autotune(self, h, reps=2):
+        """Time the batch on device-resident inputs under candidate tunings;
+        the fastest is set on the context and returned."""
+        t = self.torch
+        subs_dev = h["subs0"].clone()
+        out_i = t.empty(max(h["total"], 1), dtype=t.int64, device=f"cuda:{self.device}")
+        out_e = t.empty(max(h["total"], 1), dtype=t.float32, device=f"cuda:{self.device}")
+        t.cuda.synchronize(self.device)
+        chosen = np.zeros(1, abi.TUNING_DTYPE)
+        rc = self.lib.hbt_autotune(self.ctx, ctypes.byref(h["ps"]), subs_dev.data_ptr(),
+                                   h["subs_host"].ctypes.data, len(h["subs_host"]), out_i.data_ptr(),
+                                   out_e.data_ptr(), int(reps), chosen.ctypes.data)
+        check(rc, "hbt_autotune")
+        return chosen[0]
+
     # snapshot + ID lists (lists resolved on the device) ---------------------
     def upload_snapshot(self, snap_cols, list_ids, subs):
         """Snapshot columns, the concatenated source-ID lists and the records
@@ -139,11 +165,11 @@ class GpuUnbinder:
 
     def profile_read(self):
         ms = ctypes.c_double()
-        vals = [ctypes.c_int64() for _ in range(4)]
+        vals = [ctypes.c_int64() for _ in range(5)]
         check(self.lib.hbt_profile_read(self.ctx, ctypes.byref(ms), *[ctypes.byref(v) for v in vals]),
               "hbt_profile_read")
         return dict(walk_ms=ms.value, walk_launches=vals[0].value, interactions=vals[1].value,
-                    opened=vals[2].value, targets=vals[3].value)
+                    opened=vals[2].value, targets=vals[3].value, lane_slots=vals[4].value)
 
     # per-subhalo form with host buffers --------------------------------------
     def unbind_one(self, cols: dict, sub: np.ndarray):

@@ -226,14 +226,48 @@ int hbt_id_to_index(hbt_ctx *ctx, int64_t n_table, const int64_t *table_ids,
 int hbt_last_batch_stats(hbt_ctx *ctx, double *device_ms, int64_t *n_evals,
                          int64_t *n_passes);
 
+/* Execution tuning (no counterpart in the reference).  Results are identical
+ * for every setting; only the speed changes.
+ *   walk_group[b]: lanes that walk the tree together (8, 16, 32 or 64) for
+ *                  subhalos of size bin b: list length < bin_edge[0] (b = 0),
+ *                  < bin_edge[1] (b = 1), otherwise b = 2
+ *   graph_passes : unbinding passes per HIP-graph launch (>= 1)
+ *   use_graph    : 1 = replay each level's pass sequence as a captured HIP graph
+ *                  (batches of more than one subhalo), 0 = direct launches */
+typedef struct hbt_tuning {
+  int32_t walk_group[3];
+  int32_t bin_edge[2];
+  int32_t graph_passes;
+  int32_t use_graph;
+  int32_t reserved;
+} hbt_tuning;
+
+int hbt_get_tuning(hbt_ctx *ctx, hbt_tuning *out);
+/* Validates and sets; HBT_ERR_INVALID_ARG for an unsupported value. */
+int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *tuning);
+
+/* Autotuner: times hbt_unbind_batch (same arguments) under candidate tunings
+ * -- one group size for all bins, then a separate group for the small-subhalo
+ * bin at two bin edges, then the passes per graph launch -- keeping at each
+ * step the setting with the smallest device time over `reps` runs, sets the
+ * winner on the context and returns it in *chosen.  The outputs of the last
+ * run are left in subs/out_index/out_energy (identical for every tuning). */
+int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs,
+                 const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
+                 float *out_energy, int reps, hbt_tuning *chosen);
+
 /* Instrumentation of the potential-walk kernel (no counterpart in the
  * reference; used by the benchmark for the roofline).  mode 0: off;
  * 1: time every walk launch with HIP events on the context stream;
  * 2: count, in every walk launch, the summed sources (leaves and accepted
- * nodes), the opened nodes and the targets.  Resets the accumulators. */
+ * nodes), the opened nodes, the targets and the lane slots spent by the lane
+ * groups (items a group visited times its lanes, whether or not a lane
+ * needed the item).
+ * Resets the accumulators. */
 int hbt_profile(hbt_ctx *ctx, int mode);
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches,
-                     int64_t *interactions, int64_t *opened, int64_t *targets);
+                     int64_t *interactions, int64_t *opened, int64_t *targets,
+                     int64_t *lane_slots);
 
 #ifdef __cplusplus
 }

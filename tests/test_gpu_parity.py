@@ -190,13 +190,17 @@ def _unbinder_with_env(params, **env):
 
 @pytest.mark.parametrize("prec", [abi.HBT_PREC_FAST, abi.HBT_PREC_REPLAY])
 def test_walk_modes_and_graph_are_identical(prec):
-    """wave-shared vs per-lane walk, hipGraph replay vs direct launches,
-    1 vs 3 passes per graph launch: byte-identical outputs"""
+    """group-shared walk with 64/32/16/8-lane groups vs per-lane walk, hipGraph
+    replay vs direct launches, 1 vs 3 passes per graph launch: byte-identical
+    outputs"""
     cat = synth.config2(seed=61, nsub=200, hi=20000)
     p = _with(cat.params, precision=prec)
     runs = []
     for env in (dict(HBT_WALK_MODE=0, HBT_GRAPH=1, HBT_GRAPH_PASSES=2),
                 dict(HBT_WALK_MODE=1, HBT_GRAPH=1, HBT_GRAPH_PASSES=2),
+                dict(HBT_WALK_MODE=0, HBT_WALK_GROUP=32),
+                dict(HBT_WALK_MODE=0, HBT_WALK_GROUP=16),
+                dict(HBT_WALK_MODE=0, HBT_WALK_GROUP=8),
                 dict(HBT_WALK_MODE=0, HBT_GRAPH=0),
                 dict(HBT_WALK_MODE=0, HBT_GRAPH=1, HBT_GRAPH_PASSES=3)):
         U = _unbinder_with_env(p, **env)
@@ -235,3 +239,29 @@ def test_indexed_batch_rejects_unresolved(gpu):
     idx[3] = -1
     with pytest.raises(RuntimeError, match="not snapshot positions"):
         U.unbind_indexed(h, idx)
+
+
+def test_tuning_api_and_bins_are_identical(gpu):
+    """per-bin group sizes set through hbt_set_tuning, and the autotuner's
+    choice: byte-identical outputs; invalid tunings are rejected"""
+    cat = synth.config2(seed=62, nsub=150, hi=20000)
+    for prec in (abi.HBT_PREC_FAST, abi.HBT_PREC_REPLAY):
+        U = gpu(_with(cat.params, precision=prec))
+        base = U.unbind_batch(cat)
+        t = U.get_tuning()
+        t["walk_group"] = (64, 32, 8)
+        t["bin_edge"] = (300, 3000)
+        t["graph_passes"] = 3
+        U.set_tuning(t)
+        got = U.unbind_batch(cat)
+        for x, y in zip(base, got):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+        chosen = U.autotune(U.upload(cat), reps=1)
+        assert int(chosen["walk_group"][2]) in (8, 16, 32, 64)
+        got = U.unbind_batch(cat)
+        for x, y in zip(base, got):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    bad = U.get_tuning()
+    bad["walk_group"] = (64, 12, 64)
+    with pytest.raises(RuntimeError):
+        U.set_tuning(bad)
