@@ -44,9 +44,9 @@ SUB_DTYPE = np.dtype([
 assert PARAMS_DTYPE.itemsize == 64, PARAMS_DTYPE.itemsize
 assert SUB_DTYPE.itemsize == 240, SUB_DTYPE.itemsize
 
-# hbt_tuning (32 bytes): walk group sizes per size bin, bin edges, graph settings
+# hbt_tuning (32 bytes): walk group sizes per size bin, bin edges, graph settings, XCD run length
 TUNING_DTYPE = np.dtype([("walk_group", "<i4", (3,)), ("bin_edge", "<i4", (2,)),
-                         ("graph_passes", "<i4"), ("use_graph", "<i4"), ("reserved", "<i4")])
+                         ("graph_passes", "<i4"), ("use_graph", "<i4"), ("xcd_run", "<i4")])
 assert TUNING_DTYPE.itemsize == 32
 
 OUTPUT_FIELDS = ("nbound", "list_len", "nsource", "nbound_type", "mbound_type", "mbound",

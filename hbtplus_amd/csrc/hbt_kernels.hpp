@@ -23,6 +23,14 @@
 
 namespace hbt {
 
+// One tree item (leaf or node) in DFS pre-order, one 32-byte record: the
+// walk reads both halves of an item at once.
+struct __align__(32) Item {
+  float4 pm;   // x, y, z, mass (a node: centre of mass and total mass)
+  int2 ax;     // float bits of len^2 (0 for a leaf), skip (first item after the subtree)
+  int2 pad;
+};
+
 struct Work {
   // per level
   Seg *seg;
@@ -47,9 +55,7 @@ struct Work {
   signed char *dl;
   u32 *bits;
   int *pre, *nlo, *nhi, *npos;
-  int *npar, *nint, *ncnt;   // [2*nslots] per item: parent item, internal children, arrivals
-  float4 *ipm;               // [2*nslots]
-  int2 *iax;                 // [2*nslots]: (float bits of len^2, skip)
+  Item *items;               // [2*nslots] tree items in pre-order
   u64 *scan64;
   int *pk, *gk;
   unsigned char *isb;
@@ -323,10 +329,6 @@ __global__ void __launch_bounds__(BLOCK) delta_kernel(Work W, int full_copy_sp) 
     W.dl[s] = (signed char)((l + 1 < S.n_src) ? key_delta(W.k0[s], W.k0[s + 1], S.D) : -1);
     W.bits[s] = 0u;
     if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
-    // item positions 2l and 2l+1 of the segment (items < 2*n_src)
-    const int64_t it = 2 * S.ibase + 2 * l;
-    W.nint[it] = 0; W.nint[it + 1] = 0;
-    W.ncnt[it] = 0; W.ncnt[it + 1] = 0;
   }
 }
 
@@ -403,8 +405,8 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
         const u32 bl = W.bits[s];
         const int pos = l + W.pre[s] + __popc(bl);
         const int li = (int)W.v0[s];
-        W.ipm[ib + pos] = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
-        W.iax[ib + pos] = make_int2(0, (int)(ib + pos + 1));
+        W.items[ib + pos].pm = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
+        W.items[ib + pos].ax = make_int2(0, (int)(ib + pos + 1));
         if (l + 1 == S.n_src) W.seg[k].n_items = pos + 1;
         const int lo = W.nlo[s];
         if (lo >= 0) {
@@ -414,20 +416,9 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
           const int hpos = hi + W.pre[b + hi] + __popc(W.bits[b + hi]);
           const float len = node_len(S, v);
           const float lsq = len * len;
-          W.ipm[ib + npos] = make_float4(0.f, 0.f, 0.f, 0.f);
-          W.iax[ib + npos] = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
+          W.items[ib + npos].pm = make_float4(0.f, 0.f, 0.f, 0.f);
+          W.items[ib + npos].ax = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
           W.npos[s] = (int)(ib + npos);
-          // parent: the enclosing node at the deeper of the two boundary splits
-          const int vl = lo > 0 ? W.dl[b + lo - 1] : -1;
-          const int vr = W.dl[b + hi];  // -1 for the last source of the segment
-          const int pv = vl > vr ? vl : vr;
-          int ppos = -1;
-          if (pv >= 0) {
-            const int plo = (vl == pv) ? prefix_lower(W.k0 + b, 0, lo - 1, pv, key_prefix(W.k0[b + lo], pv)) : lo;
-            ppos = (int)ib + plo + W.pre[b + plo] + __popc(W.bits[b + plo] & ((1u << pv) - 1u));
-            atomicAdd(&W.nint[ppos], 1);
-          }
-          W.npar[ib + npos] = ppos;
         }
       }
     }
@@ -439,48 +430,33 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
 // (UpdateInternalNodes/ProcessNode/FillNodeCenter, src/gravity_tree.cpp:18-77).
 __device__ __forceinline__ void node_moments(const Work &W, int g) {
 #pragma clang fp contract(off)
-  const int end = W.iax[g].y;
+  const int end = W.items[g].ax.y;
   double mass = 0., cx = 0., cy = 0., cz = 0.;
   int c = g + 1;
   while (c < end) {
-    const float4 q = W.ipm[c];
+    const float4 q = W.items[c].pm;
     const double m = q.w;
     mass += m;
     cx += (double)q.x * m;
     cy += (double)q.y * m;
     cz += (double)q.z * m;
-    const int nx = W.iax[c].y;
+    const int nx = W.items[c].ax.y;
     c = nx > c ? nx : end;  // skips always advance
   }
-  W.ipm[g] = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
+  W.items[g].pm = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
 }
 
-// All nodes in one launch, bottom up: a thread starts at each node whose
-// children are all leaves; after finishing a node it counts itself in at the
-// parent, and the thread that completes the parent's count goes on with the
-// parent (device-scope fences order the children's moments before the count,
-// and the count before the parent reads them).  No thread waits on another.
-__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W) {
+// Moments of the nodes at depth v (one launch per depth, deepest first: a
+// kernel boundary orders the children's moments before their parents read
+// them).  Each node is handled by the slot of its leftmost split pair.
+__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
-    if (l + 1 >= S.n_src || W.nlo[s] < 0) continue;
-    int g = W.npos[s];
-    if (W.nint[g] != 0) continue;
-    // a parent precedes its child in pre-order, so g decreases on every
-    // step and the climb is at most one node per depth
-    for (int step = 0; step <= NLEVEL; step++) {
-      node_moments(W, g);
-      const int p = W.npar[g];
-      if (p < 0 || p >= g) break;
-      __threadfence();
-      const int arrived = atomicAdd(&W.ncnt[p], 1) + 1;
-      if (arrived < W.nint[p]) break;
-      __threadfence();
-      g = p;
-    }
+    if (l + 1 >= S.n_src || W.dl[s] != v || W.nlo[s] < 0) continue;
+    node_moments(W, W.npos[s]);
   }
 }
 
@@ -501,8 +477,8 @@ __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int
   const int end = (int)(ib + n_items);
   int j = (int)ib;
   while (j < end) {
-    const float4 q = W.ipm[j];
-    const int2 a = W.iax[j];
+    const float4 q = W.items[j].pm;
+    const int2 a = W.items[j].ax;
     double dx = (double)q.x - x, dy = (double)q.y - y, dz = (double)q.z - z;
     if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
     const double r2 = dx * dx + dy * dy + dz * dz;
@@ -527,8 +503,8 @@ __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n
   const int end = (int)(ib + n_items);
   int j = (int)ib;
   while (j < end) {
-    const float4 q = W.ipm[j];
-    const int2 a = W.iax[j];
+    const float4 q = W.items[j].pm;
+    const int2 a = W.items[j].ax;
     float dx, dy, dz;
     if (STRADDLE) {
       dx = (float)nearest_d((double)q.x - (double)tx, P);
@@ -591,11 +567,13 @@ __device__ __forceinline__ bool group_any(bool p, u64 lanes) {
   return (__ballot(p) & lanes) != 0ull;
 }
 
-// Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN).
-// Straddling segments (a periodic subhalo wider than a quarter box) form the
-// separation in double.
+// Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN):
+// sep2 = squared separation, opens = the opening test, add = the softened
+// monopole term.  Straddling segments (a periodic subhalo wider than a
+// quarter box) form the separation in double.
 template <bool STRADDLE, bool KAHAN>
 struct FastTerm {
+  using real = float;
   float h2, hinv;
   Kahan<KAHAN> acc;
   __device__ __forceinline__ void init(const DevParams &P, bool live, float tmass) {
@@ -605,9 +583,8 @@ struct FastTerm {
     hinv = 1.0f / h;
     acc.init(live ? tmass / P.eps : 0.f);
   }
-  // returns true when the item is opened
-  __device__ __forceinline__ bool visit(const DevParams &P, const float4 &q, float lsq, float tx,
-                                        float ty, float tz) {
+  __device__ __forceinline__ float sep2(const DevParams &P, const float4 &q, float tx, float ty,
+                                        float tz) const {
 #pragma clang fp contract(off)
     float dx, dy, dz;
     if (STRADDLE) {
@@ -617,16 +594,19 @@ struct FastTerm {
     } else {
       dx = q.x - tx; dy = q.y - ty; dz = q.z - tz;
     }
-    const float r2 = pair_r2(dx, dy, dz);
-    if (lsq > r2 * P.theta2) return true;
-    acc.add(kernel_term_f(q.w, r2, h2, hinv));
-    return false;
+    return pair_r2(dx, dy, dz);
   }
+  __device__ __forceinline__ bool opens(const DevParams &P, float lsq, float r2) const {
+#pragma clang fp contract(off)
+    return lsq > r2 * P.theta2;
+  }
+  __device__ __forceinline__ void add(float m, float r2) { acc.add(kernel_term_f(m, r2, h2, hinv)); }
   __device__ __forceinline__ double result() const { return (double)acc.s; }
 };
 
 // Reference arithmetic (src/gravity_tree.cpp:79-164): FP64 over float data.
 struct ReplayTerm {
+  using real = double;
   double h, hinv, pot;
   __device__ __forceinline__ void init(const DevParams &P, bool live, float tmass) {
 #pragma clang fp contract(off)
@@ -634,116 +614,137 @@ struct ReplayTerm {
     hinv = 1.0 / h;
     pot = live ? (double)(float)(tmass / P.eps) : 0.0;
   }
-  __device__ __forceinline__ bool visit(const DevParams &P, const float4 &q, float lsq, float tx,
-                                        float ty, float tz) {
+  __device__ __forceinline__ double sep2(const DevParams &P, const float4 &q, float tx, float ty,
+                                         float tz) const {
 #pragma clang fp contract(off)
     double dx = (double)q.x - (double)tx, dy = (double)q.y - (double)ty, dz = (double)q.z - (double)tz;
     if (P.periodic) { dx = nearest_d(dx, P); dy = nearest_d(dy, P); dz = nearest_d(dz, P); }
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if ((double)lsq > r2 * (double)P.theta2) return true;
-    pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
-    return false;
+    return dx * dx + dy * dy + dz * dz;
+  }
+  __device__ __forceinline__ bool opens(const DevParams &P, float lsq, double r2) const {
+#pragma clang fp contract(off)
+    return (double)lsq > r2 * (double)P.theta2;
+  }
+  __device__ __forceinline__ void add(float m, double r2) {
+#pragma clang fp contract(off)
+    pot += kernel_term_d((double)m, sqrt(r2), h, hinv);
   }
   __device__ __forceinline__ double result() const { return pot; }
 };
 
-// The item arrays come in as restrict-qualified kernel arguments: nothing in
-// the walk kernel writes them, so reads at a group-uniform index compile to
-// scalar loads when GROUP = 64.  Both possible successors of item j (j+1 and
-// skip(j)) are fetched while j is evaluated, so one load latency is exposed per
-// step instead of one per dependent read.  Fetch indices past the end are
-// clamped to ib (a valid item whose value is then not used).
+// The items come in as a restrict-qualified kernel argument: nothing in the
+// walk kernel writes them, so reads at a group-uniform index compile to
+// scalar loads when GROUP = 64.  Then both possible successors of item j (j+1
+// and skip(j)) are fetched while j is evaluated, so one load latency is exposed
+// per step instead of one per dependent read (fetch indices past the end are
+// clamped to ib, a valid item whose value is not used).  Smaller groups read
+// with vector loads: every lane loads and tests its group's item before any
+// branch (lanes off their path discard the result), so the item's loads issue
+// together at the top of the step.
 template <int GROUP, bool COUNT, class Term>
-__device__ double walk_group(const float4 *__restrict__ ipm, const int2 *__restrict__ iax,
-                             const DevParams &P, int ib, int end, bool live, float tx, float ty,
-                             float tz, float tmass, unsigned &ni, unsigned &no, unsigned &ns) {
+__device__ double walk_group(const Item *__restrict__ it, const DevParams &P, int ib, int end,
+                             bool live, float tx, float ty, float tz, float tmass, unsigned &ni,
+                             unsigned &no, unsigned &ns) {
   Term t;
   t.init(P, live, tmass);
   const u64 lanes = group_lanes<GROUP>();
   const bool leader = (threadIdx.x & (GROUP - 1)) == 0;
   int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
   int j = group_any<GROUP>(live, lanes) ? ib : end;
-  if (GROUP == WAVE) j = __builtin_amdgcn_readfirstlane(j);
-  float4 q = ipm[j < end ? j : ib];
-  int2 a = iax[j < end ? j : ib];
-  while (true) {
-    if (GROUP == WAVE) {
-      if (j >= end) break;
-    } else if (!__any(j < end)) {
-      break;
-    }
-    const bool on = GROUP == WAVE || j < end;
-    const int j1 = j + 1;
-    const int js = a.y > j ? a.y : end;  // skips always advance
-    const int c1 = j1 < end ? j1 : ib, cs = js < end ? js : ib;
-    const float4 q1 = ipm[c1], qs = ipm[cs];
-    const int2 a1 = iax[c1], as = iax[cs];
-    bool opened = false;
-    if (on) {
+  if (GROUP == WAVE) {
+    j = __builtin_amdgcn_readfirstlane(j);
+    float4 q = it[j < end ? j : ib].pm;
+    int2 a = it[j < end ? j : ib].ax;
+    while (j < end) {
+      const int j1 = j + 1;
+      const int js = a.y > j ? a.y : end;  // skips always advance
+      const int c1 = j1 < end ? j1 : ib, cs = js < end ? js : ib;
+      const float4 q1 = it[c1].pm, qs = it[cs].pm;
+      const int2 a1 = it[c1].ax, as = it[cs].ax;
+      bool opened = false;
       if (COUNT && leader) ns += GROUP;
       if (until <= j) {
-        opened = t.visit(P, q, __int_as_float(a.x), tx, ty, tz);
+        const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
+        opened = t.opens(P, __int_as_float(a.x), r2);
         if (opened) {
           if (COUNT) no++;
         } else {
+          t.add(q.w, r2);
           until = a.y;
           if (COUNT) ni++;
         }
       }
-    }
-    const bool go = group_any<GROUP>(opened, lanes);
-    if (on) {
+      const bool go = __any(opened);
       j = go ? j1 : js;
       q = go ? q1 : qs;
       a = go ? a1 : as;
+    }
+  } else {
+    while (__any(j < end)) {
+      const bool on = j < end;
+      const int jj = on ? j : ib;
+      const float4 q = it[jj].pm;
+      const int2 a = it[jj].ax;
+      const bool path = on & (until <= j);
+      const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
+      const bool opened = path & t.opens(P, __int_as_float(a.x), r2);
+      const bool accept = path & !opened;
+      if (accept) t.add(q.w, r2);
+      if (COUNT) {
+        if (on && leader) ns += GROUP;
+        no += opened ? 1u : 0u;
+        ni += accept ? 1u : 0u;
+      }
+      until = accept ? a.y : until;
+      const bool go = group_any<GROUP>(opened, lanes);
+      const int js = a.y > j ? a.y : end;  // skips always advance
+      j = on ? (go ? j + 1 : js) : j;
     }
   }
   return t.result();
 }
 
 template <int GROUP, bool COUNT, bool REPLAY>
-__device__ __forceinline__ double walk_group_dispatch(const float4 *__restrict__ ipm,
-                                                      const int2 *__restrict__ iax,
+__device__ __forceinline__ double walk_group_dispatch(const Item *__restrict__ it,
                                                       const DevParams &P, const Seg &S, int ib,
                                                       int end, bool live, float x, float y, float z,
                                                       float tm, unsigned &ni, unsigned &no,
                                                       unsigned &ns) {
   if (REPLAY)
-    return walk_group<GROUP, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    return walk_group<GROUP, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
   if (!P.kahan) {
     if (S.straddle)
-      return walk_group<GROUP, COUNT, FastTerm<true, false>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    return walk_group<GROUP, COUNT, FastTerm<false, false>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      return walk_group<GROUP, COUNT, FastTerm<true, false>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    return walk_group<GROUP, COUNT, FastTerm<false, false>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
   }
   if (S.straddle)
-    return walk_group<GROUP, COUNT, FastTerm<true, true>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-  return walk_group<GROUP, COUNT, FastTerm<false, true>>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    return walk_group<GROUP, COUNT, FastTerm<true, true>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+  return walk_group<GROUP, COUNT, FastTerm<false, true>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
 }
 
 // Lanes per group chosen per segment (S.wgroup, set from its size bin).  The
 // straddling and uncompensated forms (rare; precision study) keep 64.
 template <bool COUNT, bool REPLAY>
-__device__ __forceinline__ double walk_binned(const float4 *__restrict__ ipm,
-                                              const int2 *__restrict__ iax, const DevParams &P,
+__device__ __forceinline__ double walk_binned(const Item *__restrict__ it, const DevParams &P,
                                               const Seg &S, int ib, int end, bool live, float x,
                                               float y, float z, float tm, unsigned &ni,
                                               unsigned &no, unsigned &ns) {
   if (REPLAY) {
     switch (S.wgroup) {
-      case 8: return walk_group<8, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-      case 16: return walk_group<16, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-      case 32: return walk_group<32, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-      default: return walk_group<64, COUNT, ReplayTerm>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      case 8: return walk_group<8, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      case 16: return walk_group<16, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      case 32: return walk_group<32, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      default: return walk_group<64, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
     }
   }
   if (!P.kahan || S.straddle)
-    return walk_group_dispatch<64, COUNT, false>(ipm, iax, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
+    return walk_group_dispatch<64, COUNT, false>(it, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
   using T = FastTerm<false, true>;
   switch (S.wgroup) {
-    case 8: return walk_group<8, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    case 16: return walk_group<16, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    case 32: return walk_group<32, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    default: return walk_group<64, COUNT, T>(ipm, iax, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    case 8: return walk_group<8, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    case 16: return walk_group<16, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    case 32: return walk_group<32, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    default: return walk_group<64, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
   }
 }
 
@@ -751,10 +752,10 @@ __device__ __forceinline__ double walk_binned(const float4 *__restrict__ ipm,
 // for every group size).  GROUP = 0: lanes per group from each segment's bin.
 template <int GROUP, bool COUNT, bool REPLAY>
 __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
-                                                            const float4 *__restrict__ ipm,
-                                                            const int2 *__restrict__ iax) {
+                                                            const Item *__restrict__ it, int xcd_run) {
   unsigned ni = 0, no = 0, nt = 0, ns = 0;
-  SLOT_LOOP(s, W) {
+  const int vb = xcd_block(xcd_run);
+  for (int64_t s = (int64_t)vb * BLOCK + threadIdx.x; s < W.nslots; s += (int64_t)gridDim.x * BLOCK) {
 #pragma clang fp contract(off)
     // segments are wave-aligned: the whole wave belongs to one segment
     const int k = __builtin_amdgcn_readfirstlane(W.seg_of_slot[s]);
@@ -771,9 +772,9 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
     const int ib = (int)(2 * b), end = ib + S.n_items;
     double pot;
     if (GROUP == 0)
-      pot = walk_binned<COUNT, REPLAY>(ipm, iax, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
+      pot = walk_binned<COUNT, REPLAY>(it, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
     else
-      pot = walk_group_dispatch<GROUP == 0 ? 64 : GROUP, COUNT, REPLAY>(ipm, iax, P, S, ib, end, live, x, y,
+      pot = walk_group_dispatch<GROUP == 0 ? 64 : GROUP, COUNT, REPLAY>(it, P, S, ib, end, live, x, y,
                                                                        z, tm, ni, no, ns);
     pot = pot * (double)P.G / (double)P.a;
     if (!live) continue;

@@ -76,6 +76,22 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T *total) {
   return off + inc - x;
 }
 
+// ---- XCD-aware block order -----------------------------------------------------
+// MI355X has 8 accelerator dies (XCDs), each with its own L2; the workgroups of
+// a launch are dispatched to them round-robin (block b runs on XCD b % 8).
+// xcd_block returns a virtual block index under which each XCD receives runs
+// of `run` consecutive virtual blocks, so neighbouring slots (spatially close
+// targets of one subhalo) share an L2.  Identity when run <= 1 or the grid is
+// not a multiple of 8*run.
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ int xcd_block(int run) {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  if (run <= 1 || G % (NXCD * run) != 0) return b;
+  const int x = b % NXCD, i = b / NXCD;
+  return ((i / run) * NXCD + x) * run + (i % run);
+}
+
 // ---- segmented sort ----------------------------------------------------------
 
 // Sorts each tile's live elements [start, min(start+TILE, n[seg])) from

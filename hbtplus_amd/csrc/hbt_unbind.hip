@@ -95,7 +95,7 @@ struct hbt_ctx {
   int cus = 256;
   // per-slot work arrays
   DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
-      npos, npar, nint, ncnt, ipm, iax, scan64, pk, gk, isb, seg_of_slot;
+      npos, items, scan64, pk, gk, isb, seg_of_slot;
   // per-level tables
   DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal, tpart;
   DBuf lvl, active, counters;
@@ -104,7 +104,7 @@ struct hbt_ctx {
   int walk_mode = 0;  // 0: group-shared traversal, 1: independent per-lane walks
   // lanes per traversal group by size bin, graph capture of the pass sequence
   // (extra passes of a graph launch on a converged level are no-ops)
-  hbt_tuning tune = {{64, 16, 16}, {1024, 1 << 30}, 2, 1, 0};
+  hbt_tuning tune = {{16, 16, 16}, {0, 1 << 30}, 2, 1, 1};
   hipEvent_t evw0 = nullptr, evw1 = nullptr;
   double walk_ms = 0;
   int64_t walk_launches = 0;
@@ -117,7 +117,7 @@ struct hbt_ctx {
   int64_t last_evals = 0, last_passes = 0;
   ~hbt_ctx() {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
-                    &bits, &pre, &nlo, &nhi, &npos, &npar, &nint, &ncnt, &ipm, &iax, &scan64, &pk, &gk,
+                    &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &seg, &ibase, &nbuf, &tiles, &first_tile, &ntile, &tsum,
                     &stot, &child_ptr, &child_list, &capal, &tpart, &lvl, &active, &counters, &st_cols, &st_subs,
                     &st_out_i, &st_out_e, &st_phi})
@@ -309,7 +309,7 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     (u64 *)nullptr));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    for (int v = MAX_DEPTH; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
     return HBT_OK;
   }
 
@@ -317,11 +317,11 @@ struct Runner {
   template <int GROUP>
   void launch_shared_walk(bool count) {
     if (P.replay) {
-      if (count) walk_shared_kernel<GROUP, true, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
-      else walk_shared_kernel<GROUP, false, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
+      if (count) walk_shared_kernel<GROUP, true, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
+      else walk_shared_kernel<GROUP, false, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
     } else {
-      if (count) walk_shared_kernel<GROUP, true, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
-      else walk_shared_kernel<GROUP, false, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.ipm, W.iax);
+      if (count) walk_shared_kernel<GROUP, true, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
+      else walk_shared_kernel<GROUP, false, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
     }
   }
 
@@ -414,11 +414,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->nlo.ensure(s * 4));
   RETCHK(c->nhi.ensure(s * 4));
   RETCHK(c->npos.ensure(s * 4));
-  RETCHK(c->npar.ensure(s * 2 * 4));
-  RETCHK(c->nint.ensure(s * 2 * 4));
-  RETCHK(c->ncnt.ensure(s * 2 * 4));
-  RETCHK(c->ipm.ensure(s * 2 * 16));
-  RETCHK(c->iax.ensure(s * 2 * 8));
+  RETCHK(c->items.ensure(s * 2 * sizeof(Item)));
   RETCHK(c->scan64.ensure(s * 8));
   RETCHK(c->pk.ensure((s + 1) * 4));
   RETCHK(c->gk.ensure((s + 1) * 4));
@@ -462,8 +458,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.bits = c->bits.as<u32>();
   W.pre = c->pre.as<int>(); W.nlo = c->nlo.as<int>(); W.nhi = c->nhi.as<int>();
   W.npos = c->npos.as<int>();
-  W.npar = c->npar.as<int>(); W.nint = c->nint.as<int>(); W.ncnt = c->ncnt.as<int>();
-  W.ipm = c->ipm.as<float4>(); W.iax = c->iax.as<int2>();
+  W.items = c->items.as<Item>();
   W.scan64 = c->scan64.as<u64>();
   W.pk = c->pk.as<int>(); W.gk = c->gk.as<int>();
   W.isb = c->isb.as<unsigned char>();
@@ -742,7 +737,8 @@ static bool tuning_valid(const hbt_tuning &t) {
     if (g != 8 && g != 16 && g != 32 && g != 64) return false;
   }
   return t.bin_edge[0] >= 0 && t.bin_edge[1] >= t.bin_edge[0] && t.graph_passes >= 1 &&
-         t.graph_passes <= 16 && (t.use_graph == 0 || t.use_graph == 1);
+         t.graph_passes <= 16 && (t.use_graph == 0 || t.use_graph == 1) && t.xcd_run >= 1 &&
+         t.xcd_run <= 64 && (t.xcd_run & (t.xcd_run - 1)) == 0;
 }
 
 int hbt_get_tuning(hbt_ctx *ctx, hbt_tuning *out) {
@@ -755,9 +751,9 @@ int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *tuning) {
   if (!ctx || !tuning) return fail(HBT_ERR_INVALID_ARG, "null argument");
   if (!tuning_valid(*tuning))
     return fail(HBT_ERR_INVALID_ARG, "tuning: walk groups must be 8/16/32/64, bin edges ascending, "
-                                     "graph_passes in 1..16, use_graph 0 or 1");
+                                     "graph_passes in 1..16, use_graph 0 or 1, xcd_run a power "
+                                     "of two in 1..64");
   ctx->tune = *tuning;
-  ctx->tune.reserved = 0;
   return HBT_OK;
 }
 
@@ -808,7 +804,15 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
       rc = trial(t, false);
     }
   }
-  // 3. passes per graph launch
+  // 3. XCD runs of the walk's blocks
+  const int runs[3] = {1, 4, 16};
+  for (int i = 0; i < 3 && rc == HBT_OK; i++) {
+    if (runs[i] == best.xcd_run) continue;
+    hbt_tuning t = best;
+    t.xcd_run = runs[i];
+    rc = trial(t, false);
+  }
+  // 4. passes per graph launch
   for (int gp = 1; gp <= 4 && rc == HBT_OK; gp++) {
     if (gp == best.graph_passes) continue;
     hbt_tuning t = best;

@@ -252,6 +252,7 @@ def test_tuning_api_and_bins_are_identical(gpu):
         t["walk_group"] = (64, 32, 8)
         t["bin_edge"] = (300, 3000)
         t["graph_passes"] = 3
+        t["xcd_run"] = 4
         U.set_tuning(t)
         got = U.unbind_batch(cat)
         for x, y in zip(base, got):
@@ -261,7 +262,8 @@ def test_tuning_api_and_bins_are_identical(gpu):
         got = U.unbind_batch(cat)
         for x, y in zip(base, got):
             assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
-    bad = U.get_tuning()
-    bad["walk_group"] = (64, 12, 64)
-    with pytest.raises(RuntimeError):
-        U.set_tuning(bad)
+    for field, value in (("walk_group", (64, 12, 64)), ("xcd_run", 3), ("graph_passes", 0)):
+        bad = U.get_tuning()
+        bad[field] = value
+        with pytest.raises(RuntimeError):
+            U.set_tuning(bad)

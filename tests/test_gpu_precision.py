@@ -46,8 +46,6 @@ def test_c3_precision_study(gpu):
         report[f"potential_rel_rms_{name}"] = dict(median=float(np.median(per)),
                                                    p99=float(np.quantile(per, 0.99)),
                                                    max=float(per.max()))
-    assert report["potential_rel_rms_fp32_kahan"]["max"] < 1e-6
-    assert report["potential_rel_rms_fp32"]["max"] < 1e-5
 
     runs = {}
     for prec, name in ((abi.HBT_PREC_REPLAY, "fp64"), (abi.HBT_PREC_FAST, "fp32_kahan"),
@@ -69,3 +67,8 @@ def test_c3_precision_study(gpu):
     if os.path.isdir("gpurun_out"):
         with open("gpurun_out/precision_c3.json", "w") as f:
             json.dump(report, f, indent=1)
+    # the FAST form's bound: per-subhalo potential RMS error <= 1e-4 against FP64;
+    # compensation must not make the sum worse than plain FP32
+    for name in ("fp32_kahan", "fp32"):
+        assert report[f"potential_rel_rms_{name}"]["max"] < 1e-4
+    assert report["potential_rel_rms_fp32_kahan"]["median"] <= report["potential_rel_rms_fp32"]["median"]
