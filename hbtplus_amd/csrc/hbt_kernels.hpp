@@ -25,6 +25,11 @@ namespace hbt {
 
 // One tree item (leaf or node) in DFS pre-order, one 32-byte record: the
 // walk reads both halves of an item at once.
+// Rows of Work::ncount, each written by the kernel that decides the count:
+// sources of the tree (pass setup), the partition scan (pass setup), the bound
+// target list (partition), the energy sort (frames).
+enum { NC_SRC = 0, NC_PART = 1, NC_BOUND = 2, NC_SORT = 3, NC_ROWS = 4 };
+
 struct __align__(32) Item {
   float4 pm;   // x, y, z, mass (a node: centre of mass and total mass)
   int2 ax;     // float bits of len^2 (0 for a leaf), skip (first item after the subtree)
@@ -38,7 +43,7 @@ struct Work {
   int64_t nslots;
   const int *seg_of_slot;
   const int64_t *ibase;
-  int *nbuf;                 // per-segment element counts for sort/scan calls
+  int *ncount;               // [NC_ROWS][nseg] element counts of the pass's scans and sorts
   // particle columns of the caller (device)
   const int64_t *g_id;
   const float *g_x, *g_y, *g_z, *g_vx, *g_vy, *g_vz, *g_m;
@@ -59,7 +64,6 @@ struct Work {
   u64 *scan64;
   int *pk, *gk;
   unsigned char *isb;
-  int *lvl;                  // [3*NLEVEL+1]: counts, offsets, cursors
   int *active;               // [1]
   unsigned long long *counters;  // [4]: interactions, opened nodes, targets (instrumented passes)
   // caller outputs
@@ -177,10 +181,7 @@ __global__ void el_init_kernel(Work W) {
 // Pass: setup
 
 __global__ void pass_setup_kernel(Work W, DevParams P) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (int i = 0; i < 3 * NLEVEL + 1; i++) W.lvl[i] = 0;
-    *W.active = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *W.active = 0;
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     S.flags &= ~F_FINAL;
@@ -206,6 +207,14 @@ __global__ void pass_setup_kernel(Work W, DevParams P) {
     S.evals += S.n_tgt;
     S.nsort = S.n_src;
     for (int j = 0; j < 3; j++) { S.bmin[j] = 0xffffffffu; S.bmax[j] = 0u; }
+  }
+  SEG_LOOP(k, W) {
+    const Seg &S = W.seg[k];
+    const bool act = (S.flags & F_ACTIVE) != 0;
+    W.ncount[NC_SRC * W.nseg + k] = S.n_src;
+    W.ncount[NC_PART * W.nseg + k] = act ? S.nlast : 0;
+    W.ncount[NC_BOUND * W.nseg + k] = 0;
+    W.ncount[NC_SORT * W.nseg + k] = 0;
   }
 }
 
@@ -303,6 +312,7 @@ __global__ void __launch_bounds__(BLOCK) keys_kernel(Work W) {
     key <<= 3 * (MAX_DEPTH - S.D);
     W.k0[s] = key;
     W.v0[s] = (u32)li;
+    W.bits[s] = 0u;  // node-start masks, set by emit_kernel
   }
 }
 
@@ -319,19 +329,6 @@ __device__ __forceinline__ int key_delta(u64 a, u64 b, int D) {
   return d < D ? d : D;
 }
 
-__global__ void __launch_bounds__(BLOCK) delta_kernel(Work W, int full_copy_sp) {
-  SLOT_LOOP(s, W) {
-    int k;
-    int l = seg_lane_local(W, s, k);
-    if (l < 0) continue;
-    const Seg &S = W.seg[k];
-    if (l >= S.n_src) continue;
-    W.dl[s] = (signed char)((l + 1 < S.n_src) ? key_delta(W.k0[s], W.k0[s + 1], S.D) : -1);
-    W.bits[s] = 0u;
-    if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
-  }
-}
-
 // First v octant digits of a key (digit 0 in bits 62..60).
 __device__ __forceinline__ u64 key_prefix(u64 k, int v) { return v == 0 ? 0ull : k >> (63 - 3 * v); }
 
@@ -345,14 +342,13 @@ __device__ __forceinline__ int prefix_lower(const u64 *key, int lo, int hi, int 
   return lo;
 }
 
-// The node split by pair l: the sources sharing the first v = dl[l] digits
-// with source l, found by binary search over the sorted keys.  Returns false
-// when pair l is not the node's leftmost split (only the leftmost split emits
-// the node).
-__device__ __forceinline__ bool node_of_pair(const Work &W, int64_t b, int l, int n, int v, int D,
-                                             int &lo, int &hi) {
-  const u64 *key = W.k0 + b;
-  const int vl = l > 0 ? W.dl[b + l - 1] : -1;
+// The node split by pair l (at depth v, the pair's delta): the sources sharing
+// the first v digits with source l, found by binary search over the sorted
+// keys.  vl, vr: deltas of the pairs l-1 and l+1 (-1 where there is none).
+// Returns false when pair l is not the node's leftmost split (only the
+// leftmost split emits the node).
+__device__ __forceinline__ bool node_of_pair(const u64 *key, int l, int n, int v, int vl, int vr,
+                                             int D, int &lo, int &hi) {
   if (vl == v) return false;               // source l-1 is in the node, in another child
   if (vl < v) {
     lo = l;                                // the node starts at source l
@@ -361,25 +357,32 @@ __device__ __forceinline__ bool node_of_pair(const Work &W, int64_t b, int l, in
     // leftmost split iff sources lo and l are in the same child
     if (key_delta(key[lo], key[l], D) <= v) return false;
   }
-  const int vr = l + 2 < n ? W.dl[b + l + 1] : -1;
   if (vr < v) hi = l + 1;
   else hi = prefix_lower(key, l + 2, n - 1, v, key_prefix(key[l], v) + 1) - 1;
   return true;
 }
 
-// Emits the node of which pair l is the leftmost split.
-__global__ void __launch_bounds__(BLOCK) emit_kernel(Work W) {
+// Per pair l of sorted sources: its delta, and the node of which it is the
+// leftmost split.  The neighbouring pairs' deltas come from the keys.  In a
+// full pass the sorted order also becomes the spatial target order.
+__global__ void __launch_bounds__(BLOCK) emit_kernel(Work W, int full_copy_sp) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
-    if (l >= S.n_src) continue;
+    const int n = S.n_src;
+    if (l >= n) continue;
+    if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
     W.nlo[s] = -1;
-    if (l + 1 >= S.n_src) continue;
-    const int v = W.dl[s];
+    const u64 *key = W.k0 + S.ibase;
+    const int v = (l + 1 < n) ? key_delta(key[l], key[l + 1], S.D) : -1;
+    W.dl[s] = (signed char)v;
+    if (v < 0) continue;
+    const int vl = l > 0 ? key_delta(key[l - 1], key[l], S.D) : -1;
+    const int vr = l + 2 < n ? key_delta(key[l + 1], key[l + 2], S.D) : -1;
     int lo, hi;
-    if (!node_of_pair(W, S.ibase, l, S.n_src, v, S.D, lo, hi)) continue;
+    if (!node_of_pair(key, l, n, v, vl, vr, S.D, lo, hi)) continue;
     W.nlo[s] = lo;
     W.nhi[s] = hi;
     atomicOr(&W.bits[S.ibase + lo], 1u << v);
@@ -636,11 +639,12 @@ struct ReplayTerm {
 // walk kernel writes them, so reads at a group-uniform index compile to
 // scalar loads when GROUP = 64.  Then both possible successors of item j (j+1
 // and skip(j)) are fetched while j is evaluated, so one load latency is exposed
-// per step instead of one per dependent read (fetch indices past the end are
-// clamped to ib, a valid item whose value is not used).  Smaller groups read
-// with vector loads: every lane loads and tests its group's item before any
-// branch (lanes off their path discard the result), so the item's loads issue
-// together at the top of the step.
+// per step instead of one per dependent read.  Smaller groups read with vector
+// loads: every lane loads and tests its group's item before any branch (lanes
+// off their path discard the result), so the item's loads issue together at
+// the top of the step.  Reads past the end are clamped to the segment's last
+// item, a leaf whose skip is `end`: a finished group keeps reading it and
+// stays finished.  Skips always advance (max with j+1).
 template <int GROUP, bool COUNT, class Term>
 __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, int ib, int end,
                              bool live, float tx, float ty, float tz, float tmass, unsigned &ni,
@@ -651,14 +655,15 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
   const bool leader = (threadIdx.x & (GROUP - 1)) == 0;
   int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
   int j = group_any<GROUP>(live, lanes) ? ib : end;
+  const int last = end - 1 > ib ? end - 1 : ib;
   if (GROUP == WAVE) {
     j = __builtin_amdgcn_readfirstlane(j);
-    float4 q = it[j < end ? j : ib].pm;
-    int2 a = it[j < end ? j : ib].ax;
+    float4 q = it[(unsigned)min(j, last)].pm;
+    int2 a = it[(unsigned)min(j, last)].ax;
     while (j < end) {
       const int j1 = j + 1;
-      const int js = a.y > j ? a.y : end;  // skips always advance
-      const int c1 = j1 < end ? j1 : ib, cs = js < end ? js : ib;
+      const int js = max(a.y, j1);
+      const unsigned c1 = (unsigned)min(j1, last), cs = (unsigned)min(js, last);
       const float4 q1 = it[c1].pm, qs = it[cs].pm;
       const int2 a1 = it[c1].ax, as = it[cs].ax;
       bool opened = false;
@@ -682,9 +687,9 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
   } else {
     while (__any(j < end)) {
       const bool on = j < end;
-      const int jj = on ? j : ib;
-      const float4 q = it[jj].pm;
-      const int2 a = it[jj].ax;
+      const Item *pj = it + (unsigned)min(j, last);
+      const float4 q = pj->pm;
+      const int2 a = pj->ax;
       const bool path = on & (until <= j);
       const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
       const bool opened = path & t.opens(P, __int_as_float(a.x), r2);
@@ -697,8 +702,7 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
       }
       until = accept ? a.y : until;
       const bool go = group_any<GROUP>(opened, lanes);
-      const int js = a.y > j ? a.y : end;  // skips always advance
-      j = on ? (go ? j + 1 : js) : j;
+      j = go ? j + 1 : max(a.y, j + 1);
     }
   }
   return t.result();
@@ -939,6 +943,7 @@ __global__ void partition_done_kernel(Work W, DevParams P) {
       S.nlast = 1;
     }
     S.nsort = 0;  // the energy sort follows the frames
+    W.ncount[NC_BOUND * W.nseg + k] = (S.flags & F_ACTIVE) ? S.nlast : 0;
   }
 }
 
@@ -1094,6 +1099,7 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
     S.nsort = S.nlast - nb;
     atomicAdd(W.active, 1);
   }
+  W.ncount[NC_SORT * W.nseg + (int)(&S - W.seg)] = S.nsort;
 }
 
 __global__ void frames_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part) {

@@ -12,7 +12,7 @@
 //     tile_sort_kernel    : LDS bitonic sort of each tile
 //     merge_pass_kernel   : one bottom-up merge level (co-rank by binary search)
 //   segmented exclusive scan (load/store functors):
-//     scan_tiles_kernel / scan_tile_offsets_kernel / scan_add_kernel
+//     scan_tiles_kernel / scan_add_kernel
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -94,26 +94,37 @@ __device__ __forceinline__ int xcd_block(int run) {
 
 // ---- segmented sort ----------------------------------------------------------
 
+// Number of merge levels a segment of n elements needs after the tile sort.
+__device__ __forceinline__ int merge_levels(int n) {
+  int m = 0;
+  for (int w = TILE; w < n; w *= 2) m++;
+  return m;
+}
+
 // Sorts each tile's live elements [start, min(start+TILE, n[seg])) from
-// (kin, vin) into (kout, vout) (the two may be the same arrays).
+// (k0, v0) into the buffer from which the segment's merge levels end in
+// (k0, v0): (k1, v1) when it needs an odd number of levels, else in place.
 __global__ void __launch_bounds__(BLOCK)
 tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__restrict__ base,
-                 const int *__restrict__ n, const u64 *kin, const u32 *vin, u64 *kout, u32 *vout) {
+                 const int *__restrict__ n, u64 *k0, u32 *v0, u64 *k1, u32 *v1) {
   __shared__ u64 sk[TILE];
   __shared__ u32 sv[TILE];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
-    const int cnt = min(TILE, n[tl.seg] - tl.start);
+    const int nseg = n[tl.seg];
+    const int cnt = min(TILE, nseg - tl.start);
     if (cnt <= 0) continue;  // uniform across the block
     const int64_t off = base[tl.seg] + tl.start;
+    u64 *kout = (merge_levels(nseg) & 1) ? k1 : k0;
+    u32 *vout = (merge_levels(nseg) & 1) ? v1 : v0;
     if (cnt == 1) {
-      if (threadIdx.x == 0) { kout[off] = kin[off]; vout[off] = vin[off]; }
+      if (threadIdx.x == 0) { kout[off] = k0[off]; vout[off] = v0[off]; }
       continue;
     }
     int P = 1;
     while (P < cnt) P <<= 1;
     for (int i = threadIdx.x; i < P; i += BLOCK) {
-      if (i < cnt) { sk[i] = kin[off + i]; sv[i] = vin[off + i]; }
+      if (i < cnt) { sk[i] = k0[off + i]; sv[i] = v0[off + i]; }
       else { sk[i] = ~0ull; sv[i] = 0xffffffffu; }
     }
     __syncthreads();
@@ -137,54 +148,60 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
   }
 }
 
-// One merge level: sorted runs of `width` elements are merged pairwise from
-// (kin, vin) into (kout, vout).  Elements of segments with n <= width are
-// copied.  seg_of_slot maps a slot to its segment.
+// Merge level `level` (runs of width TILE << level merged pairwise), one block
+// per tile.  A segment that needs m levels takes part in levels 0..m-1 and
+// ping-pongs between the two buffers so that level m-1 writes (k0, v0);
+// segments with m <= level are done and are not touched.
 __global__ void __launch_bounds__(BLOCK)
-merge_pass_kernel(int64_t nslots, const int *__restrict__ seg_of_slot,
-                  const int64_t *__restrict__ base, const int *__restrict__ n, int width,
-                  const u64 *__restrict__ kin, const u32 *__restrict__ vin, u64 *__restrict__ kout,
-                  u32 *__restrict__ vout) {
-  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < nslots;
-       s += (int64_t)gridDim.x * BLOCK) {
-    const int k = seg_of_slot[s];
-    if (k < 0) continue;
-    const int64_t b = base[k];
-    const int l = (int)(s - b), cnt = n[k];
-    if (l >= cnt) continue;
-    const u64 key = kin[s];
-    const u32 val = vin[s];
-    if (cnt <= width) { kout[s] = key; vout[s] = val; continue; }
-    const int run = l / width;
-    const int a0 = (run & ~1) * width;  // start of the merged pair
-    int pos;
-    if ((run & 1) == 0) {
-      // count elements of the right run that precede (key,val)
-      int lo = a0 + width, hi = min(a0 + 2 * width, cnt);
-      const int b0 = lo;
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
+merge_pass_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__restrict__ base,
+                  const int *__restrict__ n, int level, u64 *k0, u32 *v0, u64 *k1, u32 *v1) {
+  const int width = TILE << level;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const int cnt = n[tl.seg];
+    if (cnt <= width || tl.start >= cnt) continue;  // uniform across the block
+    const int m = merge_levels(cnt);
+    // buffer holding the input of this level: the tile sort wrote parity m&1
+    const bool in1 = ((m + level) & 1) != 0;
+    const u64 *kin = in1 ? k1 : k0;
+    const u32 *vin = in1 ? v1 : v0;
+    u64 *kout = in1 ? k0 : k1;
+    u32 *vout = in1 ? v0 : v1;
+    const int64_t b = base[tl.seg];
+    const int lend = min(tl.start + TILE, cnt);
+    for (int l = tl.start + (int)threadIdx.x; l < lend; l += BLOCK) {
+      const u64 key = kin[b + l];
+      const u32 val = vin[b + l];
+      const int run = l / width;
+      const int a0 = (run & ~1) * width;  // start of the merged pair
+      int pos;
+      if ((run & 1) == 0) {
+        // count elements of the right run that precede (key,val)
+        int lo = a0 + width, hi = min(a0 + 2 * width, cnt);
+        const int b0 = lo;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
+        }
+        pos = (l - a0) + (lo - b0);
+      } else {
+        int lo = a0, hi = a0 + width;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
+        }
+        pos = (l - a0 - width) + (lo - a0);
       }
-      pos = (l - a0) + (lo - b0);
-    } else {
-      int lo = a0, hi = a0 + width;
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
-      }
-      pos = (l - a0 - width) + (lo - a0);
+      kout[b + a0 + pos] = key;
+      vout[b + a0 + pos] = val;
     }
-    kout[b + a0 + pos] = key;
-    vout[b + a0 + pos] = val;
   }
 }
 
 // ---- segmented exclusive scan ----------------------------------------------
 // Load(seg, local) -> T for local < n[seg]; Store(seg, local, exclusive prefix).
 // Pass 1 scans each tile and stores tile-local prefixes and the tile total;
-// pass 2 scans the tile totals of each segment (tile_off, seg_total);
-// pass 3 adds the tile offsets.
+// pass 2 adds to every tile the totals of the tiles before it in its segment.
 
 template <class T, class Load, class Store>
 __global__ void __launch_bounds__(BLOCK)
@@ -217,32 +234,23 @@ scan_tiles_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restr
   }
 }
 
-template <class T>
-__global__ void scan_tile_offsets_kernel(int nseg, const int *__restrict__ first_tile,
-                                         const int *__restrict__ ntile_of, const int *__restrict__ n,
-                                         T *__restrict__ tile_sum, T *__restrict__ seg_total) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nseg; k += gridDim.x * blockDim.x) {
-    const int f = first_tile[k];
-    const int nt = min(ntile_of[k], (n[k] + TILE - 1) / TILE);
-    T run = 0;
-    for (int t = f; t < f + nt; t++) {
-      T x = tile_sum[t];
-      tile_sum[t] = run;
-      run += x;
-    }
-    if (seg_total) seg_total[k] = run;
-  }
-}
-
+// Pass 2: adds to each tile the sum of the earlier tiles of its segment (the
+// tiles of a segment are consecutive in the table; tile t is the
+// start/TILE-th of its segment).
 template <class T, class Update>
 __global__ void __launch_bounds__(BLOCK)
 scan_add_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restrict__ n,
-                const T *__restrict__ tile_off, Update update) {
+                const T *__restrict__ tile_sum, Update update) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     if (tl.start == 0) continue;  // first tile of a segment has offset 0
     const int cnt = min(TILE, n[tl.seg] - tl.start);
-    const T o = tile_off[t];
+    if (cnt <= 0) continue;
+    const int t0 = t - tl.start / TILE;
+    T part = 0;
+    for (int u = t0 + (int)threadIdx.x; u < t; u += BLOCK) part += tile_sum[u];
+    T o;
+    (void)block_exclusive_scan(part, &o);
     for (int l = threadIdx.x; l < cnt; l += BLOCK) update(tl.seg, tl.start + l, o);
   }
 }

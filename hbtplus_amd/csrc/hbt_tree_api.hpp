@@ -8,8 +8,8 @@ namespace {
 
 __global__ void tree_only_setup_kernel(Work W, int n) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (int i = 0; i < 3 * NLEVEL + 1; i++) W.lvl[i] = 0;
     Seg &S = W.seg[0];
+    W.ncount[NC_SRC * W.nseg] = n;
     S.flags = F_ACTIVE;
     S.n_src = n; S.src_lo = 0; S.n_tgt = 0; S.nsort = n; S.nlast = n; S.nbound = n;
     for (int j = 0; j < 3; j++) { S.bmin[j] = 0xffffffffu; S.bmax[j] = 0u; }
@@ -49,12 +49,7 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
   Level L;
   build_level(&h, cap, children, std::vector<int>{0}, ctx->tune, L);
   RETCHK(ensure_slots(ctx, L.nslots, 1, (int64_t)L.tiles.size()));
-  RETCHK(upload(ctx->seg, L.seg, ctx->stream));
-  RETCHK(upload(ctx->ibase, L.ibase, ctx->stream));
-  RETCHK(upload(ctx->capal, L.capal, ctx->stream));
-  RETCHK(upload(ctx->tiles, L.tiles, ctx->stream));
-  RETCHK(upload(ctx->first_tile, L.first_tile, ctx->stream));
-  RETCHK(upload(ctx->ntile, L.ntile, ctx->stream));
+  RETCHK(stage_level(ctx, L));
   Runner R;
   R.c = ctx;
   R.L = &L;

@@ -86,6 +86,12 @@ struct DBuf {
   template <class T> T *as() const { return (T *)p; }
 };
 
+// A typed view into a device buffer owned elsewhere.
+struct DView {
+  void *p = nullptr;
+  template <class T> T *as() const { return (T *)p; }
+};
+
 }  // namespace
 
 struct hbt_ctx {
@@ -96,9 +102,13 @@ struct hbt_ctx {
   // per-slot work arrays
   DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
       npos, items, scan64, pk, gk, isb, seg_of_slot;
-  // per-level tables
-  DBuf seg, ibase, nbuf, tiles, first_tile, ntile, tsum, stot, child_ptr, child_list, capal, tpart;
-  DBuf lvl, active, counters;
+  // per-level tables: staged through one pinned host buffer and one copy
+  DBuf lvlbuf;
+  void *h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+  DView seg, ibase, tiles, first_tile, ntile, child_ptr, child_list, capal;
+  DBuf ncount, tsum, tpart;
+  DBuf active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
   int walk_mode = 0;  // 0: group-shared traversal, 1: independent per-lane walks
@@ -118,11 +128,11 @@ struct hbt_ctx {
   ~hbt_ctx() {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
-                    &isb, &seg_of_slot, &seg, &ibase, &nbuf, &tiles, &first_tile, &ntile, &tsum,
-                    &stot, &child_ptr, &child_list, &capal, &tpart, &lvl, &active, &counters, &st_cols, &st_subs,
-                    &st_out_i, &st_out_e, &st_phi})
+                    &isb, &seg_of_slot, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
+    if (h_stage) (void)hipHostFree(h_stage);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (evw0) (void)hipEventDestroy(evw0);
@@ -188,6 +198,47 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
   L.nslots = off;
 }
 
+// Copies the level's tables to the device: packed into the pinned staging
+// buffer, one asynchronous copy.  The stream is drained first, so no earlier
+// copy still reads the staging buffer (normally it is idle already: each level
+// ends with a stream synchronisation).
+int stage_level(hbt_ctx *c, const Level &L) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  size_t off = 0;
+  auto place = [&](size_t bytes) {
+    const size_t o = off;
+    off += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t n = L.seg.size();
+  const size_t o_seg = place(n * sizeof(Seg)), o_ib = place(n * 8), o_cap = place(n * 4),
+               o_tl = place(L.tiles.size() * sizeof(Tile)), o_ft = place(n * 4), o_nt = place(n * 4),
+               o_cp = place(L.child_ptr.size() * 4), o_cl = place(L.child_list.size() * 4);
+  if (off > c->h_stage_bytes) {
+    if (c->h_stage) HIPCHK(hipHostFree(c->h_stage));
+    c->h_stage = nullptr;
+    c->h_stage_bytes = 0;
+    HIPCHK(hipHostMalloc(&c->h_stage, off, hipHostMallocDefault));
+    c->h_stage_bytes = off;
+  }
+  RETCHK(c->lvlbuf.ensure(off));
+  char *h = (char *)c->h_stage;
+  auto put = [&](size_t o, const void *src, size_t bytes) { if (bytes) memcpy(h + o, src, bytes); };
+  put(o_seg, L.seg.data(), n * sizeof(Seg));
+  put(o_ib, L.ibase.data(), n * 8);
+  put(o_cap, L.capal.data(), n * 4);
+  put(o_tl, L.tiles.data(), L.tiles.size() * sizeof(Tile));
+  put(o_ft, L.first_tile.data(), n * 4);
+  put(o_nt, L.ntile.data(), n * 4);
+  put(o_cp, L.child_ptr.data(), L.child_ptr.size() * 4);
+  put(o_cl, L.child_list.data(), L.child_list.size() * 4);
+  HIPCHK(hipMemcpyAsync(c->lvlbuf.p, c->h_stage, off, hipMemcpyHostToDevice, c->stream));
+  char *d = (char *)c->lvlbuf.p;
+  c->seg.p = d + o_seg; c->ibase.p = d + o_ib; c->capal.p = d + o_cap; c->tiles.p = d + o_tl;
+  c->first_tile.p = d + o_ft; c->ntile.p = d + o_nt; c->child_ptr.p = d + o_cp; c->child_list.p = d + o_cl;
+  return HBT_OK;
+}
+
 template <class T>
 int upload(DBuf &b, const std::vector<T> &v, hipStream_t st) {
   RETCHK(b.ensure(std::max<size_t>(v.size() * sizeof(T), 16)));
@@ -237,28 +288,29 @@ struct BoundLoad {
   __device__ u64 operator()(int k, int l) const { return (u64)isb[ibase[k] + sp[ibase[k] + l]]; }
 };
 
-enum NMode { N_SORT = 0, N_SRC = 1, N_NLAST_ACTIVE = 2 };
 
-__global__ void set_n_kernel(Work W, int mode) {
-  SEG_LOOP(k, W) {
-    const Seg &S = W.seg[k];
-    int v = 0;
-    if (mode == N_SORT) v = S.nsort;
-    else if (mode == N_SRC) v = S.n_src;
-    else v = (S.flags & F_ACTIVE) ? S.nlast : 0;
-    W.nbuf[k] = v;
-  }
-}
-
-__global__ void partition_counts_kernel(Work W, const u64 *stot) {
+// Partition totals of each active segment: the sum of its tile totals.
+__global__ void partition_counts_kernel(Work W, const int *first_tile, const u64 *tile_sum) {
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     if (!(S.flags & F_ACTIVE)) { S.nP = S.nG = 0; continue; }
-    const u64 t = stot[k];
+    const int nt = (W.ncount[NC_PART * W.nseg + k] + TILE - 1) / TILE;
+    u64 t = 0;
+    for (int u = first_tile[k]; u < first_tile[k] + nt; u++) t += tile_sum[u];
     S.nP = (int)(t & 0xffffffffu);
     S.nG = (int)(t >> 32);
     S.K = 0;
   }
+}
+
+// Sum of n_evals over the records (one block).
+__global__ void __launch_bounds__(BLOCK) eval_sum_kernel(const hbt_subhalo *subs, int64_t nsub,
+                                                         unsigned long long *out) {
+  unsigned long long v = 0;
+  for (int64_t i = threadIdx.x; i < nsub; i += BLOCK) v += (unsigned long long)subs[i].n_evals;
+  unsigned long long tot;
+  (void)block_exclusive_scan(v, &tot);
+  if (threadIdx.x == 0) *out = tot;
 }
 
 struct Runner {
@@ -268,46 +320,37 @@ struct Runner {
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
 
-  int sort_pairs() {  // sorts (k0,v0) by nbuf, result in (k0,v0)
-    // merge levels for the largest segment of the level; the tile sort writes
-    // to the buffer from which the ping-pong of the merges ends in (k0,v0)
-    int merges = 0;
-    for (int w = TILE; w < L->maxcap; w *= 2) merges++;
-    u64 *ki = (merges & 1) ? W.k1 : W.k0, *ko = (merges & 1) ? W.k0 : W.k1;
-    u32 *vi = (merges & 1) ? W.v1 : W.v0, *vo = (merges & 1) ? W.v0 : W.v1;
-    tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), (int)L->tiles.size(),
-                                                     W.ibase, W.nbuf, W.k0, W.v0, ki, vi);
-    for (int w = TILE; w < L->maxcap; w *= 2) {
-      merge_pass_kernel<<<gs, BLOCK, 0, c->stream>>>(W.nslots, W.seg_of_slot, W.ibase, W.nbuf, w, ki,
-                                                     vi, ko, vo);
-      std::swap(ki, ko);
-      std::swap(vi, vo);
-    }
+  int sort_pairs(const int *n) {  // sorts (k0,v0) by the counts n[seg], result in (k0,v0)
+    const Tile *tl = c->tiles.as<Tile>();
+    const int nt = (int)L->tiles.size();
+    tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
+    // one launch per merge level of the largest segment of the level
+    int level = 0;
+    for (int w = TILE; w < L->maxcap; w *= 2, level++)
+      merge_pass_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
     return HBT_OK;
   }
 
+  // segmented exclusive scan over the counts n[seg]; the tile totals stay in tsum
   template <class Load, class Store, class Add>
-  int seg_scan(Load ld, Store st, Add ad, u64 *seg_total) {
+  int seg_scan(Load ld, Store st, Add ad, const int *n) {
     const int nt = (int)L->tiles.size();
     u64 *ts = c->tsum.as<u64>();
-    scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, W.nbuf, ts, ld, st);
-    scan_tile_offsets_kernel<u64><<<gseg, BLOCK, 0, c->stream>>>(
-        W.nseg, c->first_tile.as<int>(), c->ntile.as<int>(), W.nbuf, ts, seg_total);
-    scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, W.nbuf, ts, ad);
+    scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, n, ts, ld, st);
+    scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, n, ts, ad);
     return HBT_OK;
   }
+  const int *count_row(int row) const { return W.ncount + (size_t)row * W.nseg; }
 
   // tree over the pass's sources: keys, sort, structure, moments
   int build_tree(bool copy_sp) {
     bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, c->tiles.as<Tile>(), (int)L->tiles.size());
     tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SRC);
-    RETCHK(sort_pairs());
-    delta_kernel<<<gs, BLOCK, 0, c->stream>>>(W, copy_sp ? 1 : 0);
-    emit_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    RETCHK(sort_pairs(count_row(NC_SRC)));
+    emit_kernel<<<gs, BLOCK, 0, c->stream>>>(W, copy_sp ? 1 : 0);
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
-                    (u64 *)nullptr));
+                    count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     for (int v = MAX_DEPTH; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
     return HBT_OK;
@@ -349,11 +392,9 @@ struct Runner {
     launch_walk(c->prof_mode == 2);
     if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evw1, c->stream));
     // partition
-    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_NLAST_ACTIVE);
-    u64 *stot = c->stot.as<u64>();
     RETCHK(seg_scan(PartLoad{W.ibase, W.el, W.E}, S64Store{W.ibase, W.scan64},
-                    S64Add{W.ibase, W.scan64}, stot));
-    partition_counts_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, stot);
+                    S64Add{W.ibase, W.scan64}, count_row(NC_PART)));
+    partition_counts_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, c->first_tile.as<int>(), c->tsum.as<u64>());
     partition_rank_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_k_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_hole_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
@@ -362,9 +403,8 @@ struct Runner {
     partition_done_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     // spatial target list keeps the bound particles
     bound_flags_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_NLAST_ACTIVE);
     RETCHK(seg_scan(BoundLoad{W.ibase, W.sp, W.isb}, PreStore{W.ibase, W.pre},
-                    PreAdd{W.ibase, W.pre}, (u64 *)nullptr));
+                    PreAdd{W.ibase, W.pre}, count_row(NC_BOUND)));
     spatial_compact_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     spatial_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     // frames, convergence (the frame sums read the bound block only, in Elist order)
@@ -377,8 +417,7 @@ struct Runner {
     }
     // energy sort of the removed block, and of the bound block on convergence
     energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    set_n_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, N_SORT);
-    RETCHK(sort_pairs());
+    RETCHK(sort_pairs(count_row(NC_SORT)));
     energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     output_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
     if (P.replay) {
@@ -420,11 +459,9 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->gk.ensure((s + 1) * 4));
   RETCHK(c->isb.ensure(s));
   RETCHK(c->seg_of_slot.ensure(s * 4));
-  RETCHK(c->nbuf.ensure((size_t)(nseg + 1) * 4));
+  RETCHK(c->ncount.ensure((size_t)(NC_ROWS * nseg + 1) * 4));
   RETCHK(c->tsum.ensure((size_t)(ntiles + 1) * 8));
   RETCHK(c->tpart.ensure((size_t)(ntiles + 1) * 8 * REC_SUMS));
-  RETCHK(c->stot.ensure((size_t)(nseg + 1) * 8));
-  RETCHK(c->lvl.ensure((3 * NLEVEL + 1) * 4));
   RETCHK(c->active.ensure(16));
   RETCHK(c->counters.ensure(64));
   return HBT_OK;
@@ -438,7 +475,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.nslots = L.nslots;
   W.seg_of_slot = c->seg_of_slot.as<int>();
   W.ibase = c->ibase.as<int64_t>();
-  W.nbuf = c->nbuf.as<int>();
+  W.ncount = c->ncount.as<int>();
   W.g_index = c->list_index;
   if (parts) {
     W.g_id = parts->id; W.g_x = parts->x; W.g_y = parts->y; W.g_z = parts->z;
@@ -462,7 +499,6 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.scan64 = c->scan64.as<u64>();
   W.pk = c->pk.as<int>(); W.gk = c->gk.as<int>();
   W.isb = c->isb.as<unsigned char>();
-  W.lvl = c->lvl.as<int>();
   W.active = c->active.as<int>();
   W.counters = c->counters.as<unsigned long long>();
   W.subs = subs;
@@ -478,14 +514,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   if (L.seg.empty()) return HBT_OK;
   const int64_t nseg = (int64_t)L.seg.size();
   RETCHK(ensure_slots(c, L.nslots, nseg, (int64_t)L.tiles.size()));
-  RETCHK(upload(c->seg, L.seg, c->stream));
-  RETCHK(upload(c->ibase, L.ibase, c->stream));
-  RETCHK(upload(c->capal, L.capal, c->stream));
-  RETCHK(upload(c->tiles, L.tiles, c->stream));
-  RETCHK(upload(c->first_tile, L.first_tile, c->stream));
-  RETCHK(upload(c->ntile, L.ntile, c->stream));
-  RETCHK(upload(c->child_ptr, L.child_ptr, c->stream));
-  RETCHK(upload(c->child_list, L.child_list, c->stream));
+  RETCHK(stage_level(c, L));
   Runner R;
   R.c = c;
   R.L = &L;
@@ -897,12 +926,13 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;
   ctx->last_passes = passes;
-  // evaluation count
-  std::vector<hbt_subhalo> back(nsub);
-  HIPCHK(hipMemcpy(back.data(), subs, nsub * sizeof(hbt_subhalo), hipMemcpyDeviceToHost));
-  int64_t ev = 0;
-  for (auto &r : back) ev += r.n_evals;
-  ctx->last_evals = ev;
+  // evaluation count, summed on the device
+  long long *h_ev = (long long *)(ctx->h_active + 2);
+  eval_sum_kernel<<<1, BLOCK, 0, ctx->stream>>>(subs, nsub, ctx->counters.as<unsigned long long>() + 4);
+  HIPCHK(hipMemcpyAsync(h_ev, ctx->counters.as<unsigned long long>() + 4, 8, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->last_evals = *h_ev;
   return HBT_OK;
 }
 
