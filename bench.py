@@ -118,8 +118,8 @@ def bench_c5(args):
         # tuned on the sample (same size distribution), before the timed region
         t = U.autotune(hb, reps=1)
         tuning = {"walk_group": [int(v) for v in t["walk_group"]],
-                  "bin_edge": [int(v) for v in t["bin_edge"]], "graph_passes": int(t["graph_passes"]),
-                  "xcd_run": int(t["xcd_run"])}
+                  "bin_edge": [int(v) for v in t["bin_edge"]], "use_graph": int(t["use_graph"]),
+                  "graph_passes": int(t["graph_passes"]), "xcd_run": int(t["xcd_run"])}
         print(f"[bench] autotuned: {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
@@ -223,8 +223,8 @@ def main():
         # chosen by timing before the timed region; results are identical for every choice
         t = U.autotune(h, reps=1)
         tuning = {"walk_group": [int(v) for v in t["walk_group"]],
-                  "bin_edge": [int(v) for v in t["bin_edge"]], "graph_passes": int(t["graph_passes"]),
-                  "xcd_run": int(t["xcd_run"])}
+                  "bin_edge": [int(v) for v in t["bin_edge"]], "use_graph": int(t["use_graph"]),
+                  "graph_passes": int(t["graph_passes"]), "xcd_run": int(t["xcd_run"])}
         print(f"[bench] autotuned: {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         U.unbind_device(h)

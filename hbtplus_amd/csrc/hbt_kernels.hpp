@@ -64,7 +64,8 @@ struct Work {
   u64 *scan64;
   int *pk, *gk;
   unsigned char *isb;
-  int *active;               // [1]
+  int *active;               // [4]: segments active after the pass, the largest Nlast among
+                             // them, the deepest tree depth D of the pass
   unsigned long long *counters;  // [4]: interactions, opened nodes, targets (instrumented passes)
   // caller outputs
   hbt_subhalo *subs;
@@ -181,7 +182,7 @@ __global__ void el_init_kernel(Work W) {
 // Pass: setup
 
 __global__ void pass_setup_kernel(Work W, DevParams P) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *W.active = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { W.active[0] = 0; W.active[1] = 0; W.active[2] = 0; }
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     S.flags &= ~F_FINAL;
@@ -279,6 +280,7 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
     int D = dlen < 1 ? 1 : dlen;
     if (D > MAX_DEPTH) { D = MAX_DEPTH; S.flags |= F_DEEP; }
     S.D = D;
+    atomicMax(&W.active[2], D);
     S.straddle = (P.periodic && Len > 0.5 * (double)P.box_half) ? 1 : 0;
   }
 }
@@ -1097,7 +1099,8 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   } else {
     S.sort_lo = nb;
     S.nsort = S.nlast - nb;
-    atomicAdd(W.active, 1);
+    atomicAdd(&W.active[0], 1);
+    atomicMax(&W.active[1], S.nlast);  // bounds every count of the next pass
   }
   W.ncount[NC_SORT * W.nseg + (int)(&S - W.seg)] = S.nsort;
 }

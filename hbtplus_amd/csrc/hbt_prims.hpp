@@ -149,17 +149,20 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
 }
 
 // Merge level `level` (runs of width TILE << level merged pairwise), one block
-// per tile.  A segment that needs m levels takes part in levels 0..m-1 and
-// ping-pongs between the two buffers so that level m-1 writes (k0, v0);
-// segments with m <= level are done and are not touched.
+// per BLOCK-element part of a tile, one element per thread.  A segment that
+// needs m levels takes part in levels 0..m-1 and ping-pongs between the two
+// buffers so that level m-1 writes (k0, v0); segments with m <= level are
+// done and are not touched.
 __global__ void __launch_bounds__(BLOCK)
 merge_pass_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__restrict__ base,
                   const int *__restrict__ n, int level, u64 *k0, u32 *v0, u64 *k1, u32 *v1) {
   const int width = TILE << level;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const Tile tl = tiles[t];
+  constexpr int PARTS = TILE / BLOCK;
+  for (int vb = blockIdx.x; vb < ntiles * PARTS; vb += gridDim.x) {
+    const Tile tl = tiles[vb / PARTS];
     const int cnt = n[tl.seg];
-    if (cnt <= width || tl.start >= cnt) continue;  // uniform across the block
+    const int start = tl.start + (vb % PARTS) * BLOCK;
+    if (cnt <= width || start >= cnt) continue;  // uniform across the block
     const int m = merge_levels(cnt);
     // buffer holding the input of this level: the tile sort wrote parity m&1
     const bool in1 = ((m + level) & 1) != 0;
@@ -168,8 +171,9 @@ merge_pass_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__r
     u64 *kout = in1 ? k0 : k1;
     u32 *vout = in1 ? v0 : v1;
     const int64_t b = base[tl.seg];
-    const int lend = min(tl.start + TILE, cnt);
-    for (int l = tl.start + (int)threadIdx.x; l < lend; l += BLOCK) {
+    {
+      const int l = start + (int)threadIdx.x;
+      if (l >= cnt) continue;
       const u64 key = kin[b + l];
       const u32 val = vin[b + l];
       const int run = l / width;

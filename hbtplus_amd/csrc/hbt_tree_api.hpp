@@ -55,6 +55,7 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
   R.L = &L;
   R.P = make_dev_params(ctx->params);
   R.W = make_work(ctx, L, nullptr, nullptr, nullptr, nullptr);
+  R.max_n = L.maxcap;
   R.gs = grid_for(L.nslots, ctx->cus);
   R.gseg = 1;
   R.gblk = 1;

@@ -114,7 +114,7 @@ struct hbt_ctx {
   int walk_mode = 0;  // 0: group-shared traversal, 1: independent per-lane walks
   // lanes per traversal group by size bin, graph capture of the pass sequence
   // (extra passes of a graph launch on a converged level are no-ops)
-  hbt_tuning tune = {{16, 16, 16}, {0, 1 << 30}, 2, 1, 1};
+  hbt_tuning tune = {{16, 16, 16}, {0, 1 << 30}, 2, 0, 1};
   hipEvent_t evw0 = nullptr, evw1 = nullptr;
   double walk_ms = 0;
   int64_t walk_launches = 0;
@@ -320,14 +320,20 @@ struct Runner {
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
 
+  // Bounds for the pass from the previous pass of the level (a segment's
+  // counts and tree depth never grow from pass to pass): merge levels for
+  // segments of up to max_n elements, node depths up to max_depth.
+  int max_n = 0, max_depth = MAX_DEPTH;
+
   int sort_pairs(const int *n) {  // sorts (k0,v0) by the counts n[seg], result in (k0,v0)
     const Tile *tl = c->tiles.as<Tile>();
     const int nt = (int)L->tiles.size();
+    const int gmerge = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nt * (TILE / BLOCK), (int64_t)c->cus * 16));
     tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
     // one launch per merge level of the largest segment of the level
     int level = 0;
-    for (int w = TILE; w < L->maxcap; w *= 2, level++)
-      merge_pass_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
+    for (int w = TILE; w < max_n; w *= 2, level++)
+      merge_pass_kernel<<<gmerge, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
     return HBT_OK;
   }
 
@@ -352,7 +358,7 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    for (int v = MAX_DEPTH; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
+    for (int v = max_depth; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
     return HBT_OK;
   }
 
@@ -520,6 +526,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   R.L = &L;
   R.P = make_dev_params(c->params);
   R.W = make_work(c, L, parts, subs, out_index, out_energy);
+  R.max_n = L.maxcap;
   R.gs = grid_for(L.nslots, c->cus);
   R.gseg = grid_for(nseg, c->cus);
   R.gtile = (int)std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)c->cus * 8);
@@ -606,8 +613,11 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
       for (size_t i = 0; i < el.size(); i++) fprintf(stderr, " %d", el[i]);
       fprintf(stderr, "\n");
     }
-    HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    // the next pass is bounded by this one
+    R.max_n = c->h_active[1];
+    R.max_depth = std::min(MAX_DEPTH, std::max(0, c->h_active[2]));
     if (c->prof_mode == 1) {
       float wms = 0;
       HIPCHK(hipEventElapsedTime(&wms, c->evw0, c->evw1));
@@ -693,7 +703,7 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   if (const char *g = getenv("HBT_GRAPH_PASSES")) c->tune.graph_passes = std::max(1, atoi(g));
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 8, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipEventCreate(&c->evw0);
@@ -841,8 +851,14 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
     t.xcd_run = runs[i];
     rc = trial(t, false);
   }
-  // 4. passes per graph launch
-  for (int gp = 1; gp <= 4 && rc == HBT_OK; gp++) {
+  // 4. direct launches (each pass bounded by the previous one) or a HIP graph
+  //    of 1-4 passes per launch
+  if (rc == HBT_OK) {
+    hbt_tuning t = best;
+    t.use_graph = 1 - best.use_graph;
+    rc = trial(t, false);
+  }
+  for (int gp = 1; gp <= 4 && rc == HBT_OK && best.use_graph; gp++) {
     if (gp == best.graph_passes) continue;
     hbt_tuning t = best;
     t.graph_passes = gp;
@@ -927,7 +943,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   ctx->last_ms = ms;
   ctx->last_passes = passes;
   // evaluation count, summed on the device
-  long long *h_ev = (long long *)(ctx->h_active + 2);
+  long long *h_ev = (long long *)(ctx->h_active + 4);
   eval_sum_kernel<<<1, BLOCK, 0, ctx->stream>>>(subs, nsub, ctx->counters.as<unsigned long long>() + 4);
   HIPCHK(hipMemcpyAsync(h_ev, ctx->counters.as<unsigned long long>() + 4, 8, hipMemcpyDeviceToHost,
                         ctx->stream));

@@ -233,7 +233,9 @@ int hbt_last_batch_stats(hbt_ctx *ctx, double *device_ms, int64_t *n_evals,
  *                  < bin_edge[1] (b = 1), otherwise b = 2
  *   graph_passes : unbinding passes per HIP-graph launch (>= 1)
  *   use_graph    : 1 = replay each level's pass sequence as a captured HIP graph
- *                  (batches of more than one subhalo), 0 = direct launches
+ *                  (batches of more than one subhalo), 0 = direct launches, each
+ *                  pass sized by the previous one (merge levels for its largest
+ *                  active subhalo, node depths of its deepest tree)
  *   xcd_run      : the walk's workgroups reach each of the 8 XCDs in runs of
  *                  this many consecutive slot blocks (1, 2, 4, ... 64; 1 = the
  *                  hardware's round-robin order) */
@@ -251,8 +253,8 @@ int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *tuning);
 
 /* Autotuner: times hbt_unbind_batch (same arguments) under candidate tunings
  * -- one group size for all bins, then a separate group for the small-subhalo
- * bin at two bin edges, then the XCD run length, then the passes per graph
- * launch -- keeping at each
+ * bin at two bin edges, then the XCD run length, then direct launches or a
+ * graph and the passes per graph launch -- keeping at each
  * step the setting with the smallest device time over `reps` runs, sets the
  * winner on the context and returns it in *chosen.  The outputs of the last
  * run are left in subs/out_index/out_energy (identical for every tuning). */
