@@ -30,6 +30,20 @@ sys.path.insert(0, ROOT)
 METRIC = "subhalo-particle energy evals/s: batched mixed-prec vs plain GPU, same bound sets"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+LANE_SLOTS_PER_S = 256 * 4 * 16 * 2.4e9
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_c2_traffic.md")
+
+
+def walk_traffic(config, launches):
+    """HBM bytes per walk launch for the default workload, from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE passes of rocprofv3, each
+    scaled by its reading for a copy of known size; see that file)."""
+    if config != "c2" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def make_workload(name):
@@ -55,12 +69,15 @@ def make_workload(name):
     return cat, desc
 
 
-def cpu_baseline(cat, budget_s=20.0):
+def cpu_baseline(cat, budget_s=20.0, threads=None, with_reference=False):
     """The oracle (CPU restatement of the reference, OpenMP over subhalos) on
     a bounded sample: subhalos taken in a fixed random order until the
-    estimated work fills about budget_s seconds."""
+    estimated work fills about budget_s seconds.  with_reference: also time
+    the reference itself (oracle/_ref, built from its sources where they are
+    present) on the same sample and threads; the evaluation count is the
+    oracle's (the two follow the same trajectory bit for bit)."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads = threads or min(16, os.cpu_count() or 1)
     rng = np.random.default_rng(0)
     order = rng.permutation(len(cat.subs))
     top = cat.subs["parent"] < 0
@@ -76,9 +93,17 @@ def cpu_baseline(cat, budget_s=20.0):
         if elapsed > budget_s / 4 or take >= len(order):
             break
         take = min(len(order), int(take * max(2.0, budget_s / 4 / max(elapsed, 1e-3))))
-    return dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
-                sample=f"{take} of {len(order)} top-level subhalos ({sub.n_particles} particles, "
-                       f"{evals} evals) in {elapsed:.1f} s, OpenMP schedule(dynamic,1)")
+    sample = (f"{take} of {len(order)} top-level subhalos ({sub.n_particles} particles, {evals} evals), "
+              f"OpenMP over nests, schedule(dynamic,1)")
+    port = dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
+                sample=f"{sample}; {elapsed:.1f} s")
+    if not (with_reference and oracle.have_reference()):
+        return port
+    # the reference's own unbinding, compiled from its sources, on the same sample
+    secs = oracle.ref_time_unbind(sub, threads)
+    return dict(value=evals / secs, unit="evals/s", cores=threads, kind="reference",
+                sample=f"{sample}; {secs:.1f} s (evaluation count from the port, same trajectory)",
+                port=port)
 
 
 def bench_c5(args):
@@ -181,10 +206,18 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--replay", action="store_true", help="REPLAY precision instead of FAST")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default execution tuning")
+    ap.add_argument("--cpu-only", type=int, default=0, metavar="THREADS",
+                    help="no GPU: time the oracle and, where it is built, the reference on the "
+                         "CPU sample with THREADS threads (cross-calibration)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times (a kernel of known "
                          "traffic that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.cpu_only:
+        cat, desc = make_workload(args.config)
+        print(json.dumps({"config": args.config, "workload": desc,
+                          "cpu": cpu_baseline(cat, threads=args.cpu_only, with_reference=True)}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the unbinding of one rank's share runs on one MI355X; "
               "multi-GPU runs are not part of this build", file=sys.stderr)
@@ -254,12 +287,23 @@ def main():
     U.profile(0)
     I, O, T = prof_c["interactions"], prof_c["opened"], prof_c["targets"]
     lane_eff = (I + O) / max(prof_c["lane_slots"], 1)
+    # op-mix ceiling: FP32 VALU issue slots (one per lane per instruction,
+    # 3.93e13 lane-slots/s on 256 CUs x 4 SIMD16 at 2.4 GHz; an FMA counts 2
+    # FLOPs, so 78.6 TFLOP/s is the non-packed FMA peak) for the algorithmic
+    # mix: a summed source costs 3 sub + 3 (r^2) + 4 (rsq, quarter rate) + 1 mul
+    # + 1 add = 12 slots for 11 FLOPs, an opened node 3 + 3 + 1 mul + 1 cmp =
+    # 8 slots for 10 FLOPs, an evaluation 20 slots for 20 FLOPs
     flops = 11.0 * I + 10.0 * O + 20.0 * T
+    slots = 12.0 * I + 8.0 * O + 20.0 * T
+    op_mix_ceiling = LANE_SLOTS_PER_S * flops / slots / 1e12 if slots > 0 else 0.0
     walk_s = prof_t["walk_ms"] / 1e3
     launches = max(prof_t["walk_launches"], 1)
     achieved = flops / walk_s / 1e12 if walk_s > 0 else 0.0
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
+                    frac=round(achieved / PEAK_FP32_TFLOPS, 4),
+                    traffic=walk_traffic(args.config, launches),
+                    op_mix_ceiling=round(op_mix_ceiling, 2),
+                    frac_of_op_mix_ceiling=round(achieved / op_mix_ceiling, 4) if op_mix_ceiling else None,
                     kernel="walk_kernel", launches_per_step=launches,
                     avg_launch_ms=round(prof_t["walk_ms"] / launches, 4),
                     flops_per_step=flops, interactions_per_eval=round(I / max(T, 1), 1),
@@ -293,7 +337,7 @@ def main():
         print(f"[bench] plain: {pt * 1e3:.1f} ms, identical outputs {same}", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(cat)
+        result["cpu_baseline"] = cpu_baseline(cat, with_reference=True)
         print(f"[bench] cpu: {result['cpu_baseline']}", file=sys.stderr, flush=True)
     U.close()
     print(json.dumps(result), flush=True)

@@ -65,7 +65,7 @@ struct Work {
   int *pk, *gk;
   unsigned char *isb;
   int *active;               // [4]: segments active after the pass, the largest Nlast among
-                             // them, the deepest tree depth D of the pass
+                             // them, a bound on the tree depth D of the next pass
   unsigned long long *counters;  // [4]: interactions, opened nodes, targets (instrumented passes)
   // caller outputs
   hbt_subhalo *subs;
@@ -266,7 +266,10 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
   SEG_LOOP(k, W) {
 #pragma clang fp contract(off)
     Seg &S = W.seg[k];
-    if (S.n_src <= 0) continue;
+    if (S.n_src <= 0) {
+      if (S.flags & F_ACTIVE) atomicMax(&W.active[2], S.Dfull);
+      continue;
+    }
     double xmin[3], xmax[3];
     for (int j = 0; j < 3; j++) { xmin[j] = ord2f(S.bmin[j]); xmax[j] = ord2f(S.bmax[j]); }
     double Len = xmax[0] - xmin[0];
@@ -280,7 +283,11 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
     int D = dlen < 1 ? 1 : dlen;
     if (D > MAX_DEPTH) { D = MAX_DEPTH; S.flags |= F_DEEP; }
     S.D = D;
-    atomicMax(&W.active[2], D);
+    // a correction pass builds its tree over the block just removed, which
+    // lies inside the latest full pass's sources but not inside the previous
+    // correction pass's: the latest full pass bounds the depth of later passes
+    if (!(S.flags & F_CORR)) S.Dfull = D;
+    atomicMax(&W.active[2], S.Dfull > D ? S.Dfull : D);
     S.straddle = (P.periodic && Len > 0.5 * (double)P.box_half) ? 1 : 0;
   }
 }

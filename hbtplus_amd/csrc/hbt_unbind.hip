@@ -320,9 +320,11 @@ struct Runner {
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
 
-  // Bounds for the pass from the previous pass of the level (a segment's
-  // counts and tree depth never grow from pass to pass): merge levels for
-  // segments of up to max_n elements, node depths up to max_depth.
+  // Bounds for the pass from the previous pass of the level: merge levels for
+  // segments of up to max_n elements (no count of a pass exceeds the Nlast of
+  // the pass before), node depths up to max_depth (the tree depth of a pass is
+  // at most that of the latest full pass of its segment: every later source
+  // set lies inside that pass's sources).
   int max_n = 0, max_depth = MAX_DEPTH;
 
   int sort_pairs(const int *n) {  // sorts (k0,v0) by the counts n[seg], result in (k0,v0)

@@ -117,22 +117,39 @@ def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+def _write_catalogue(path, cat, params):
+    cols = cat.cols
+    n = len(cols["x"])
+    with open(path, "wb") as f:
+        f.write(np.ascontiguousarray(params, PARAMS_DTYPE).tobytes())
+        f.write(np.array([len(cat.subs), n], np.int64).tobytes())
+        f.write(np.ascontiguousarray(cols["id"], np.int64).tobytes())
+        for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):
+            f.write(_f32(cols[k]).tobytes())
+        f.write(np.ascontiguousarray(cols["type"], np.int32).tobytes())
+        f.write(np.ascontiguousarray(cat.subs, SUB_DTYPE).tobytes())
+
+
+def ref_time_unbind(cat, nthreads, params=None):
+    """Wall seconds of the reference's RecursiveUnbind over the catalogue's
+    nests, OpenMP over nests with schedule(dynamic,1) as RefineParticles runs
+    them (timed inside oracle/_ref/hbt_ref_harness, input/output excluded)."""
+    params = cat.params if params is None else params
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        _write_catalogue(fin, cat, params)
+        env = dict(os.environ, OMP_NUM_THREADS=str(int(nthreads)))
+        subprocess.run([REF_HARNESS, "time", fin, fout], check=True, env=env)
+        return float(np.fromfile(fout, np.float64, 1)[0])
+
+
 def ref_unbind_batch(cat, params=None):
     """Run the reference's own RecursiveUnbind/Unbind/TruncateSource on the
     catalogue through oracle/_ref/hbt_ref_harness (single thread)."""
     params = cat.params if params is None else params
-    cols = cat.cols
-    n = len(cols["x"])
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
-        with open(fin, "wb") as f:
-            f.write(np.ascontiguousarray(params, PARAMS_DTYPE).tobytes())
-            f.write(np.array([len(cat.subs), n], np.int64).tobytes())
-            f.write(np.ascontiguousarray(cols["id"], np.int64).tobytes())
-            for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):
-                f.write(_f32(cols[k]).tobytes())
-            f.write(np.ascontiguousarray(cols["type"], np.int32).tobytes())
-            f.write(np.ascontiguousarray(cat.subs, SUB_DTYPE).tobytes())
+        _write_catalogue(fin, cat, params)
         env = dict(os.environ, OMP_NUM_THREADS="1")
         subprocess.run([REF_HARNESS, "unbind", fin, fout], check=True, env=env)
         raw = open(fout, "rb").read()

@@ -16,6 +16,8 @@
 // then TruncateSource on every subhalo.  Run with OMP_NUM_THREADS=1.
 // `tree` builds GravityTree_t over the sources and calls EvaluatePotential.
 
+#include <omp.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -82,7 +84,7 @@ void wr(FILE *f, const T *p, size_t n) {
   if (n && fwrite(p, sizeof(T), n, f) != n) { fprintf(stderr, "short write\n"); exit(2); }
 }
 
-int run_unbind(FILE *in, FILE *out) {
+int run_unbind(FILE *in, FILE *out, bool timed) {
   hbt_params p;
   int64_t nsub, npart;
   rd(in, &p, 1); rd(in, &nsub, 1); rd(in, &npart, 1);
@@ -132,6 +134,17 @@ int run_unbind(FILE *in, FILE *out) {
     for (int j = 0; j < 3; j++) S.SpecificAngularMomentum[j] = r.specific_angular_momentum[j];
     if (r.parent >= 0) subs[r.parent].NestedSubhalos.push_back(s);
     else roots.push_back(s);
+  }
+  if (timed) {
+    // RefineParticles' halo-parallel layout (src/subhalo_unbind.cpp:479-513):
+    // the nests are independent, one per OpenMP iteration, dynamic schedule
+    const int64_t nroots = (int64_t)roots.size();
+    const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < nroots; i++) subs[roots[i]].RecursiveUnbind(subs, epoch);
+    const double secs = omp_get_wtime() - t0;
+    wr(out, &secs, 1);
+    return 0;
   }
   for (int64_t s : roots) subs[s].RecursiveUnbind(subs, epoch);
 
@@ -204,12 +217,13 @@ int run_tree(FILE *in, FILE *out) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s unbind|tree in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|tree in.bin out.bin\n", argv[0]); return 1; }
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
   if (!in || !out) { fprintf(stderr, "cannot open files\n"); return 1; }
   int rc = 1;
-  if (!strcmp(argv[1], "unbind")) rc = run_unbind(in, out);
+  if (!strcmp(argv[1], "unbind")) rc = run_unbind(in, out, false);
+  else if (!strcmp(argv[1], "time")) rc = run_unbind(in, out, true);
   else if (!strcmp(argv[1], "tree")) rc = run_tree(in, out);
   fclose(in);
   fclose(out);
