@@ -304,7 +304,7 @@ def main():
                     traffic=walk_traffic(args.config, launches),
                     op_mix_ceiling=round(op_mix_ceiling, 2),
                     frac_of_op_mix_ceiling=round(achieved / op_mix_ceiling, 4) if op_mix_ceiling else None,
-                    kernel="walk_kernel", launches_per_step=launches,
+                    kernel="walk_shared_kernel", launches_per_step=launches,
                     avg_launch_ms=round(prof_t["walk_ms"] / launches, 4),
                     flops_per_step=flops, interactions_per_eval=round(I / max(T, 1), 1),
                     opened_per_eval=round(O / max(T, 1), 1),

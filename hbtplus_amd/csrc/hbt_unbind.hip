@@ -114,7 +114,7 @@ struct hbt_ctx {
   int walk_mode = 0;  // 0: group-shared traversal, 1: independent per-lane walks
   // lanes per traversal group by size bin, graph capture of the pass sequence
   // (extra passes of a graph launch on a converged level are no-ops)
-  hbt_tuning tune = {{16, 16, 16}, {0, 1 << 30}, 2, 0, 1};
+  hbt_tuning tune = {{8, 8, 8}, {0, 1 << 30}, 2, 0, 16};
   hipEvent_t evw0 = nullptr, evw1 = nullptr;
   double walk_ms = 0;
   int64_t walk_launches = 0;
@@ -703,6 +703,10 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   }
   if (const char *g = getenv("HBT_GRAPH")) c->tune.use_graph = atoi(g) ? 1 : 0;
   if (const char *g = getenv("HBT_GRAPH_PASSES")) c->tune.graph_passes = std::max(1, atoi(g));
+  if (const char *g = getenv("HBT_XCD_RUN")) {
+    const int r = atoi(g);
+    c->tune.xcd_run = (r >= 1 && r <= 64 && (r & (r - 1)) == 0) ? r : 1;
+  }
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_active, sizeof(int) * 8, hipHostMallocDefault);
@@ -821,7 +825,8 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
     }
     return HBT_OK;
   };
-  int rc = HBT_OK;
+  // 0. the current setting is the one to beat
+  int rc = trial(start, true);
   // 1. one group size for every bin
   const int groups[4] = {64, 32, 16, 8};
   for (int i = 0; i < 4 && rc == HBT_OK; i++) {
@@ -829,7 +834,8 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
     for (int b = 0; b < 3; b++) t.walk_group[b] = groups[i];
     t.bin_edge[0] = 0;
     t.bin_edge[1] = 1 << 30;
-    rc = trial(t, i == 0);
+    if (!memcmp(&t, &best, sizeof t)) continue;
+    rc = trial(t, false);
   }
   // 2. a separate group size for small subhalos
   const int edges[2] = {256, 2048};
@@ -846,7 +852,7 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
     }
   }
   // 3. XCD runs of the walk's blocks
-  const int runs[3] = {1, 4, 16};
+  const int runs[3] = {16, 4, 1};
   for (int i = 0; i < 3 && rc == HBT_OK; i++) {
     if (runs[i] == best.xcd_run) continue;
     hbt_tuning t = best;

@@ -251,8 +251,8 @@ int hbt_get_tuning(hbt_ctx *ctx, hbt_tuning *out);
 /* Validates and sets; HBT_ERR_INVALID_ARG for an unsupported value. */
 int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *tuning);
 
-/* Autotuner: times hbt_unbind_batch (same arguments) under candidate tunings
- * -- one group size for all bins, then a separate group for the small-subhalo
+/* Autotuner: times hbt_unbind_batch (same arguments) under the context's
+ * current tuning and then candidate tunings -- one group size for all bins, then a separate group for the small-subhalo
  * bin at two bin edges, then the XCD run length, then direct launches or a
  * graph and the passes per graph launch -- keeping at each
  * step the setting with the smallest device time over `reps` runs, sets the
