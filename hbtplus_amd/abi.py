@@ -44,6 +44,15 @@ SUB_DTYPE = np.dtype([
 assert PARAMS_DTYPE.itemsize == 64, PARAMS_DTYPE.itemsize
 assert SUB_DTYPE.itemsize == 240, SUB_DTYPE.itemsize
 
+# hbt_properties (80 bytes): CalculateProfileProperties + CalculateShape outputs
+PROPS_DTYPE = np.dtype([("rmax_comoving", "<f4"), ("vmax_physical", "<f4"), ("rhalf_comoving", "<f4"),
+                        ("r2sigma_comoving", "<f4"), ("bound_r200crit_comoving", "<f4"),
+                        ("bound_m200crit", "<f4"), ("last_max_vmax_physical", "<f4"),
+                        ("snapshot_index_of_last_max_vmax", "<i4"), ("inertial_tensor", "<f4", (6,)),
+                        ("inertial_tensor_weighted", "<f4", (6,))])
+assert PROPS_DTYPE.itemsize == 80
+PROPS_FIELDS = PROPS_DTYPE.names
+
 # hbt_tuning (32 bytes): walk group sizes per size bin, bin edges, graph settings, XCD run length
 TUNING_DTYPE = np.dtype([("walk_group", "<i4", (3,)), ("bin_edge", "<i4", (2,)),
                          ("graph_passes", "<i4"), ("use_graph", "<i4"), ("xcd_run", "<i4")])
@@ -126,6 +135,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_last_batch_stats": ([vp, vp, vp, vp], i32),
         "hbt_profile": ([vp, i32], i32),
         "hbt_profile_read": ([vp, vp, vp, vp, vp, vp, vp], i32),
+        "hbt_subhalo_properties": ([vp, vp, vp, i64, vp, vp], i32),
         "hbt_get_tuning": ([vp, vp], i32),
         "hbt_set_tuning": ([vp, vp], i32),
         "hbt_autotune": ([vp, vp, vp, vp, i64, vp, vp, i32, vp], i32),

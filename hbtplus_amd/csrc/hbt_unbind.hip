@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -198,22 +199,24 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
   L.nslots = off;
 }
 
-// Copies the level's tables to the device: packed into the pinned staging
-// buffer, one asynchronous copy.  The stream is drained first, so no earlier
-// copy still reads the staging buffer (normally it is idle already: each level
-// ends with a stream synchronisation).
-int stage_level(hbt_ctx *c, const Level &L) {
+// Copies host tables to the device: packed (256-byte aligned) into the pinned
+// staging buffer, one asynchronous copy into lvlbuf; dev[i] receives the
+// device address of blob i.  The stream is drained first, so no earlier copy
+// still reads the staging buffer (normally it is idle already: every level
+// and every entry point ends with a stream synchronisation).
+struct Blob {
+  const void *p;
+  size_t bytes;
+};
+
+int stage_blobs(hbt_ctx *c, std::initializer_list<Blob> blobs, void **dev) {
   HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<size_t> at;
   size_t off = 0;
-  auto place = [&](size_t bytes) {
-    const size_t o = off;
-    off += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
-    return o;
-  };
-  const size_t n = L.seg.size();
-  const size_t o_seg = place(n * sizeof(Seg)), o_ib = place(n * 8), o_cap = place(n * 4),
-               o_tl = place(L.tiles.size() * sizeof(Tile)), o_ft = place(n * 4), o_nt = place(n * 4),
-               o_cp = place(L.child_ptr.size() * 4), o_cl = place(L.child_list.size() * 4);
+  for (const Blob &b : blobs) {
+    at.push_back(off);
+    off += (std::max<size_t>(b.bytes, 1) + 255) & ~(size_t)255;
+  }
   if (off > c->h_stage_bytes) {
     if (c->h_stage) HIPCHK(hipHostFree(c->h_stage));
     c->h_stage = nullptr;
@@ -222,20 +225,33 @@ int stage_level(hbt_ctx *c, const Level &L) {
     c->h_stage_bytes = off;
   }
   RETCHK(c->lvlbuf.ensure(off));
-  char *h = (char *)c->h_stage;
-  auto put = [&](size_t o, const void *src, size_t bytes) { if (bytes) memcpy(h + o, src, bytes); };
-  put(o_seg, L.seg.data(), n * sizeof(Seg));
-  put(o_ib, L.ibase.data(), n * 8);
-  put(o_cap, L.capal.data(), n * 4);
-  put(o_tl, L.tiles.data(), L.tiles.size() * sizeof(Tile));
-  put(o_ft, L.first_tile.data(), n * 4);
-  put(o_nt, L.ntile.data(), n * 4);
-  put(o_cp, L.child_ptr.data(), L.child_ptr.size() * 4);
-  put(o_cl, L.child_list.data(), L.child_list.size() * 4);
+  char *h = (char *)c->h_stage, *d = (char *)c->lvlbuf.p;
+  size_t i = 0;
+  for (const Blob &b : blobs) {
+    if (b.bytes) memcpy(h + at[i], b.p, b.bytes);
+    dev[i] = d + at[i];
+    i++;
+  }
   HIPCHK(hipMemcpyAsync(c->lvlbuf.p, c->h_stage, off, hipMemcpyHostToDevice, c->stream));
-  char *d = (char *)c->lvlbuf.p;
-  c->seg.p = d + o_seg; c->ibase.p = d + o_ib; c->capal.p = d + o_cap; c->tiles.p = d + o_tl;
-  c->first_tile.p = d + o_ft; c->ntile.p = d + o_nt; c->child_ptr.p = d + o_cp; c->child_list.p = d + o_cl;
+  return HBT_OK;
+}
+
+// The level's tables: segment records, slot bases, aligned capacities, tile
+// table, first tile and tile count per segment, nest children (CSR).
+int stage_level(hbt_ctx *c, const Level &L) {
+  const size_t n = L.seg.size();
+  void *d[8];
+  RETCHK(stage_blobs(c, {{L.seg.data(), n * sizeof(Seg)},
+                         {L.ibase.data(), n * 8},
+                         {L.capal.data(), n * 4},
+                         {L.tiles.data(), L.tiles.size() * sizeof(Tile)},
+                         {L.first_tile.data(), n * 4},
+                         {L.ntile.data(), n * 4},
+                         {L.child_ptr.data(), L.child_ptr.size() * 4},
+                         {L.child_list.data(), L.child_list.size() * 4}},
+                     d));
+  c->seg.p = d[0]; c->ibase.p = d[1]; c->capal.p = d[2]; c->tiles.p = d[3];
+  c->first_tile.p = d[4]; c->ntile.p = d[5]; c->child_ptr.p = d[6]; c->child_list.p = d[7];
   return HBT_OK;
 }
 
@@ -1055,5 +1071,6 @@ int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *su
 }  // extern "C"
 
 #include "hbt_tree_api.hpp"
+#include "hbt_props.hpp"
 #include "hbt_direct.hpp"
 #include "hbt_idhash.hpp"

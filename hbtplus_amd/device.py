@@ -93,6 +93,32 @@ class GpuUnbinder:
         total = h["total"]
         return subs, out_i.cpu().numpy()[:total].copy(), out_e.cpu().numpy()[:total].copy()
 
+    # post-unbind properties -------------------------------------------------
+    def properties_device(self, h, out, props_in=None):
+        """CalculateProfileProperties + CalculateShape on the outputs of
+        unbind_device (out = (records, out_index, out_energy), device); props_in:
+        numpy array of PROPS_DTYPE (in/out fields) or None.  Returns numpy."""
+        t = self.torch
+        subs_dev, out_i, _ = out
+        n = len(h["subs_host"])
+        pin = (np.zeros(n, abi.PROPS_DTYPE) if props_in is None
+               else np.ascontiguousarray(props_in, abi.PROPS_DTYPE))
+        pd = self._dev(pin.view(np.uint8))
+        t.cuda.synchronize(self.device)
+        rc = self.lib.hbt_subhalo_properties(self.ctx, ctypes.byref(h["ps"]), subs_dev.data_ptr(), n,
+                                             out_i.data_ptr(), pd.data_ptr())
+        check(rc, "hbt_subhalo_properties")
+        return pd.cpu().numpy().view(abi.PROPS_DTYPE).copy()
+
+    def unbind_properties(self, cat, props_in=None, plain=False):
+        """unbind_batch followed by the properties: (subs, out_index, out_energy, props)."""
+        h = self.upload(cat)
+        out = self.unbind_device(h, plain=plain)
+        props = self.properties_device(h, out, props_in)
+        subs = out[0].cpu().numpy().view(SUB_DTYPE).copy()
+        total = h["total"]
+        return subs, out[1].cpu().numpy()[:total].copy(), out[2].cpu().numpy()[:total].copy(), props
+
     # execution tuning (results do not depend on it) -----------------------
     def get_tuning(self):
         t = np.zeros(1, abi.TUNING_DTYPE)

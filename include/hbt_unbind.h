@@ -215,6 +215,31 @@ int hbt_direct_potential(hbt_ctx *ctx, int64_t nseg, const int64_t *seg_offset,
                          const float *x, const float *y, const float *z,
                          const float *mass, int mode, double *out_phi);
 
+/* Post-unbind properties of one subhalo: the Subhalo_t fields set by
+ * CalculateProfileProperties and CalculateShape (src/subhalo.h:48-77,
+ * src/subhalo.cpp:242-398).  Fields marked in/out keep their input value
+ * where the reference leaves them unchanged. */
+typedef struct hbt_properties {
+  float   rmax_comoving;            /* RmaxComoving */
+  float   vmax_physical;            /* VmaxPhysical */
+  float   rhalf_comoving;           /* RHalfComoving */
+  float   r2sigma_comoving;         /* R2SigmaComoving (95.5% containment) */
+  float   bound_r200crit_comoving;  /* BoundR200CritComoving (in/out: kept when no radius qualifies) */
+  float   bound_m200crit;           /* BoundM200Crit (in/out, as above) */
+  float   last_max_vmax_physical;   /* LastMaxVmaxPhysical (in/out) */
+  int32_t snapshot_index_of_last_max_vmax;  /* SnapshotIndexOfLastMaxVmax (in/out) */
+  float   inertial_tensor[6];       /* InertialTensor {xx, xy, xz, yy, yz, zz} / Mbound */
+  float   inertial_tensor_weighted[6];  /* InertialTensorWeighted */
+} hbt_properties;
+
+/* CalculateProfileProperties + CalculateShape (src/subhalo.cpp:242-398) for
+ * nsub subhalos after hbt_unbind_batch: subhalo i's bound particles are
+ * out_index[subs[i].out_offset .. + subs[i].nbound) (energy order, most bound
+ * first), indices into the particle columns; the centre is mostbound_pos.
+ * particles, subs, out_index, props: device pointers; props is in/out. */
+int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particles, const hbt_subhalo *subs,
+                           int64_t nsub, const int64_t *out_index, hbt_properties *props);
+
 /* ID -> index resolution: builds a device hash table of the n_table IDs
  * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs
  * and negative IDs give -1 (src/hash.tpp:52).  Device pointers. */

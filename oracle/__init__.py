@@ -13,7 +13,7 @@ import tempfile
 
 import numpy as np
 
-from hbtplus_amd.abi import PARAMS_DTYPE, SUB_DTYPE, particles_struct, plan_batch
+from hbtplus_amd.abi import PARAMS_DTYPE, PROPS_DTYPE, SUB_DTYPE, particles_struct, plan_batch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -40,6 +40,7 @@ def lib():
         L.oracle_tree_ncell.restype = i64
         L.oracle_direct_potential.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, i32]
         L.oracle_id_to_index.argtypes = [i64, vp, i64, vp, vp]
+        L.oracle_subhalo_properties.argtypes = [vp, vp, vp, i64, vp, vp, i32]
         L.oracle_abi_sizes.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
@@ -97,6 +98,22 @@ def direct_potential(params, seg_offset, x, y, z, m, mode, nthreads=1):
     return out
 
 
+def subhalo_properties(params, cols, subs, out_index, props_in=None, nthreads=1):
+    """CalculateProfileProperties + CalculateShape restated, from unbinding
+    outputs (records and lists); props_in holds the in/out fields."""
+    c = {k: np.ascontiguousarray(v) for k, v in cols.items()}
+    ps = particles_struct(c)
+    subs = np.ascontiguousarray(subs, SUB_DTYPE)
+    oi = np.ascontiguousarray(out_index, np.int64)
+    props = (np.zeros(len(subs), PROPS_DTYPE) if props_in is None
+             else np.ascontiguousarray(props_in, PROPS_DTYPE).copy())
+    rc = lib().oracle_subhalo_properties(_p(params), ctypes.byref(ps), _p(subs), len(subs), _p(oi),
+                                         _p(props), nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle_subhalo_properties failed")
+    return props
+
+
 def id_to_index(table_ids, query_ids):
     t = np.ascontiguousarray(table_ids, np.int64)
     q = np.ascontiguousarray(query_ids, np.int64)
@@ -143,15 +160,22 @@ def ref_time_unbind(cat, nthreads, params=None):
         return float(np.fromfile(fout, np.float64, 1)[0])
 
 
-def ref_unbind_batch(cat, params=None):
+def ref_unbind_batch(cat, params=None, props_in=None):
     """Run the reference's own RecursiveUnbind/Unbind/TruncateSource on the
-    catalogue through oracle/_ref/hbt_ref_harness (single thread)."""
+    catalogue through oracle/_ref/hbt_ref_harness (single thread).  With
+    props_in (an array of PROPS_DTYPE): also CalculateProfileProperties and
+    CalculateShape on every subhalo after unbinding; returns the properties
+    as a fourth element."""
     params = cat.params if params is None else params
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         _write_catalogue(fin, cat, params)
+        if props_in is not None:
+            with open(fin, "ab") as f:
+                f.write(np.ascontiguousarray(props_in, PROPS_DTYPE).tobytes())
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        subprocess.run([REF_HARNESS, "unbind", fin, fout], check=True, env=env)
+        mode = "unbind" if props_in is None else "props"
+        subprocess.run([REF_HARNESS, mode, fin, fout], check=True, env=env)
         raw = open(fout, "rb").read()
     nsub = int(np.frombuffer(raw, np.int64, 1)[0])
     pos = 16
@@ -171,6 +195,9 @@ def ref_unbind_batch(cat, params=None):
         pos += 8 * L
         out_energy[o:o + L] = np.frombuffer(raw, np.float32, L, pos)
         pos += 4 * L
+    if props_in is not None:
+        props = np.frombuffer(raw, PROPS_DTYPE, nsub, pos).copy()
+        return subs, out_index, out_energy, props
     return subs, out_index, out_energy
 
 

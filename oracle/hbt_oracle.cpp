@@ -505,9 +505,116 @@ void recursive_unbind(Batch &B, Int s, Drand48 &rng, Tree &tree) {
   }
 }
 
+// ---- post-unbind properties (src/subhalo.cpp:242-398) ------------------------
+// PeriodicDistance, src/config_parser.h:143-156: float differences, NEAREST,
+// float sqrt of the float sum of squares.
+inline float periodic_distance(const Params &P, const float c[3], float x, float y, float z) {
+  float d[3] = {c[0] - x, c[1] - y, c[2] - z};
+  if (P.periodic)
+    for (int j = 0; j < 3; j++) d[j] = P.nearest(d[j]);
+  return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+struct RadMassVel { float r, m, v; };
+
+// Subhalo_t::CalculateProfileProperties (src/subhalo.cpp:242-333) with the
+// spherical-overdensity size of Snapshot_t (src/snapshot.cpp:259-280) at the
+// c200 factor (src/snapshot.cpp:328-336).
+void profile_properties(const Params &P, const PartCols &C, const hbt_subhalo &s, const Int *list,
+                        hbt_properties &o) {
+  const Int nb = s.nbound;
+  if (nb <= 1) {
+    o.rmax_comoving = 0.f; o.vmax_physical = 0.f; o.r2sigma_comoving = 0.f; o.rhalf_comoving = 0.f;
+    o.bound_r200crit_comoving = 0.f; o.bound_m200crit = 0.f;
+    return;
+  }
+  const float vel_unit = P.G / P.a;
+  std::vector<RadMassVel> prof(nb);
+  for (Int i = 0; i < nb; i++) {
+    const Int g = list[i];
+    prof[i].r = periodic_distance(P, s.mostbound_pos, C.p->x[g], C.p->y[g], C.p->z[g]);
+    prof[i].m = C.p->mass[g];
+  }
+  std::sort(prof.begin(), prof.end(), [](const RadMassVel &a, const RadMassVel &b) { return a.r < b.r; });
+  double m_cum = 0.;
+  for (auto &p : prof) p.m = (float)(m_cum += p.m);
+  for (auto &p : prof) {
+    if (p.r < P.eps) p.r = P.eps;  // SofteningHalo
+    p.v = p.m / p.r;
+  }
+  auto maxprof = std::max_element(prof.begin(), prof.end(),
+                                  [](const RadMassVel &a, const RadMassVel &b) { return a.v < b.v; });
+  o.rmax_comoving = maxprof->r;
+  o.vmax_physical = std::sqrt(maxprof->v * vel_unit);
+  o.rhalf_comoving = prof[nb / 2].r;
+  o.r2sigma_comoving = prof[(Int)(nb * 0.955)].r;
+  const float virial_c200 = 200.f;
+  const float rho = (float)((double)(virial_c200 * P.hz * P.hz) / 2.0 / (double)P.G * (double)P.a *
+                            (double)P.a * (double)P.a);
+  for (auto p = prof.rbegin(); p != prof.rend(); ++p) {
+    if (p->m > rho * p->r * p->r * p->r) {
+      o.bound_m200crit = p->m;
+      o.bound_r200crit_comoving = (float)std::pow((double)(p->m / rho), 1.0 / 3);
+      break;
+    }
+  }
+  if (o.vmax_physical >= o.last_max_vmax_physical) {
+    o.snapshot_index_of_last_max_vmax = P.snap;
+    o.last_max_vmax_physical = o.vmax_physical;
+  }
+}
+
+// Subhalo_t::CalculateShape (src/subhalo.cpp:337-398), serial order (the
+// reference's OpenMP reduction with one thread), without the GSL eigenvectors.
+void shape(const Params &P, const PartCols &C, const hbt_subhalo &s, const Int *list, hbt_properties &o) {
+  const Int nb = s.nbound;
+  if (nb <= 1) {
+    for (int j = 0; j < 6; j++) { o.inertial_tensor[j] = 0.f; o.inertial_tensor_weighted[j] = 0.f; }
+    return;
+  }
+  const float *c = s.mostbound_pos;
+  double Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
+  double Ixxw = 0, Iyyw = 0, Izzw = 0, Ixyw = 0, Ixzw = 0, Iyzw = 0;
+  for (Int i = 1; i < nb; i++) {
+    const Int g = list[i];
+    const float m = C.p->mass[g];
+    float dx = C.p->x[g] - c[0], dy = C.p->y[g] - c[1], dz = C.p->z[g] - c[2];
+    if (P.periodic) { dx = P.nearest(dx); dy = P.nearest(dy); dz = P.nearest(dz); }
+    const float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    Ixx += dx2 * m; Iyy += dy2 * m; Izz += dz2 * m;
+    Ixy += dx * dy * m; Ixz += dx * dz * m; Iyz += dy * dz * m;
+    float dr2 = dx2 + dy2 + dz2;
+    dr2 /= m;
+    Ixxw += dx2 / dr2; Iyyw += dy2 / dr2; Izzw += dz2 / dr2;
+    Ixyw += dx * dy / dr2; Ixzw += dx * dz / dr2; Iyzw += dy * dz / dr2;
+  }
+  const double t[6] = {Ixx, Ixy, Ixz, Iyy, Iyz, Izz}, w[6] = {Ixxw, Ixyw, Ixzw, Iyyw, Iyzw, Izzw};
+  for (int j = 0; j < 6; j++) {
+    o.inertial_tensor[j] = (float)t[j];
+    o.inertial_tensor[j] /= s.mbound;
+    o.inertial_tensor_weighted[j] = (float)w[j];
+    o.inertial_tensor_weighted[j] /= s.mbound;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int oracle_subhalo_properties(const hbt_params *hp, const hbt_particles *parts, const hbt_subhalo *subs,
+                              int64_t nsub, const int64_t *out_index, hbt_properties *props,
+                              int nthreads) {
+  if (!hp || !parts || (nsub > 0 && (!subs || !out_index || !props))) return HBT_ERR_INVALID_ARG;
+  Params P(*hp);
+  PartCols C{parts};
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (Int i = 0; i < nsub; i++) {
+    const Int *list = out_index + subs[i].out_offset;
+    profile_properties(P, C, subs[i], list, props[i]);
+    shape(P, C, subs[i], list, props[i]);
+  }
+  return HBT_OK;
+}
 
 int oracle_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
   if (nsub < 0 || (nsub > 0 && !subs) || !total) return HBT_ERR_INVALID_ARG;

@@ -84,7 +84,11 @@ void wr(FILE *f, const T *p, size_t n) {
   if (n && fwrite(p, sizeof(T), n, f) != n) { fprintf(stderr, "short write\n"); exit(2); }
 }
 
-int run_unbind(FILE *in, FILE *out, bool timed) {
+// mode 0: unbind and write the outputs; 1: time the nest-parallel unbinding;
+// 2: as 0, then CalculateProfileProperties + CalculateShape of every subhalo
+// (src/subhalo_tracking.cpp:904-905), with hbt_properties in and out.
+int run_unbind(FILE *in, FILE *out, int mode) {
+  const bool timed = mode == 1;
   hbt_params p;
   int64_t nsub, npart;
   rd(in, &p, 1); rd(in, &nsub, 1); rd(in, &npart, 1);
@@ -97,6 +101,8 @@ int run_unbind(FILE *in, FILE *out, bool timed) {
   rd(in, m.data(), npart); rd(in, type.data(), npart);
   std::vector<hbt_subhalo> rec(nsub);
   rd(in, rec.data(), nsub);
+  std::vector<hbt_properties> props(mode == 2 ? nsub : 0);
+  if (mode == 2) rd(in, props.data(), nsub);
 
   Epoch epoch;
   configure(p, epoch);
@@ -147,6 +153,28 @@ int run_unbind(FILE *in, FILE *out, bool timed) {
     return 0;
   }
   for (int64_t s : roots) subs[s].RecursiveUnbind(subs, epoch);
+  for (size_t s = 0; s < props.size(); s++) {
+    Subhalo_t &S = subs[s];
+    hbt_properties &q = props[s];
+    S.BoundR200CritComoving = q.bound_r200crit_comoving;
+    S.BoundM200Crit = q.bound_m200crit;
+    S.LastMaxVmaxPhysical = q.last_max_vmax_physical;
+    S.SnapshotIndexOfLastMaxVmax = q.snapshot_index_of_last_max_vmax;
+    S.CalculateProfileProperties(epoch);
+    S.CalculateShape();
+    q.rmax_comoving = S.RmaxComoving;
+    q.vmax_physical = S.VmaxPhysical;
+    q.rhalf_comoving = S.RHalfComoving;
+    q.r2sigma_comoving = S.R2SigmaComoving;
+    q.bound_r200crit_comoving = S.BoundR200CritComoving;
+    q.bound_m200crit = S.BoundM200Crit;
+    q.last_max_vmax_physical = S.LastMaxVmaxPhysical;
+    q.snapshot_index_of_last_max_vmax = S.SnapshotIndexOfLastMaxVmax;
+    for (int j = 0; j < 6; j++) {
+      q.inertial_tensor[j] = S.InertialTensor[j];
+      q.inertial_tensor_weighted[j] = S.InertialTensorWeighted[j];
+    }
+  }
 
   std::vector<int64_t> list_len(nsub);
   int64_t total = 0;
@@ -185,6 +213,7 @@ int run_unbind(FILE *in, FILE *out, bool timed) {
     wr(out, idx.data(), idx.size());
     wr(out, en.data(), en.size());
   }
+  if (!props.empty()) wr(out, props.data(), nsub);
   return 0;
 }
 
@@ -217,13 +246,14 @@ int run_tree(FILE *in, FILE *out) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|tree in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|props|tree in.bin out.bin\n", argv[0]); return 1; }
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
   if (!in || !out) { fprintf(stderr, "cannot open files\n"); return 1; }
   int rc = 1;
-  if (!strcmp(argv[1], "unbind")) rc = run_unbind(in, out, false);
-  else if (!strcmp(argv[1], "time")) rc = run_unbind(in, out, true);
+  if (!strcmp(argv[1], "unbind")) rc = run_unbind(in, out, 0);
+  else if (!strcmp(argv[1], "time")) rc = run_unbind(in, out, 1);
+  else if (!strcmp(argv[1], "props")) rc = run_unbind(in, out, 2);
   else if (!strcmp(argv[1], "tree")) rc = run_tree(in, out);
   fclose(in);
   fclose(out);

@@ -9,7 +9,12 @@ Each fixture (<name>.npz) holds the inputs (particle columns, hbt_subhalo
 records, hbt_params) and what the reference's RecursiveUnbind/Unbind/
 TruncateSource produced for them (records, particle lists, binding
 energies), or, for the tree fixtures, GravityTree_t::EvaluatePotential at
-the given targets.  Runs are single-threaded with sampling off.
+the given targets.  props_<name>.npz: for the catalogue of the unbind fixture
+<name>, the hbt_properties given in (previous LastMaxVmax, R200, M200) and
+what CalculateProfileProperties + CalculateShape produced after unbinding.
+Runs are single-threaded with sampling off.
+
+    python tests/golden/make_golden.py --props-only   # only the props_ files
 """
 from __future__ import annotations
 
@@ -24,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 import oracle  # noqa: E402
 from hbtplus_amd import synth  # noqa: E402
-from hbtplus_amd.abi import SUB_DTYPE, make_params  # noqa: E402
+from hbtplus_amd.abi import PROPS_DTYPE, SUB_DTYPE, make_params  # noqa: E402
 
 
 def edge_catalogue():
@@ -125,14 +130,40 @@ def save_tree(name, cat):
     print(f"{name}: {len(x)} targets")
 
 
+PROPS_CASES = ["c2_small", "c3_small", "c4_small", "hubble", "edge"]
+
+
+def save_props(name):
+    sys.path.insert(0, os.path.dirname(HERE))
+    from fixtures import load_unbind
+    cat, (ref_subs, ref_idx, _) = load_unbind(name)
+    rng = np.random.Generator(np.random.Philox(key=777))
+    pin = np.zeros(len(cat.subs), PROPS_DTYPE)
+    pin["last_max_vmax_physical"] = rng.uniform(0.0, 40.0, len(pin)).astype(np.float32)
+    pin["snapshot_index_of_last_max_vmax"] = rng.integers(0, 9, len(pin))
+    pin["bound_r200crit_comoving"] = 111.0
+    pin["bound_m200crit"] = 2.5
+    subs, idx, _, props = oracle.ref_unbind_batch(cat, props_in=pin)
+    assert np.array_equal(subs.view(np.uint8), ref_subs.view(np.uint8)) and np.array_equal(idx, ref_idx)
+    np.savez_compressed(os.path.join(HERE, "props_" + name + ".npz"), kind="props", name=name,
+                        props_in=pin.view(np.uint8), ref_props=props.view(np.uint8))
+    print(f"props_{name}: {len(props)} subhalos")
+
+
 def main():
     if not oracle.have_reference():
         raise SystemExit("oracle/_ref/hbt_ref_harness is missing (make -C oracle ref)")
+    if "--props-only" in sys.argv:
+        for name in PROPS_CASES:
+            save_props(name)
+        return
     for name, make in CASES.items():
         save_unbind(name, make())
     save_tree("tree_c1", synth.config1(n_nfw=3000, n_cont=600))
     save_tree("tree_straddle", straddle_catalogue())
     save_tree("tree_floor", floor_catalogue())
+    for name in PROPS_CASES:
+        save_props(name)
 
 
 if __name__ == "__main__":

@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import TREE_CASES, UNBIND_CASES, load_tree, load_unbind
+from fixtures import PROPS_CASES, TREE_CASES, UNBIND_CASES, load_props, load_tree, load_unbind
+from hbtplus_amd.abi import PROPS_FIELDS
 from helpers import assert_exact
 
 
@@ -39,3 +40,14 @@ def test_fixture_outcomes_are_nontrivial():
         if ((cat.subs["nbound_prev"] <= 1) & (rs["list_len"] > cat.subs["count"])).any():
             seen.add("orphan-with-tail")
     assert {1, 2, 4, "multi-pass", "nested", "orphan-with-tail"} <= seen
+
+
+@pytest.mark.parametrize("name", PROPS_CASES)
+def test_oracle_properties_match_reference(name):
+    """CalculateProfileProperties + CalculateShape restated: bit-identical to
+    the reference's on its own unbinding outputs"""
+    cat, (subs, idx, _), pin, ref = load_props(name)
+    got = oracle.subhalo_properties(cat.params, cat.cols, subs, idx, pin)
+    for f in PROPS_FIELDS:
+        assert np.array_equal(np.ascontiguousarray(got[f]).view(np.uint8),
+                              np.ascontiguousarray(ref[f]).view(np.uint8)), f
