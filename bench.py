@@ -336,11 +336,50 @@ def main():
         result["batched_over_plain"] = round(value / (evals / pt), 2)
         print(f"[bench] plain: {pt * 1e3:.1f} ms, identical outputs {same}", file=sys.stderr, flush=True)
 
+    # post-unbind properties (profile + shape) on the step's outputs
+    result["properties"] = bench_properties(U, h, out, cat, args)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cat, with_reference=True)
         print(f"[bench] cpu: {result['cpu_baseline']}", file=sys.stderr, flush=True)
     U.close()
     print(json.dumps(result), flush=True)
+
+
+def bench_properties(U, h, out, cat, args):
+    """hbt_subhalo_properties on the unbinding outputs: wall time per call
+    (records read back for the layout, distance sort, FP64 scans and sums),
+    against the oracle restatement on the host's cores (OpenMP over
+    subhalos).  Bytes: each bound particle's index, position and mass read
+    once (24 B) is the algorithmic minimum."""
+    import torch
+
+    from hbtplus_amd.abi import SUB_DTYPE
+    subs = out[0].cpu().numpy().view(SUB_DTYPE)
+    nbp = int(subs["nbound"][subs["nbound"] > 1].sum())
+    U.properties_device(h, out)  # warm
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        props = U.properties_device(h, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    res = {"ms_per_call": round(ms, 3), "subhalos": int(len(subs)), "bound_particles": nbp,
+           "bound_particles_per_s": nbp / (ms / 1e3),
+           "algorithmic_GBps": round(24.0 * nbp / (ms / 1e3) / 1e9, 2)}
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        idx = out[1].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.subhalo_properties(cat.params, cat.cols, subs, idx, nthreads=threads)
+        tc = time.perf_counter() - t0
+        same = all(np.array_equal(np.ascontiguousarray(ref[f]).view(np.uint8),
+                                  np.ascontiguousarray(props[f]).view(np.uint8)) for f in ref.dtype.names)
+        res["cpu_port"] = {"ms": round(tc * 1e3, 1), "cores": threads, "identical": bool(same)}
+    print(f"[bench] properties: {res}", file=sys.stderr, flush=True)
+    return res
 
 
 if __name__ == "__main__":
