@@ -136,6 +136,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_profile": ([vp, i32], i32),
         "hbt_profile_read": ([vp, vp, vp, vp, vp, vp, vp], i32),
         "hbt_subhalo_properties": ([vp, vp, vp, i64, vp, vp], i32),
+        "hbt_mask_subhalos": ([vp, vp, vp, vp, i64, vp, vp], i32),
         "hbt_get_tuning": ([vp, vp], i32),
         "hbt_set_tuning": ([vp, vp], i32),
         "hbt_autotune": ([vp, vp, vp, vp, i64, vp, vp, i32, vp], i32),

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hbt_unbind.h"
@@ -1072,5 +1073,6 @@ int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *su
 
 #include "hbt_tree_api.hpp"
 #include "hbt_props.hpp"
+#include "hbt_mask.hpp"
 #include "hbt_direct.hpp"
 #include "hbt_idhash.hpp"

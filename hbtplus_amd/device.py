@@ -119,6 +119,24 @@ class GpuUnbinder:
         total = h["total"]
         return subs, out[1].cpu().numpy()[:total].copy(), out[2].cpu().numpy()[:total].copy(), props
 
+    # exclusive ownership inside host groups ----------------------------------
+    def mask_subhalos(self, cols, subs, host_group):
+        """MaskSubhalos on host columns/records: returns (out_list laid out
+        like the columns, new counts), numpy."""
+        t = self.torch
+        c = {k: self._dev(np.ascontiguousarray(cols[k])) for k in PARTICLE_COLUMNS}
+        ps = particles_struct(c, ptr=lambda x: x.data_ptr())
+        subs = np.ascontiguousarray(subs, SUB_DTYPE)
+        grp = np.ascontiguousarray(host_group, np.int64)
+        n = len(cols["id"])
+        out = t.full((max(n, 1),), -1, dtype=t.int64, device=f"cuda:{self.device}")
+        cnt = np.zeros(len(subs), np.int64)
+        t.cuda.synchronize(self.device)
+        rc = self.lib.hbt_mask_subhalos(self.ctx, ctypes.byref(ps), subs.ctypes.data, grp.ctypes.data,
+                                        len(subs), out.data_ptr(), cnt.ctypes.data)
+        check(rc, "hbt_mask_subhalos")
+        return out.cpu().numpy()[:n].copy(), cnt
+
     # execution tuning (results do not depend on it) -----------------------
     def get_tuning(self):
         t = np.zeros(1, abi.TUNING_DTYPE)

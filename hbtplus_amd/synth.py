@@ -392,3 +392,52 @@ def config5(seed: int = 1005, nsub: int = 200_000, background_frac: float = 0.1)
 
 
 CONFIGS = {"c1": config1, "c2": config2, "c3": config3, "c4": config4, "c5": config5}
+
+
+def masking_catalogue(seed=1006, ngroups=40, max_subs=12, pool=4000, big_group=0):
+    """Host groups whose subhalo source lists overlap, as they do before
+    MaskSubhalos: each group draws its lists from a shared ID pool (the same
+    ID appears as several rows of the columns), with nesting inside the group,
+    orphans (nbound_prev <= 1), repeated IDs inside a list, null IDs (-1) and
+    field subhalos (host group -1).  big_group > 0 adds one group whose lists
+    hold about that many entries in total.  Returns (cols, subs, host_group)."""
+    from .abi import SUB_DTYPE
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    ids, subs, groups = [], [], []
+    next_id = 1
+    off = 0
+
+    def add(g, parent, nprev, lst):
+        nonlocal off
+        r = np.zeros(1, SUB_DTYPE)[0]
+        r["offset"], r["count"], r["parent"], r["nbound_prev"] = off, len(lst), parent, nprev
+        subs.append(r)
+        groups.append(g)
+        ids.append(np.asarray(lst, np.int64))
+        off += len(lst)
+        return len(subs) - 1
+
+    sizes = [(g, int(rng.integers(1, max_subs + 1)), pool) for g in range(ngroups)]
+    if big_group:
+        sizes.append((ngroups, 6, big_group))
+    for g, ns, psize in sizes:
+        pool_ids = np.arange(next_id, next_id + psize, dtype=np.int64)
+        next_id += psize
+        members = []
+        for k in range(ns):
+            n = int(rng.integers(0, max(2, psize // max(ns, 1)) * 2))
+            lst = rng.choice(pool_ids, size=n, replace=True) if n else np.zeros(0, np.int64)
+            if n and rng.random() < 0.2:
+                lst[rng.integers(0, n, max(1, n // 50))] = -1
+            parent = -1 if (k == 0 or rng.random() < 0.3) else int(members[int(rng.integers(0, len(members)))])
+            nprev = 1 if rng.random() < 0.15 else int(max(2, n))
+            members.append(add(g, parent, nprev, lst))
+    for k in range(5):  # field subhalos
+        lst = rng.integers(1, next_id, 50)
+        add(-1, -1, 50, lst)
+    ids = np.concatenate(ids) if ids else np.zeros(0, np.int64)
+    n = len(ids)
+    cols = dict(id=ids, x=rng.random(n, dtype=np.float32), y=rng.random(n, dtype=np.float32),
+                z=rng.random(n, dtype=np.float32), vx=np.zeros(n, np.float32), vy=np.zeros(n, np.float32),
+                vz=np.zeros(n, np.float32), mass=np.ones(n, np.float32), type=np.ones(n, np.int32))
+    return cols, np.array(subs, SUB_DTYPE), np.array(groups, np.int64)

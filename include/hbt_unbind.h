@@ -240,6 +240,26 @@ typedef struct hbt_properties {
 int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particles, const hbt_subhalo *subs,
                            int64_t nsub, const int64_t *out_index, hbt_properties *props);
 
+/* Exclusive particle ownership inside each host halo (SubhaloMasker_t and
+ * SubhaloSnapshot_t::MaskSubhalos, src/subhalo_tracking.cpp:793-841).
+ * A host group's subhalos are visited from its central -- the first head
+ * (parent < 0) of the group in index order -- in post-order: a subhalo's nested
+ * subhalos (index order) before it, the group's other heads counting as nested
+ * in the central after its own nested subhalos.  A visited subhalo with
+ * nbound_prev > 1 keeps the list entries whose particle ID no earlier entry of
+ * the group has kept (list order); orphans (nbound_prev <= 1) and field
+ * subhalos (host_group < 0) keep their lists and claim nothing.
+ *   particles  : device columns (id read)
+ *   subs_host  : HOST records; offset/count/parent/nbound_prev read
+ *   host_group : HOST int64[nsub]; a nested subhalo must share its parent's group
+ *   out_list   : DEVICE int64[sum of counts]; subhalo i's retained entries, as
+ *                particle column positions, at [offset_i, offset_i + new_count[i])
+ *                (so out_list is laid out like the input columns)
+ *   new_count  : HOST int64[nsub], output */
+int hbt_mask_subhalos(hbt_ctx *ctx, const hbt_particles *particles, const hbt_subhalo *subs_host,
+                      const int64_t *host_group, int64_t nsub, int64_t *out_list,
+                      int64_t *new_count);
+
 /* ID -> index resolution: builds a device hash table of the n_table IDs
  * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs
  * and negative IDs give -1 (src/hash.tpp:52).  Device pointers. */

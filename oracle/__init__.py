@@ -41,6 +41,7 @@ def lib():
         L.oracle_direct_potential.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, i32]
         L.oracle_id_to_index.argtypes = [i64, vp, i64, vp, vp]
         L.oracle_subhalo_properties.argtypes = [vp, vp, vp, i64, vp, vp, i32]
+        L.oracle_mask_subhalos.argtypes = [vp, vp, vp, i64, vp, vp]
         L.oracle_abi_sizes.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
@@ -112,6 +113,20 @@ def subhalo_properties(params, cols, subs, out_index, props_in=None, nthreads=1)
     if rc != 0:
         raise RuntimeError("oracle_subhalo_properties failed")
     return props
+
+
+def mask_subhalos(cols, subs, host_group):
+    """MaskSubhalos restated: (out_list laid out like the columns, new counts)."""
+    c = {k: np.ascontiguousarray(v) for k, v in cols.items()}
+    ps = particles_struct(c)
+    subs = np.ascontiguousarray(subs, SUB_DTYPE)
+    grp = np.ascontiguousarray(host_group, np.int64)
+    out = np.full(len(c["id"]), -1, np.int64)
+    cnt = np.zeros(len(subs), np.int64)
+    rc = lib().oracle_mask_subhalos(ctypes.byref(ps), _p(subs), _p(grp), len(subs), _p(out), _p(cnt))
+    if rc != 0:
+        raise RuntimeError("oracle_mask_subhalos failed")
+    return out, cnt
 
 
 def id_to_index(table_ids, query_ids):

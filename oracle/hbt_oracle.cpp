@@ -35,6 +35,9 @@
 #include <limits>
 #include <numeric>
 #include <vector>
+#include <unordered_set>
+#include <functional>
+#include <map>
 #include <omp.h>
 
 #include "../include/hbt_unbind.h"
@@ -612,6 +615,50 @@ int oracle_subhalo_properties(const hbt_params *hp, const hbt_particles *parts, 
     const Int *list = out_index + subs[i].out_offset;
     profile_properties(P, C, subs[i], list, props[i]);
     shape(P, C, subs[i], list, props[i]);
+  }
+  return HBT_OK;
+}
+
+// MaskSubhalos (src/subhalo_tracking.cpp:793-841): per host group a masker
+// with an exclusion set, Mask(central) recursing into NestedSubhalos first
+// (the central's list temporarily extended by the group's other heads,
+// :828-834), orphans skipped (:808), first claim of an ID kept (:811-819).
+// subs: offset/count/parent/nbound_prev; out_list laid out like the columns.
+int oracle_mask_subhalos(const hbt_particles *parts, const hbt_subhalo *subs, const int64_t *group,
+                         int64_t nsub, int64_t *out_list, int64_t *new_count) {
+  if (!parts || (nsub > 0 && (!subs || !group || !out_list || !new_count))) return HBT_ERR_INVALID_ARG;
+  std::vector<std::vector<Int>> nested(nsub);
+  std::map<Int, std::vector<Int>> heads;
+  for (Int s = 0; s < nsub; s++) {
+    const Int p = subs[s].parent;
+    if (p >= 0) {
+      if (p >= nsub || group[p] != group[s]) return HBT_ERR_INVALID_ARG;
+      nested[p].push_back(s);
+    } else if (group[s] >= 0) {
+      heads[group[s]].push_back(s);
+    }
+  }
+  for (Int s = 0; s < nsub; s++) {
+    new_count[s] = subs[s].count;
+    for (Int i = 0; i < subs[s].count; i++) out_list[subs[s].offset + i] = subs[s].offset + i;
+  }
+  for (auto &g : heads) {
+    const std::vector<Int> &hd = g.second;
+    const Int central = hd[0];
+    std::vector<Int> cnest = nested[central];
+    cnest.insert(cnest.end(), hd.begin() + 1, hd.end());
+    std::unordered_set<int64_t> excl;
+    std::function<void(Int)> mask = [&](Int s) {
+      for (Int c : (s == central ? cnest : nested[s])) mask(c);
+      if (subs[s].nbound_prev <= 1) return;
+      Int keep = 0;
+      for (Int i = 0; i < subs[s].count; i++) {
+        const Int col = subs[s].offset + i;
+        if (excl.insert(parts->id[col]).second) out_list[subs[s].offset + keep++] = col;
+      }
+      new_count[s] = keep;
+    };
+    mask(central);
   }
   return HBT_OK;
 }

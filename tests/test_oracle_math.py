@@ -172,3 +172,64 @@ def test_properties_closed_forms():
     # Hz = 0: the critical density vanishes, so the reference's overdensity
     # radius is infinite and M200 is the whole bound mass (src/snapshot.cpp:268-279)
     assert np.isinf(p["bound_r200crit_comoving"]) and abs(p["bound_m200crit"] - mc[-1]) <= 1e-6 * mc[-1]
+
+
+def _mask_case(lists, parents, nprev, groups):
+    from hbtplus_amd.abi import SUB_DTYPE
+    ids = np.concatenate([np.asarray(x, np.int64) for x in lists])
+    n = len(ids)
+    cols = dict(id=ids, x=np.zeros(n, np.float32), y=np.zeros(n, np.float32), z=np.zeros(n, np.float32),
+                vx=np.zeros(n, np.float32), vy=np.zeros(n, np.float32), vz=np.zeros(n, np.float32),
+                mass=np.ones(n, np.float32), type=np.ones(n, np.int32))
+    subs = np.zeros(len(lists), SUB_DTYPE)
+    off = 0
+    for i, x in enumerate(lists):
+        subs[i]["offset"], subs[i]["count"] = off, len(x)
+        off += len(x)
+    subs["parent"], subs["nbound_prev"] = parents, nprev
+    out, cnt = oracle.mask_subhalos(cols, subs, np.asarray(groups, np.int64))
+    return [list(ids[out[s["offset"]:s["offset"] + c]]) for s, c in zip(subs, cnt)]
+
+
+def test_mask_visit_order_and_orphans():
+    """Hand-derived from SubhaloMasker_t::Mask (src/subhalo_tracking.cpp:803-822)
+    and MaskSubhalos (:826-840): nested subhalos claim before their host, the
+    group's other heads are masked after the central's own nested subhalos but
+    before the central, orphans neither claim nor lose, a repeated ID inside one
+    list keeps its first entry, groups do not interact, field subhalos keep all."""
+    # group 0: central A (0) with nested C (2); second head B (1) nested-in-central
+    got = _mask_case([[1, 2, 3, 4], [3, 4, 5], [4, 6, 6], [2, 5, 9], [1, 2], [1, 7]],
+                     parents=[-1, -1, 0, -1, 0, -1], nprev=[10, 10, 10, 1, 10, 10],
+                     groups=[0, 0, 0, 1, 0, -1])
+    # visit: C(2) [4,6], then E(4) nested in A [1,2], then B(1) [3,5], then A(0) [] (1,2,3,4 taken)
+    assert got[2] == [4, 6]
+    assert got[4] == [1, 2]
+    assert got[1] == [3, 5]
+    assert got[0] == []
+    assert got[3] == [2, 5, 9]    # group 1: an orphan keeps its list
+    assert got[5] == [1, 7]       # field
+    # an orphan does not claim: B keeps 2 although the orphan nested in A lists it first
+    got = _mask_case([[1], [2, 3], [2]], parents=[-1, -1, 0], nprev=[5, 5, 1], groups=[7, 7, 7])
+    assert got == [[1], [2, 3], [2]]
+
+
+def test_mask_invariants():
+    """Random host groups: within a group the kept IDs of the subhalos that
+    claim are disjoint, every ID those subhalos listed is kept exactly once,
+    each kept list is a subsequence of its input list."""
+    from hbtplus_amd import synth
+    cols, subs, grp = synth.masking_catalogue(seed=9, ngroups=25)
+    out, cnt = oracle.mask_subhalos(cols, subs, grp)
+    ids = cols["id"]
+    for g in np.unique(grp[grp >= 0]):
+        claim = [s for s in np.nonzero(grp == g)[0] if subs["nbound_prev"][s] > 1]
+        kept = np.concatenate([ids[out[subs["offset"][s]:subs["offset"][s] + cnt[s]]] for s in claim] + [[]])
+        listed = np.concatenate([ids[subs["offset"][s]:subs["offset"][s] + subs["count"][s]] for s in claim] + [[]])
+        assert len(kept) == len(np.unique(kept))
+        assert set(kept.tolist()) == set(listed.tolist())
+    for s in range(len(subs)):
+        o = subs["offset"][s]
+        pos = out[o:o + cnt[s]]
+        assert np.all(np.diff(pos) > 0) and np.all((pos >= o) & (pos < o + subs["count"][s]))
+        if grp[s] < 0 or subs["nbound_prev"][s] <= 1:
+            assert cnt[s] == subs["count"][s]
