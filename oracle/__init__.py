@@ -18,6 +18,7 @@ from hbtplus_amd.abi import PARAMS_DTYPE, PROPS_DTYPE, SUB_DTYPE, particles_stru
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_HARNESS = os.path.join(HERE, "_ref", "hbt_ref_harness")
+REF_TRACKING = os.path.join(os.path.dirname(REF_HARNESS), "hbt_ref_tracking")
 _lib = None
 
 
@@ -214,6 +215,24 @@ def ref_unbind_batch(cat, params=None, props_in=None):
         props = np.frombuffer(raw, PROPS_DTYPE, nsub, pos).copy()
         return subs, out_index, out_energy, props
     return subs, out_index, out_energy
+
+
+def ref_mask_subhalos(cols, subs, host_group):
+    """The reference's own SubhaloSnapshot_t::MaskSubhalos on the catalogue
+    (oracle/_ref/hbt_ref_tracking): (out_list laid out like the columns, new
+    counts), as oracle.mask_subhalos returns them."""
+    ids = np.ascontiguousarray(cols["id"], np.int64)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([len(subs), len(ids)], np.int64).tobytes())
+            f.write(ids.tobytes())
+            f.write(np.ascontiguousarray(subs, SUB_DTYPE).tobytes())
+            f.write(np.ascontiguousarray(host_group, np.int64).tobytes())
+        subprocess.run([REF_TRACKING, "mask", fin, fout], check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        raw = np.fromfile(fout, np.int64)
+    return raw[:len(ids)].copy(), raw[len(ids):].copy()
 
 
 def ref_tree_potential(params, src, tgt, tmass=None):

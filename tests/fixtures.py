@@ -12,6 +12,7 @@ UNBIND_CASES = ["c1_small", "c1_small_bmp05", "c2_small", "c3_small", "c4_small"
                 "floor"]
 TREE_CASES = ["tree_c1", "tree_straddle", "tree_floor"]
 PROPS_CASES = ["c2_small", "c3_small", "c4_small", "hubble", "edge"]
+MASK_CASES = ["mask_a", "mask_b"]
 
 
 def load_unbind(name):
@@ -35,3 +36,13 @@ def load_props(name):
     cat, ref = load_unbind(name)
     z = np.load(os.path.join(GOLDEN, "props_" + name + ".npz"))
     return cat, ref, z["props_in"].view(PROPS_DTYPE).copy(), z["ref_props"].view(PROPS_DTYPE).copy()
+
+
+def load_mask(name):
+    """(columns with the ID column, records, host groups, reference out_list, reference counts)"""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    n = len(z["id"])
+    cols = dict(id=z["id"], x=np.zeros(n, np.float32), y=np.zeros(n, np.float32), z=np.zeros(n, np.float32),
+                vx=np.zeros(n, np.float32), vy=np.zeros(n, np.float32), vz=np.zeros(n, np.float32),
+                mass=np.ones(n, np.float32), type=np.ones(n, np.int32))
+    return cols, z["subs"].view(SUB_DTYPE).copy(), z["host_group"], z["ref_out"], z["ref_count"]

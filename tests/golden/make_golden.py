@@ -15,6 +15,11 @@ what CalculateProfileProperties + CalculateShape produced after unbinding.
 Runs are single-threaded with sampling off.
 
     python tests/golden/make_golden.py --props-only   # only the props_ files
+    python tests/golden/make_golden.py --mask-only    # only the mask_ files
+
+mask_<name>.npz: a catalogue of overlapping source lists in host groups (ID
+column, records, host groups) and what the reference's own
+SubhaloSnapshot_t::MaskSubhalos kept (oracle/_ref/hbt_ref_tracking).
 """
 from __future__ import annotations
 
@@ -150,9 +155,24 @@ def save_props(name):
     print(f"props_{name}: {len(props)} subhalos")
 
 
+MASK_CASES = {"mask_a": dict(seed=21, ngroups=20), "mask_b": dict(seed=22, ngroups=6, big_group=3000)}
+
+
+def save_mask(name, kw):
+    cols, subs, grp = synth.masking_catalogue(**kw)
+    out, cnt = oracle.ref_mask_subhalos(cols, subs, grp)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), kind="mask", id=cols["id"],
+                        subs=subs.view(np.uint8), host_group=grp, ref_out=out, ref_count=cnt)
+    print(f"{name}: {len(subs)} subhalos, {len(cols['id'])} entries, kept {cnt.sum()}")
+
+
 def main():
     if not oracle.have_reference():
         raise SystemExit("oracle/_ref/hbt_ref_harness is missing (make -C oracle ref)")
+    if "--mask-only" in sys.argv:
+        for name, kw in MASK_CASES.items():
+            save_mask(name, kw)
+        return
     if "--props-only" in sys.argv:
         for name in PROPS_CASES:
             save_props(name)
@@ -164,6 +184,8 @@ def main():
     save_tree("tree_floor", floor_catalogue())
     for name in PROPS_CASES:
         save_props(name)
+    for name, kw in MASK_CASES.items():
+        save_mask(name, kw)
 
 
 if __name__ == "__main__":

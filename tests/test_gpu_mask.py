@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
+from fixtures import MASK_CASES, load_mask
 from hbtplus_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -34,3 +35,13 @@ def test_mask_rejects_cross_group_nesting(gpu):
     grp[child] = grp[subs["parent"][child]] + 1000
     with pytest.raises(RuntimeError, match="host group"):
         _run(gpu, cols, subs, grp)
+
+
+@pytest.mark.parametrize("name", MASK_CASES)
+def test_mask_matches_reference(gpu, name):
+    """against the reference's own MaskSubhalos (golden fixtures)"""
+    cols, subs, grp, ref_out, ref_cnt = load_mask(name)
+    out, cnt = _run(gpu, cols, subs, grp)
+    assert np.array_equal(cnt, ref_cnt)
+    for o, c in zip(subs["offset"], cnt):
+        assert np.array_equal(out[o:o + c], ref_out[o:o + c])

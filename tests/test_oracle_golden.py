@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import PROPS_CASES, TREE_CASES, UNBIND_CASES, load_props, load_tree, load_unbind
+from fixtures import (MASK_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_props, load_tree,
+                      load_unbind)
 from hbtplus_amd.abi import PROPS_FIELDS
 from helpers import assert_exact
 
@@ -51,3 +52,14 @@ def test_oracle_properties_match_reference(name):
     for f in PROPS_FIELDS:
         assert np.array_equal(np.ascontiguousarray(got[f]).view(np.uint8),
                               np.ascontiguousarray(ref[f]).view(np.uint8)), f
+
+
+@pytest.mark.parametrize("name", MASK_CASES)
+def test_oracle_mask_matches_reference(name):
+    """MaskSubhalos restated: the kept lists of the reference's own
+    SubhaloSnapshot_t::MaskSubhalos, entry for entry"""
+    cols, subs, grp, ref_out, ref_cnt = load_mask(name)
+    out, cnt = oracle.mask_subhalos(cols, subs, grp)
+    assert np.array_equal(cnt, ref_cnt)
+    for o, c in zip(subs["offset"], cnt):
+        assert np.array_equal(out[o:o + c], ref_out[o:o + c])
