@@ -12,6 +12,11 @@ import os
 import numpy as np
 
 HBT_OK = 0
+HBT_ERR_INVALID_ARG = -1
+HBT_ERR_OUT_OF_MEMORY = -2
+HBT_ERR_DEVICE = -3
+HBT_ERR_UNSUPPORTED = -4
+HBT_ERR_CAPACITY = -5
 HBT_PREC_FAST = 0
 HBT_PREC_REPLAY = 1
 HBT_PREC_FP32 = 2
@@ -137,6 +142,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_profile_read": ([vp, vp, vp, vp, vp, vp, vp], i32),
         "hbt_subhalo_properties": ([vp, vp, vp, i64, vp, vp], i32),
         "hbt_mask_subhalos": ([vp, vp, vp, vp, i64, vp, vp], i32),
+        "hbt_merge_subhalos": ([vp, vp, vp, i64, vp, vp, i64, i32, vp, i64, vp], i32),
         "hbt_get_tuning": ([vp, vp], i32),
         "hbt_set_tuning": ([vp, vp], i32),
         "hbt_autotune": ([vp, vp, vp, vp, i64, vp, vp, i32, vp], i32),

@@ -441,3 +441,79 @@ def masking_catalogue(seed=1006, ngroups=40, max_subs=12, pool=4000, big_group=0
                 z=rng.random(n, dtype=np.float32), vx=np.zeros(n, np.float32), vy=np.zeros(n, np.float32),
                 vz=np.zeros(n, np.float32), mass=np.ones(n, np.float32), type=np.ones(n, np.int32))
     return cols, np.array(subs, SUB_DTYPE), np.array(groups, np.int64)
+
+
+def merging_catalogue(seed=1007, ngroups=3, periodic=True, box=1000.0, hz=0.0, scale_factor=1.0,
+                      snapshot_index=5, scale=1.0):
+    """Host groups laid out so that merging has work to do.  Per group: a
+    central NFW halo; nested in it a satellite sinking onto its core (with a
+    subhalo nested in the satellite and sinking onto the satellite's core), a
+    satellite orbiting far out (with one nested subhalo sinking onto that
+    satellite and one lying at the central's core; in the first group also
+    an empty one), a hot subhalo that disrupts and a one-particle orphan at
+    the centre; a second head of the
+    group sinking onto the central and a head far out.  Plus a field halo with
+    a nested subhalo sinking onto it.  The first group sits across the
+    periodic boundary.  `scale` multiplies every particle count.  Returns
+    (Catalogue, host_group)."""
+    rng = _rng(seed)
+    jeans = _JeansTable(5.0, 20.0)
+    G = G_DEFAULT
+
+    def halo(n, c):
+        n = max(1, int(round(n * scale))) if n > 1 else 1
+        r200 = 200.0 * (n / 2.0e4) ** (1.0 / 3.0)
+        rp, rv = _nfw_members(rng, np.zeros(n, np.int64), np.array([c]), np.array([r200]), jeans, G,
+                              PARTICLE_MASS)
+        return rp, rv, r200, np.sqrt(G * n * PARTICLE_MASS / r200)
+
+    entries, parent, group, nprev = [], [], [], []
+
+    def add(cen, bulk, rp, rv, par, grp, orphan=False):
+        entries.append((np.asarray(cen, np.float64), np.asarray(bulk, np.float64), rp, rv))
+        parent.append(par)
+        group.append(grp)
+        nprev.append(1 if orphan else len(rp))
+        return len(entries) - 1
+
+    def near(c, r):
+        return np.asarray(c) + _isotropic(rng, 1)[0] * r
+
+    for g in range(ngroups):
+        C = np.array([box - 3.0, 2.0, 0.5 * box]) if (g == 0 and periodic) else rng.uniform(0.2, 0.8, 3) * box
+        B = rng.normal(size=3) * 50.0
+        rp, rv, r200, v200 = halo(rng.integers(1200, 1800), 8.0)
+        cen = add(C, B, rp, rv, -1, g)
+        c1, b1 = near(C, 0.3), B + rng.normal(size=3) * 0.5
+        rp, rv, _, _ = halo(rng.integers(150, 300), 15.0)
+        s1 = add(c1, b1, rp, rv, cen, g)
+        rp, rv, _, _ = halo(60, 15.0)
+        add(near(c1, 0.1), b1 + rng.normal(size=3) * 0.3, rp, rv, s1, g)
+        c2, b2 = near(C, 0.5 * r200), B + _isotropic(rng, 1)[0] * v200
+        rp, rv, _, _ = halo(rng.integers(300, 500), 12.0)
+        s2 = add(c2, b2, rp, rv, cen, g)
+        rp, rv, _, _ = halo(rng.integers(50, 80), 15.0)
+        add(near(c2, 0.1), b2 + rng.normal(size=3) * 0.2, rp, rv, s2, g)
+        rp, rv, _, _ = halo(rng.integers(50, 80), 15.0)
+        add(near(C, 0.2), B + rng.normal(size=3) * 0.3, rp, rv, s2, g)
+        if g == 0:  # an empty list nested in the far satellite
+            add(C, B, np.zeros((0, 3)), np.zeros((0, 3)), s2, g)
+        rp, rv, _, _ = halo(40, 10.0)
+        add(near(C, 5.0), B, rp, rv * 20.0, cen, g)  # hot: disrupts
+        add(C, B, np.zeros((1, 3)), np.zeros((1, 3)), cen, g, orphan=True)
+        rp, rv, _, _ = halo(rng.integers(100, 200), 12.0)
+        add(near(C, 0.3), B + rng.normal(size=3) * 0.5, rp, rv, -1, g)
+        rp, rv, _, _ = halo(100, 12.0)
+        add(near(C, 0.8 * r200), B + _isotropic(rng, 1)[0] * 2.0 * v200, rp, rv, -1, g)
+    F = rng.uniform(0.2, 0.8, 3) * box
+    BF = rng.normal(size=3) * 50.0
+    rp, rv, _, _ = halo(600, 8.0)
+    f = add(F, BF, rp, rv, -1, -1)
+    rp, rv, _, _ = halo(60, 15.0)
+    add(near(F, 0.2), BF + rng.normal(size=3) * 0.3, rp, rv, f, -1)
+    params = make_params(softening=0.5, open_angle=0.45, periodic=1 if periodic else 0, box_size=box,
+                         hz=hz, scale_factor=scale_factor, snapshot_index=snapshot_index)
+    cat = _assemble(rng, entries, box, periodic, params, f"merging {ngroups} groups",
+                    parent=np.array(parent, np.int64))
+    cat.subs["nbound_prev"] = np.array(nprev, np.int64)
+    return cat, np.array(group, np.int64)

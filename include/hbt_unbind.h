@@ -260,6 +260,49 @@ int hbt_mask_subhalos(hbt_ctx *ctx, const hbt_particles *particles, const hbt_su
                       const int64_t *host_group, int64_t nsub, int64_t *out_list,
                       int64_t *new_count);
 
+/* Merging of trapped subhalos after the unbinding of a snapshot
+ * (SubhaloSnapshot_t::MergeSubhalos with DetectTraps, MergeRecursive and
+ * Subhalo_t::MergeTo, src/subhalo_merge.cpp:136-262).
+ *   1. DetectTraps: every subhalo not yet trapped looks up its chain of hosts
+ *      (its parent; a group's other heads count as nested in the group's
+ *      central -- the first head in index order, as in hbt_mask_subhalos) for
+ *      the first host with nbound > 1 whose core (mass-weighted mean position
+ *      and velocity of its min(nbound, 20) most bound particles) lies within
+ *      phase-space distance 2 of the subhalo's most bound particle, in units
+ *      of the core's dispersions; that host becomes its sink
+ *      (sink_track_id, snapshot_index_of_sink = the context's snapshot index).
+ *   2. With merge_trapped (HBTConfig.MergeTrappedSubhalos), each group is
+ *      walked in post-order from its central; a trapped subhalo that is alive
+ *      is marked dead at this snapshot and, when nbound > 1, its list is
+ *      appended to its sink's without the IDs the sink already holds and its
+ *      own list shrinks to its first particle (nbound 1).  Every host that
+ *      received a subhalo with nbound > 1 is then unbound again (plain Unbind)
+ *      and its list truncated (TruncateSource).
+ *   snapshot   : device particle columns (every column read)
+ *   list_index : DEVICE int64[n_list]; subhalo i's particle list is
+ *                list_index[subs_host[i].offset .. + count) as snapshot
+ *                positions, its first nbound entries bound and most bound first
+ *                (e.g. the out_index of hbt_unbind_batch_indexed with
+ *                offset = out_offset, count = nsource)
+ *   subs_host  : HOST records, in/out.  Read: offset, count, parent, nbound,
+ *                sink_track_id, snapshot indices, most bound position and
+ *                velocity, and the Unbind inputs of the re-unbound hosts.
+ *                Written: the merged state; offset/count then give each list
+ *                in out_list (list_len = nsource = count, out_offset = offset).
+ *                A trapped, alive subhalo with nbound > 1 on entry is rejected
+ *                (unbinding never leaves one: it clears the trap of a subhalo
+ *                that stays bound and marks the others dead).
+ *   host_group : HOST int64[nsub], -1 for field subhalos; a nested subhalo
+ *                shares its parent's group
+ *   out_list   : DEVICE int64[out_capacity]; the lists after merging,
+ *                compacted in subhalo order.  Sum of counts + nsub always
+ *                suffices; *out_total (may be NULL) receives the length used
+ *                (HBT_ERR_CAPACITY, with *out_total set, when it does not fit). */
+int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
+                       int64_t n_list, hbt_subhalo *subs_host, const int64_t *host_group,
+                       int64_t nsub, int merge_trapped, int64_t *out_list, int64_t out_capacity,
+                       int64_t *out_total);
+
 /* ID -> index resolution: builds a device hash table of the n_table IDs
  * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs
  * and negative IDs give -1 (src/hash.tpp:52).  Device pointers. */

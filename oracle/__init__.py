@@ -43,6 +43,7 @@ def lib():
         L.oracle_id_to_index.argtypes = [i64, vp, i64, vp, vp]
         L.oracle_subhalo_properties.argtypes = [vp, vp, vp, i64, vp, vp, i32]
         L.oracle_mask_subhalos.argtypes = [vp, vp, vp, i64, vp, vp]
+        L.oracle_merge_subhalos.argtypes = [vp, vp, vp, i64, vp, vp, i64, i32, vp, vp, i64, vp]
         L.oracle_abi_sizes.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
@@ -128,6 +129,27 @@ def mask_subhalos(cols, subs, host_group):
     if rc != 0:
         raise RuntimeError("oracle_mask_subhalos failed")
     return out, cnt
+
+
+def merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True):
+    """MergeSubhalos restated (one thread): (records after merging, their
+    lists as column positions, packed in subhalo order, and the binding
+    energies along the lists of the subhalos unbound again, NaN elsewhere)."""
+    c = {k: np.ascontiguousarray(v) for k, v in cols.items()}
+    ps = particles_struct(c)
+    li = np.ascontiguousarray(list_index, np.int64)
+    subs = np.ascontiguousarray(subs, SUB_DTYPE).copy()
+    grp = np.ascontiguousarray(host_group, np.int64)
+    cap = int(subs["count"].sum()) + len(subs)
+    out = np.full(max(cap, 1), -1, np.int64)
+    en = np.full(max(cap, 1), np.nan, np.float32)
+    used = ctypes.c_int64()
+    rc = lib().oracle_merge_subhalos(_p(params), ctypes.byref(ps), _p(li), len(li), _p(subs), _p(grp),
+                                     len(subs), 1 if merge_trapped else 0, _p(out), _p(en), cap,
+                                     ctypes.byref(used))
+    if rc != 0:
+        raise RuntimeError(f"oracle_merge_subhalos failed: {rc}")
+    return subs, out[:used.value].copy(), en[:used.value].copy()
 
 
 def id_to_index(table_ids, query_ids):
@@ -233,6 +255,57 @@ def ref_mask_subhalos(cols, subs, host_group):
                        env=dict(os.environ, OMP_NUM_THREADS="1"))
         raw = np.fromfile(fout, np.int64)
     return raw[:len(ids)].copy(), raw[len(ids):].copy()
+
+
+def merge_input(cat, subs_out, out_index):
+    """The state MergeSubhalos starts from, built from an unbinding's outputs
+    (records and output lists): each subhalo's list truncated to nsource
+    (TruncateSource), the lists concatenated into list_index and the records'
+    offset/count pointed at them.  Returns (list_index, records)."""
+    subs = np.ascontiguousarray(subs_out, SUB_DTYPE).copy()
+    parts, off = [], 0
+    for s in range(len(subs)):
+        o, n = int(subs["out_offset"][s]), int(subs["nsource"][s])
+        parts.append(np.asarray(out_index[o:o + n], np.int64))
+        subs["offset"][s], subs["count"][s] = off, n
+        off += n
+    subs["parent"] = cat.subs["parent"]
+    lst = np.concatenate(parts) if parts else np.zeros(0, np.int64)
+    return lst, subs
+
+
+def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True):
+    """The reference's own SubhaloSnapshot_t::MergeSubhalos
+    (oracle/_ref/hbt_ref_tracking, single thread) on the state given by the
+    columns, the lists (column positions) and the records.  Returns (records
+    after merging, their lists as column positions, per-entry energies: the
+    Energies of the bound block of every subhalo unbound again, NaN
+    elsewhere)."""
+    n = len(cols["x"])
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.ascontiguousarray(params, PARAMS_DTYPE).tobytes())
+            f.write(np.array([1 if merge_trapped else 0, len(subs), n, len(list_index)], np.int64).tobytes())
+            f.write(np.ascontiguousarray(cols["id"], np.int64).tobytes())
+            for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):
+                f.write(_f32(cols[k]).tobytes())
+            f.write(np.ascontiguousarray(cols["type"], np.int32).tobytes())
+            f.write(np.ascontiguousarray(list_index, np.int64).tobytes())
+            f.write(np.ascontiguousarray(subs, SUB_DTYPE).tobytes())
+            f.write(np.ascontiguousarray(host_group, np.int64).tobytes())
+        subprocess.run([REF_TRACKING, "merge", fin, fout], check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        raw = open(fout, "rb").read()
+    ns = len(subs)
+    out = np.frombuffer(raw, SUB_DTYPE, ns, 0).copy()
+    pos = ns * SUB_DTYPE.itemsize
+    total = int(np.frombuffer(raw, np.int64, 1, pos)[0])
+    pos += 8
+    lst = np.frombuffer(raw, np.int64, total, pos).copy()
+    pos += 8 * total
+    en = np.frombuffer(raw, np.float32, total, pos).copy()
+    return out, lst, en
 
 
 def ref_tree_potential(params, src, tgt, tmass=None):

@@ -663,6 +663,181 @@ int oracle_mask_subhalos(const hbt_particles *parts, const hbt_subhalo *subs, co
   return HBT_OK;
 }
 
+// MergeSubhalos (src/subhalo_merge.cpp:187-220): GlueHeadNests (the group's
+// other heads appended to the central's nest, src/subhalo_tracking.cpp:843-861),
+// FillHelpers (HostTrackId from the nests, :163-186; cores from the first
+// min(Nbound, 20) particles, BuildPosition/BuildVelocity :29-127),
+// DetectTraps (:136-161 with SinkDistance :129-134), then with merge_trapped
+// MergeRecursive from every group's central (:222-232) with MergeTo
+// (:244-262), Unbind of every IsMerged subhalo in index order and their
+// TruncateSource (:207-214).  list_index: the lists as column positions;
+// subs in/out (offset/count then index out_list, packed in subhalo order);
+// out_energy (optional): the re-unbound subhalos' binding energies along
+// their lists, NaN elsewhere.
+int oracle_merge_subhalos(const hbt_params *hp, const hbt_particles *parts, const int64_t *list_index,
+                          int64_t n_list, hbt_subhalo *subs, const int64_t *group, int64_t nsub,
+                          int merge_trapped, int64_t *out_list, float *out_energy, int64_t out_capacity,
+                          int64_t *out_total) {
+  if (!hp || !parts || (nsub > 0 && (!subs || !group || !list_index || !out_list))) return HBT_ERR_INVALID_ARG;
+  Params P(*hp);
+  PartCols C{parts};
+  std::vector<std::vector<Int>> nested(nsub);
+  std::map<Int, std::vector<Int>> heads;
+  for (Int s = 0; s < nsub; s++) {
+    if (subs[s].offset < 0 || subs[s].count < 0 || subs[s].offset + subs[s].count > n_list) return HBT_ERR_INVALID_ARG;
+    const Int p = subs[s].parent;
+    if (p >= 0) {
+      if (p >= nsub || group[p] != group[s]) return HBT_ERR_INVALID_ARG;
+      nested[p].push_back(s);
+    } else if (group[s] >= 0) {
+      heads[group[s]].push_back(s);
+    }
+  }
+  for (auto &g : heads)  // GlueHeadNests
+    nested[g.second[0]].insert(nested[g.second[0]].end(), g.second.begin() + 1, g.second.end());
+  std::vector<Int> host(nsub, -1);  // FillHostTrackIds
+  for (Int s = 0; s < nsub; s++)
+    for (Int c : nested[s]) host[c] = s;
+  std::vector<Sub> S(nsub);
+  for (Int s = 0; s < nsub; s++) {
+    S[s].rec = &subs[s];
+    S[s].list.assign(list_index + subs[s].offset, list_index + subs[s].offset + subs[s].count);
+    S[s].nbound = subs[s].nbound;
+    S[s].energy.assign(S[s].list.size(), std::numeric_limits<float>::quiet_NaN());
+  }
+  // FillCores: SubHelper_t::BuildPosition / BuildVelocity
+  struct Helper { float pos[3], vel[3], sig_r, sig_v; bool merged; };
+  std::vector<Helper> H(nsub);
+  for (Int s = 0; s < nsub; s++) {
+    Helper &h = H[s];
+    h.merged = false;
+    h.sig_r = h.sig_v = 0.f;
+    for (int j = 0; j < 3; j++) h.pos[j] = h.vel[j] = 0.f;
+    const Int nb = S[s].nbound;
+    if (nb == 0) continue;
+    const Int g0 = S[s].list[0];
+    if (nb == 1) {
+      for (int j = 0; j < 3; j++) { h.pos[j] = C.x(g0, j); h.vel[j] = C.v(g0, j); }
+      continue;
+    }
+    const Int n = nb > 20 ? 20 : nb;
+    double sx[3] = {0, 0, 0}, sx2[3] = {0, 0, 0}, msum = 0, origin[3];
+    for (int j = 0; j < 3; j++) origin[j] = C.x(g0, j);
+    for (Int i = 0; i < n; i++) {
+      const Int g = S[s].list[i];
+      const float m = C.m(g);
+      msum += m;
+      for (int j = 0; j < 3; j++) {
+        double dx;
+        if (P.periodic) dx = P.nearest((double)C.x(g, j) - origin[j]);  // float - double
+        else dx = C.x(g, j);
+        sx[j] += dx * m;
+        sx2[j] += dx * dx * m;
+      }
+    }
+    for (int j = 0; j < 3; j++) {
+      sx[j] /= msum;
+      sx2[j] /= msum;
+      h.pos[j] = (float)sx[j];
+      if (P.periodic) h.pos[j] = (float)((double)h.pos[j] + origin[j]);  // float += double
+      sx2[j] -= sx[j] * sx[j];
+    }
+    h.sig_r = (float)std::sqrt(sx2[0] + sx2[1] + sx2[2]);
+    double sv[3] = {0, 0, 0}, sv2[3] = {0, 0, 0};
+    msum = 0;
+    for (Int i = 0; i < n; i++) {
+      const Int g = S[s].list[i];
+      const float m = C.m(g);
+      msum += m;
+      for (int j = 0; j < 3; j++) {
+        const double dv = C.v(g, j);
+        sv[j] += dv * m;
+        sv2[j] += dv * dv * m;
+      }
+    }
+    for (int j = 0; j < 3; j++) {
+      sv[j] /= msum;
+      sv2[j] /= msum;
+      h.vel[j] = (float)sv[j];
+      sv2[j] -= sv[j] * sv[j];
+    }
+    h.sig_v = (float)std::sqrt(sv2[0] + sv2[1] + sv2[2]);
+  }
+  // DetectTraps
+  for (Int i = 0; i < nsub; i++) {
+    hbt_subhalo &r = subs[i];
+    if (r.sink_track_id != -1) continue;
+    for (Int h = host[i]; h >= 0; h = host[h]) {
+      if (S[h].nbound <= 1) continue;
+      const float d = periodic_distance(P, H[h].pos, r.mostbound_pos[0], r.mostbound_pos[1], r.mostbound_pos[2]);
+      float e[3];
+      for (int j = 0; j < 3; j++) e[j] = H[h].vel[j] - r.mostbound_vel[j];
+      const float v = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+      const float delta = d / H[h].sig_r + v / H[h].sig_v;
+      if (delta < 2.f) {
+        r.sink_track_id = h;
+        r.snapshot_index_of_sink = P.snap;
+        if (S[i].nbound > 1) H[h].merged = true;
+        break;
+      }
+    }
+  }
+  if (merge_trapped) {
+    std::function<void(Int)> merge_rec = [&](Int s) {
+      for (Int c : nested[s]) merge_rec(c);
+      hbt_subhalo &r = subs[s];
+      if (r.sink_track_id == -1 || r.snapshot_index_of_death != -1) return;
+      if (S[s].nbound > 1) {  // MergeTo
+        if (r.sink_track_id < 0 || r.sink_track_id >= nsub) return;
+        Sub &T = S[r.sink_track_id];
+        std::unordered_set<int64_t> ids;
+        for (Int g : T.list) ids.insert(parts->id[g]);
+        for (Int g : S[s].list)
+          if (ids.insert(parts->id[g]).second) T.list.push_back(g);
+        T.nbound += S[s].nbound;
+        S[s].list.resize(1);
+        S[s].energy.resize(1);
+        S[s].nbound = 1;
+        count_types(C, S[s], false);
+        float m = 0.f;
+        for (int t = 0; t < HBT_TYPE_MAX; t++) m += r.mbound_type[t];
+        r.mbound = m;
+      }
+      r.snapshot_index_of_death = P.snap;
+    };
+    for (auto &g : heads) merge_rec(g.second[0]);
+    Drand48 rng;
+    Tree tree;
+    for (Int s = 0; s < nsub; s++) {
+      if (!H[s].merged) continue;
+      subs[s].nbound_prev = INT64_MAX;
+      unbind(P, C, S[s], rng, tree);
+      Int ns = S[s].nbound <= 1 ? S[s].nbound : (Int)((float)S[s].nbound * P.relax);
+      if (ns > (Int)S[s].list.size()) ns = (Int)S[s].list.size();
+      S[s].list.resize(ns);
+      S[s].energy.resize(ns);
+    }
+  }
+  Int total = 0;
+  for (Int s = 0; s < nsub; s++) total += (Int)S[s].list.size();
+  if (out_total) *out_total = total;
+  if (total > out_capacity) return HBT_ERR_CAPACITY;
+  Int off = 0;
+  for (Int s = 0; s < nsub; s++) {
+    hbt_subhalo &r = subs[s];
+    r.nbound = S[s].nbound;
+    r.offset = off;
+    r.count = (Int)S[s].list.size();
+    r.list_len = r.nsource = r.count;
+    r.out_offset = off;
+    for (size_t i = 0; i < S[s].list.size(); i++) {
+      if (out_energy) out_energy[off] = S[s].energy[i];
+      out_list[off++] = S[s].list[i];
+    }
+  }
+  return HBT_OK;
+}
+
 int oracle_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
   if (nsub < 0 || (nsub > 0 && !subs) || !total) return HBT_ERR_INVALID_ARG;
   std::vector<Int> cap(nsub, 0);

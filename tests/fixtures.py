@@ -13,6 +13,7 @@ UNBIND_CASES = ["c1_small", "c1_small_bmp05", "c2_small", "c3_small", "c4_small"
 TREE_CASES = ["tree_c1", "tree_straddle", "tree_floor"]
 PROPS_CASES = ["c2_small", "c3_small", "c4_small", "hubble", "edge"]
 MASK_CASES = ["mask_a", "mask_b"]
+MERGE_CASES = ["merge_a", "merge_b"]
 
 
 def load_unbind(name):
@@ -46,3 +47,24 @@ def load_mask(name):
                 vx=np.zeros(n, np.float32), vy=np.zeros(n, np.float32), vz=np.zeros(n, np.float32),
                 mass=np.ones(n, np.float32), type=np.ones(n, np.int32))
     return cols, z["subs"].view(SUB_DTYPE).copy(), z["host_group"], z["ref_out"], z["ref_count"]
+
+
+class MergeFixture:
+    """A merge fixture: params, columns, the unbinding's state (list_index,
+    records, host groups) and the reference's MergeSubhalos outputs with
+    MergeTrappedSubhalos on (out_*: records, lists, energies) and off."""
+
+    def __init__(self, name):
+        z = np.load(os.path.join(GOLDEN, name + ".npz"))
+        self.name = name
+        self.params = z["params"].view(PARAMS_DTYPE).copy()
+        self.cols = {k: z[k] for k in ("id", "x", "y", "z", "vx", "vy", "vz", "mass", "type")}
+        self.list_index = z["list_index"].copy()
+        self.subs = z["subs"].view(SUB_DTYPE).copy()
+        self.host_group = z["host_group"].copy()
+        self.out = (z["out_subs"].view(SUB_DTYPE).copy(), z["out_list"].copy(), z["out_energy"].copy())
+        self.off = (z["off_subs"].view(SUB_DTYPE).copy(), z["off_list"].copy())
+
+
+def load_merge(name):
+    return MergeFixture(name)

@@ -16,10 +16,17 @@ Runs are single-threaded with sampling off.
 
     python tests/golden/make_golden.py --props-only   # only the props_ files
     python tests/golden/make_golden.py --mask-only    # only the mask_ files
+    python tests/golden/make_golden.py --merge-only   # only the merge_ files
 
 mask_<name>.npz: a catalogue of overlapping source lists in host groups (ID
 column, records, host groups) and what the reference's own
 SubhaloSnapshot_t::MaskSubhalos kept (oracle/_ref/hbt_ref_tracking).
+
+merge_<name>.npz: a catalogue of host groups with sinking satellites
+(synth.merging_catalogue), the state after the reference's unbinding (lists
+truncated to nsource, as column positions, with a few hand-set trap and death
+fields), and what the reference's own SubhaloSnapshot_t::MergeSubhalos left,
+with MergeTrappedSubhalos on (out_*) and off (off_*).
 """
 from __future__ import annotations
 
@@ -166,12 +173,54 @@ def save_mask(name, kw):
     print(f"{name}: {len(subs)} subhalos, {len(cols['id'])} entries, kept {cnt.sum()}")
 
 
+MERGE_CASES = {
+    "merge_a": dict(seed=1007, periodic=True),
+    "merge_b": dict(seed=1009, periodic=False, hz=0.07, scale_factor=0.8, snapshot_index=9),
+}
+
+
+def merge_state(cat, grp):
+    """The reference's unbinding of the catalogue as MergeSubhalos' input,
+    plus hand-set fields: the orphan of the second group alive (trapped but
+    too small to merge), the hot subhalo of the third group already trapped
+    (dead, skipped)."""
+    rs, ri, _ = oracle.ref_unbind_batch(cat)
+    lst, subs = oracle.merge_input(cat, rs, ri)
+    g1 = np.nonzero(grp == 1)[0]
+    g2 = np.nonzero(grp == 2)[0]
+    orphan = g1[cat.subs["nbound_prev"][g1] == 1][0]
+    subs["snapshot_index_of_death"][orphan] = -1
+    hot = g2[(subs["nbound"][g2] == 1) & (cat.subs["nbound_prev"][g2] > 1)][0]
+    subs["sink_track_id"][hot] = g2[0]
+    subs["snapshot_index_of_sink"][hot] = int(cat.params["snapshot_index"][0]) - 1
+    return lst, subs
+
+
+def save_merge(name, kw):
+    cat, grp = synth.merging_catalogue(**kw)
+    lst, subs = merge_state(cat, grp)
+    on = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, True)
+    off = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, False)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), kind="merge", params=cat.params.view(np.uint8),
+                        **{k: cat.cols[k] for k in cat.cols}, parent=cat.subs["parent"],
+                        list_index=lst, subs=subs.view(np.uint8), host_group=grp,
+                        out_subs=on[0].view(np.uint8), out_list=on[1], out_energy=on[2],
+                        off_subs=off[0].view(np.uint8), off_list=off[1])
+    merged = int(((on[0]["count"] == 1) & (subs["count"] > 1)).sum())
+    print(f"{name}: {len(subs)} subhalos, {cat.n_particles} particles, {merged} merged away, "
+          f"{int((on[0]['sink_track_id'] != subs['sink_track_id']).sum())} newly trapped")
+
+
 def main():
     if not oracle.have_reference():
         raise SystemExit("oracle/_ref/hbt_ref_harness is missing (make -C oracle ref)")
     if "--mask-only" in sys.argv:
         for name, kw in MASK_CASES.items():
             save_mask(name, kw)
+        return
+    if "--merge-only" in sys.argv:
+        for name, kw in MERGE_CASES.items():
+            save_merge(name, kw)
         return
     if "--props-only" in sys.argv:
         for name in PROPS_CASES:
@@ -186,6 +235,8 @@ def main():
         save_props(name)
     for name, kw in MASK_CASES.items():
         save_mask(name, kw)
+    for name, kw in MERGE_CASES.items():
+        save_merge(name, kw)
 
 
 if __name__ == "__main__":

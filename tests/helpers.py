@@ -95,3 +95,37 @@ def assert_tolerant(ref, got, band_rel=1e-5):
             run = np.maximum.accumulate(eg)
             assert np.all(run - eg <= band), f"subhalo {s}: bound order differs beyond the band"
     return stats
+
+
+MERGE_FIELDS = ["count", "nbound", "mbound", "nbound_type", "mbound_type", "sink_track_id",
+                "snapshot_index_of_death", "snapshot_index_of_sink", "ref_pos", "ref_vel", "mostbound_pos",
+                "mostbound_vel", "specific_self_potential_energy", "specific_self_kinetic_energy",
+                "specific_angular_momentum"]
+
+
+def merge_lists(subs, lst):
+    return [lst[int(o):int(o) + int(c)] for o, c in zip(subs["offset"], subs["count"])]
+
+
+def assert_merge(ref, got, ties=False):
+    """MergeSubhalos outputs: record fields bit for bit and every list entry
+    for entry.  ref = (records, lists, energies or None), got = (records,
+    lists); with ties, runs of exactly equal reference energy may come in any
+    order (the reference's std::sort leaves them unspecified)."""
+    rs, rl = ref[0], ref[1]
+    re_ = ref[2] if len(ref) > 2 else None
+    gs, gl = got
+    for f in MERGE_FIELDS:
+        a = np.ascontiguousarray(rs[f]).view(np.uint8).reshape(len(rs), -1)
+        b = np.ascontiguousarray(gs[f]).view(np.uint8).reshape(len(gs), -1)
+        bad = np.nonzero(~np.all(a == b, axis=1))[0]
+        assert len(bad) == 0, f"field {f} differs for subhalos {bad[:10]}: ref {rs[f][bad[:3]]} got {gs[f][bad[:3]]}"
+    ra, ga = merge_lists(rs, rl), merge_lists(gs, gl)
+    ea = merge_lists(rs, re_) if re_ is not None else None
+    for s in range(len(rs)):
+        a, b = ra[s], ga[s]
+        if ties and ea is not None and np.isfinite(ea[s]).any():
+            e_of = dict(zip(a.tolist(), ea[s].tolist()))
+            eb = np.array([e_of.get(p, np.nan) for p in b.tolist()], np.float32)
+            a, b = canonical_order(a, ea[s]), canonical_order(b, eb)
+        assert np.array_equal(a, b), f"list of subhalo {s} differs"

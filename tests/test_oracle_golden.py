@@ -6,10 +6,10 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import (MASK_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_props, load_tree,
-                      load_unbind)
-from hbtplus_amd.abi import PROPS_FIELDS
-from helpers import assert_exact
+from fixtures import (MASK_CASES, MERGE_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_merge,
+                      load_props, load_tree, load_unbind)
+from hbtplus_amd.abi import HBT_STATUS_CONVERGED, PROPS_FIELDS
+from helpers import assert_exact, assert_merge
 
 
 @pytest.mark.parametrize("name", UNBIND_CASES)
@@ -63,3 +63,20 @@ def test_oracle_mask_matches_reference(name):
     assert np.array_equal(cnt, ref_cnt)
     for o, c in zip(subs["offset"], cnt):
         assert np.array_equal(out[o:o + c], ref_out[o:o + c])
+
+
+@pytest.mark.parametrize("name", MERGE_CASES)
+@pytest.mark.parametrize("merge_trapped", [True, False])
+def test_oracle_merge_matches_reference(name, merge_trapped):
+    """MergeSubhalos restated (cores, trap detection, MergeRecursive/MergeTo,
+    re-unbinding, TruncateSource): records and lists of the reference's own
+    SubhaloSnapshot_t::MergeSubhalos, bit for bit and entry for entry"""
+    F = load_merge(name)
+    got = oracle.merge_subhalos(F.params, F.cols, F.list_index, F.subs, F.host_group, merge_trapped)
+    ref = F.out if merge_trapped else F.off
+    assert_merge(ref, got[:2])
+    if merge_trapped:  # binding energies of the bound blocks of the re-unbound subhalos
+        rs = ref[0]
+        for s in np.nonzero(got[0]["status"] & HBT_STATUS_CONVERGED)[0]:
+            o, nb = int(rs["offset"][s]), int(rs["nbound"][s])
+            assert np.array_equal(ref[2][o:o + nb], got[2][o:o + nb]), s

@@ -233,3 +233,14 @@ def test_mask_invariants():
         assert np.all(np.diff(pos) > 0) and np.all((pos >= o) & (pos < o + subs["count"][s]))
         if grp[s] < 0 or subs["nbound_prev"][s] <= 1:
             assert cnt[s] == subs["count"][s]
+
+
+@pytest.mark.parametrize("merge_trapped", [True, False])
+def test_merge_hand_derived(merge_trapped):
+    """MergeSubhalos restated on a configuration worked out by hand: trap
+    decisions from the core dispersions, the orphan host skipped in the
+    chain, glued heads, field halos, the kept-first MergeTo union"""
+    from merge_cases import check_hand_merge, hand_merge_case
+    params, cols, lst, subs, grp = hand_merge_case()
+    out, ol, _ = oracle.merge_subhalos(params, cols, lst, subs, grp, merge_trapped)
+    check_hand_merge(subs, out, ol, merge_trapped)
