@@ -138,6 +138,55 @@ def bench_mask(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_merge(args):
+    """MergeSubhalos after one snapshot's unbinding: 300 host groups laid out
+    with sinking satellites (about 2.3 M particles, scale 3 of the merging
+    test catalogue).  The state is the GPU's own FAST unbinding of the
+    catalogue; a call detects the traps, merges the trapped subhalos'
+    lists into their hosts and unbinds the merged hosts again.  Columns and
+    lists are resident on the device; the call's wall time is timed, against
+    the reference's own SubhaloSnapshot_t::MergeSubhalos on the host cores."""
+    import torch
+
+    from hbtplus_amd import abi, synth
+    from hbtplus_amd.device import GpuUnbinder
+    import oracle
+
+    cat, grp = synth.merging_catalogue(seed=1010, ngroups=300, scale=3.0)
+    params = cat.params.copy()
+    params["precision"] = abi.HBT_PREC_REPLAY if args.replay else abi.HBT_PREC_FAST
+    U = GpuUnbinder(params, device=0)
+    subs, idx, _ = U.unbind_batch(cat)
+    lst, sin = oracle.merge_input(cat, subs, idx)
+    hm = U.upload_merge(cat.cols, lst)
+    print(f"[bench] merge: {len(sin)} subhalos, {cat.n_particles} particles, {len(lst)} list entries",
+          file=sys.stderr, flush=True)
+    out = None
+    for _ in range(args.warmup):
+        got, out, used = U.merge_device(hm, sin, grp, True, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got, out, used = U.merge_device(hm, sin, grp, True, out)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    dev_ms, evals, passes = U.last_stats()
+    away = int(((got["count"] == 1) & (sin["count"] > 1)).sum())
+    trapped = int((got["sink_track_id"] != sin["sink_track_id"]).sum())
+    res = {"metric": "MergeSubhalos subhalos/s", "value": len(sin) / (ms / 1e3), "unit": "subhalos/s",
+           "ms_per_call": round(ms, 3), "subhalos": int(len(sin)), "particles": int(cat.n_particles),
+           "list_entries": int(len(lst)), "newly_trapped": trapped, "merged_away": away,
+           "reunbind": {"device_ms": round(dev_ms, 3), "evals": int(evals), "passes": int(passes)},
+           "dtype": "f32+f64" if not args.replay else "replay",
+           "note": "wall time per call; columns and lists resident on the device"}
+    if not args.no_cpu and oracle.have_reference():
+        threads = min(16, os.cpu_count() or 1)
+        secs = oracle.ref_merge_subhalos(params, cat.cols, lst, sin, grp, True, time_threads=threads)
+        res["cpu_baseline"] = {"kind": "reference", "ms_per_call": round(secs * 1e3, 1), "cores": threads,
+                               "value": len(sin) / secs, "unit": "subhalos/s"}
+    U.close()
+    print(json.dumps(res), flush=True)
+
+
 def bench_c5(args):
     """One rank's share: 200,000 subhalos given as source-ID lists into a
     shuffled snapshot (with a uniform background).  A step resolves every list
@@ -275,6 +324,8 @@ def main():
         return bench_c5(args)
     if args.config == "mask":
         return bench_mask(args)
+    if args.config == "merge":
+        return bench_merge(args)
     cat, desc = make_workload(args.config)
     params = cat.params.copy()
     params["precision"] = abi.HBT_PREC_REPLAY if args.replay else abi.HBT_PREC_FAST

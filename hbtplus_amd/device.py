@@ -137,26 +137,38 @@ class GpuUnbinder:
         check(rc, "hbt_mask_subhalos")
         return out.cpu().numpy()[:n].copy(), cnt
 
+    def upload_merge(self, cols, list_index):
+        """Particle columns and the lists (column positions) to the device."""
+        c = {k: self._dev(np.ascontiguousarray(cols[k])) for k in PARTICLE_COLUMNS}
+        return dict(cols=c, ps=particles_struct(c, ptr=lambda x: x.data_ptr()),
+                    li=self._dev(np.ascontiguousarray(list_index, np.int64)), n_list=len(list_index))
+
+    def merge_device(self, hm, subs, host_group, merge_trapped=True, out=None):
+        """hbt_merge_subhalos on uploaded columns and lists.  Returns (records
+        after merging, the device output tensor, entries used)."""
+        t = self.torch
+        subs = np.ascontiguousarray(subs, SUB_DTYPE).copy()
+        grp = np.ascontiguousarray(host_group, np.int64)
+        cap = int(subs["count"].sum()) + len(subs)
+        if out is None or out.numel() < cap:
+            out = t.full((max(cap, 1),), -1, dtype=t.int64, device=f"cuda:{self.device}")
+        used = ctypes.c_int64()
+        t.cuda.synchronize(self.device)
+        rc = self.lib.hbt_merge_subhalos(self.ctx, ctypes.byref(hm["ps"]), hm["li"].data_ptr(), hm["n_list"],
+                                         subs.ctypes.data, grp.ctypes.data, len(subs),
+                                         1 if merge_trapped else 0, out.data_ptr(), out.numel(),
+                                         ctypes.byref(used))
+        check(rc, "hbt_merge_subhalos")
+        return subs, out, used.value
+
     def merge_subhalos(self, cols, list_index, subs, host_group, merge_trapped=True):
         """MergeSubhalos after unbinding.  cols: host particle columns;
         list_index: the lists as column positions (subs offset/count index
         it); subs: post-unbinding records.  Returns (records after merging,
         their lists as column positions laid out by the records), numpy."""
-        t = self.torch
-        c = {k: self._dev(np.ascontiguousarray(cols[k])) for k in PARTICLE_COLUMNS}
-        ps = particles_struct(c, ptr=lambda x: x.data_ptr())
-        li = self._dev(np.ascontiguousarray(list_index, np.int64))
-        subs = np.ascontiguousarray(subs, SUB_DTYPE).copy()
-        grp = np.ascontiguousarray(host_group, np.int64)
-        cap = int(subs["count"].sum()) + len(subs)
-        out = t.full((max(cap, 1),), -1, dtype=t.int64, device=f"cuda:{self.device}")
-        used = ctypes.c_int64()
-        t.cuda.synchronize(self.device)
-        rc = self.lib.hbt_merge_subhalos(self.ctx, ctypes.byref(ps), li.data_ptr(), len(list_index),
-                                         subs.ctypes.data, grp.ctypes.data, len(subs),
-                                         1 if merge_trapped else 0, out.data_ptr(), cap, ctypes.byref(used))
-        check(rc, "hbt_merge_subhalos")
-        return subs, out.cpu().numpy()[:used.value].copy()
+        hm = self.upload_merge(cols, list_index)
+        subs, out, used = self.merge_device(hm, subs, host_group, merge_trapped)
+        return subs, out.cpu().numpy()[:used].copy()
 
     # execution tuning (results do not depend on it) -----------------------
     def get_tuning(self):

@@ -8,6 +8,8 @@
 //   Subhalo_t::RecursiveUnbind      src/subhalo.h:112, src/subhalo_unbind.cpp:432-447
 //   Subhalo_t::TruncateSource       src/subhalo.h:114, src/subhalo_unbind.cpp:449-458
 //   RefineParticles                 src/subhalo.h:245, src/subhalo_unbind.cpp:460-516
+//   SubhaloSnapshot_t::MergeSubhalos
+//                                   src/subhalo.h:248, src/subhalo_merge.cpp:187-262
 //   GravityTree_t::Build / EvaluatePotential / BindingEnergy
 //                                   src/oct_tree.h:62-63, src/gravity_tree.h:15-16
 // Particle lists stay AoS on the host (as in the reference) and are converted
@@ -148,6 +150,108 @@ class Unbinder {
       }
     }
     run(ptr, parent, plain, /*truncate=*/true, /*recursive=*/true);
+  }
+
+  // SubhaloSnapshot_t::MergeSubhalos after RefineParticles: trap detection
+  // along the (glued) nests, MergeTo in post-order from every group's central,
+  // re-unbinding and TruncateSource of the hosts that received subhalos.
+  // HostHaloId: the subhalos' host halos (-1 for field subhalos); the central
+  // of a group is its first subhalo in index order that no other nests, and
+  // a nested subhalo shares its host's halo.
+  void MergeSubhalos(std::vector<Subhalo_t> &subs, const std::vector<int64_t> &HostHaloId,
+                     bool MergeTrappedSubhalos = true) {
+    const int64_t nsub = (int64_t)subs.size();
+    if ((int64_t)HostHaloId.size() != nsub) throw Error(HBT_ERR_INVALID_ARG, "one host halo per subhalo");
+    std::vector<int64_t> parent(nsub, -1);
+    for (int64_t i = 0; i < nsub; i++)
+      for (int64_t c : subs[i].NestedSubhalos) {
+        if (c < 0 || c >= nsub) throw Error(HBT_ERR_INVALID_ARG, "bad nested index");
+        parent[c] = i;
+      }
+    int64_t n = 0;
+    for (auto &S : subs) n += (int64_t)S.Particles.size();
+    std::vector<int64_t> id(n), lst(n);
+    std::vector<float> col[7];
+    for (auto &c : col) c.resize(n);
+    std::vector<int32_t> type(n);
+    std::vector<const Particle_t *> src(n);
+    std::vector<hbt_subhalo> rec(nsub);
+    int64_t off = 0;
+    for (int64_t s = 0; s < nsub; s++) {
+      const Subhalo_t &S = subs[s];
+      hbt_subhalo &r = rec[s];
+      std::memset(&r, 0, sizeof(r));
+      r.offset = off;
+      r.count = (int64_t)S.Particles.size();
+      r.parent = parent[s];
+      r.nbound_prev = S.Nbound;
+      r.nbound = S.Nbound;
+      r.mbound = S.Mbound;
+      for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = S.NboundType[t]; r.mbound_type[t] = S.MboundType[t]; }
+      r.sink_track_id = S.SinkTrackId;
+      r.snapshot_index_of_death = S.SnapshotIndexOfDeath;
+      r.snapshot_index_of_sink = S.SnapshotIndexOfSink;
+      for (int j = 0; j < 3; j++) {
+        r.ref_pos[j] = S.ComovingAveragePosition[j];
+        r.ref_vel[j] = S.PhysicalAverageVelocity[j];
+        r.mostbound_pos[j] = S.ComovingMostBoundPosition[j];
+        r.mostbound_vel[j] = S.PhysicalMostBoundVelocity[j];
+        r.specific_angular_momentum[j] = S.SpecificAngularMomentum[j];
+      }
+      r.specific_self_potential_energy = S.SpecificSelfPotentialEnergy;
+      r.specific_self_kinetic_energy = S.SpecificSelfKineticEnergy;
+      for (const Particle_t &p : S.Particles) {
+        id[off] = p.Id;
+        for (int j = 0; j < 3; j++) { col[j][off] = p.ComovingPosition[j]; col[3 + j][off] = p.PhysicalVelocity[j]; }
+        col[6][off] = p.Mass;
+        type[off] = p.Type;
+        src[off] = &p;
+        lst[off] = off;
+        off++;
+      }
+    }
+    const int64_t cap = n + nsub;
+    DevArray<int64_t> d_id(n), d_lst(n), d_out(cap);
+    DevArray<float> d_col[7] = {DevArray<float>(n), DevArray<float>(n), DevArray<float>(n), DevArray<float>(n),
+                                DevArray<float>(n), DevArray<float>(n), DevArray<float>(n)};
+    DevArray<int32_t> d_type(n);
+    d_id.put(id);
+    d_lst.put(lst);
+    for (int i = 0; i < 7; i++) d_col[i].put(col[i]);
+    d_type.put(type);
+    hbt_particles P{n, d_id.p, d_col[0].p, d_col[1].p, d_col[2].p, d_col[3].p, d_col[4].p, d_col[5].p,
+                    d_col[6].p, d_type.p};
+    int64_t used = 0;
+    check(hbt_merge_subhalos(ctx_, &P, d_lst.p, n, rec.data(), HostHaloId.data(), nsub,
+                             MergeTrappedSubhalos ? 1 : 0, d_out.p, cap, &used),
+          "hbt_merge_subhalos");
+    std::vector<int64_t> out(used);
+    d_out.get(out);
+    std::vector<std::vector<Particle_t>> lists(nsub);
+    for (int64_t s = 0; s < nsub; s++) {
+      lists[s].resize(rec[s].count);
+      for (int64_t i = 0; i < rec[s].count; i++) lists[s][i] = *src[out[rec[s].offset + i]];
+    }
+    for (int64_t s = 0; s < nsub; s++) {
+      Subhalo_t &S = subs[s];
+      const hbt_subhalo &r = rec[s];
+      S.Particles.swap(lists[s]);
+      S.Nbound = r.nbound;
+      S.Mbound = r.mbound;
+      for (int t = 0; t < HBT_TYPE_MAX; t++) { S.NboundType[t] = r.nbound_type[t]; S.MboundType[t] = r.mbound_type[t]; }
+      S.SnapshotIndexOfDeath = r.snapshot_index_of_death;
+      S.SnapshotIndexOfSink = r.snapshot_index_of_sink;
+      S.SinkTrackId = r.sink_track_id;
+      S.SpecificSelfPotentialEnergy = r.specific_self_potential_energy;
+      S.SpecificSelfKineticEnergy = r.specific_self_kinetic_energy;
+      for (int j = 0; j < 3; j++) {
+        S.SpecificAngularMomentum[j] = r.specific_angular_momentum[j];
+        S.ComovingAveragePosition[j] = r.ref_pos[j];
+        S.PhysicalAverageVelocity[j] = r.ref_vel[j];
+        S.ComovingMostBoundPosition[j] = r.mostbound_pos[j];
+        S.PhysicalMostBoundVelocity[j] = r.mostbound_vel[j];
+      }
+    }
   }
 
   hbt_ctx *context() { return ctx_; }

@@ -274,13 +274,14 @@ def merge_input(cat, subs_out, out_index):
     return lst, subs
 
 
-def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True):
+def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True, time_threads=0):
     """The reference's own SubhaloSnapshot_t::MergeSubhalos
     (oracle/_ref/hbt_ref_tracking, single thread) on the state given by the
     columns, the lists (column positions) and the records.  Returns (records
     after merging, their lists as column positions, per-entry energies: the
     Energies of the bound block of every subhalo unbound again, NaN
-    elsewhere)."""
+    elsewhere).  time_threads > 0: returns only the wall seconds the
+    reference spends in MergeSubhalos on that many OpenMP threads."""
     n = len(cols["x"])
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -294,9 +295,12 @@ def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped
             f.write(np.ascontiguousarray(list_index, np.int64).tobytes())
             f.write(np.ascontiguousarray(subs, SUB_DTYPE).tobytes())
             f.write(np.ascontiguousarray(host_group, np.int64).tobytes())
-        subprocess.run([REF_TRACKING, "merge", fin, fout], check=True,
-                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        nt = str(time_threads) if time_threads > 0 else "1"
+        subprocess.run([REF_TRACKING, "merge_time" if time_threads > 0 else "merge", fin, fout], check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS=nt))
         raw = open(fout, "rb").read()
+    if time_threads > 0:
+        return float(np.frombuffer(raw, np.float64, 1)[0])
     ns = len(subs)
     out = np.frombuffer(raw, SUB_DTYPE, ns, 0).copy()
     pos = ns * SUB_DTYPE.itemsize

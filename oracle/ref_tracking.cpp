@@ -26,12 +26,16 @@
 //          below); int64 total; int64 list[total] (column positions);
 //          float energy[total] (Energies of the bound block where the
 //          subhalo was unbound again, NaN elsewhere)
+//   ref_tracking merge_time in.bin out.bin
+//     as merge; out: double seconds spent in MergeSubhalos (OMP_NUM_THREADS
+//     threads, the reference's own parallel loops)
 //
 // The member table is built by the reference (MemberShipTable_t::Build, which
 // orders each group by Mbound): every subhalo gets an Mbound that decreases
 // with its index and the group's first head the largest, so the group's
 // central and the order of its heads are those of hbt_mask_subhalos.
 #include <mpi.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -157,7 +161,9 @@ void configure(const hbt_params &p, Snapshot_t &epoch) {
   epoch.Cosmology.OmegaZ = 0.3f;
 }
 
-int run_merge(FILE *in, FILE *out) {
+// timed: only the wall seconds of MergeSubhalos (OpenMP as the reference runs
+// it) are written
+int run_merge(FILE *in, FILE *out, bool timed) {
   hbt_params p;
   int64_t flag, nsub, ncol, nlist;
   rd(in, &p, 1); rd(in, &flag, 1); rd(in, &nsub, 1); rd(in, &ncol, 1); rd(in, &nlist, 1);
@@ -216,7 +222,13 @@ int run_merge(FILE *in, FILE *out) {
     else if (grp[s] >= 0) heads[grp[s]].push_back(s);
   }
   build_members(snap, grp, heads);
+  const double t0 = omp_get_wtime();
   snap.MergeSubhalos();
+  const double secs = omp_get_wtime() - t0;
+  if (timed) {
+    wr(out, &secs, 1);
+    return 0;
+  }
   int64_t total = 0;
   for (auto &S : snap.Subhalos) total += (int64_t)S.Particles.size();
   std::vector<int64_t> olist;
@@ -258,14 +270,15 @@ int run_merge(FILE *in, FILE *out) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge|merge_time in.bin out.bin\n", argv[0]); return 1; }
   MPI_Init(&argc, &argv);
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
   int rc = 1;
   if (!in || !out) fprintf(stderr, "cannot open files\n");
   else if (!strcmp(argv[1], "mask")) rc = run_mask(in, out);
-  else if (!strcmp(argv[1], "merge")) rc = run_merge(in, out);
+  else if (!strcmp(argv[1], "merge")) rc = run_merge(in, out, false);
+  else if (!strcmp(argv[1], "merge_time")) rc = run_merge(in, out, true);
   if (in) fclose(in);
   if (out) fclose(out);
   MPI_Finalize();

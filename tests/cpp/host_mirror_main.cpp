@@ -2,11 +2,15 @@
 // reference caller would: fills Subhalo_t objects with AoS Particle_t lists,
 // calls RefineParticles (or Unbind per subhalo), writes the results.
 //
-// usage: hbt_host_test <in.bin> <out.bin> <mode: batch|plain|unbind>
+// usage: hbt_host_test <in.bin> <out.bin> <mode: batch|plain|unbind|merge>
 // in.bin : hbt_params, int64 nsub, int64 npart, columns (id, x..vz, mass, type),
 //          hbt_subhalo records (offset/count/parent/nbound_prev/frames/markers)
+//          merge: then int64 HostHaloId[nsub]; RefineParticles followed by
+//          MergeSubhalos
 // out.bin: per subhalo: int64 nbound, int64 nparticles, float mbound,
-//          float ref_pos[3], float ref_vel[3], then the particle IDs in order
+//          float ref_pos[3], float ref_vel[3], int64 SinkTrackId,
+//          int32 SnapshotIndexOfDeath, int32 SnapshotIndexOfSink, then the
+//          particle IDs in order
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -35,6 +39,9 @@ int main(int argc, char **argv) {
   rd(in, type.data(), n);
   std::vector<hbt_subhalo> rec(nsub);
   rd(in, rec.data(), nsub);
+  const bool merge = !strcmp(argv[3], "merge");
+  std::vector<int64_t> host(merge ? nsub : 0);
+  if (merge) rd(in, host.data(), nsub);
   fclose(in);
 
   hbt::UnbindConfig cfg;
@@ -77,6 +84,7 @@ int main(int argc, char **argv) {
       for (auto &S : subs) U.Unbind(S);
     } else {
       U.RefineParticles(subs, !strcmp(argv[3], "plain"));
+      if (merge) U.MergeSubhalos(subs, host, true);
     }
   } catch (const hbt::Error &e) {
     fprintf(stderr, "error %d: %s\n", e.code, e.what());
@@ -90,6 +98,9 @@ int main(int argc, char **argv) {
     fwrite(&S.Mbound, 4, 1, out);
     fwrite(S.ComovingAveragePosition.data(), 4, 3, out);
     fwrite(S.PhysicalAverageVelocity.data(), 4, 3, out);
+    fwrite(&S.SinkTrackId, 8, 1, out);
+    fwrite(&S.SnapshotIndexOfDeath, 4, 1, out);
+    fwrite(&S.SnapshotIndexOfSink, 4, 1, out);
     for (auto &q : S.Particles) fwrite(&q.Id, 8, 1, out);
   }
   fclose(out);

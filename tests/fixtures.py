@@ -52,7 +52,8 @@ def load_mask(name):
 class MergeFixture:
     """A merge fixture: params, columns, the unbinding's state (list_index,
     records, host groups) and the reference's MergeSubhalos outputs with
-    MergeTrappedSubhalos on (out_*: records, lists, energies) and off."""
+    MergeTrappedSubhalos on (out_*: records, lists, energies) and off; pipe:
+    merging on the unbinding's state without the hand-set fields."""
 
     def __init__(self, name):
         z = np.load(os.path.join(GOLDEN, name + ".npz"))
@@ -64,6 +65,8 @@ class MergeFixture:
         self.host_group = z["host_group"].copy()
         self.out = (z["out_subs"].view(SUB_DTYPE).copy(), z["out_list"].copy(), z["out_energy"].copy())
         self.off = (z["off_subs"].view(SUB_DTYPE).copy(), z["off_list"].copy())
+        self.pipe = (z["pipe_subs"].view(SUB_DTYPE).copy(), z["pipe_list"].copy(), z["pipe_energy"].copy())
+        self.parent = z["parent"].copy()
 
 
 def load_merge(name):

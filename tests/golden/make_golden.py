@@ -26,7 +26,8 @@ merge_<name>.npz: a catalogue of host groups with sinking satellites
 (synth.merging_catalogue), the state after the reference's unbinding (lists
 truncated to nsource, as column positions, with a few hand-set trap and death
 fields), and what the reference's own SubhaloSnapshot_t::MergeSubhalos left,
-with MergeTrappedSubhalos on (out_*) and off (off_*).
+with MergeTrappedSubhalos on (out_*) and off (off_*); pipe_*: the same on
+the unbinding's state as it is (RefineParticles then MergeSubhalos).
 """
 from __future__ import annotations
 
@@ -180,12 +181,13 @@ MERGE_CASES = {
 
 
 def merge_state(cat, grp):
-    """The reference's unbinding of the catalogue as MergeSubhalos' input,
-    plus hand-set fields: the orphan of the second group alive (trapped but
-    too small to merge), the hot subhalo of the third group already trapped
-    (dead, skipped)."""
+    """The reference's unbinding of the catalogue as MergeSubhalos' input
+    (lists, records), and a copy with hand-set fields: the orphan of the
+    second group alive (trapped but too small to merge), the hot subhalo of
+    the third group already trapped (dead, skipped)."""
     rs, ri, _ = oracle.ref_unbind_batch(cat)
-    lst, subs = oracle.merge_input(cat, rs, ri)
+    lst, plain = oracle.merge_input(cat, rs, ri)
+    subs = plain.copy()
     g1 = np.nonzero(grp == 1)[0]
     g2 = np.nonzero(grp == 2)[0]
     orphan = g1[cat.subs["nbound_prev"][g1] == 1][0]
@@ -193,19 +195,21 @@ def merge_state(cat, grp):
     hot = g2[(subs["nbound"][g2] == 1) & (cat.subs["nbound_prev"][g2] > 1)][0]
     subs["sink_track_id"][hot] = g2[0]
     subs["snapshot_index_of_sink"][hot] = int(cat.params["snapshot_index"][0]) - 1
-    return lst, subs
+    return lst, subs, plain
 
 
 def save_merge(name, kw):
     cat, grp = synth.merging_catalogue(**kw)
-    lst, subs = merge_state(cat, grp)
+    lst, subs, plain = merge_state(cat, grp)
     on = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, True)
     off = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, False)
+    pipe = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, plain, grp, True)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), kind="merge", params=cat.params.view(np.uint8),
                         **{k: cat.cols[k] for k in cat.cols}, parent=cat.subs["parent"],
                         list_index=lst, subs=subs.view(np.uint8), host_group=grp,
                         out_subs=on[0].view(np.uint8), out_list=on[1], out_energy=on[2],
-                        off_subs=off[0].view(np.uint8), off_list=off[1])
+                        off_subs=off[0].view(np.uint8), off_list=off[1],
+                        pipe_subs=pipe[0].view(np.uint8), pipe_list=pipe[1], pipe_energy=pipe[2])
     merged = int(((on[0]["count"] == 1) & (subs["count"] > 1)).sum())
     print(f"{name}: {len(subs)} subhalos, {cat.n_particles} particles, {merged} merged away, "
           f"{int((on[0]['sink_track_id'] != subs['sink_track_id']).sum())} newly trapped")
