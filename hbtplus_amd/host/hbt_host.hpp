@@ -9,7 +9,7 @@
 //   Subhalo_t::TruncateSource       src/subhalo.h:114, src/subhalo_unbind.cpp:449-458
 //   RefineParticles                 src/subhalo.h:245, src/subhalo_unbind.cpp:460-516
 //   SubhaloSnapshot_t::MergeSubhalos
-//                                   src/subhalo.h:248, src/subhalo_merge.cpp:187-262
+//                                   src/subhalo.h:247, src/subhalo_merge.cpp:136-262
 //   GravityTree_t::Build / EvaluatePotential / BindingEnergy
 //                                   src/oct_tree.h:62-63, src/gravity_tree.h:15-16
 // Particle lists stay AoS on the host (as in the reference) and are converted
