@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
         const int pos = l + W.pre[s] + __popc(bl);
         const int li = (int)W.v0[s];
         W.items[ib + pos].pm = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
-        W.items[ib + pos].ax = make_int2(0, (int)(ib + pos + 1));
+        W.items[ib + pos].ax = make_int2(0, pos + 1);
         if (l + 1 == S.n_src) W.seg[k].n_items = pos + 1;
         const int lo = W.nlo[s];
         if (lo >= 0) {
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
           const float len = node_len(S, v);
           const float lsq = len * len;
           W.items[ib + npos].pm = make_float4(0.f, 0.f, 0.f, 0.f);
-          W.items[ib + npos].ax = make_int2(__float_as_int(lsq), (int)(ib + hpos + 1));
+          W.items[ib + npos].ax = make_int2(__float_as_int(lsq), hpos + 1);
           W.npos[s] = (int)(ib + npos);
         }
       }
@@ -440,9 +440,10 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
 // Node moments: mass and centre of mass accumulated in double over the float
 // values of the children in son order, stored as float
 // (UpdateInternalNodes/ProcessNode/FillNodeCenter, src/gravity_tree.cpp:18-77).
-__device__ __forceinline__ void node_moments(const Work &W, int g) {
+// Skips are segment-local item positions; ib is the segment's first item.
+__device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
 #pragma clang fp contract(off)
-  const int end = W.items[g].ax.y;
+  const int end = (int)(ib + W.items[g].ax.y);
   double mass = 0., cx = 0., cy = 0., cz = 0.;
   int c = g + 1;
   while (c < end) {
@@ -452,7 +453,7 @@ __device__ __forceinline__ void node_moments(const Work &W, int g) {
     cx += (double)q.x * m;
     cy += (double)q.y * m;
     cz += (double)q.z * m;
-    const int nx = W.items[c].ax.y;
+    const int nx = (int)(ib + W.items[c].ax.y);
     c = nx > c ? nx : end;  // skips always advance
   }
   W.items[g].pm = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
@@ -460,15 +461,35 @@ __device__ __forceinline__ void node_moments(const Work &W, int g) {
 
 // Moments of the nodes at depth v (one launch per depth, deepest first: a
 // kernel boundary orders the children's moments before their parents read
-// them).  Each node is handled by the slot of its leftmost split pair.
-__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v) {
-  SLOT_LOOP(s, W) {
+// them), over the slots [0, nslots) of the large segments.  Each node is
+// handled by the slot of its leftmost split pair.
+__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v, int64_t nslots) {
+  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * BLOCK) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
     if (l + 1 >= S.n_src || W.dl[s] != v || W.nlo[s] < 0) continue;
-    node_moments(W, W.npos[s]);
+    node_moments(W, 2 * S.ibase, W.npos[s]);
+  }
+}
+
+// Moments of the small segments (from k_small on): one workgroup per
+// segment walks its depths deepest first, a workgroup barrier ordering each
+// depth's moments before the next depth reads them.
+__global__ void __launch_bounds__(BLOCK) moments_small_kernel(Work W, int k_small) {
+  for (int k = k_small + (int)blockIdx.x; k < W.nseg; k += (int)gridDim.x) {
+    const Seg &S = W.seg[k];
+    const int n = S.n_src;
+    if (n < 2) continue;  // uniform across the block
+    const int64_t b = S.ibase, ib = 2 * b;
+    for (int v = min(max(S.D, S.Dfull), MAX_DEPTH); v >= 0; v--) {
+      for (int l = threadIdx.x; l + 1 < n; l += BLOCK) {
+        const int64_t s = b + l;
+        if (W.dl[s] == v && W.nlo[s] >= 0) node_moments(W, ib, W.npos[s]);
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -496,7 +517,8 @@ __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int
     const double r2 = dx * dx + dy * dy + dz * dz;
     const float lsq = __int_as_float(a.x);
     if ((double)lsq > r2 * (double)P.theta2) { j++; if (COUNT) no++; continue; }
-    j = a.y > j ? a.y : end;  // skips always advance
+    const int nj = (int)(ib + a.y);
+    j = nj > j ? nj : end;  // skips always advance
     if (COUNT) ni++;
     pot += kernel_term_d((double)q.w, sqrt(r2), h, hinv);
   }
@@ -527,7 +549,8 @@ __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n
     }
     const float r2 = pair_r2(dx, dy, dz);
     if (__int_as_float(a.x) > r2 * P.theta2) { j++; if (COUNT) no++; continue; }
-    j = a.y > j ? a.y : end;  // skips always advance
+    const int nj = (int)(ib + a.y);
+    j = nj > j ? nj : end;  // skips always advance
     if (COUNT) ni++;
     acc.add(kernel_term_f(q.w, r2, h2, hinv));
   }
@@ -570,13 +593,29 @@ template <int GROUP>
 __device__ __forceinline__ u64 group_lanes() {
   if (GROUP == WAVE) return ~0ull;
   const int base = (int)(threadIdx.x & (WAVE - 1)) & ~(GROUP - 1);
-  return ((1ull << GROUP) - 1ull) << base;
+  return (GROUP >= WAVE ? ~0ull : ((1ull << (GROUP & 63)) - 1ull)) << base;
 }
 
 template <int GROUP>
 __device__ __forceinline__ bool group_any(bool p, u64 lanes) {
   if (GROUP == WAVE) return __any(p);
   return (__ballot(p) & lanes) != 0ull;
+}
+
+// The same vote with the whole wave active, on the scalar unit: the wave's
+// ballot is OR-folded within each aligned group of GROUP lanes, spread back
+// over the group, and read per lane as a lane mask (inverse ballot).
+template <int GROUP>
+__device__ __forceinline__ bool group_any_wave(bool p) {
+  u64 b = __builtin_amdgcn_ballot_w64(p);
+  if (GROUP == WAVE) return b != 0ull;
+#pragma unroll
+  for (int d = 1; d < GROUP; d <<= 1) b |= b >> d;
+  constexpr u64 first = GROUP == 8 ? 0x0101010101010101ull
+                                   : (GROUP == 16 ? 0x0001000100010001ull : 0x0000000100000001ull);
+  constexpr u64 fill = GROUP >= 32 ? 0xffffffffull : ((1ull << (GROUP & 31)) - 1ull);
+  b = (b & first) * fill;
+  return __builtin_amdgcn_inverse_ballot_w64(b);
 }
 
 // Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN):
@@ -644,14 +683,17 @@ struct ReplayTerm {
   __device__ __forceinline__ double result() const { return pot; }
 };
 
-// The items come in as a restrict-qualified kernel argument: nothing in the
-// walk kernel writes them, so reads at a group-uniform index compile to
-// scalar loads when GROUP = 64.  Then both possible successors of item j (j+1
-// and skip(j)) are fetched while j is evaluated, so one load latency is exposed
-// per step instead of one per dependent read.  Smaller groups read with vector
-// loads: every lane loads and tests its group's item before any branch (lanes
-// off their path discard the result), so the item's loads issue together at
-// the top of the step.  Reads past the end are clamped to the segment's last
+// The items come in as a restrict-qualified kernel argument pointing at the
+// segment's first item (wave-uniform; item positions and skips are
+// segment-local): nothing in the walk kernel writes them, so reads at a
+// group-uniform index compile to scalar loads when GROUP = 64, and both
+// possible successors of item j (j+1 and skip(j)) are fetched while j is
+// evaluated.  Smaller groups read with vector loads at a 32-bit byte offset
+// from the scalar base: every lane loads and tests its group's item before
+// any branch (lanes off their path discard the result), so the item's loads
+// issue together at the top of the step, and the group's vote runs on the
+// scalar unit.  (Prefetching both successors there, or loading through the
+// group's first lane and a crossbar broadcast, measured slower.)  Reads past the end are clamped to the segment's last
 // item, a leaf whose skip is `end`: a finished group keeps reading it and
 // stays finished.  Skips always advance (max with j+1).
 template <int GROUP, bool COUNT, class Term>
@@ -694,15 +736,18 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
       a = go ? a1 : as;
     }
   } else {
+    const char *base = (const char *)it;
     while (__any(j < end)) {
       const bool on = j < end;
-      const Item *pj = it + (unsigned)min(j, last);
-      const float4 q = pj->pm;
-      const int2 a = pj->ax;
+      const unsigned off = (unsigned)min(j, last) * (unsigned)sizeof(Item);
+      const float4 q = *(const float4 *)(base + off);
+      const int2 a = *(const int2 *)(base + off + 16);
+      const int j1 = j + 1, js = max(a.y, j1);
       const bool path = on & (until <= j);
       const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
       const bool opened = path & t.opens(P, __int_as_float(a.x), r2);
       const bool accept = path & !opened;
+      const bool go = group_any_wave<GROUP>(opened);  // voted before the branch below
       if (accept) t.add(q.w, r2);
       if (COUNT) {
         if (on && leader) ns += GROUP;
@@ -710,8 +755,7 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
         ni += accept ? 1u : 0u;
       }
       until = accept ? a.y : until;
-      const bool go = group_any<GROUP>(opened, lanes);
-      j = go ? j + 1 : max(a.y, j + 1);
+      j = go ? j1 : js;
     }
   }
   return t.result();
@@ -761,6 +805,59 @@ __device__ __forceinline__ double walk_binned(const Item *__restrict__ it, const
   }
 }
 
+// A target's binding energy from its potential.
+// Full pass: E = 0.5|dv|^2 + phi (BindingEnergy, src/subhalo_unbind.cpp:341-354).
+// Correction pass: E += OldVel.RefVelDiff + dK - phi_removed (:319-328).
+template <bool REPLAY>
+__device__ __forceinline__ void finish_target(const Work &W, const DevParams &P, const Seg &S, int64_t b,
+                                              int li, float x, float y, float z, double pot) {
+#pragma clang fp contract(off)
+  pot = pot * (double)P.G / (double)P.a;
+  const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
+  if (!(S.flags & F_CORR)) {
+    float dv[3];
+    rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
+    if (REPLAY) {
+      W.E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
+    } else {
+      const double kin = 0.5 * ((double)dv[0] * dv[0] + (double)dv[1] * dv[1] + (double)dv[2] * dv[2]);
+      W.E[b + li] = (float)(kin + pot);
+    }
+  } else {
+    float ov[3];
+    rel_vel(P, x, y, z, vx, vy, vz, S.old_pos, S.old_vel, ov);
+    const float kin = vdot_f(ov, S.rvd) + S.dK;
+    W.E[b + li] = (float)((double)W.E[b + li] + ((double)kin - pot));
+  }
+}
+
+// The targets of one wave-aligned chunk of slots, one per lane.
+template <int GROUP, bool COUNT, bool REPLAY>
+__device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, const Item *__restrict__ it,
+                                           int64_t s, unsigned &ni, unsigned &no, unsigned &nt,
+                                           unsigned &ns) {
+  // segments are wave-aligned: the whole wave belongs to one segment
+  const int k = __builtin_amdgcn_readfirstlane(W.seg_of_slot[s]);
+  const Seg &S = W.seg[k];
+  const int64_t b = S.ibase;
+  const int l = (int)(s - b);
+  const bool live = l < S.n_tgt;
+  if (!__any(live)) return;
+  if (COUNT && live) nt++;
+  const int li = live ? W.sp[s] : 0;
+  const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
+  const float tm = (S.flags & F_CORR) ? 0.f : W.pm[b + li];
+  const Item *seg_it = it + 2 * b;  // item positions and skips are segment-local
+  const int end = S.n_items;
+  double pot;
+  if (GROUP == 0)
+    pot = walk_binned<COUNT, REPLAY>(seg_it, P, S, 0, end, live, x, y, z, tm, ni, no, ns);
+  else
+    pot = walk_group_dispatch<GROUP == 0 ? 64 : GROUP, COUNT, REPLAY>(seg_it, P, S, 0, end, live, x, y, z,
+                                                                     tm, ni, no, ns);
+  if (live) finish_target<REPLAY>(W, P, S, b, li, x, y, z, pot);
+}
+
 // Binding energies with the group-shared walk (same results as walk_kernel
 // for every group size).  GROUP = 0: lanes per group from each segment's bin.
 template <int GROUP, bool COUNT, bool REPLAY>
@@ -768,46 +865,8 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
                                                             const Item *__restrict__ it, int xcd_run) {
   unsigned ni = 0, no = 0, nt = 0, ns = 0;
   const int vb = xcd_block(xcd_run);
-  for (int64_t s = (int64_t)vb * BLOCK + threadIdx.x; s < W.nslots; s += (int64_t)gridDim.x * BLOCK) {
-#pragma clang fp contract(off)
-    // segments are wave-aligned: the whole wave belongs to one segment
-    const int k = __builtin_amdgcn_readfirstlane(W.seg_of_slot[s]);
-    const Seg &S = W.seg[k];
-    const int64_t b = S.ibase;
-    const int l = (int)(s - b);
-    const bool live = l < S.n_tgt;
-    if (!__any(live)) continue;
-    if (COUNT && live) nt++;
-    const int li = live ? W.sp[s] : 0;
-    const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
-    const bool corr = (S.flags & F_CORR) != 0;
-    const float tm = corr ? 0.f : W.pm[b + li];
-    const int ib = (int)(2 * b), end = ib + S.n_items;
-    double pot;
-    if (GROUP == 0)
-      pot = walk_binned<COUNT, REPLAY>(it, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
-    else
-      pot = walk_group_dispatch<GROUP == 0 ? 64 : GROUP, COUNT, REPLAY>(it, P, S, ib, end, live, x, y,
-                                                                       z, tm, ni, no, ns);
-    pot = pot * (double)P.G / (double)P.a;
-    if (!live) continue;
-    const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
-    if (!corr) {
-      float dv[3];
-      rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
-      if (REPLAY) {
-        W.E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
-      } else {
-        const double kin = 0.5 * ((double)dv[0] * dv[0] + (double)dv[1] * dv[1] + (double)dv[2] * dv[2]);
-        W.E[b + li] = (float)(kin + pot);
-      }
-    } else {
-      float ov[3];
-      rel_vel(P, x, y, z, vx, vy, vz, S.old_pos, S.old_vel, ov);
-      const float kin = vdot_f(ov, S.rvd) + S.dK;
-      W.E[b + li] = (float)((double)W.E[b + li] + ((double)kin - pot));
-    }
-  }
+  for (int64_t s = (int64_t)vb * BLOCK + threadIdx.x; s < W.nslots; s += (int64_t)gridDim.x * BLOCK)
+    walk_chunk<GROUP, COUNT, REPLAY>(W, P, it, s, ni, no, nt, ns);
   if (COUNT) flush_counts(W, ni, no, nt, ns);
 }
 

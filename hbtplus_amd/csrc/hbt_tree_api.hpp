@@ -57,6 +57,7 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
   R.W = make_work(ctx, L, nullptr, nullptr, nullptr, nullptr);
   R.max_n = L.maxcap;
   R.gs = grid_for(L.nslots, ctx->cus);
+  R.gw = walk_grid(L.nslots, ctx->tune.xcd_run);
   R.gseg = 1;
   R.gblk = 1;
   R.gtile = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)ctx->cus * 8));

@@ -145,6 +145,13 @@ struct hbt_ctx {
 
 namespace {
 
+// Capacity up to which a segment's node moments are built by one workgroup
+// (HBT_SMALL_TREE overrides it; 0 = every segment by per-depth launches).
+int small_tree() {
+  static const int v = getenv("HBT_SMALL_TREE") ? atoi(getenv("HBT_SMALL_TREE")) : 1024;
+  return v;
+}
+
 // Level description built on the host.
 struct Level {
   std::vector<int> subs;        // subhalo indices
@@ -154,6 +161,11 @@ struct Level {
   std::vector<Tile> tiles;
   int64_t nslots = 0;
   int maxcap = 0;
+  // segments are ordered by capacity, largest first; those from k_small on
+  // (capacity <= small_tree()) build their node moments one workgroup each,
+  // the others (slots [0, big_slots)) one launch per depth
+  int k_small = 0;
+  int64_t big_slots = 0;
 };
 
 int walk_group_for(const hbt_tuning &t, int64_t len) {
@@ -164,7 +176,10 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
                  const std::vector<std::vector<int>> &children, const std::vector<int> &members,
                  const hbt_tuning &tune, Level &L) {
   const int n = (int)members.size();
-  L.subs = members;
+  // largest first: merge levels and per-depth launches then cover a prefix
+  std::vector<int> order(members);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cap[a] > cap[b]; });
+  L.subs = order;
   L.seg.assign(n, Seg{});
   L.ibase.resize(n);
   L.capal.resize(n);
@@ -175,10 +190,15 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
   L.tiles.clear();
   int64_t off = 0;
   L.maxcap = 0;
+  L.k_small = n;
   for (int k = 0; k < n; k++) {
-    const int s = members[k];
+    const int s = order[k];
     const int c = (int)cap[s];
     const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
+    if (c <= small_tree() && L.k_small == n) {
+      L.k_small = k;
+      L.big_slots = off;
+    }
     Seg &S = L.seg[k];
     S.ibase = off;
     S.out_off = subs[s].out_offset;
@@ -198,6 +218,7 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
     L.child_ptr[k + 1] = (int)L.child_list.size();
   }
   L.nslots = off;
+  if (L.k_small == n) L.big_slots = off;
 }
 
 // Copies host tables to the device: packed (256-byte aligned) into the pinned
@@ -261,6 +282,17 @@ int upload(DBuf &b, const std::vector<T> &v, hipStream_t st) {
   RETCHK(b.ensure(std::max<size_t>(v.size() * sizeof(T), 16)));
   if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
   return HBT_OK;
+}
+
+// The walk's grid covers every slot block once: its blocks finish at very
+// different times and the hardware hands the remaining ones to the CUs that
+// come free; the count is rounded up to whole XCD runs so that the XCD-aware
+// block order applies (the extra blocks find no slots).
+int walk_grid(int64_t nslots, int xcd_run) {
+  int64_t g = (nslots + BLOCK - 1) / BLOCK;
+  const int64_t m = (int64_t)NXCD * std::max(1, xcd_run);
+  g = (g + m - 1) / m * m;
+  return (int)std::max<int64_t>(g, 1);
 }
 
 int grid_for(int64_t n, int cus) {
@@ -336,6 +368,7 @@ struct Runner {
   DevParams P;
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
+  int gw;                      // walk grid: every slot block once
 
   // Bounds for the pass from the previous pass of the level: merge levels for
   // segments of up to max_n elements (no count of a pass exceeds the Nlast of
@@ -344,15 +377,31 @@ struct Runner {
   // set lies inside that pass's sources).
   int max_n = 0, max_depth = MAX_DEPTH;
 
+  // tiles of the segments with capacity above w (a prefix of the tile table)
+  int tiles_above(int w) const {
+    const std::vector<Seg> &sg = L->seg;
+    int lo = 0, hi = (int)sg.size();  // first segment with cap <= w
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (sg[mid].cap > w) lo = mid + 1; else hi = mid;
+    }
+    return lo < (int)sg.size() ? L->first_tile[lo] : (int)L->tiles.size();
+  }
+
   int sort_pairs(const int *n) {  // sorts (k0,v0) by the counts n[seg], result in (k0,v0)
     const Tile *tl = c->tiles.as<Tile>();
     const int nt = (int)L->tiles.size();
-    const int gmerge = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nt * (TILE / BLOCK), (int64_t)c->cus * 16));
     tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
-    // one launch per merge level of the largest segment of the level
+    // one launch per merge level of the largest segment of the level, over
+    // the tiles of the segments whose capacity exceeds the level's width
+    // (segments are ordered largest first, so those tiles are a prefix)
     int level = 0;
-    for (int w = TILE; w < max_n; w *= 2, level++)
-      merge_pass_kernel<<<gmerge, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
+    for (int w = TILE; w < max_n; w *= 2, level++) {
+      const int ntw = tiles_above(w);
+      if (ntw == 0) break;
+      const int gmerge = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ntw * (TILE / BLOCK), (int64_t)c->cus * 16));
+      merge_pass_kernel<<<gmerge, BLOCK, 0, c->stream>>>(tl, ntw, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
+    }
     return HBT_OK;
   }
 
@@ -377,7 +426,13 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    for (int v = max_depth; v >= 0; v--) moments_kernel<<<gs, BLOCK, 0, c->stream>>>(W, v);
+    const int nsmall = (int)L->seg.size() - L->k_small;
+    if (nsmall > 0)
+      moments_small_kernel<<<std::min(nsmall, c->cus * 8), BLOCK, 0, c->stream>>>(W, L->k_small);
+    if (L->big_slots > 0) {
+      const int gb = grid_for(L->big_slots, c->cus);
+      for (int v = max_depth; v >= 0; v--) moments_kernel<<<gb, BLOCK, 0, c->stream>>>(W, v, L->big_slots);
+    }
     return HBT_OK;
   }
 
@@ -385,11 +440,11 @@ struct Runner {
   template <int GROUP>
   void launch_shared_walk(bool count) {
     if (P.replay) {
-      if (count) walk_shared_kernel<GROUP, true, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
-      else walk_shared_kernel<GROUP, false, true><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
+      if (count) walk_shared_kernel<GROUP, true, true><<<gw, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
+      else walk_shared_kernel<GROUP, false, true><<<gw, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
     } else {
-      if (count) walk_shared_kernel<GROUP, true, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
-      else walk_shared_kernel<GROUP, false, false><<<gs, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
+      if (count) walk_shared_kernel<GROUP, true, false><<<gw, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
+      else walk_shared_kernel<GROUP, false, false><<<gw, BLOCK, 0, c->stream>>>(W, P, W.items, c->tune.xcd_run);
     }
   }
 
@@ -547,6 +602,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   R.W = make_work(c, L, parts, subs, out_index, out_energy);
   R.max_n = L.maxcap;
   R.gs = grid_for(L.nslots, c->cus);
+  R.gw = walk_grid(L.nslots, c->tune.xcd_run);
   R.gseg = grid_for(nseg, c->cus);
   R.gtile = (int)std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)c->cus * 8);
   if (R.gtile < 1) R.gtile = 1;
