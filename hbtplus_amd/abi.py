@@ -143,6 +143,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_subhalo_properties": ([vp, vp, vp, i64, vp, vp], i32),
         "hbt_mask_subhalos": ([vp, vp, vp, vp, i64, vp, vp], i32),
         "hbt_merge_subhalos": ([vp, vp, vp, i64, vp, vp, i64, i32, vp, i64, vp], i32),
+        "hbt_nest_subhalos": ([vp, i64, vp, vp, vp, vp], i32),
         "hbt_get_tuning": ([vp, vp], i32),
         "hbt_set_tuning": ([vp, vp], i32),
         "hbt_autotune": ([vp, vp, vp, vp, i64, vp, vp, i32, vp], i32),

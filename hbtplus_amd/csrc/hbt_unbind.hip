@@ -1131,5 +1131,6 @@ int hbt_unbind_one(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *su
 #include "hbt_props.hpp"
 #include "hbt_mask.hpp"
 #include "hbt_merge.hpp"
+#include "hbt_nest.hpp"
 #include "hbt_direct.hpp"
 #include "hbt_idhash.hpp"

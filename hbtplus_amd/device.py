@@ -137,6 +137,19 @@ class GpuUnbinder:
         check(rc, "hbt_mask_subhalos")
         return out.cpu().numpy()[:n].copy(), cnt
 
+    def nest_subhalos(self, host_group, is_central, parent, rank):
+        """NestSubhalos' LevelUpDetachedSubhalos: returns (parent, rank) after
+        the detached subhalos were levelled up, numpy."""
+        t = self.torch
+        g = self._dev(np.ascontiguousarray(host_group, np.int64))
+        c = self._dev(np.ascontiguousarray(is_central, np.int32))
+        p = self._dev(np.ascontiguousarray(parent, np.int64))
+        r = self._dev(np.ascontiguousarray(rank, np.int32))
+        t.cuda.synchronize(self.device)
+        check(self.lib.hbt_nest_subhalos(self.ctx, len(host_group), g.data_ptr(), c.data_ptr(), p.data_ptr(),
+                                         r.data_ptr()), "hbt_nest_subhalos")
+        return p.cpu().numpy(), r.cpu().numpy()
+
     def upload_merge(self, cols, list_index):
         """Particle columns and the lists (column positions) to the device."""
         c = {k: self._dev(np.ascontiguousarray(cols[k])) for k in PARTICLE_COLUMNS}

@@ -517,3 +517,42 @@ def merging_catalogue(seed=1007, ngroups=3, periodic=True, box=1000.0, hz=0.0, s
                     parent=np.array(parent, np.int64))
     cat.subs["nbound_prev"] = np.array(nprev, np.int64)
     return cat, np.array(group, np.int64)
+
+
+def nesting_catalogue(seed=1011, ngroups=60, nsub=600, nfield=40):
+    """Inputs of the nest bookkeeping: host groups with one flagged central
+    each, field subhalos, and nests that cross groups (a subhalo nested in a
+    host of another halo), with Ranks as a previous snapshot left them: roots
+    mostly heads (Rank 0) but some not, nested subhalos mostly satellites
+    (Rank >= 1) but some Rank 0.  Indices are shuffled; nests are acyclic.
+    Returns (host_group, is_central, parent, rank)."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    n = nsub + nfield
+    group = np.concatenate([rng.integers(0, ngroups, nsub), np.full(nfield, -1)]).astype(np.int64)
+    parent = np.full(n, -1, np.int64)
+    rank = np.zeros(n, np.int32)
+    u = rng.random(n)
+    pick = rng.random(n)
+    members = {}
+    for s in range(n):
+        same = members.setdefault(int(group[s]), [])
+        if s > 0 and u[s] >= 0.25:  # else a root
+            if u[s] < 0.85 and same:
+                parent[s] = same[int(pick[s] * len(same))]
+            else:
+                parent[s] = int(pick[s] * s)
+        same.append(s)
+    roots = parent < 0
+    rank[~roots] = rng.integers(1, 9, int((~roots).sum()))
+    rank[~roots & (rng.random(n) < 0.08)] = 0
+    rank[roots & (rng.random(n) < 0.1)] = rng.integers(1, 5)
+    central = np.zeros(n, np.int32)
+    for g in range(ngroups):
+        m = np.nonzero(group == g)[0]
+        if len(m):
+            central[m[rng.integers(0, len(m))]] = 1
+    perm = rng.permutation(n)      # new index of old subhalo i: inv[i]
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    parent_new = np.where(parent[perm] >= 0, inv[np.maximum(parent[perm], 0)], -1)
+    return group[perm], central[perm], parent_new.astype(np.int64), rank[perm]

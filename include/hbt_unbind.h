@@ -21,6 +21,14 @@
  *   hbt_id_to_index     ParticleSnapshot_t::GetIndices / MappedIndexTable_t::GetIndices
  *                                                        src/hash_remote.tpp:58-109,
  *                                                        src/hash.tpp:18-56
+ *   hbt_unbind_batch_indexed  the RefineParticles loop body on ID-resolved lists
+ *   hbt_subhalo_properties    CalculateProfileProperties + CalculateShape
+ *                                                        src/subhalo.cpp:242-398
+ *   hbt_nest_subhalos   NestSubhalos / LevelUpDetachedSubhalos
+ *                                                        src/subhalo_tracking.cpp:669-688,735-786
+ *   hbt_mask_subhalos   MaskSubhalos / SubhaloMasker_t   src/subhalo_tracking.cpp:793-841
+ *   hbt_merge_subhalos  MergeSubhalos (DetectTraps, MergeRecursive, MergeTo)
+ *                                                        src/subhalo_merge.cpp:136-262
  *
  * Conventions
  *   - Plain pointers and sizes only.  Every function returns an int status:
@@ -259,6 +267,24 @@ int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particles, const h
 int hbt_mask_subhalos(hbt_ctx *ctx, const hbt_particles *particles, const hbt_subhalo *subs_host,
                       const int64_t *host_group, int64_t nsub, int64_t *out_list,
                       int64_t *new_count);
+
+/* Nest bookkeeping before masking (SubhaloSnapshot_t::NestSubhalos ->
+ * LevelUpDetachedSubhalos, Subhalo_t::LevelUpDetachedMembers,
+ * src/subhalo_tracking.cpp:669-688, 735-786), for one rank's subhalos with
+ * local nest indices.  Every group's central and every field subhalo gets
+ * Rank 0; then, in the nests below the subhalos that were heads (Rank 0) on
+ * entry, a nested subhalo leaves its host's nest (parent -1, Rank 0) when its
+ * host halo differs from its host's or its Rank is 0.
+ *   host_group : DEVICE int64[nsub], HostHaloId (-1: field)
+ *   is_central : DEVICE int32[nsub], 1 for each group's central (its most
+ *                massive member, MemberTable.SubGroups[g][0])
+ *   parent     : DEVICE int64[nsub], in/out: the subhalo whose NestedSubhalos
+ *                holds this one, -1 if none
+ *   rank       : DEVICE int32[nsub], in/out: Rank
+ * Hosts outside [0, nsub) and nest chains deeper than 4096 are rejected with
+ * HBT_ERR_INVALID_ARG (parent and rank then keep their input). */
+int hbt_nest_subhalos(hbt_ctx *ctx, int64_t nsub, const int64_t *host_group,
+                      const int32_t *is_central, int64_t *parent, int32_t *rank);
 
 /* Merging of trapped subhalos after the unbinding of a snapshot
  * (SubhaloSnapshot_t::MergeSubhalos with DetectTraps, MergeRecursive and

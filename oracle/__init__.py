@@ -44,6 +44,7 @@ def lib():
         L.oracle_subhalo_properties.argtypes = [vp, vp, vp, i64, vp, vp, i32]
         L.oracle_mask_subhalos.argtypes = [vp, vp, vp, i64, vp, vp]
         L.oracle_merge_subhalos.argtypes = [vp, vp, vp, i64, vp, vp, i64, i32, vp, vp, i64, vp]
+        L.oracle_nest_subhalos.argtypes = [i64, vp, vp, vp, vp]
         L.oracle_abi_sizes.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
@@ -129,6 +130,35 @@ def mask_subhalos(cols, subs, host_group):
     if rc != 0:
         raise RuntimeError("oracle_mask_subhalos failed")
     return out, cnt
+
+
+def nest_subhalos(host_group, is_central, parent, rank):
+    """LevelUpDetachedSubhalos restated: (parent, rank) afterwards."""
+    g = np.ascontiguousarray(host_group, np.int64)
+    c = np.ascontiguousarray(is_central, np.int32)
+    p = np.ascontiguousarray(parent, np.int64).copy()
+    r = np.ascontiguousarray(rank, np.int32).copy()
+    rc = lib().oracle_nest_subhalos(len(g), _p(g), _p(c), _p(p), _p(r))
+    if rc != 0:
+        raise RuntimeError(f"oracle_nest_subhalos failed: {rc}")
+    return p, r
+
+
+def ref_nest_subhalos(host_group, is_central, parent, rank):
+    """The reference's own LevelUpDetachedSubhalos
+    (oracle/_ref/hbt_ref_tracking mode nest): (parent, rank) afterwards."""
+    n = len(host_group)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([n], np.int64).tobytes())
+            f.write(np.ascontiguousarray(host_group, np.int64).tobytes())
+            f.write(np.ascontiguousarray(is_central, np.int32).tobytes())
+            f.write(np.ascontiguousarray(parent, np.int64).tobytes())
+            f.write(np.ascontiguousarray(rank, np.int32).tobytes())
+        subprocess.run([REF_TRACKING, "nest", fin, fout], check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        raw = open(fout, "rb").read()
+    return np.frombuffer(raw, np.int64, n, 0).copy(), np.frombuffer(raw, np.int32, n, 8 * n).copy()
 
 
 def merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True):

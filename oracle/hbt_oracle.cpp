@@ -663,6 +663,51 @@ int oracle_mask_subhalos(const hbt_particles *parts, const hbt_subhalo *subs, co
   return HBT_OK;
 }
 
+// LevelUpDetachedSubhalos (src/subhalo_tracking.cpp:735-769) with
+// Subhalo_t::LevelUpDetachedMembers (:771-786), the nest step of
+// NestSubhalos (:669-688) on local indices: heads recorded first (Rank 0),
+// centrals and field subhalos promoted to Rank 0, then each head's nest walked
+// recursively, a member leaving the nest when its host halo differs from its
+// host's or its Rank is 0.  Every head on entry walks its nest, also one that
+// is itself nested.  Children are the subhalos naming a host, in index order.
+// parent and rank in/out.
+int oracle_nest_subhalos(int64_t nsub, const int64_t *group, const int32_t *is_central, int64_t *parent,
+                         int32_t *rank) {
+  if (nsub < 0 || (nsub > 0 && (!group || !is_central || !parent || !rank))) return HBT_ERR_INVALID_ARG;
+  std::vector<std::vector<Int>> nested(nsub);
+  for (Int s = 0; s < nsub; s++) {
+    if (parent[s] < -1 || parent[s] >= nsub || parent[s] == s) return HBT_ERR_INVALID_ARG;
+    if (parent[s] >= 0) nested[parent[s]].push_back(s);
+  }
+  for (Int s = 0; s < nsub; s++) {  // acyclic: every chain of hosts ends
+    Int h = s, d = 0;
+    while (parent[h] >= 0 && d <= nsub) { h = parent[h]; d++; }
+    if (d > nsub) return HBT_ERR_INVALID_ARG;
+  }
+  std::vector<char> head(nsub);
+  for (Int s = 0; s < nsub; s++) head[s] = rank[s] == 0;
+  for (Int s = 0; s < nsub; s++)
+    if (is_central[s] || group[s] < 0) rank[s] = 0;
+  // every head on entry, nested or not, walks its nest (a second visit
+  // through a host's nest changes nothing)
+  std::function<void(Int)> level_up = [&](Int h) {
+    std::vector<Int> keep;
+    for (Int c : nested[h]) {
+      if (group[c] != group[h] || rank[c] == 0) {
+        rank[c] = 0;
+        parent[c] = -1;
+      } else {
+        keep.push_back(c);
+      }
+      level_up(c);
+    }
+    nested[h].swap(keep);
+  };
+  for (Int s = 0; s < nsub; s++)
+    if (head[s]) level_up(s);
+  return HBT_OK;
+}
+
 // MergeSubhalos (src/subhalo_merge.cpp:187-220): GlueHeadNests (the group's
 // other heads appended to the central's nest, src/subhalo_tracking.cpp:843-861),
 // FillHelpers (HostTrackId from the nests, :163-186; cores from the first

@@ -26,6 +26,11 @@
 //          below); int64 total; int64 list[total] (column positions);
 //          float energy[total] (Energies of the bound block where the
 //          subhalo was unbound again, NaN elsewhere)
+//   ref_tracking nest in.bin out.bin
+//     in : int64 nsub; int64 host_group[nsub]; int32 is_central[nsub];
+//          int64 parent[nsub]; int32 rank[nsub]
+//     out: int64 parent[nsub]; int32 rank[nsub] after LevelUpDetachedSubhalos
+//
 //   ref_tracking merge_time in.bin out.bin
 //     as merge; out: double seconds spent in MergeSubhalos (OMP_NUM_THREADS
 //     threads, the reference's own parallel loops)
@@ -163,6 +168,44 @@ void configure(const hbt_params &p, Snapshot_t &epoch) {
 
 // timed: only the wall seconds of MergeSubhalos (OpenMP as the reference runs
 // it) are written
+// LevelUpDetachedSubhalos (src/subhalo_tracking.cpp:735-769) on a snapshot
+// whose member table has each group's flagged central first.
+int run_nest(FILE *in, FILE *out) {
+  int64_t nsub;
+  rd(in, &nsub, 1);
+  std::vector<int64_t> grp(nsub), parent(nsub);
+  std::vector<int32_t> central(nsub), rank(nsub);
+  rd(in, grp.data(), nsub);
+  rd(in, central.data(), nsub);
+  rd(in, parent.data(), nsub);
+  rd(in, rank.data(), nsub);
+  int64_t nhalos = 0;
+  for (int64_t g : grp) nhalos = std::max(nhalos, g + 1);
+  SubhaloSnapshot_t snap;
+  snap.Subhalos.resize(nsub);
+  std::vector<std::vector<int64_t>> heads(nhalos);
+  for (int64_t s = 0; s < nsub; s++) {
+    Subhalo_t &S = snap.Subhalos[s];
+    S.TrackId = s;
+    S.Rank = rank[s];
+    if (parent[s] >= 0) snap.Subhalos[parent[s]].NestedSubhalos.push_back(s);
+  }
+  // the flagged central of a group as SubGroups[g][0]
+  for (int64_t s = 0; s < nsub; s++)
+    if (central[s] && grp[s] >= 0) heads[grp[s]].insert(heads[grp[s]].begin(), s);
+  build_members(snap, grp, heads);
+  snap.LevelUpDetachedSubhalos();
+  std::vector<int64_t> np(nsub, -1);
+  std::vector<int32_t> nr(nsub);
+  for (int64_t s = 0; s < nsub; s++) {
+    for (auto c : snap.Subhalos[s].NestedSubhalos) np[c] = s;
+    nr[s] = snap.Subhalos[s].Rank;
+  }
+  wr(out, np.data(), nsub);
+  wr(out, nr.data(), nsub);
+  return 0;
+}
+
 int run_merge(FILE *in, FILE *out, bool timed) {
   hbt_params p;
   int64_t flag, nsub, ncol, nlist;
@@ -270,7 +313,7 @@ int run_merge(FILE *in, FILE *out, bool timed) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge|merge_time in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge|merge_time|nest in.bin out.bin\n", argv[0]); return 1; }
   MPI_Init(&argc, &argv);
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
@@ -279,6 +322,7 @@ int main(int argc, char **argv) {
   else if (!strcmp(argv[1], "mask")) rc = run_mask(in, out);
   else if (!strcmp(argv[1], "merge")) rc = run_merge(in, out, false);
   else if (!strcmp(argv[1], "merge_time")) rc = run_merge(in, out, true);
+  else if (!strcmp(argv[1], "nest")) rc = run_nest(in, out);
   if (in) fclose(in);
   if (out) fclose(out);
   MPI_Finalize();

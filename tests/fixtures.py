@@ -14,6 +14,7 @@ TREE_CASES = ["tree_c1", "tree_straddle", "tree_floor"]
 PROPS_CASES = ["c2_small", "c3_small", "c4_small", "hubble", "edge"]
 MASK_CASES = ["mask_a", "mask_b"]
 MERGE_CASES = ["merge_a", "merge_b"]
+NEST_CASES = ["nest_a", "nest_b"]
 
 
 def load_unbind(name):
@@ -71,3 +72,9 @@ class MergeFixture:
 
 def load_merge(name):
     return MergeFixture(name)
+
+
+def load_nest(name):
+    """(host_group, is_central, parent, rank, reference parent, reference rank)"""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    return tuple(z[k] for k in ("host_group", "is_central", "parent", "rank", "ref_parent", "ref_rank"))

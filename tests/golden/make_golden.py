@@ -17,6 +17,7 @@ Runs are single-threaded with sampling off.
     python tests/golden/make_golden.py --props-only   # only the props_ files
     python tests/golden/make_golden.py --mask-only    # only the mask_ files
     python tests/golden/make_golden.py --merge-only   # only the merge_ files
+    python tests/golden/make_golden.py --nest-only    # only the nest_ files
 
 mask_<name>.npz: a catalogue of overlapping source lists in host groups (ID
 column, records, host groups) and what the reference's own
@@ -28,6 +29,10 @@ truncated to nsource, as column positions, with a few hand-set trap and death
 fields), and what the reference's own SubhaloSnapshot_t::MergeSubhalos left,
 with MergeTrappedSubhalos on (out_*) and off (off_*); pipe_*: the same on
 the unbinding's state as it is (RefineParticles then MergeSubhalos).
+
+nest_<name>.npz: host groups, central flags, nest hosts and Ranks
+(synth.nesting_catalogue) and the hosts and Ranks after the reference's own
+LevelUpDetachedSubhalos.
 """
 from __future__ import annotations
 
@@ -174,6 +179,20 @@ def save_mask(name, kw):
     print(f"{name}: {len(subs)} subhalos, {len(cols['id'])} entries, kept {cnt.sum()}")
 
 
+NEST_CASES = {
+    "nest_a": dict(seed=1011, ngroups=60, nsub=600, nfield=40),
+    "nest_b": dict(seed=1012, ngroups=8, nsub=300, nfield=5),
+}
+
+
+def save_nest(name, kw):
+    g, c, p, r = synth.nesting_catalogue(**kw)
+    rp, rr = oracle.ref_nest_subhalos(g, c, p, r)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), kind="nest", host_group=g, is_central=c, parent=p,
+                        rank=r, ref_parent=rp, ref_rank=rr)
+    print(f"{name}: {len(g)} subhalos, {int(((p >= 0) & (rp < 0)).sum())} levelled up")
+
+
 MERGE_CASES = {
     "merge_a": dict(seed=1007, periodic=True),
     "merge_b": dict(seed=1009, periodic=False, hz=0.07, scale_factor=0.8, snapshot_index=9),
@@ -222,6 +241,10 @@ def main():
         for name, kw in MASK_CASES.items():
             save_mask(name, kw)
         return
+    if "--nest-only" in sys.argv:
+        for name, kw in NEST_CASES.items():
+            save_nest(name, kw)
+        return
     if "--merge-only" in sys.argv:
         for name, kw in MERGE_CASES.items():
             save_merge(name, kw)
@@ -241,6 +264,8 @@ def main():
         save_mask(name, kw)
     for name, kw in MERGE_CASES.items():
         save_merge(name, kw)
+    for name, kw in NEST_CASES.items():
+        save_nest(name, kw)
 
 
 if __name__ == "__main__":

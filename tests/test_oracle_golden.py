@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import (MASK_CASES, MERGE_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_merge,
-                      load_props, load_tree, load_unbind)
+from fixtures import (MASK_CASES, MERGE_CASES, NEST_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_merge,
+                      load_nest, load_props, load_tree, load_unbind)
 from hbtplus_amd.abi import HBT_STATUS_CONVERGED, PROPS_FIELDS
 from helpers import assert_exact, assert_merge
 
@@ -80,3 +80,12 @@ def test_oracle_merge_matches_reference(name, merge_trapped):
         for s in np.nonzero(got[0]["status"] & HBT_STATUS_CONVERGED)[0]:
             o, nb = int(rs["offset"][s]), int(rs["nbound"][s])
             assert np.array_equal(ref[2][o:o + nb], got[2][o:o + nb]), s
+
+
+@pytest.mark.parametrize("name", NEST_CASES)
+def test_oracle_nest_matches_reference(name):
+    """NestSubhalos' level-up restated: the hosts and Ranks the reference's
+    own LevelUpDetachedSubhalos leaves"""
+    g, c, p, r, rp, rr = load_nest(name)
+    op, orr = oracle.nest_subhalos(g, c, p, r)
+    assert np.array_equal(op, rp) and np.array_equal(orr, rr)

@@ -244,3 +244,29 @@ def test_merge_hand_derived(merge_trapped):
     params, cols, lst, subs, grp = hand_merge_case()
     out, ol, _ = oracle.merge_subhalos(params, cols, lst, subs, grp, merge_trapped)
     check_hand_merge(subs, out, ol, merge_trapped)
+
+
+# A nest worked out by hand from LevelUpDetachedSubhalos/LevelUpDetachedMembers
+# (src/subhalo_tracking.cpp:735-786): columns are host group, central flag,
+# host (parent) and Rank on entry, then host and Rank afterwards.
+NEST_HAND = [
+    (0, 1, -1, 0, -1, 0),   # 0: central of group 0, a head
+    (0, 0, 0, 1, 0, 1),     # 1: same group as its host: stays
+    (1, 0, 0, 2, -1, 0),    # 2: in another group than its host: levelled up
+    (0, 0, 2, 1, -1, 0),    # 3: below 2 (reached through it), group differs from 2's
+    (1, 1, 1, 3, -1, 0),    # 4: central of group 1 (promoted to Rank 0), nested below 0
+    (0, 0, -1, 2, -1, 2),   # 5: a root that is no head: its nest is never walked
+    (1, 0, 5, 1, 5, 1),     # 6: below 5: left as it is although its group differs
+    (-1, 0, 1, 1, -1, 0),   # 7: a field subhalo (promoted), nested below 1
+    (1, 0, 5, 0, 5, 0),     # 8: a head nested in 5: stays there, but walks its own nest
+    (0, 0, 8, 1, -1, 0),    # 9: below 8, other group: levelled up
+    (2, 1, -1, 5, -1, 0),   # 10: central of group 2, a root with Rank 5: promoted
+]
+
+
+def test_nest_hand_derived():
+    """NestSubhalos' level-up restated on a nest worked out by hand"""
+    a = np.array(NEST_HAND, np.int64)
+    p, r = oracle.nest_subhalos(a[:, 0], a[:, 1].astype(np.int32), a[:, 2], a[:, 3].astype(np.int32))
+    assert p.tolist() == a[:, 4].tolist()
+    assert r.tolist() == a[:, 5].tolist()
