@@ -41,6 +41,11 @@ struct Work {
   Seg *seg;
   int nseg;
   int64_t nslots;
+  // the slots a pass visits: all (act_chunk null, nact = nslots) or the
+  // wave-sized chunks of the segments still active (nact = 64 * chunks)
+  const int *act_chunk;
+  int64_t nact;
+  int *segact;               // [nseg] 1: the segment stays active after the pass
   const int *seg_of_slot;
   const int64_t *ibase;
   int *ncount;               // [NC_ROWS][nseg] element counts of the pass's scans and sorts
@@ -87,9 +92,14 @@ __device__ __forceinline__ int seg_lane_local(const Work &W, int64_t s, int &k) 
   return (int)(s - W.ibase[k]);
 }
 
-#define SLOT_LOOP(s, W)                                                              \
-  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < (W).nslots;       \
-       s += (int64_t)gridDim.x * BLOCK)
+// i-th visited slot (chunks are wave-sized and aligned, so a wave stays in one segment)
+__device__ __forceinline__ int64_t slot_at(const Work &W, int64_t i) {
+  return (W.act_chunk && i < W.nact) ? (int64_t)W.act_chunk[i >> 6] * WAVE + (i & (WAVE - 1)) : i;
+}
+
+#define SLOT_LOOP(s, W)                                                                        \
+  for (int64_t s##_i = (int64_t)blockIdx.x * BLOCK + threadIdx.x, s = slot_at((W), s##_i);    \
+       s##_i < (W).nact; s##_i += (int64_t)gridDim.x * BLOCK, s = slot_at((W), s##_i))
 
 #define SEG_LOOP(k, W) \
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < (W).nseg; k += gridDim.x * blockDim.x)
@@ -185,6 +195,7 @@ __global__ void pass_setup_kernel(Work W, DevParams P) {
   if (blockIdx.x == 0 && threadIdx.x == 0) { W.active[0] = 0; W.active[1] = 0; W.active[2] = 0; }
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
+    W.segact[k] = 0;
     S.flags &= ~F_FINAL;
     S.n_src = 0; S.n_tgt = 0; S.src_lo = 0; S.nsort = 0; S.sort_lo = 0; S.n_items = 0;
     S.K = 0; S.nP = 0; S.nG = 0;
@@ -461,10 +472,11 @@ __device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
 
 // Moments of the nodes at depth v (one launch per depth, deepest first: a
 // kernel boundary orders the children's moments before their parents read
-// them), over the slots [0, nslots) of the large segments.  Each node is
-// handled by the slot of its leftmost split pair.
+// them), over the first nslots visited slots (those of the large segments).
+// Each node is handled by the slot of its leftmost split pair.
 __global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v, int64_t nslots) {
-  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * BLOCK) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nslots; i += (int64_t)gridDim.x * BLOCK) {
+    const int64_t s = slot_at(W, i);
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
@@ -865,8 +877,8 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
                                                             const Item *__restrict__ it, int xcd_run) {
   unsigned ni = 0, no = 0, nt = 0, ns = 0;
   const int vb = xcd_block(xcd_run);
-  for (int64_t s = (int64_t)vb * BLOCK + threadIdx.x; s < W.nslots; s += (int64_t)gridDim.x * BLOCK)
-    walk_chunk<GROUP, COUNT, REPLAY>(W, P, it, s, ni, no, nt, ns);
+  for (int64_t i = (int64_t)vb * BLOCK + threadIdx.x; i < W.nact; i += (int64_t)gridDim.x * BLOCK)
+    walk_chunk<GROUP, COUNT, REPLAY>(W, P, it, slot_at(W, i), ni, no, nt, ns);
   if (COUNT) flush_counts(W, ni, no, nt, ns);
 }
 
@@ -1167,6 +1179,7 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
     S.nsort = S.nlast - nb;
     atomicAdd(&W.active[0], 1);
     atomicMax(&W.active[1], S.nlast);  // bounds every count of the next pass
+    W.segact[&S - W.seg] = 1;
   }
   W.ncount[NC_SORT * W.nseg + (int)(&S - W.seg)] = S.nsort;
 }

@@ -56,11 +56,9 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
   R.P = make_dev_params(ctx->params);
   R.W = make_work(ctx, L, nullptr, nullptr, nullptr, nullptr);
   R.max_n = L.maxcap;
-  R.gs = grid_for(L.nslots, ctx->cus);
-  R.gw = walk_grid(L.nslots, ctx->tune.xcd_run);
+  R.list_all();
   R.gseg = 1;
   R.gblk = 1;
-  R.gtile = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)ctx->cus * 8));
   hipStream_t st = ctx->stream;
   fill_seg_of_slot_kernel<<<1, BLOCK, 0, st>>>(1, R.W.ibase, ctx->capal.as<int>(), ctx->seg_of_slot.as<int>());
   tree_only_setup_kernel<<<1, 1, 0, st>>>(R.W, (int)n_src);

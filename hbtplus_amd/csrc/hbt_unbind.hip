@@ -110,6 +110,7 @@ struct hbt_ctx {
   size_t h_stage_bytes = 0;
   DView seg, ibase, tiles, first_tile, ntile, child_ptr, child_list, capal;
   DBuf ncount, tsum, tpart;
+  DBuf segact, actbuf;  // segment activity after a pass; lists of the active segments
   DBuf active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
@@ -131,7 +132,7 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
-                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi})
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -144,6 +145,13 @@ struct hbt_ctx {
 };
 
 namespace {
+
+// Later passes visit only the active segments' slots and tiles
+// (HBT_COMPACT=0 keeps every pass on the whole level).
+bool compact_passes() {
+  static const bool v = getenv("HBT_COMPACT") ? atoi(getenv("HBT_COMPACT")) != 0 : true;
+  return v;
+}
 
 // Capacity up to which a segment's node moments are built by one workgroup
 // (HBT_SMALL_TREE overrides it; 0 = every segment by per-depth launches).
@@ -369,6 +377,75 @@ struct Runner {
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
   int gw;                      // walk grid: every slot block once
+  // the tile table a pass uses: the level's, or that of its active segments
+  // (then W.act_chunk lists their slots); caps and first tiles of the listed
+  // segments in order, for the merge-level prefixes
+  const Tile *tl_cur = nullptr;
+  const int *ft_cur = nullptr;
+  int nt_cur = 0;
+  std::vector<int> cur_cap, cur_first;
+  int64_t n_big = 0;           // visited slots that belong to large segments (per-depth moments)
+  int nlisted = 0;             // segments in the current lists
+
+  void list_all() {
+    tl_cur = c->tiles.as<Tile>();
+    ft_cur = c->first_tile.as<int>();
+    nt_cur = (int)L->tiles.size();
+    cur_cap.resize(L->seg.size());
+    for (size_t k = 0; k < L->seg.size(); k++) cur_cap[k] = L->seg[k].cap;
+    cur_first = L->first_tile;
+    n_big = L->big_slots;
+    nlisted = (int)L->seg.size();
+    W.act_chunk = nullptr;
+    W.nact = L->nslots;
+    set_grids();
+  }
+
+  void set_grids() {
+    gs = grid_for(W.nact, c->cus);
+    gw = walk_grid(W.nact, c->tune.xcd_run);
+    gtile = (int)std::max<int64_t>(1, std::min<int64_t>(nt_cur, (int64_t)c->cus * 8));
+  }
+
+  // Lists the segments still active after a pass: their wave-sized slot
+  // chunks, their tiles (a table of its own, with a first-tile index per
+  // segment), caps and first tiles in level order.
+  int list_active() {
+    const int nseg = (int)L->seg.size();
+    std::vector<int> act(nseg);
+    HIPCHK(hipMemcpyAsync(act.data(), W.segact, nseg * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int> chunks, ft(nseg, 0);
+    std::vector<Tile> tl;
+    cur_cap.clear();
+    cur_first.clear();
+    n_big = 0;
+    nlisted = 0;
+    for (int k = 0; k < nseg; k++) {
+      if (!act[k]) continue;
+      nlisted++;
+      cur_cap.push_back(L->seg[k].cap);
+      cur_first.push_back((int)tl.size());
+      ft[k] = (int)tl.size();
+      for (int t = 0; t < L->ntile[k]; t++) tl.push_back(L->tiles[L->first_tile[k] + t]);
+      const int64_t c0 = L->ibase[k] / WAVE, c1 = (L->ibase[k] + L->capal[k]) / WAVE;
+      for (int64_t q = c0; q < c1; q++) chunks.push_back((int)q);
+      if (k < L->k_small) n_big += (c1 - c0) * WAVE;
+    }
+    const size_t b_ch = (chunks.size() * 4 + 255) & ~(size_t)255, b_tl = (tl.size() * sizeof(Tile) + 255) & ~(size_t)255;
+    RETCHK(c->actbuf.ensure(b_ch + b_tl + nseg * 4 + 256));
+    char *d = (char *)c->actbuf.p;
+    if (!chunks.empty()) HIPCHK(hipMemcpy(d, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
+    if (!tl.empty()) HIPCHK(hipMemcpy(d + b_ch, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + b_ch + b_tl, ft.data(), nseg * 4, hipMemcpyHostToDevice));
+    W.act_chunk = (const int *)d;
+    W.nact = (int64_t)chunks.size() * WAVE;
+    tl_cur = (const Tile *)(d + b_ch);
+    nt_cur = (int)tl.size();
+    ft_cur = (const int *)(d + b_ch + b_tl);
+    set_grids();
+    return HBT_OK;
+  }
 
   // Bounds for the pass from the previous pass of the level: merge levels for
   // segments of up to max_n elements (no count of a pass exceeds the Nlast of
@@ -379,18 +456,17 @@ struct Runner {
 
   // tiles of the segments with capacity above w (a prefix of the tile table)
   int tiles_above(int w) const {
-    const std::vector<Seg> &sg = L->seg;
-    int lo = 0, hi = (int)sg.size();  // first segment with cap <= w
+    int lo = 0, hi = (int)cur_cap.size();  // first listed segment with cap <= w
     while (lo < hi) {
       const int mid = (lo + hi) / 2;
-      if (sg[mid].cap > w) lo = mid + 1; else hi = mid;
+      if (cur_cap[mid] > w) lo = mid + 1; else hi = mid;
     }
-    return lo < (int)sg.size() ? L->first_tile[lo] : (int)L->tiles.size();
+    return lo < (int)cur_cap.size() ? cur_first[lo] : nt_cur;
   }
 
   int sort_pairs(const int *n) {  // sorts (k0,v0) by the counts n[seg], result in (k0,v0)
-    const Tile *tl = c->tiles.as<Tile>();
-    const int nt = (int)L->tiles.size();
+    const Tile *tl = tl_cur;
+    const int nt = nt_cur;
     tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
     // one launch per merge level of the largest segment of the level, over
     // the tiles of the segments whose capacity exceeds the level's width
@@ -408,17 +484,16 @@ struct Runner {
   // segmented exclusive scan over the counts n[seg]; the tile totals stay in tsum
   template <class Load, class Store, class Add>
   int seg_scan(Load ld, Store st, Add ad, const int *n) {
-    const int nt = (int)L->tiles.size();
     u64 *ts = c->tsum.as<u64>();
-    scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, n, ts, ld, st);
-    scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(c->tiles.as<Tile>(), nt, n, ts, ad);
+    scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nt_cur, n, ts, ld, st);
+    scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nt_cur, n, ts, ad);
     return HBT_OK;
   }
   const int *count_row(int row) const { return W.ncount + (size_t)row * W.nseg; }
 
   // tree over the pass's sources: keys, sort, structure, moments
   int build_tree(bool copy_sp) {
-    bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, c->tiles.as<Tile>(), (int)L->tiles.size());
+    bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
     tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(sort_pairs(count_row(NC_SRC)));
@@ -429,9 +504,9 @@ struct Runner {
     const int nsmall = (int)L->seg.size() - L->k_small;
     if (nsmall > 0)
       moments_small_kernel<<<std::min(nsmall, c->cus * 8), BLOCK, 0, c->stream>>>(W, L->k_small);
-    if (L->big_slots > 0) {
-      const int gb = grid_for(L->big_slots, c->cus);
-      for (int v = max_depth; v >= 0; v--) moments_kernel<<<gb, BLOCK, 0, c->stream>>>(W, v, L->big_slots);
+    if (n_big > 0) {
+      const int gb = grid_for(n_big, c->cus);
+      for (int v = max_depth; v >= 0; v--) moments_kernel<<<gb, BLOCK, 0, c->stream>>>(W, v, n_big);
     }
     return HBT_OK;
   }
@@ -474,7 +549,7 @@ struct Runner {
     // partition
     RETCHK(seg_scan(PartLoad{W.ibase, W.el, W.E}, S64Store{W.ibase, W.scan64},
                     S64Add{W.ibase, W.scan64}, count_row(NC_PART)));
-    partition_counts_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, c->first_tile.as<int>(), c->tsum.as<u64>());
+    partition_counts_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, ft_cur, c->tsum.as<u64>());
     partition_rank_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_k_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_hole_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
@@ -491,9 +566,8 @@ struct Runner {
     if (P.replay) {
       frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     } else {
-      frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, c->tiles.as<Tile>(), (int)L->tiles.size(),
-                                                            c->tpart.as<double>());
-      frames_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, c->first_tile.as<int>(), c->tpart.as<double>());
+      frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
+      frames_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
     }
     // energy sort of the removed block, and of the bound block on convergence
     energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
@@ -503,9 +577,8 @@ struct Runner {
     if (P.replay) {
       output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
     } else {
-      output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, c->tiles.as<Tile>(), (int)L->tiles.size(),
-                                                            c->tpart.as<double>());
-      output_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, c->first_tile.as<int>(), c->tpart.as<double>());
+      output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
+      output_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
     }
     HIPCHK(hipGetLastError());
     return HBT_OK;
@@ -540,6 +613,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->isb.ensure(s));
   RETCHK(c->seg_of_slot.ensure(s * 4));
   RETCHK(c->ncount.ensure((size_t)(NC_ROWS * nseg + 1) * 4));
+  RETCHK(c->segact.ensure((size_t)(nseg + 1) * 4));
   RETCHK(c->tsum.ensure((size_t)(ntiles + 1) * 8));
   RETCHK(c->tpart.ensure((size_t)(ntiles + 1) * 8 * REC_SUMS));
   RETCHK(c->active.ensure(16));
@@ -553,6 +627,9 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.seg = c->seg.as<Seg>();
   W.nseg = (int)L.seg.size();
   W.nslots = L.nslots;
+  W.act_chunk = nullptr;
+  W.nact = L.nslots;
+  W.segact = c->segact.as<int>();
   W.seg_of_slot = c->seg_of_slot.as<int>();
   W.ibase = c->ibase.as<int64_t>();
   W.ncount = c->ncount.as<int>();
@@ -601,11 +678,8 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   R.P = make_dev_params(c->params);
   R.W = make_work(c, L, parts, subs, out_index, out_energy);
   R.max_n = L.maxcap;
-  R.gs = grid_for(L.nslots, c->cus);
-  R.gw = walk_grid(L.nslots, c->tune.xcd_run);
+  R.list_all();
   R.gseg = grid_for(nseg, c->cus);
-  R.gtile = (int)std::min<int64_t>((int64_t)L.tiles.size(), (int64_t)c->cus * 8);
-  if (R.gtile < 1) R.gtile = 1;
   const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
   R.gblk = gsb;
   fill_seg_of_slot_kernel<<<gsb, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, c->capal.as<int>(),
@@ -693,6 +767,9 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     // the next pass is bounded by this one
     R.max_n = c->h_active[1];
     R.max_depth = std::min(MAX_DEPTH, std::max(0, c->h_active[2]));
+    // once a quarter of the listed segments have finished, the next passes
+    // visit only the slots and tiles of the active ones
+    if (c->h_active[0] > 0 && 4 * c->h_active[0] <= 3 * R.nlisted && compact_passes()) RETCHK(R.list_active());
     if (c->prof_mode == 1) {
       float wms = 0;
       HIPCHK(hipEventElapsedTime(&wms, c->evw0, c->evw1));
