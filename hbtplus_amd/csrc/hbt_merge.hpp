@@ -240,11 +240,11 @@ __global__ void merge_first_kernel(const int64_t *__restrict__ list, const int64
 }  // namespace
 
 extern "C" int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
-                                  int64_t n_list, hbt_subhalo *subs_host, const int64_t *host_group,
+                                  int64_t n_list, hbt_subhalo *subs_io, const int64_t *host_group,
                                   int64_t nsub, int merge_trapped, int64_t *out_list, int64_t out_capacity,
                                   int64_t *out_total) {
   if (!ctx || !snapshot || n_list < 0 || nsub < 0 || out_capacity < 0 ||
-      (nsub > 0 && (!subs_host || !host_group || !list_index || !out_list)))
+      (nsub > 0 && (!subs_io || !host_group || !list_index || !out_list)))
     return fail(HBT_ERR_INVALID_ARG, "bad argument");
   if (out_total) *out_total = 0;
   if (nsub == 0) return HBT_OK;
@@ -257,6 +257,9 @@ extern "C" int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, c
   hipStream_t st = ctx->stream;
   const int isnap = ctx->params.snapshot_index;
   const DevParams P = make_dev_params(ctx->params);
+  // the records are worked on in a copy: a call that fails leaves them as they were
+  std::vector<hbt_subhalo> work(subs_io, subs_io + nsub);
+  hbt_subhalo *subs_host = work.data();
 
   // ---- host checks, nests and glued hosts ----------------------------------
   std::vector<std::vector<int>> nested(nsub);
@@ -669,5 +672,6 @@ extern "C" int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, c
     r.nsource = r.count;
     r.out_offset = r.offset;
   }
+  std::copy(work.begin(), work.end(), subs_io);
   return HBT_OK;
 }

@@ -315,6 +315,7 @@ int hbt_nest_subhalos(hbt_ctx *ctx, int64_t nsub, const int64_t *host_group,
  *                velocity, and the Unbind inputs of the re-unbound hosts.
  *                Written: the merged state; offset/count then give each list
  *                in out_list (list_len = nsource = count, out_offset = offset).
+ *                A failed call leaves them unchanged.
  *                A trapped, alive subhalo with nbound > 1 on entry is rejected
  *                (unbinding never leaves one: it clears the trap of a subhalo
  *                that stays bound and marks the others dead).
