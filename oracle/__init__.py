@@ -45,6 +45,7 @@ def lib():
         L.oracle_mask_subhalos.argtypes = [vp, vp, vp, i64, vp, vp]
         L.oracle_merge_subhalos.argtypes = [vp, vp, vp, i64, vp, vp, i64, i32, vp, vp, i64, vp]
         L.oracle_nest_subhalos.argtypes = [i64, vp, vp, vp, vp]
+        L.oracle_merge_cores.argtypes = [vp, vp, vp, vp, i64, vp]
         L.oracle_abi_sizes.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
@@ -130,6 +131,37 @@ def mask_subhalos(cols, subs, host_group):
     if rc != 0:
         raise RuntimeError("oracle_mask_subhalos failed")
     return out, cnt
+
+
+def merge_cores(params, cols, list_index, subs):
+    """The merging cores restated: float[nsub, 8] (position, velocity,
+    sigma_R, sigma_V)."""
+    c = {k: np.ascontiguousarray(v) for k, v in cols.items()}
+    ps = particles_struct(c)
+    li = np.ascontiguousarray(list_index, np.int64)
+    subs = np.ascontiguousarray(subs, SUB_DTYPE)
+    out = np.zeros((len(subs), 8), np.float32)
+    rc = lib().oracle_merge_cores(_p(params), ctypes.byref(ps), _p(li), _p(subs), len(subs), _p(out))
+    if rc != 0:
+        raise RuntimeError("oracle_merge_cores failed")
+    return out
+
+
+def ref_merge_cores(params, cols, list_index, subs):
+    """The reference's own SubHelper_t::BuildPosition / BuildVelocity
+    (oracle/_ref/hbt_ref_tracking mode cores): float[nsub, 8]."""
+    n = len(cols["x"])
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.ascontiguousarray(params, PARAMS_DTYPE).tobytes())
+            f.write(np.array([len(subs), n, len(list_index)], np.int64).tobytes())
+            for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):
+                f.write(_f32(cols[k]).tobytes())
+            f.write(np.ascontiguousarray(list_index, np.int64).tobytes())
+            f.write(np.ascontiguousarray(subs, SUB_DTYPE).tobytes())
+        subprocess.run([REF_TRACKING, "cores", fin, fout], check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        return np.fromfile(fout, np.float32).reshape(len(subs), 8)
 
 
 def nest_subhalos(host_group, is_central, parent, rank):

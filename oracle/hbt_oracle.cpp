@@ -600,6 +600,65 @@ void shape(const Params &P, const PartCols &C, const hbt_subhalo &s, const Int *
   }
 }
 
+// SubHelper_t::BuildPosition / BuildVelocity (src/subhalo_merge.cpp:29-127):
+// the core of the first min(nbound, 20) list entries.  The periodic origin is
+// a double, so differences, NEAREST and the final `+= origin` are in double.
+struct MergeCore { float pos[3], vel[3], sig_r, sig_v; bool merged; };
+
+void merge_core(const Params &P, const PartCols &C, const Int *list, Int nb, MergeCore &h) {
+  h.merged = false;
+  h.sig_r = h.sig_v = 0.f;
+  for (int j = 0; j < 3; j++) h.pos[j] = h.vel[j] = 0.f;
+  if (nb <= 0) return;
+  const Int g0 = list[0];
+  if (nb == 1) {
+    for (int j = 0; j < 3; j++) { h.pos[j] = C.x(g0, j); h.vel[j] = C.v(g0, j); }
+    return;
+  }
+  const Int n = nb > 20 ? 20 : nb;
+  double sx[3] = {0, 0, 0}, sx2[3] = {0, 0, 0}, msum = 0, origin[3];
+  for (int j = 0; j < 3; j++) origin[j] = C.x(g0, j);
+  for (Int i = 0; i < n; i++) {
+    const Int g = list[i];
+    const float m = C.m(g);
+    msum += m;
+    for (int j = 0; j < 3; j++) {
+      double dx;
+      if (P.periodic) dx = P.nearest((double)C.x(g, j) - origin[j]);  // float - double
+      else dx = C.x(g, j);
+      sx[j] += dx * m;
+      sx2[j] += dx * dx * m;
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    sx[j] /= msum;
+    sx2[j] /= msum;
+    h.pos[j] = (float)sx[j];
+    if (P.periodic) h.pos[j] = (float)((double)h.pos[j] + origin[j]);  // float += double
+    sx2[j] -= sx[j] * sx[j];
+  }
+  h.sig_r = (float)std::sqrt(sx2[0] + sx2[1] + sx2[2]);
+  double sv[3] = {0, 0, 0}, sv2[3] = {0, 0, 0};
+  msum = 0;
+  for (Int i = 0; i < n; i++) {
+    const Int g = list[i];
+    const float m = C.m(g);
+    msum += m;
+    for (int j = 0; j < 3; j++) {
+      const double dv = C.v(g, j);
+      sv[j] += dv * m;
+      sv2[j] += dv * dv * m;
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    sv[j] /= msum;
+    sv2[j] /= msum;
+    h.vel[j] = (float)sv[j];
+    sv2[j] -= sv[j] * sv[j];
+  }
+  h.sig_v = (float)std::sqrt(sv2[0] + sv2[1] + sv2[2]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -751,63 +810,8 @@ int oracle_merge_subhalos(const hbt_params *hp, const hbt_particles *parts, cons
     S[s].energy.assign(S[s].list.size(), std::numeric_limits<float>::quiet_NaN());
   }
   // FillCores: SubHelper_t::BuildPosition / BuildVelocity
-  struct Helper { float pos[3], vel[3], sig_r, sig_v; bool merged; };
-  std::vector<Helper> H(nsub);
-  for (Int s = 0; s < nsub; s++) {
-    Helper &h = H[s];
-    h.merged = false;
-    h.sig_r = h.sig_v = 0.f;
-    for (int j = 0; j < 3; j++) h.pos[j] = h.vel[j] = 0.f;
-    const Int nb = S[s].nbound;
-    if (nb == 0) continue;
-    const Int g0 = S[s].list[0];
-    if (nb == 1) {
-      for (int j = 0; j < 3; j++) { h.pos[j] = C.x(g0, j); h.vel[j] = C.v(g0, j); }
-      continue;
-    }
-    const Int n = nb > 20 ? 20 : nb;
-    double sx[3] = {0, 0, 0}, sx2[3] = {0, 0, 0}, msum = 0, origin[3];
-    for (int j = 0; j < 3; j++) origin[j] = C.x(g0, j);
-    for (Int i = 0; i < n; i++) {
-      const Int g = S[s].list[i];
-      const float m = C.m(g);
-      msum += m;
-      for (int j = 0; j < 3; j++) {
-        double dx;
-        if (P.periodic) dx = P.nearest((double)C.x(g, j) - origin[j]);  // float - double
-        else dx = C.x(g, j);
-        sx[j] += dx * m;
-        sx2[j] += dx * dx * m;
-      }
-    }
-    for (int j = 0; j < 3; j++) {
-      sx[j] /= msum;
-      sx2[j] /= msum;
-      h.pos[j] = (float)sx[j];
-      if (P.periodic) h.pos[j] = (float)((double)h.pos[j] + origin[j]);  // float += double
-      sx2[j] -= sx[j] * sx[j];
-    }
-    h.sig_r = (float)std::sqrt(sx2[0] + sx2[1] + sx2[2]);
-    double sv[3] = {0, 0, 0}, sv2[3] = {0, 0, 0};
-    msum = 0;
-    for (Int i = 0; i < n; i++) {
-      const Int g = S[s].list[i];
-      const float m = C.m(g);
-      msum += m;
-      for (int j = 0; j < 3; j++) {
-        const double dv = C.v(g, j);
-        sv[j] += dv * m;
-        sv2[j] += dv * dv * m;
-      }
-    }
-    for (int j = 0; j < 3; j++) {
-      sv[j] /= msum;
-      sv2[j] /= msum;
-      h.vel[j] = (float)sv[j];
-      sv2[j] -= sv[j] * sv[j];
-    }
-    h.sig_v = (float)std::sqrt(sv2[0] + sv2[1] + sv2[2]);
-  }
+  std::vector<MergeCore> H(nsub);
+  for (Int s = 0; s < nsub; s++) merge_core(P, C, S[s].list.data(), S[s].nbound, H[s]);
   // DetectTraps
   for (Int i = 0; i < nsub; i++) {
     hbt_subhalo &r = subs[i];
@@ -879,6 +883,24 @@ int oracle_merge_subhalos(const hbt_params *hp, const hbt_particles *parts, cons
       if (out_energy) out_energy[off] = S[s].energy[i];
       out_list[off++] = S[s].list[i];
     }
+  }
+  return HBT_OK;
+}
+
+// The merging cores alone (BuildPosition / BuildVelocity of every subhalo):
+// out[8*i ..] = position[3], velocity[3], sigma_R, sigma_V.
+int oracle_merge_cores(const hbt_params *hp, const hbt_particles *parts, const int64_t *list_index,
+                       const hbt_subhalo *subs, int64_t nsub, float *out) {
+  if (!hp || !parts || (nsub > 0 && (!subs || !list_index || !out))) return HBT_ERR_INVALID_ARG;
+  Params P(*hp);
+  PartCols C{parts};
+  for (Int s = 0; s < nsub; s++) {
+    MergeCore h;
+    merge_core(P, C, list_index + subs[s].offset, subs[s].nbound, h);
+    float *o = out + 8 * s;
+    for (int j = 0; j < 3; j++) { o[j] = h.pos[j]; o[3 + j] = h.vel[j]; }
+    o[6] = h.sig_r;
+    o[7] = h.sig_v;
   }
   return HBT_OK;
 }

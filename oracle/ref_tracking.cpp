@@ -31,6 +31,13 @@
 //          int64 parent[nsub]; int32 rank[nsub]
 //     out: int64 parent[nsub]; int32 rank[nsub] after LevelUpDetachedSubhalos
 //
+//   ref_tracking cores in.bin out.bin
+//     in : hbt_params; int64 nsub, ncol, nlist; x y z vx vy vz mass [ncol]
+//          (float); int64 list_index[nlist]; hbt_subhalo subs[nsub]
+//          (offset/count index list_index; nbound read)
+//     out: float[8 * nsub]: each subhalo's merging core (position[3],
+//          velocity[3], sigma_R, sigma_V; zeros where nbound = 0)
+//
 //   ref_tracking merge_time in.bin out.bin
 //     as merge; out: double seconds spent in MergeSubhalos (OMP_NUM_THREADS
 //     threads, the reference's own parallel loops)
@@ -65,6 +72,21 @@
 #include "subhalo.h"
 #undef private
 #include "../include/hbt_unbind.h"
+
+// The merging helper of src/subhalo_merge.cpp:14-27, declared here with the
+// same members so that the harness can call the reference's compiled
+// SubHelper_t::BuildPosition / BuildVelocity.
+struct SubHelper_t {
+  HBTInt HostTrackId;
+  bool IsMerged;
+  HBTxyz ComovingPosition;
+  HBTxyz PhysicalVelocity;
+  float ComovingSigmaR;
+  float PhysicalSigmaV;
+  void BuildPosition(const Subhalo_t &sub);
+  void BuildVelocity(const Subhalo_t &sub);
+  SubHelper_t() : HostTrackId(-1), IsMerged(false) {}
+};
 
 namespace {
 
@@ -164,6 +186,45 @@ void configure(const hbt_params &p, Snapshot_t &epoch) {
   epoch.Cosmology.OmegaM0 = 0.3f;
   epoch.Cosmology.OmegaLambda0 = 0.7f;
   epoch.Cosmology.OmegaZ = 0.3f;
+}
+
+// The cores of the merging (SubHelper_t::BuildPosition / BuildVelocity) of
+// every subhalo: position[3], velocity[3], sigma_R, sigma_V as floats.
+int run_cores(FILE *in, FILE *out) {
+  hbt_params p;
+  int64_t nsub, ncol, nlist;
+  rd(in, &p, 1); rd(in, &nsub, 1); rd(in, &ncol, 1); rd(in, &nlist, 1);
+  std::vector<float> c[7];
+  for (auto &v : c) v.resize(ncol);
+  for (auto &v : c) rd(in, v.data(), ncol);
+  std::vector<int64_t> list(nlist);
+  rd(in, list.data(), nlist);
+  std::vector<hbt_subhalo> rec(nsub);
+  rd(in, rec.data(), nsub);
+  SubhaloSnapshot_t snap;
+  configure(p, snap);
+  std::vector<float> o(8 * nsub, 0.f);
+  for (int64_t s = 0; s < nsub; s++) {
+    Subhalo_t S;
+    S.Particles.resize(rec[s].count);
+    for (int64_t i = 0; i < rec[s].count; i++) {
+      const int64_t g = list[rec[s].offset + i];
+      S.Particles[i].ComovingPosition = {{c[0][g], c[1][g], c[2][g]}};
+      S.Particles[i].PhysicalVelocity = {{c[3][g], c[4][g], c[5][g]}};
+      S.Particles[i].Mass = c[6][g];
+    }
+    S.Nbound = rec[s].nbound;
+    SubHelper_t h;
+    h.ComovingPosition = {{0.f, 0.f, 0.f}};
+    h.PhysicalVelocity = {{0.f, 0.f, 0.f}};
+    h.BuildPosition(S);
+    h.BuildVelocity(S);
+    for (int j = 0; j < 3; j++) { o[8 * s + j] = h.ComovingPosition[j]; o[8 * s + 3 + j] = h.PhysicalVelocity[j]; }
+    o[8 * s + 6] = h.ComovingSigmaR;
+    o[8 * s + 7] = h.PhysicalSigmaV;
+  }
+  wr(out, o.data(), o.size());
+  return 0;
 }
 
 // timed: only the wall seconds of MergeSubhalos (OpenMP as the reference runs
@@ -313,7 +374,7 @@ int run_merge(FILE *in, FILE *out, bool timed) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge|merge_time|nest in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge|merge_time|nest|cores in.bin out.bin\n", argv[0]); return 1; }
   MPI_Init(&argc, &argv);
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
@@ -323,6 +384,7 @@ int main(int argc, char **argv) {
   else if (!strcmp(argv[1], "merge")) rc = run_merge(in, out, false);
   else if (!strcmp(argv[1], "merge_time")) rc = run_merge(in, out, true);
   else if (!strcmp(argv[1], "nest")) rc = run_nest(in, out);
+  else if (!strcmp(argv[1], "cores")) rc = run_cores(in, out);
   if (in) fclose(in);
   if (out) fclose(out);
   MPI_Finalize();

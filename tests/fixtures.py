@@ -68,6 +68,7 @@ class MergeFixture:
         self.off = (z["off_subs"].view(SUB_DTYPE).copy(), z["off_list"].copy())
         self.pipe = (z["pipe_subs"].view(SUB_DTYPE).copy(), z["pipe_list"].copy(), z["pipe_energy"].copy())
         self.parent = z["parent"].copy()
+        self.cores = z["cores"].copy()
 
 
 def load_merge(name):

@@ -28,7 +28,9 @@ merge_<name>.npz: a catalogue of host groups with sinking satellites
 truncated to nsource, as column positions, with a few hand-set trap and death
 fields), and what the reference's own SubhaloSnapshot_t::MergeSubhalos left,
 with MergeTrappedSubhalos on (out_*) and off (off_*); pipe_*: the same on
-the unbinding's state as it is (RefineParticles then MergeSubhalos).
+the unbinding's state as it is (RefineParticles then MergeSubhalos); cores:
+every subhalo's merging core from the reference's own
+SubHelper_t::BuildPosition / BuildVelocity.
 
 nest_<name>.npz: host groups, central flags, nest hosts and Ranks
 (synth.nesting_catalogue) and the hosts and Ranks after the reference's own
@@ -223,12 +225,14 @@ def save_merge(name, kw):
     on = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, True)
     off = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, False)
     pipe = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, plain, grp, True)
+    cores = oracle.ref_merge_cores(cat.params, cat.cols, lst, subs)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), kind="merge", params=cat.params.view(np.uint8),
                         **{k: cat.cols[k] for k in cat.cols}, parent=cat.subs["parent"],
                         list_index=lst, subs=subs.view(np.uint8), host_group=grp,
                         out_subs=on[0].view(np.uint8), out_list=on[1], out_energy=on[2],
                         off_subs=off[0].view(np.uint8), off_list=off[1],
-                        pipe_subs=pipe[0].view(np.uint8), pipe_list=pipe[1], pipe_energy=pipe[2])
+                        pipe_subs=pipe[0].view(np.uint8), pipe_list=pipe[1], pipe_energy=pipe[2],
+                        cores=cores)
     merged = int(((on[0]["count"] == 1) & (subs["count"] > 1)).sum())
     print(f"{name}: {len(subs)} subhalos, {cat.n_particles} particles, {merged} merged away, "
           f"{int((on[0]['sink_track_id'] != subs['sink_track_id']).sum())} newly trapped")
