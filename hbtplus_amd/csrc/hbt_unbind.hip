@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <initializer_list>
 #include <string>
 #include <unordered_map>
@@ -185,8 +186,11 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
                  const hbt_tuning &tune, Level &L) {
   const int n = (int)members.size();
   // largest first: merge levels and per-depth launches then cover a prefix
-  std::vector<int> order(members);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cap[a] > cap[b]; });
+  std::vector<std::pair<int64_t, int>> key(members.size());
+  for (size_t i = 0; i < members.size(); i++) key[i] = {-cap[members[i]], (int)i};
+  std::sort(key.begin(), key.end());  // by capacity descending, then input order
+  std::vector<int> order(members.size());
+  for (size_t i = 0; i < members.size(); i++) order[i] = members[key[i].second];
   L.subs = order;
   L.seg.assign(n, Seg{});
   L.ibase.resize(n);
@@ -670,6 +674,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
               int64_t *out_index, float *out_energy, int64_t *passes) {
   if (L.seg.empty()) return HBT_OK;
   const int64_t nseg = (int64_t)L.seg.size();
+  const auto ts0 = std::chrono::steady_clock::now();
   RETCHK(ensure_slots(c, L.nslots, nseg, (int64_t)L.tiles.size()));
   RETCHK(stage_level(c, L));
   Runner R;
@@ -691,6 +696,11 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   output_list_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 1);
   output_record_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P, 1);
   HIPCHK(hipGetLastError());
+  if (getenv("HBT_PASS_TIMES") && atoi(getenv("HBT_PASS_TIMES"))) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    fprintf(stderr, "[hbt] level set up (staging, lists, gather) in %.3f ms\n",
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
+  }
   // every pass that does not converge removes at least one particle
   static const bool debug = getenv("HBT_DEBUG") != nullptr;
   // The pass sequence has static launch shapes and arguments for the whole
@@ -736,9 +746,20 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     release();
     return HBT_OK;
   }
+  // HBT_PASS_TIMES=1: per pass, the GPU time between its first and last
+  // launch and the host time, printed to stderr (diagnostics)
+  static const bool pass_times = getenv("HBT_PASS_TIMES") && atoi(getenv("HBT_PASS_TIMES"));
+  hipEvent_t pe0 = nullptr, pe1 = nullptr;
+  if (pass_times) {
+    HIPCHK(hipEventCreate(&pe0));
+    HIPCHK(hipEventCreate(&pe1));
+  }
   // every pass that does not converge removes at least one particle
   for (int it = 0; it < L.maxcap + 2; it++) {
+    const auto h0 = std::chrono::steady_clock::now();
+    if (pass_times) HIPCHK(hipEventRecord(pe0, c->stream));
     RETCHK(R.pass());
+    if (pass_times) HIPCHK(hipEventRecord(pe1, c->stream));
     (*passes)++;
     if (debug) {
       std::vector<Seg> hs(nseg);
@@ -764,6 +785,13 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     }
     HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (pass_times) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, pe0, pe1));
+      const double hms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+      fprintf(stderr, "[hbt] level of %zu: pass %d gpu %.3f ms host %.3f ms, active after %d, listed %d\n",
+              L.seg.size(), it + 1, ms, hms, c->h_active[0], R.nlisted);
+    }
     // the next pass is bounded by this one
     R.max_n = c->h_active[1];
     R.max_depth = std::min(MAX_DEPTH, std::max(0, c->h_active[2]));
@@ -778,6 +806,8 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     }
     if (*c->h_active == 0) break;
   }
+  if (pe0) (void)hipEventDestroy(pe0);
+  if (pe1) (void)hipEventDestroy(pe1);
   return HBT_OK;
 }
 
@@ -787,7 +817,8 @@ int plan_nests(const hbt_subhalo *subs, int64_t nsub, std::vector<int64_t> &cap,
                std::vector<int> &roots) {
   cap.assign(nsub, 0);
   height.assign(nsub, 0);
-  children.assign(nsub, {});
+  children.resize(nsub);
+  for (auto &ch : children) ch.clear();  // keeps each list's memory
   roots.clear();
   for (int64_t s = 0; s < nsub; s++) {
     if (subs[s].count < 0 || subs[s].parent >= nsub || subs[s].parent == s)
@@ -1063,9 +1094,12 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   if (nsub == 0) return HBT_OK;
   if (particles->n >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 particles");
   HIPCHK(hipSetDevice(ctx->device));
-  std::vector<int64_t> cap;
-  std::vector<int> height, order;
-  std::vector<std::vector<int>> children;
+  // host tables kept per thread from call to call (their memory is reused,
+  // so a repeated batch pays no page faults for them)
+  static thread_local std::vector<int64_t> cap;
+  static thread_local std::vector<int> height, order;
+  static thread_local std::vector<std::vector<int>> children;
+  static thread_local Level L;
   RETCHK(plan_nests(subs_host, nsub, cap, height, children, order));
   for (int64_t s = 0; s < nsub; s++) {
     if (subs_host[s].offset < 0 || subs_host[s].offset + subs_host[s].count > n_list)
@@ -1077,7 +1111,6 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   if (plain) {
     for (int s : order) {
-      Level L;
       build_level(subs_host, cap, children, std::vector<int>{s}, ctx->tune, L);
       RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
     }
@@ -1085,12 +1118,15 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     int hmax = 0;
     for (int64_t s = 0; s < nsub; s++) hmax = std::max(hmax, height[s]);
     std::vector<std::vector<int>> by_h(hmax + 1);
-    // largest first within a level keeps the heavy segments' blocks early
+    // build_level lays each level out largest first
     for (int s : order) by_h[height[s]].push_back(s);
+    static const bool pass_times = getenv("HBT_PASS_TIMES") && atoi(getenv("HBT_PASS_TIMES"));
     for (int h = 0; h <= hmax; h++) {
-      std::stable_sort(by_h[h].begin(), by_h[h].end(), [&](int a, int b) { return cap[a] > cap[b]; });
-      Level L;
+      const auto t0 = std::chrono::steady_clock::now();
       build_level(subs_host, cap, children, by_h[h], ctx->tune, L);
+      if (pass_times)
+        fprintf(stderr, "[hbt] level %d of %zu segments built on the host in %.3f ms\n", h, L.seg.size(),
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
       RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
     }
   }
