@@ -270,3 +270,17 @@ def test_nest_hand_derived():
     p, r = oracle.nest_subhalos(a[:, 0], a[:, 1].astype(np.int32), a[:, 2], a[:, 3].astype(np.int32))
     assert p.tolist() == a[:, 4].tolist()
     assert r.tolist() == a[:, 5].tolist()
+
+
+def test_id_to_index_against_a_dictionary():
+    """ID -> position restated (MappedIndexTable_t, src/hash.tpp:18-56):
+    every known ID gives its position, unknown IDs and negative (null) IDs
+    give -1, checked against a plain dictionary"""
+    rng = np.random.default_rng(12)
+    table = synth.unique_ids(np.random.Generator(np.random.Philox(key=9)), 5000)
+    q = np.concatenate([table[rng.integers(0, len(table), 3000)], rng.integers(0, 2 ** 62, 300),
+                        -rng.integers(1, 50, 20), np.array([-1, 0], np.int64)])
+    rng.shuffle(q)
+    where = {int(v): i for i, v in enumerate(table.tolist())}
+    want = np.array([where.get(int(v), -1) if v >= 0 else -1 for v in q.tolist()], np.int64)
+    assert np.array_equal(oracle.id_to_index(table, q), want)
