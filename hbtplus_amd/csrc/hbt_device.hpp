@@ -49,6 +49,8 @@ struct Seg {
   long long evals;
   int D, dlen, straddle;
   int Dfull;                  // D of the latest full pass: bounds D of every later pass
+  int sfull;                  // straddle of the latest full pass, whose sources hold
+                              // every later pass's sources and targets
   int wgroup;                 // lanes per traversal group in the walk
   float ref_pos[3], ref_vel[3], old_pos[3], old_vel[3];
   float rvd[3], dK;

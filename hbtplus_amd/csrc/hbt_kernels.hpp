@@ -299,7 +299,14 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
     // correction pass's: the latest full pass bounds the depth of later passes
     if (!(S.flags & F_CORR)) S.Dfull = D;
     atomicMax(&W.active[2], S.Dfull > D ? S.Dfull : D);
-    S.straddle = (P.periodic && Len > 0.5 * (double)P.box_half) ? 1 : 0;
+    // the FP32 walk drops the periodic wrap unless some target-source pair can
+    // be more than half a box apart: a correction pass's sources (the block
+    // just removed) may lie on one side of the box edge while its targets lie
+    // on both, so its flag also takes the latest full pass's, whose sources
+    // hold both sets
+    const int st = (P.periodic && Len > 0.5 * (double)P.box_half) ? 1 : 0;
+    if (!(S.flags & F_CORR)) S.sfull = st;
+    S.straddle = st | S.sfull;
   }
 }
 

@@ -26,7 +26,9 @@ __global__ void tree_sources_kernel(Work W, int n, const float *x, const float *
 
 __global__ void walk_points_kernel(Work W, DevParams P, int64_t nt, const float *tx, const float *ty,
                                    const float *tz, const float *tm, double *out) {
-  const Seg &S = W.seg[0];
+  // the targets may lie anywhere: a periodic box always takes the wrapped walk
+  Seg S = W.seg[0];
+  S.straddle = P.periodic;
   unsigned ni = 0, no = 0;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nt; i += (int64_t)gridDim.x * BLOCK)
     out[i] = walk_dispatch<false>(W, P, S, tx[i], ty[i], tz[i], tm ? tm[i] : 0.f, ni, no);

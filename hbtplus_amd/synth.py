@@ -304,16 +304,25 @@ def config1(seed: int = 1001, bound_mass_precision: float = 0.5, n_nfw=20000, n_
     return _assemble(rng, groups, 1e5, False, params, f"C1 single 24k NFW (BoundMassPrecision={bound_mass_precision})")
 
 
-def config2(seed: int = 1002, nsub: int = 5000, hi: float = 2.0e5) -> Catalogue:
+def config2(seed: int = 1002, nsub: int = 5000, hi: float = 2.0e5, at_edges: bool = False) -> Catalogue:
     """5,000 subhalos in a periodic 100 Mpc/h box, N_p ~ N^-1.9 on [20, 2e5],
     log-normal c (median 10, sigma_ln 0.3), r200 = 200 (N_p/2e4)^(1/3),
-    20% uniform contaminants at 3x the local escape speed."""
+    20% uniform contaminants at 3x the local escape speed.  at_edges: each
+    centre coordinate is put on the box edge (0) with probability 0.6, so
+    most subhalos wrap around one to three faces of the periodic box."""
     rng = _rng(seed)
     box = 1.0e5
-    pop = _power_law_population(rng, nsub, 20, hi, 0.2, "escape", 10.0, 0.3, G_DEFAULT, box,
-                                lambda n: rng.random((n, 3)) * box)
+
+    def centres(n):
+        c = rng.random((n, 3)) * box
+        if at_edges:
+            c[rng.random((n, 3)) < 0.6] = 0.0
+        return c
+
+    pop = _power_law_population(rng, nsub, 20, hi, 0.2, "escape", 10.0, 0.3, G_DEFAULT, box, centres)
     params = make_params(softening=0.5, open_angle=0.45, periodic=1, box_size=box)
-    return _assemble_arrays(rng, *pop, box, True, params, f"C2 {nsub} power-law subhalos")
+    name = f"C2 {nsub} power-law subhalos" + (" at the box edges" if at_edges else "")
+    return _assemble_arrays(rng, *pop, box, True, params, name)
 
 
 def config3(seed: int = 1003, nsub: int = 50000) -> Catalogue:

@@ -129,3 +129,45 @@ def assert_merge(ref, got, ties=False):
             eb = np.array([e_of.get(p, np.nan) for p in b.tolist()], np.float32)
             a, b = canonical_order(a, ea[s]), canonical_order(b, eb)
         assert np.array_equal(a, b), f"list of subhalo {s} differs"
+
+
+def check_unbind_invariants(cat, out, conv=None):
+    """Properties of an unbinding result that hold at any size, checked with
+    array operations (no per-subhalo loop), for catalogues without nesting:
+    every output list is a permutation of its subhalo's source particles;
+    nbound <= list_len; a converged subhalo's bound block has negative,
+    non-decreasing energies (most bound first) and its first particle is the
+    most-bound particle of the record; the per-type bound counts add up to the
+    bound particles of each type; mbound is their mass sum.  conv: which
+    subhalos converged (default: the records' status).  Returns their number."""
+    subs, idx, en = out
+    src = cat.subs
+    n = len(subs)
+    assert np.all(src["parent"] < 0), "nested catalogue"
+    L = subs["list_len"].astype(np.int64)
+    assert np.array_equal(L, src["count"]), "list length differs from the source count"
+    nb = subs["nbound"].astype(np.int64)
+    assert np.all((nb >= 0) & (nb <= L)), "nbound outside [0, list_len]"
+    seg = np.repeat(np.arange(n), L)
+    pos = np.arange(len(seg)) - np.repeat(np.cumsum(L) - L, L)
+    rows = idx[np.repeat(subs["out_offset"], L) + pos]
+    # permutation of the source rows offset .. offset+count
+    order = np.lexsort((rows, seg))
+    assert np.array_equal(rows[order], np.repeat(src["offset"], L) + pos), "list is not a permutation"
+    conv = (subs["status"] & HBT_STATUS_CONVERGED) != 0 if conv is None else np.asarray(conv, bool)
+    bound = conv[seg] & (pos < nb[seg])
+    e = en[np.repeat(subs["out_offset"], L) + pos]
+    assert np.all(e[bound] < 0), "a bound particle has E >= 0"
+    step = bound[1:] & bound[:-1] & (seg[1:] == seg[:-1])
+    assert np.all(e[1:][step] >= e[:-1][step]), "bound block is not in ascending energy"
+    first = subs["out_offset"][conv]
+    for k, c in enumerate("xyz"):
+        assert np.array_equal(subs["mostbound_pos"][conv, k], cat.cols[c][idx[first]]), "most-bound position"
+    typ = cat.cols["type"][rows].astype(np.int64)
+    tmax = subs["nbound_type"].shape[1]
+    cnt = np.bincount(seg[bound] * tmax + typ[bound], minlength=n * tmax).reshape(n, tmax)
+    assert np.array_equal(cnt[conv], subs["nbound_type"][conv]), "per-type bound counts"
+    assert np.array_equal(cnt[conv].sum(1), nb[conv]), "per-type counts do not add up to nbound"
+    mass = np.bincount(seg[bound], weights=cat.cols["mass"][rows][bound].astype(np.float64), minlength=n)
+    assert np.allclose(subs["mbound"][conv], mass[conv], rtol=1e-5, atol=0), "mbound differs from the mass sum"
+    return int(conv.sum())

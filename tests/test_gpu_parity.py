@@ -16,7 +16,7 @@ import pytest
 import oracle
 from fixtures import TREE_CASES, UNBIND_CASES, load_tree, load_unbind
 from hbtplus_amd import abi, synth
-from helpers import assert_exact, assert_tolerant
+from helpers import assert_exact, assert_tolerant, check_unbind_invariants
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,10 @@ def _with(params, **kw):
     for k, v in kw.items():
         p[k] = v
     return p
+
+
+def _same_bytes(a, b):
+    return all(np.array_equal(x.view(np.uint8), y.view(np.uint8)) for x, y in zip(a, b))
 
 
 # ---- potentials ------------------------------------------------------------
@@ -67,6 +71,31 @@ def test_tree_potential_external_targets(gpu):
     ref = oracle.tree_potential(p, src, tgt, None)
     got = gpu(p).tree_potential(src, tgt, None)
     assert np.array_equal(ref, got)
+
+
+@pytest.mark.parametrize("prec", [abi.HBT_PREC_FAST, abi.HBT_PREC_REPLAY])
+def test_tree_potential_targets_across_the_box_edge(gpu, prec):
+    """periodic box: sources just above the z = 0 face, targets just below
+    z = box (and some among the sources); the separations wrap around the
+    face in both precisions"""
+    rng = np.random.default_rng(5)
+    box = 1.0e4
+    n = 3000
+    sp = rng.normal(size=(n, 3)) * 25.0 + np.array([500.0, 500.0, 150.0])
+    assert sp[:, 2].min() > 0
+    src = tuple(sp[:, i].astype(np.float32) for i in range(3)) + (np.full(n, 1e-4, np.float32),)
+    tp = rng.normal(size=(400, 3)) * 30.0 + np.array([500.0, 500.0, box - 60.0])
+    tp = np.concatenate([tp, sp[:100]]) % box
+    tgt = tuple(tp[:, i].astype(np.float32) for i in range(3))
+    p = abi.make_params(periodic=1, box_size=box, precision=prec)
+    ref = oracle.tree_potential(p, src, tgt, None)
+    got = gpu(p).tree_potential(src, tgt, None)
+    if prec == abi.HBT_PREC_REPLAY:
+        assert np.array_equal(ref, got)
+    else:
+        rel = np.abs(got / ref - 1)
+        print(f"FAST across the edge: max rel {rel.max():.2e}")
+        assert rel.max() < 1e-5
 
 
 # ---- unbinding: golden fixtures ----------------------------------------------
@@ -127,6 +156,24 @@ def test_unbind_c2_sample_fast_and_replay(gpu):
     print("C2 sample fast:", assert_tolerant(ref, got))
     rep = gpu(_with(cat.params, precision=abi.HBT_PREC_REPLAY)).unbind_batch(cat)
     print("C2 sample replay:", assert_tolerant(ref, rep))
+
+
+def test_unbind_at_box_edges(gpu):
+    """400 subhalos centred on the faces, edges and corners of the periodic
+    box (380 wrap around it): FAST meets the tolerant criteria and the
+    size-free invariants, REPLAY is bit-identical to the oracle, and the
+    per-subhalo form gives the batched bytes.  A correction pass's sources
+    (the block just removed) can lie on one side of a face while its targets
+    lie on both."""
+    cat = synth.config2(seed=1012, nsub=400, hi=2.0e4, at_edges=True)
+    ref = oracle.unbind_batch(cat, nthreads=16)
+    U = gpu(_with(cat.params, precision=abi.HBT_PREC_FAST))
+    fast = U.unbind_batch(cat)
+    check_unbind_invariants(cat, fast)
+    print("edges fast:", assert_tolerant(ref, fast))
+    assert _same_bytes(fast, U.unbind_batch(cat, plain=True))
+    rep = gpu(_with(cat.params, precision=abi.HBT_PREC_REPLAY)).unbind_batch(cat)
+    assert_exact(ref, rep)
 
 
 def test_repeated_calls_are_identical(gpu):
@@ -289,3 +336,40 @@ def test_plain_equals_batched_c3_sample(gpu):
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
     print("C3 sample fast vs oracle:", assert_tolerant(oracle.unbind_batch(cat, nthreads=16), a))
+
+
+# ---- full sizes ------------------------------------------------------------------
+
+def test_unbind_c2_full_size(gpu):
+    """The bench workload at full size (5,000 power-law subhalos, 1.50 M
+    particles): FAST meets the tolerant criteria against the oracle on every
+    subhalo and the size-free invariants, a second call is byte-identical,
+    and REPLAY is bit-identical to the oracle"""
+    cat = synth.config2()
+    U = gpu(_with(cat.params, precision=abi.HBT_PREC_FAST))
+    got = U.unbind_batch(cat)
+    nconv = check_unbind_invariants(cat, got)
+    ref = oracle.unbind_batch(cat, nthreads=16)
+    stats = assert_tolerant(ref, got)
+    assert _same_bytes(got, U.unbind_batch(cat))
+    print(f"C2 full: {nconv} of {len(cat.subs)} converged, {stats}")
+    assert_exact(ref, gpu(_with(cat.params, precision=abi.HBT_PREC_REPLAY)).unbind_batch(cat))
+
+
+def test_unbind_c3_full_size(gpu):
+    """50,000 subhalos, 44.3 M particles (N_p log-uniform on [20, 4096], 15%
+    contaminants), FAST: the size-free invariants over the whole catalogue,
+    the tolerant criteria against the oracle on 1,000 subhalos drawn at
+    random (a flat batch's subhalos do not interact, so the oracle runs them
+    alone; lists are compared as particle IDs), and a second call
+    byte-identical"""
+    cat = synth.config3()
+    U = gpu(_with(cat.params, precision=abi.HBT_PREC_FAST))
+    got = U.unbind_batch(cat)
+    nconv = check_unbind_invariants(cat, got)
+    sel = np.sort(np.random.default_rng(7).choice(len(cat.subs), 1000, replace=False))
+    part = cat.subset(sel)
+    rs, ri, re_ = oracle.unbind_batch(part, nthreads=16)
+    stats = assert_tolerant((rs, part.cols["id"][ri], re_), (got[0][sel], cat.cols["id"][got[1]], got[2]))
+    assert _same_bytes(got, U.unbind_batch(cat))
+    print(f"C3 full: {nconv} of {len(cat.subs)} converged, sample {stats}")

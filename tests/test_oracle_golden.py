@@ -9,7 +9,7 @@ import oracle
 from fixtures import (MASK_CASES, MERGE_CASES, NEST_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_merge,
                       load_nest, load_props, load_tree, load_unbind)
 from hbtplus_amd.abi import HBT_STATUS_CONVERGED, PROPS_FIELDS
-from helpers import assert_exact, assert_merge
+from helpers import assert_exact, assert_merge, check_unbind_invariants
 
 
 @pytest.mark.parametrize("name", UNBIND_CASES)
@@ -17,6 +17,57 @@ def test_oracle_matches_reference_unbinding(name):
     cat, ref = load_unbind(name)
     got = oracle.unbind_batch(cat)
     assert_exact(ref, got)
+
+
+@pytest.mark.parametrize("name", UNBIND_CASES)
+def test_size_free_invariants_hold_for_reference_outputs(name):
+    """The properties the full-size GPU tests check hold for the reference's
+    own outputs (converged = nbound > 1: its records carry no status) and for
+    the oracle's"""
+    cat, ref = load_unbind(name)
+    if np.any(cat.subs["parent"] >= 0):
+        pytest.skip("nested catalogue")
+    assert check_unbind_invariants(cat, ref, conv=ref[0]["nbound"] > 1) >= 1
+    assert check_unbind_invariants(cat, oracle.unbind_batch(cat)) >= 1
+
+
+def test_size_free_invariants_catch_planted_mistakes():
+    cat, _ = load_unbind("c2_small")
+    good = oracle.unbind_batch(cat)
+    check_unbind_invariants(cat, good)
+    s0 = int(np.argmax(good[0]["nbound"]))
+    o = int(good[0]["out_offset"][s0])
+
+    def planted(fn):
+        subs, idx, en = good[0].copy(), good[1].copy(), good[2].copy()
+        fn(subs, idx, en)
+        with pytest.raises(AssertionError):
+            check_unbind_invariants(cat, (subs, idx, en))
+
+    def swap(subs, idx, en):  # two bound particles out of energy order
+        idx[[o + 1, o + 5]] = idx[[o + 5, o + 1]]
+        en[[o + 1, o + 5]] = en[[o + 5, o + 1]]
+
+    def foreign(subs, idx, en):  # a particle of another subhalo
+        idx[o + 2] = good[1][int(good[0]["out_offset"][(s0 + 1) % len(subs)])]
+
+    def type_count(subs, idx, en):
+        subs["nbound_type"][s0, 1] += 1
+
+    def mass(subs, idx, en):
+        subs["mbound"][s0] *= 1.001
+
+    def most_bound(subs, idx, en):
+        subs["mostbound_pos"][s0, 0] += 1.0
+
+    def unbound_in_block(subs, idx, en):
+        en[o + 3] = abs(en[o + 3])
+
+    def too_many(subs, idx, en):
+        subs["nbound"][s0] = subs["list_len"][s0] + 1
+
+    for fn in (swap, foreign, type_count, mass, most_bound, unbound_in_block, too_many):
+        planted(fn)
 
 
 @pytest.mark.parametrize("name", TREE_CASES)
