@@ -637,6 +637,16 @@ __device__ __forceinline__ bool group_any_wave(bool p) {
   return __builtin_amdgcn_inverse_ballot_w64(b);
 }
 
+// Lane K of the caller's aligned group of GROUP (8, 16 or 32) lanes, read by
+// every lane of the group (ds_swizzle, bit mode: source lane = (lane & AND) |
+// K within each half-wave).  Every lane of the wave must be active.
+template <int GROUP, int K>
+__device__ __forceinline__ int group_field(int v) {
+  static_assert(GROUP >= 8 && GROUP <= 32 && K < 8, "group_field: 8 to 32 lanes");
+  constexpr int AND = 0x1f & ~(GROUP - 1);
+  return __builtin_amdgcn_ds_swizzle(v, AND | (K << 5));
+}
+
 // Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN):
 // sep2 = squared separation, opens = the opening test, add = the softened
 // monopole term.  Straddling segments (a periodic subhalo wider than a
@@ -726,7 +736,7 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
   int until = live ? ib : INT_MAX;  // lane is on its path at j iff until <= j
   int j = group_any<GROUP>(live, lanes) ? ib : end;
   const int last = end - 1 > ib ? end - 1 : ib;
-  if (GROUP == WAVE) {
+  if constexpr (GROUP == WAVE) {
     j = __builtin_amdgcn_readfirstlane(j);
     float4 q = it[(unsigned)min(j, last)].pm;
     int2 a = it[(unsigned)min(j, last)].ax;
@@ -755,12 +765,18 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
       a = go ? a1 : as;
     }
   } else {
+    // lane r of every 8 fetches dword r of its group's item (r = 6, 7 read
+    // the padding) and the group takes the six fields from those lanes through
+    // the LDS crossbar: 4 bytes per lane through the vector memory pipe
     const char *base = (const char *)it;
+    const unsigned dw = 4u * (threadIdx.x & 7u);
     while (__any(j < end)) {
       const bool on = j < end;
-      const unsigned off = (unsigned)min(j, last) * (unsigned)sizeof(Item);
-      const float4 q = *(const float4 *)(base + off);
-      const int2 a = *(const int2 *)(base + off + 16);
+      const unsigned off = (unsigned)min(j, last) * (unsigned)sizeof(Item) + dw;
+      const int v = *(const int *)(base + off);
+      const float4 q = make_float4(__int_as_float(group_field<GROUP, 0>(v)), __int_as_float(group_field<GROUP, 1>(v)),
+                                   __int_as_float(group_field<GROUP, 2>(v)), __int_as_float(group_field<GROUP, 3>(v)));
+      const int2 a = make_int2(group_field<GROUP, 4>(v), group_field<GROUP, 5>(v));
       const int j1 = j + 1, js = max(a.y, j1);
       const bool path = on & (until <= j);
       const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);

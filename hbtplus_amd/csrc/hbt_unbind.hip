@@ -112,6 +112,7 @@ struct hbt_ctx {
   DView seg, ibase, tiles, first_tile, ntile, child_ptr, child_list, capal;
   DBuf ncount, tsum, tpart;
   DBuf segact, actbuf;  // segment activity after a pass; lists of the active segments
+  DBuf gaps;            // (subhalo, capacity) of the hosts whose output regions have unused slots
   DBuf active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
@@ -361,6 +362,21 @@ __global__ void partition_counts_kernel(Work W, const int *first_tile, const u64
     S.nP = (int)(t & 0xffffffffu);
     S.nG = (int)(t >> 32);
     S.K = 0;
+  }
+}
+
+// A host's output region holds its own list and every nested particle (the
+// planned capacity); only its subhalos' unbound tails arrive, so the slots
+// past its final list are set to -1 and NaN.  gap: (subhalo, capacity) pairs.
+__global__ void __launch_bounds__(BLOCK) fill_gaps_kernel(const hbt_subhalo *subs, const int64_t *gap, int ngap,
+                                                          int64_t *out_index, float *out_energy) {
+  for (int h = blockIdx.x; h < ngap; h += gridDim.x) {
+    const int64_t s = gap[2 * h], cap = gap[2 * h + 1];
+    const int64_t o = subs[s].out_offset, len = subs[s].list_len;
+    for (int64_t i = (len > 0 ? len : 0) + threadIdx.x; i < cap; i += BLOCK) {
+      out_index[o + i] = -1;
+      out_energy[o + i] = __builtin_nanf("");
+    }
   }
 }
 
@@ -1129,6 +1145,19 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
       RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
     }
+  }
+  static thread_local std::vector<int64_t> gap;
+  gap.clear();
+  for (int64_t s = 0; s < nsub; s++)
+    if (!children[s].empty()) { gap.push_back(s); gap.push_back(cap[s]); }
+  if (!gap.empty()) {
+    RETCHK(ctx->gaps.ensure(gap.size() * sizeof(int64_t)));
+    HIPCHK(hipMemcpyAsync(ctx->gaps.p, gap.data(), gap.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    const int ngap = (int)(gap.size() / 2);
+    fill_gaps_kernel<<<std::min(ngap, 4 * ctx->cus), BLOCK, 0, ctx->stream>>>(
+        subs, ctx->gaps.as<int64_t>(), ngap, out_index, out_energy);
+    HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -172,9 +172,12 @@ int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total);
  *   out_index : DEVICE int64[total]; for each subhalo the final particle list
  *               as indices into the particle columns: bound particles sorted by
  *               binding energy, then the removed blocks (last removed first),
- *               then appended tails, exactly the reference's Particles order
+ *               then appended tails, exactly the reference's Particles order;
+ *               a host's region keeps room for every nested particle, and the
+ *               slots past its final list_len are set to -1
  *   out_energy: DEVICE float[total]; the binding energy at each list position
  *               (the reference's Elist E, SAVE_BINDING_ENERGY), NaN where none
+ *               and in the unused slots
  *   subs_host : HOST copy of subs (same layout) used for planning; not modified
  *   plain     : 0 = batched form; 1 = plain per-subhalo form (one subhalo at a
  *               time through the same kernels, the loop RefineParticles runs) */

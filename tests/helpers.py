@@ -6,6 +6,23 @@ import numpy as np
 from hbtplus_amd.abi import HBT_STATUS_CONVERGED, OUTPUT_FIELDS
 
 
+def as_ids(ids, idx):
+    """Particle IDs of output positions; slots outside every list (the
+    unused capacity of a nested host's output region holds no position) map
+    to -1."""
+    ok = (idx >= 0) & (idx < len(ids))
+    return np.where(ok, ids[np.where(ok, idx, 0)].astype(np.int64), -1)
+
+
+def unused_slots(subs, total):
+    """Mask of the output slots outside every subhalo's list."""
+    L = subs["list_len"].astype(np.int64)
+    pos = np.repeat(subs["out_offset"], L) + np.arange(L.sum()) - np.repeat(np.cumsum(L) - L, L)
+    free = np.ones(total, bool)
+    free[pos] = False
+    return free
+
+
 def list_of(subs, out, s):
     o, L = int(subs["out_offset"][s]), int(subs["list_len"][s])
     return out[o:o + L]
@@ -133,27 +150,38 @@ def assert_merge(ref, got, ties=False):
 
 def check_unbind_invariants(cat, out, conv=None):
     """Properties of an unbinding result that hold at any size, checked with
-    array operations (no per-subhalo loop), for catalogues without nesting:
-    every output list is a permutation of its subhalo's source particles;
-    nbound <= list_len; a converged subhalo's bound block has negative,
-    non-decreasing energies (most bound first) and its first particle is the
-    most-bound particle of the record; the per-type bound counts add up to the
-    bound particles of each type; mbound is their mass sum.  conv: which
-    subhalos converged (default: the records' status).  Returns their number."""
+    array operations (no per-subhalo loop): every output list is a
+    permutation of its subhalo's source particles plus the unbound tails of
+    its children's output lists (RecursiveUnbind hands a child's unbound
+    particles to its host); nbound <= list_len; a converged subhalo's bound
+    block has negative, non-decreasing energies (most bound first) and its
+    first particle is the most-bound particle of the record; the per-type
+    bound counts add up to the bound particles of each type; mbound is their
+    mass sum.  conv: which subhalos converged (default: the records'
+    status).  Returns their number."""
     subs, idx, en = out
     src = cat.subs
     n = len(subs)
-    assert np.all(src["parent"] < 0), "nested catalogue"
     L = subs["list_len"].astype(np.int64)
-    assert np.array_equal(L, src["count"]), "list length differs from the source count"
     nb = subs["nbound"].astype(np.int64)
     assert np.all((nb >= 0) & (nb <= L)), "nbound outside [0, list_len]"
     seg = np.repeat(np.arange(n), L)
     pos = np.arange(len(seg)) - np.repeat(np.cumsum(L) - L, L)
     rows = idx[np.repeat(subs["out_offset"], L) + pos]
-    # permutation of the source rows offset .. offset+count
-    order = np.lexsort((rows, seg))
-    assert np.array_equal(rows[order], np.repeat(src["offset"], L) + pos), "list is not a permutation"
+    # expected multiset per subhalo: own rows, then each child's tail
+    cnt = src["count"].astype(np.int64)
+    own_seg = np.repeat(np.arange(n), cnt)
+    own_rows = np.repeat(src["offset"], cnt) + np.arange(len(own_seg)) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    kids = np.nonzero(src["parent"] >= 0)[0]
+    tl = (L - nb)[kids]
+    t_seg = np.repeat(src["parent"][kids], tl)
+    t_pos = np.arange(len(t_seg)) - np.repeat(np.cumsum(tl) - tl, tl)
+    t_rows = idx[np.repeat(subs["out_offset"][kids] + nb[kids], tl) + t_pos]
+    e_seg = np.concatenate([own_seg, t_seg])
+    e_rows = np.concatenate([own_rows, t_rows])
+    assert len(e_seg) == len(seg), "list lengths differ from the source counts plus the children's tails"
+    a, b = np.lexsort((rows, seg)), np.lexsort((e_rows, e_seg))
+    assert np.array_equal(seg[a], e_seg[b]) and np.array_equal(rows[a], e_rows[b]), "list is not a permutation"
     conv = (subs["status"] & HBT_STATUS_CONVERGED) != 0 if conv is None else np.asarray(conv, bool)
     bound = conv[seg] & (pos < nb[seg])
     e = en[np.repeat(subs["out_offset"], L) + pos]

@@ -10,13 +10,22 @@ FAST precision (FP32+Kahan walk, FP64 frames) meets the tolerant criteria:
 bound sets identical outside the 1e-5*|E_min| band around E=0, bound counts
 within the band count, most-bound IDs equal unless tied within the band,
 potentials within 1e-4 relative RMS of the reference tree."""
+import hashlib
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import oracle
 from fixtures import TREE_CASES, UNBIND_CASES, load_tree, load_unbind
 from hbtplus_amd import abi, synth
-from helpers import assert_exact, assert_tolerant, check_unbind_invariants
+from helpers import as_ids, assert_exact, assert_tolerant, check_unbind_invariants, unused_slots
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+from make_c4_full import catalogue_digest  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -370,6 +379,71 @@ def test_unbind_c3_full_size(gpu):
     sel = np.sort(np.random.default_rng(7).choice(len(cat.subs), 1000, replace=False))
     part = cat.subset(sel)
     rs, ri, re_ = oracle.unbind_batch(part, nthreads=16)
-    stats = assert_tolerant((rs, part.cols["id"][ri], re_), (got[0][sel], cat.cols["id"][got[1]], got[2]))
+    stats = assert_tolerant((rs, as_ids(part.cols["id"], ri), re_), (got[0][sel], as_ids(cat.cols["id"], got[1]), got[2]))
     assert _same_bytes(got, U.unbind_batch(cat))
     print(f"C3 full: {nconv} of {len(cat.subs)} converged, sample {stats}")
+
+
+def test_unbind_c4_full_size(gpu):
+    """One 4 M-particle host holding 200 satellites (5.33 M particles), FAST:
+    the size-free invariants over the nest (the host's list is its own
+    particles plus the satellites' unbound tails), the satellites against the
+    oracle run on them alone (a satellite is unbound from its own list before
+    its host), the host's unused output slots hold -1 and NaN, the
+    per-subhalo form gives the batched bytes, and every
+    subhalo, the 4 M-particle host included, meets the tolerant criteria
+    against a digest of the oracle's run of the whole catalogue."""
+    cat = synth.config4()
+    U = gpu(_with(cat.params, precision=abi.HBT_PREC_FAST))
+    got = U.unbind_batch(cat)
+    nconv = check_unbind_invariants(cat, got)
+    free = unused_slots(got[0], len(got[1]))
+    assert free.sum() > 0 and np.all(got[1][free] == -1) and np.all(np.isnan(got[2][free]))
+    sat = np.nonzero(cat.subs["parent"] >= 0)[0]
+    part = cat.subset(sat)
+    rs, ri, re_ = oracle.unbind_batch(part, nthreads=16)
+    stats = assert_tolerant((rs, as_ids(part.cols["id"], ri), re_), (got[0][sat], as_ids(cat.cols["id"], got[1]), got[2]))
+    assert _same_bytes(got, U.unbind_batch(cat, plain=True))
+    print(f"C4 full: {nconv} of {len(cat.subs)} converged, host nbound {got[0]['nbound'][0]}, satellites {stats}")
+    # every subhalo, the host included, against the oracle's run of the whole
+    # catalogue (tests/golden/make_c4_full.py): bound sets equal outside the
+    # band, Nbound within the band count, most-bound IDs equal unless tied
+    z = np.load(os.path.join(GOLDEN, "c4_full_oracle.npz"))
+    assert bytes(z["catalogue_sha1"]) == catalogue_digest(cat), "catalogue generator changed"
+    ids = cat.cols["id"].astype(np.uint64)
+    subs, idx, _ = got
+    for s in range(len(subs)):
+        o, L, nb = int(subs["out_offset"][s]), int(subs["list_len"][s]), int(subs["nbound"][s])
+        band = z["band_ids"][z["band_off"][s]:z["band_off"][s + 1]]
+        b = ids[idx[o:o + nb]]
+        outside = np.sort(b[~np.isin(b, band)])
+        assert hashlib.sha1(outside.tobytes()).digest() == bytes(z["bound_digest"][s]), f"subhalo {s}: bound set"
+        assert abs(nb - int(z["nbound"][s])) <= len(band), f"subhalo {s}: nbound {nb} vs {z['nbound'][s]}"
+        assert ids[idx[o]] == z["mostbound_id"][s] or z["top_tie"][s], f"subhalo {s}: most-bound ID"
+
+
+def test_indexed_batch_c5_tenth_size(gpu):
+    """A tenth of the one-rank share (20,000 subhalos, 6.4 M listed particles
+    in a shuffled 7.1 M-particle snapshot; the full share's catalogue takes
+    minutes to generate and is run by the bench): ID lists resolved on the
+    device and unbound from the snapshot give the direct batch's results,
+    which meet the size-free invariants and, on 500 subhalos drawn at random,
+    the tolerant criteria against the oracle"""
+    snap, ids, cat = synth.config5(seed=1015, nsub=20000)
+    U = gpu(cat.params)
+    h = U.upload_snapshot(snap, ids, cat.subs)
+    idx = U.resolve_ids(h)
+    assert np.array_equal(snap["id"][idx.cpu().numpy()], ids)
+    s1, i1, e1 = U.unbind_indexed(h, idx)
+    got = U.unbind_batch(cat)
+    s1 = s1.cpu().numpy().view(abi.SUB_DTYPE)
+    for f in abi.OUTPUT_FIELDS:
+        assert np.array_equal(s1[f], got[0][f]), f
+    assert np.array_equal(snap["id"][i1.cpu().numpy()[: len(got[1])]], as_ids(cat.cols["id"], got[1]))
+    assert np.array_equal(e1.cpu().numpy()[: len(got[2])].view(np.uint32), got[2].view(np.uint32))
+    nconv = check_unbind_invariants(cat, got)
+    sel = np.sort(np.random.default_rng(15).choice(len(cat.subs), 500, replace=False))
+    part = cat.subset(sel)
+    rs, ri, re_ = oracle.unbind_batch(part, nthreads=16)
+    stats = assert_tolerant((rs, as_ids(part.cols["id"], ri), re_), (got[0][sel], as_ids(cat.cols["id"], got[1]), got[2]))
+    print(f"C5/10: {nconv} of {len(cat.subs)} converged, sample {stats}")

@@ -25,8 +25,6 @@ def test_size_free_invariants_hold_for_reference_outputs(name):
     own outputs (converged = nbound > 1: its records carry no status) and for
     the oracle's"""
     cat, ref = load_unbind(name)
-    if np.any(cat.subs["parent"] >= 0):
-        pytest.skip("nested catalogue")
     assert check_unbind_invariants(cat, ref, conv=ref[0]["nbound"] > 1) >= 1
     assert check_unbind_invariants(cat, oracle.unbind_batch(cat)) >= 1
 
@@ -68,6 +66,20 @@ def test_size_free_invariants_catch_planted_mistakes():
 
     for fn in (swap, foreign, type_count, mass, most_bound, unbound_in_block, too_many):
         planted(fn)
+
+    # nested: a host list missing one particle of a child's unbound tail
+    cat, _ = load_unbind("c4_small")
+    subs, idx, en = oracle.unbind_batch(cat)
+    check_unbind_invariants(cat, (subs, idx, en))
+    k = int(np.nonzero(cat.subs["parent"] >= 0)[0][0])
+    h = int(cat.subs["parent"][k])
+    tail = idx[int(subs["out_offset"][k] + subs["nbound"][k])]
+    o, L = int(subs["out_offset"][h]), int(subs["list_len"][h])
+    j = o + int(np.nonzero(idx[o:o + L] == tail)[0][0])
+    idx2 = idx.copy()
+    idx2[j] = idx[int(subs["out_offset"][k])]  # a bound particle of the child instead
+    with pytest.raises(AssertionError):
+        check_unbind_invariants(cat, (subs, idx2, en))
 
 
 @pytest.mark.parametrize("name", TREE_CASES)
