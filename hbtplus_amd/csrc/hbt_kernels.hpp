@@ -647,6 +647,17 @@ __device__ __forceinline__ int group_field(int v) {
   return __builtin_amdgcn_ds_swizzle(v, AND | (K << 5));
 }
 
+// group_any_wave on a lane mask taken straight from a compare.
+template <int GROUP>
+__device__ __forceinline__ bool group_any_mask(u64 b) {
+#pragma unroll
+  for (int d = 1; d < GROUP; d <<= 1) b |= b >> d;
+  constexpr u64 first = GROUP == 8 ? 0x0101010101010101ull
+                                   : (GROUP == 16 ? 0x0001000100010001ull : 0x0000000100000001ull);
+  constexpr u64 fill = GROUP >= 32 ? 0xffffffffull : ((1ull << (GROUP & 31)) - 1ull);
+  return __builtin_amdgcn_inverse_ballot_w64((b & first) * fill);
+}
+
 // Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN):
 // sep2 = squared separation, opens = the opening test, add = the softened
 // monopole term.  Straddling segments (a periodic subhalo wider than a
@@ -680,6 +691,11 @@ struct FastTerm {
 #pragma clang fp contract(off)
     return lsq > r2 * P.theta2;
   }
+  // the same test as a wave lane mask (every lane active)
+  __device__ __forceinline__ u64 opens_mask(const DevParams &P, float lsq, float r2) const {
+#pragma clang fp contract(off)
+    return __builtin_amdgcn_fcmpf(lsq, r2 * P.theta2, 2 /* ordered > */);
+  }
   __device__ __forceinline__ void add(float m, float r2) { acc.add(kernel_term_f(m, r2, h2, hinv)); }
   __device__ __forceinline__ double result() const { return (double)acc.s; }
 };
@@ -704,6 +720,10 @@ struct ReplayTerm {
   __device__ __forceinline__ bool opens(const DevParams &P, float lsq, double r2) const {
 #pragma clang fp contract(off)
     return (double)lsq > r2 * (double)P.theta2;
+  }
+  __device__ __forceinline__ u64 opens_mask(const DevParams &P, float lsq, double r2) const {
+#pragma clang fp contract(off)
+    return __builtin_amdgcn_fcmp((double)lsq, r2 * (double)P.theta2, 2 /* ordered > */);
   }
   __device__ __forceinline__ void add(float m, double r2) {
 #pragma clang fp contract(off)
@@ -778,19 +798,23 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
                                    __int_as_float(group_field<GROUP, 2>(v)), __int_as_float(group_field<GROUP, 3>(v)));
       const int2 a = make_int2(group_field<GROUP, 4>(v), group_field<GROUP, 5>(v));
       const int j1 = j + 1, js = max(a.y, j1);
-      const bool path = on & (until <= j);
+      // lane masks straight from the compares (every lane is active here):
+      // the group's vote runs on them without a round trip through VGPRs
+      const u64 pmask = __builtin_amdgcn_sicmp(j, end, 40 /* signed < */) &
+                        __builtin_amdgcn_sicmp(until, j, 41 /* signed <= */);
       const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
-      const bool opened = path & t.opens(P, __int_as_float(a.x), r2);
-      const bool accept = path & !opened;
-      const bool go = group_any_wave<GROUP>(opened);  // voted before the branch below
-      if (accept) t.add(q.w, r2);
+      const u64 omask = pmask & t.opens_mask(P, __int_as_float(a.x), r2);
+      const bool accept = __builtin_amdgcn_inverse_ballot_w64(pmask & ~omask);
       if (COUNT) {
         if (on && leader) ns += GROUP;
-        no += opened ? 1u : 0u;
+        no += __builtin_amdgcn_inverse_ballot_w64(omask) ? 1u : 0u;
         ni += accept ? 1u : 0u;
       }
-      until = accept ? a.y : until;
-      j = go ? j1 : js;
+      j = group_any_mask<GROUP>(omask) ? j1 : js;
+      if (accept) {
+        t.add(q.w, r2);
+        until = a.y;
+      }
     }
   }
   return t.result();
