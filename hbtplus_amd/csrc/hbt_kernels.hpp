@@ -637,6 +637,11 @@ __device__ __forceinline__ bool group_any_wave(bool p) {
   return __builtin_amdgcn_inverse_ballot_w64(b);
 }
 
+// Dword `dw` (a byte offset) of item i, at a 32-bit offset from the scalar base.
+__device__ __forceinline__ int item_dword(const char *__restrict__ base, int i, unsigned dw) {
+  return *(const int *)(base + ((unsigned)i * (unsigned)sizeof(Item) + dw));
+}
+
 // Lane K of the caller's aligned group of GROUP (8, 16 or 32) lanes, read by
 // every lane of the group (ds_swizzle, bit mode: source lane = (lane & AND) |
 // K within each half-wave).  Every lane of the wave must be active.
@@ -738,13 +743,14 @@ struct ReplayTerm {
 // group-uniform index compile to scalar loads when GROUP = 64, and both
 // possible successors of item j (j+1 and skip(j)) are fetched while j is
 // evaluated.  Smaller groups read with vector loads at a 32-bit byte offset
-// from the scalar base: every lane loads and tests its group's item before
-// any branch (lanes off their path discard the result), so the item's loads
-// issue together at the top of the step, and the group's vote runs on the
-// scalar unit.  (Prefetching both successors there, or loading through the
-// group's first lane and a crossbar broadcast, measured slower.)  Reads past the end are clamped to the segment's last
-// item, a leaf whose skip is `end`: a finished group keeps reading it and
-// stays finished.  Skips always advance (max with j+1).
+// from the scalar base, one dword of the item per lane, and spread the six
+// fields over the group with crossbar swizzles; every lane of the wave takes
+// part in every step (lanes off their path discard the result), the dwords of
+// both possible successors are fetched as soon as skip(j) is known, and the
+// group's vote runs on the scalar unit over compare masks.  Reads past the end
+// are clamped to the segment's last item, a leaf whose skip is `end`: a
+// finished group keeps reading it and stays finished.  Skips always advance
+// (max with j+1).
 template <int GROUP, bool COUNT, class Term>
 __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, int ib, int end,
                              bool live, float tx, float ty, float tz, float tmass, unsigned &ni,
@@ -790,14 +796,15 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
     // the LDS crossbar: 4 bytes per lane through the vector memory pipe
     const char *base = (const char *)it;
     const unsigned dw = 4u * (threadIdx.x & 7u);
+    int v = item_dword(base, min(j, last), dw);
     while (__any(j < end)) {
       const bool on = j < end;
-      const unsigned off = (unsigned)min(j, last) * (unsigned)sizeof(Item) + dw;
-      const int v = *(const int *)(base + off);
       const float4 q = make_float4(__int_as_float(group_field<GROUP, 0>(v)), __int_as_float(group_field<GROUP, 1>(v)),
                                    __int_as_float(group_field<GROUP, 2>(v)), __int_as_float(group_field<GROUP, 3>(v)));
       const int2 a = make_int2(group_field<GROUP, 4>(v), group_field<GROUP, 5>(v));
       const int j1 = j + 1, js = max(a.y, j1);
+      // both possible successors are fetched now, under the work on item j
+      const int v1 = item_dword(base, min(j1, last), dw), vs = item_dword(base, min(js, last), dw);
       // lane masks straight from the compares (every lane is active here):
       // the group's vote runs on them without a round trip through VGPRs
       const u64 pmask = __builtin_amdgcn_sicmp(j, end, 40 /* signed < */) &
@@ -810,7 +817,9 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
         no += __builtin_amdgcn_inverse_ballot_w64(omask) ? 1u : 0u;
         ni += accept ? 1u : 0u;
       }
-      j = group_any_mask<GROUP>(omask) ? j1 : js;
+      const bool go = group_any_mask<GROUP>(omask);
+      j = go ? j1 : js;
+      v = go ? v1 : vs;
       if (accept) {
         t.add(q.w, r2);
         until = a.y;
