@@ -437,7 +437,13 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
         const int li = (int)W.v0[s];
         W.items[ib + pos].pm = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
         W.items[ib + pos].ax = make_int2(0, pos + 1);
-        if (l + 1 == S.n_src) W.seg[k].n_items = pos + 1;
+        if (l + 1 == S.n_src) {
+          W.seg[k].n_items = pos + 1;
+          // sentinel after the last item: massless, never opened, its skip is
+          // itself, so a group walk that has finished reads it and stays put
+          W.items[ib + pos + 1].pm = make_float4(0.f, 0.f, 0.f, 0.f);
+          W.items[ib + pos + 1].ax = make_int2(0, pos + 1);
+        }
         const int lo = W.nlo[s];
         if (lo >= 0) {
           const int v = W.dl[s], hi = W.nhi[s];
@@ -747,10 +753,10 @@ struct ReplayTerm {
 // fields over the group with crossbar swizzles; every lane of the wave takes
 // part in every step (lanes off their path discard the result), the dwords of
 // both possible successors are fetched as soon as skip(j) is known, and the
-// group's vote runs on the scalar unit over compare masks.  Reads past the end
-// are clamped to the segment's last item, a leaf whose skip is `end`: a
-// finished group keeps reading it and stays finished.  Skips always advance
-// (max with j+1).
+// group's vote runs on the scalar unit over compare masks; a finished group
+// sits on the sentinel item at `end`, whose skip is itself.  The 64-lane form
+// clamps its reads to the segment's last item, a leaf whose skip is `end`, and
+// its skips always advance (max with j+1).
 template <int GROUP, bool COUNT, class Term>
 __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, int ib, int end,
                              bool live, float tx, float ty, float tz, float tmass, unsigned &ni,
@@ -796,15 +802,17 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
     // the LDS crossbar: 4 bytes per lane through the vector memory pipe
     const char *base = (const char *)it;
     const unsigned dw = 4u * (threadIdx.x & 7u);
-    int v = item_dword(base, min(j, last), dw);
+    int v = item_dword(base, j, dw);
     while (__any(j < end)) {
       const bool on = j < end;
       const float4 q = make_float4(__int_as_float(group_field<GROUP, 0>(v)), __int_as_float(group_field<GROUP, 1>(v)),
                                    __int_as_float(group_field<GROUP, 2>(v)), __int_as_float(group_field<GROUP, 3>(v)));
       const int2 a = make_int2(group_field<GROUP, 4>(v), group_field<GROUP, 5>(v));
-      const int j1 = j + 1, js = max(a.y, j1);
+      // j <= end throughout: every item's skip lies in (j, end], the sentinel's is end
+      const int j1 = j + 1, js = a.y;
       // both possible successors are fetched now, under the work on item j
-      const int v1 = item_dword(base, min(j1, last), dw), vs = item_dword(base, min(js, last), dw);
+      // (j1 = end + 1 reads the next region or the buffer's two spare items)
+      const int v1 = item_dword(base, j1, dw), vs = item_dword(base, js, dw);
       // lane masks straight from the compares (every lane is active here):
       // the group's vote runs on them without a round trip through VGPRs
       const u64 pmask = __builtin_amdgcn_sicmp(j, end, 40 /* signed < */) &

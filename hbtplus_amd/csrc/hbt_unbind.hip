@@ -626,7 +626,9 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->nlo.ensure(s * 4));
   RETCHK(c->nhi.ensure(s * 4));
   RETCHK(c->npos.ensure(s * 4));
-  RETCHK(c->items.ensure(s * 2 * sizeof(Item)));
+  // two items per slot (leaves and nodes, plus each segment's sentinel) and
+  // two spare items that the group walk may read past the last segment's end
+  RETCHK(c->items.ensure((s * 2 + 2) * sizeof(Item)));
   RETCHK(c->scan64.ensure(s * 8));
   RETCHK(c->pk.ensure((s + 1) * 4));
   RETCHK(c->gk.ensure((s + 1) * 4));
