@@ -134,7 +134,7 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
-                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf})
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (h_stage) (void)hipHostFree(h_stage);
