@@ -28,7 +28,7 @@ namespace hbt {
 // Rows of Work::ncount, each written by the kernel that decides the count:
 // sources of the tree (pass setup), the partition scan (pass setup), the bound
 // target list (partition), the energy sort (frames).
-enum { NC_SRC = 0, NC_PART = 1, NC_BOUND = 2, NC_SORT = 3, NC_ROWS = 4 };
+enum { NC_SRC = 0, NC_PART = 1, NC_BOUND = 2, NC_SORT = 3, NC_CROSS = 4, NC_ROWS = 5 };
 
 struct __align__(32) Item {
   float4 pm;   // x, y, z, mass (a node: centre of mass and total mass)
@@ -65,6 +65,7 @@ struct Work {
   signed char *dl;
   u32 *bits;
   int *pre, *nlo, *nhi, *npos;
+  int *cross;                // per segment: pairs of the nodes whose leaves span tiles
   Item *items;               // [2*nslots] tree items in pre-order
   u64 *scan64;
   int *pk, *gk;
@@ -277,6 +278,7 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
   SEG_LOOP(k, W) {
 #pragma clang fp contract(off)
     Seg &S = W.seg[k];
+    W.ncount[NC_CROSS * W.nseg + k] = 0;
     if (S.n_src <= 0) {
       if (S.flags & F_ACTIVE) atomicMax(&W.active[2], S.Dfull);
       continue;
@@ -464,23 +466,31 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
 // Node moments: mass and centre of mass accumulated in double over the float
 // values of the children in son order, stored as float
 // (UpdateInternalNodes/ProcessNode/FillNodeCenter, src/gravity_tree.cpp:18-77).
-// Skips are segment-local item positions; ib is the segment's first item.
-__device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
+// pm(i) and skip(i) read item i; g is the node, end its skip.
+template <class Pm, class Skip>
+__device__ __forceinline__ float4 sum_children(Pm pm, Skip skip, int g, int end) {
 #pragma clang fp contract(off)
-  const int end = (int)(ib + W.items[g].ax.y);
   double mass = 0., cx = 0., cy = 0., cz = 0.;
   int c = g + 1;
   while (c < end) {
-    const float4 q = W.items[c].pm;
+    const float4 q = pm(c);
     const double m = q.w;
     mass += m;
     cx += (double)q.x * m;
     cy += (double)q.y * m;
     cz += (double)q.z * m;
-    const int nx = (int)(ib + W.items[c].ax.y);
+    const int nx = skip(c);
     c = nx > c ? nx : end;  // skips always advance
   }
-  W.items[g].pm = make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
+  return make_float4((float)(cx / mass), (float)(cy / mass), (float)(cz / mass), (float)mass);
+}
+
+// The same over the items in device memory.  Skips are segment-local item
+// positions; ib is the segment's first item.
+__device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
+  const Item *it = W.items;
+  W.items[g].pm = sum_children([&](int c) { return it[c].pm; }, [&](int c) { return (int)(ib + it[c].ax.y); },
+                               g, (int)(ib + W.items[g].ax.y));
 }
 
 // Moments of the nodes at depth v (one launch per depth, deepest first: a
@@ -496,6 +506,115 @@ __global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v, int64_t n
     const Seg &S = W.seg[k];
     if (l + 1 >= S.n_src || W.dl[s] != v || W.nlo[s] < 0) continue;
     node_moments(W, 2 * S.ibase, W.npos[s]);
+  }
+}
+
+// Node moments in two launches.  moments_tile_kernel: one workgroup per tile
+// stages the tile's items in LDS (its leaves, the headers of the nodes whose
+// leftmost leaf it holds: fewer than 2 TILE + MAX_DEPTH), computes the nodes
+// whose leaves all lie in the tile deepest first (their children do too),
+// depth by depth with a workgroup barrier between depths, and lists in W.cross
+// the pairs of the nodes whose leaves cross a tile edge (at most one per depth
+// per edge).  moments_cross_kernel: one workgroup per segment computes those,
+// deepest first (their children are either inside a tile, done by the first
+// launch, or listed and deeper).  Each node sums its children in son order
+// (sum_children), so the results are those of the per-depth launches.
+constexpr int TILE_ITEMS = 2 * TILE + MAX_DEPTH + 2;
+
+__global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile *__restrict__ tiles, int ntiles) {
+  __shared__ float4 spm[TILE_ITEMS];
+  __shared__ int sskip[TILE_ITEMS];
+  __shared__ int snode[TILE];            // local nodes (LDS item index), grouped by depth
+  __shared__ int sdepth[MAX_DEPTH + 2];  // nodes per depth, then the next free place
+  __shared__ int sfirst[MAX_DEPTH + 2];  // first place of each depth
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const int n = S.n_src;
+    const int cnt = min(TILE, n - 1 - tl.start);  // pairs l with l + 1 < n
+    if (cnt <= 0) continue;  // uniform across the block
+    const int64_t b = S.ibase, ib = 2 * b;
+    const int t_lo = tl.start, t_hi = tl.start + TILE - 1;
+    const int nl = min(TILE, n - tl.start);
+    const int lastl = tl.start + nl - 1;
+    const int first = tl.start + W.pre[b + tl.start];
+    const int last = lastl + W.pre[b + lastl] + __popc(W.bits[b + lastl]);
+    const int nit = last - first + 1;
+    int *ncross = W.ncount + NC_CROSS * W.nseg + tl.seg;
+    if (nit > TILE_ITEMS) {
+      // not expected (leaves plus the headers of the nodes starting in the
+      // tile); the same depth order over device memory
+      for (int v = min(max(S.D, S.Dfull), MAX_DEPTH); v >= 0; v--) {
+        for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+          const int64_t s = b + tl.start + i;
+          if (W.dl[s] != v || W.nlo[s] < 0) continue;
+          if (W.nlo[s] >= t_lo && W.nhi[s] <= t_hi) node_moments(W, ib, W.npos[s]);
+          else W.cross[b + atomicAdd(ncross, 1)] = tl.start + i;
+        }
+        __syncthreads();
+      }
+      continue;  // uniform across the block
+    }
+    for (int i = threadIdx.x; i < nit; i += BLOCK) {
+      const Item q = W.items[ib + first + i];
+      spm[i] = q.pm;
+      sskip[i] = q.ax.y - first;
+    }
+    for (int v = threadIdx.x; v <= MAX_DEPTH + 1; v += BLOCK) sdepth[v] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int64_t s = b + tl.start + i;
+      if (W.nlo[s] < 0) continue;
+      if (W.nlo[s] >= t_lo && W.nhi[s] <= t_hi) atomicAdd(&sdepth[W.dl[s]], 1);
+      else W.cross[b + atomicAdd(ncross, 1)] = tl.start + i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int o = 0;
+      for (int v = MAX_DEPTH; v >= 0; v--) { sfirst[v] = o; o += sdepth[v]; sdepth[v] = sfirst[v]; }
+      sfirst[MAX_DEPTH + 1] = o;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int64_t s = b + tl.start + i;
+      if (W.nlo[s] < 0 || W.nlo[s] < t_lo || W.nhi[s] > t_hi) continue;
+      snode[atomicAdd(&sdepth[W.dl[s]], 1)] = (int)(W.npos[s] - ib) - first;
+    }
+    __syncthreads();
+    // depths deepest first: places sfirst[v] .. sfirst[v] + count, in order
+    for (int v = MAX_DEPTH; v >= 0; v--) {
+      const int p0 = sfirst[v], p1 = (v > 0 ? sfirst[v - 1] : sfirst[MAX_DEPTH + 1]);
+      if (p0 == p1) continue;  // uniform
+      for (int p = p0 + (int)threadIdx.x; p < p1; p += BLOCK) {
+        const int g = snode[p];
+        spm[g] = sum_children([&](int c) { return spm[c]; }, [&](int c) { return sskip[c]; }, g, sskip[g]);
+      }
+      __syncthreads();
+    }
+    for (int p = threadIdx.x; p < sfirst[MAX_DEPTH + 1]; p += BLOCK) {
+      const int g = snode[p];
+      W.items[ib + first + g].pm = spm[g];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) moments_cross_kernel(Work W, const Tile *__restrict__ tiles, int ntiles) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    if (tl.start != 0) continue;  // the segment's first tile stands for it
+    const int k = tl.seg;
+    const int ng = W.ncount[NC_CROSS * W.nseg + k];
+    if (ng == 0) continue;
+    const Seg &S = W.seg[k];
+    const int64_t b = S.ibase, ib = 2 * b;
+    for (int v = min(max(S.D, S.Dfull), MAX_DEPTH); v >= 0; v--) {
+      for (int i = threadIdx.x; i < ng; i += BLOCK) {
+        const int64_t s = b + W.cross[b + i];
+        if (W.dl[s] == v) node_moments(W, ib, W.npos[s]);
+      }
+      __syncthreads();
+    }
   }
 }
 

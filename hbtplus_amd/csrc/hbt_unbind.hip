@@ -104,7 +104,7 @@ struct hbt_ctx {
   int cus = 256;
   // per-slot work arrays
   DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
-      npos, items, scan64, pk, gk, isb, seg_of_slot;
+      npos, items, scan64, pk, gk, isb, seg_of_slot, cross;
   // per-level tables: staged through one pinned host buffer and one copy
   DBuf lvlbuf;
   void *h_stage = nullptr;
@@ -133,7 +133,7 @@ struct hbt_ctx {
   ~hbt_ctx() {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
-                    &isb, &seg_of_slot, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
+                    &isb, &seg_of_slot, &cross, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
                     &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
@@ -390,6 +390,13 @@ __global__ void __launch_bounds__(BLOCK) eval_sum_kernel(const hbt_subhalo *subs
   if (threadIdx.x == 0) *out = tot;
 }
 
+// HBT_MOMENTS_PER_DEPTH=1: node moments with one launch per tree depth (the
+// former schedule, kept for comparison); default: two launches per pass
+static bool moments_per_depth() {
+  static const bool v = getenv("HBT_MOMENTS_PER_DEPTH") && atoi(getenv("HBT_MOMENTS_PER_DEPTH"));
+  return v;
+}
+
 struct Runner {
   hbt_ctx *c;
   Work W;
@@ -521,12 +528,17 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    const int nsmall = (int)L->seg.size() - L->k_small;
-    if (nsmall > 0)
-      moments_small_kernel<<<std::min(nsmall, c->cus * 8), BLOCK, 0, c->stream>>>(W, L->k_small);
-    if (n_big > 0) {
-      const int gb = grid_for(n_big, c->cus);
-      for (int v = max_depth; v >= 0; v--) moments_kernel<<<gb, BLOCK, 0, c->stream>>>(W, v, n_big);
+    if (moments_per_depth()) {
+      const int nsmall = (int)L->seg.size() - L->k_small;
+      if (nsmall > 0)
+        moments_small_kernel<<<std::min(nsmall, c->cus * 8), BLOCK, 0, c->stream>>>(W, L->k_small);
+      if (n_big > 0) {
+        const int gb = grid_for(n_big, c->cus);
+        for (int v = max_depth; v >= 0; v--) moments_kernel<<<gb, BLOCK, 0, c->stream>>>(W, v, n_big);
+      }
+    } else {
+      moments_tile_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+      moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
     }
     return HBT_OK;
   }
@@ -630,6 +642,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   // two spare items that the group walk may read past the last segment's end
   RETCHK(c->items.ensure((s * 2 + 2) * sizeof(Item)));
   RETCHK(c->scan64.ensure(s * 8));
+  RETCHK(c->cross.ensure(s * 4));
   RETCHK(c->pk.ensure((s + 1) * 4));
   RETCHK(c->gk.ensure((s + 1) * 4));
   RETCHK(c->isb.ensure(s));
@@ -675,6 +688,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.pre = c->pre.as<int>(); W.nlo = c->nlo.as<int>(); W.nhi = c->nhi.as<int>();
   W.npos = c->npos.as<int>();
   W.items = c->items.as<Item>();
+  W.cross = c->cross.as<int>();
   W.scan64 = c->scan64.as<u64>();
   W.pk = c->pk.as<int>(); W.gk = c->gk.as<int>();
   W.isb = c->isb.as<unsigned char>();
