@@ -70,8 +70,7 @@ struct Work {
   u64 *scan64;
   int *pk, *gk;
   unsigned char *isb;
-  int *active;               // [4]: segments active after the pass, the largest Nlast among
-                             // them, a bound on the tree depth D of the next pass
+  int *active;               // [4]: segments active after the pass, the largest Nlast among them
   unsigned long long *counters;  // [4]: interactions, opened nodes, targets (instrumented passes)
   // caller outputs
   hbt_subhalo *subs;
@@ -279,10 +278,7 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
 #pragma clang fp contract(off)
     Seg &S = W.seg[k];
     W.ncount[NC_CROSS * W.nseg + k] = 0;
-    if (S.n_src <= 0) {
-      if (S.flags & F_ACTIVE) atomicMax(&W.active[2], S.Dfull);
-      continue;
-    }
+    if (S.n_src <= 0) continue;
     double xmin[3], xmax[3];
     for (int j = 0; j < 3; j++) { xmin[j] = ord2f(S.bmin[j]); xmax[j] = ord2f(S.bmax[j]); }
     double Len = xmax[0] - xmin[0];
@@ -300,7 +296,6 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
     // lies inside the latest full pass's sources but not inside the previous
     // correction pass's: the latest full pass bounds the depth of later passes
     if (!(S.flags & F_CORR)) S.Dfull = D;
-    atomicMax(&W.active[2], S.Dfull > D ? S.Dfull : D);
     // the FP32 walk drops the periodic wrap unless some target-source pair can
     // be more than half a box apart: a correction pass's sources (the block
     // just removed) may lie on one side of the box edge while its targets lie
@@ -493,22 +488,6 @@ __device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
                                g, (int)(ib + W.items[g].ax.y));
 }
 
-// Moments of the nodes at depth v (one launch per depth, deepest first: a
-// kernel boundary orders the children's moments before their parents read
-// them), over the first nslots visited slots (those of the large segments).
-// Each node is handled by the slot of its leftmost split pair.
-__global__ void __launch_bounds__(BLOCK) moments_kernel(Work W, int v, int64_t nslots) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nslots; i += (int64_t)gridDim.x * BLOCK) {
-    const int64_t s = slot_at(W, i);
-    int k;
-    int l = seg_lane_local(W, s, k);
-    if (l < 0) continue;
-    const Seg &S = W.seg[k];
-    if (l + 1 >= S.n_src || W.dl[s] != v || W.nlo[s] < 0) continue;
-    node_moments(W, 2 * S.ibase, W.npos[s]);
-  }
-}
-
 // Node moments in two launches.  moments_tile_kernel: one workgroup per tile
 // stages the tile's items in LDS (its leaves, the headers of the nodes whose
 // leftmost leaf it holds: fewer than 2 TILE + MAX_DEPTH), computes the nodes
@@ -612,25 +591,6 @@ __global__ void __launch_bounds__(BLOCK) moments_cross_kernel(Work W, const Tile
       for (int i = threadIdx.x; i < ng; i += BLOCK) {
         const int64_t s = b + W.cross[b + i];
         if (W.dl[s] == v) node_moments(W, ib, W.npos[s]);
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Moments of the small segments (from k_small on): one workgroup per
-// segment walks its depths deepest first, a workgroup barrier ordering each
-// depth's moments before the next depth reads them.
-__global__ void __launch_bounds__(BLOCK) moments_small_kernel(Work W, int k_small) {
-  for (int k = k_small + (int)blockIdx.x; k < W.nseg; k += (int)gridDim.x) {
-    const Seg &S = W.seg[k];
-    const int n = S.n_src;
-    if (n < 2) continue;  // uniform across the block
-    const int64_t b = S.ibase, ib = 2 * b;
-    for (int v = min(max(S.D, S.Dfull), MAX_DEPTH); v >= 0; v--) {
-      for (int l = threadIdx.x; l + 1 < n; l += BLOCK) {
-        const int64_t s = b + l;
-        if (W.dl[s] == v && W.nlo[s] >= 0) node_moments(W, ib, W.npos[s]);
       }
       __syncthreads();
     }

@@ -155,13 +155,6 @@ bool compact_passes() {
   return v;
 }
 
-// Capacity up to which a segment's node moments are built by one workgroup
-// (HBT_SMALL_TREE overrides it; 0 = every segment by per-depth launches).
-int small_tree() {
-  static const int v = getenv("HBT_SMALL_TREE") ? atoi(getenv("HBT_SMALL_TREE")) : 1024;
-  return v;
-}
-
 // Level description built on the host.
 struct Level {
   std::vector<int> subs;        // subhalo indices
@@ -171,11 +164,6 @@ struct Level {
   std::vector<Tile> tiles;
   int64_t nslots = 0;
   int maxcap = 0;
-  // segments are ordered by capacity, largest first; those from k_small on
-  // (capacity <= small_tree()) build their node moments one workgroup each,
-  // the others (slots [0, big_slots)) one launch per depth
-  int k_small = 0;
-  int64_t big_slots = 0;
 };
 
 int walk_group_for(const hbt_tuning &t, int64_t len) {
@@ -186,7 +174,7 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
                  const std::vector<std::vector<int>> &children, const std::vector<int> &members,
                  const hbt_tuning &tune, Level &L) {
   const int n = (int)members.size();
-  // largest first: merge levels and per-depth launches then cover a prefix
+  // largest first: the merge levels of a sort then cover a prefix
   std::vector<std::pair<int64_t, int>> key(members.size());
   for (size_t i = 0; i < members.size(); i++) key[i] = {-cap[members[i]], (int)i};
   std::sort(key.begin(), key.end());  // by capacity descending, then input order
@@ -203,15 +191,10 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
   L.tiles.clear();
   int64_t off = 0;
   L.maxcap = 0;
-  L.k_small = n;
   for (int k = 0; k < n; k++) {
     const int s = order[k];
     const int c = (int)cap[s];
     const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
-    if (c <= small_tree() && L.k_small == n) {
-      L.k_small = k;
-      L.big_slots = off;
-    }
     Seg &S = L.seg[k];
     S.ibase = off;
     S.out_off = subs[s].out_offset;
@@ -231,7 +214,6 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
     L.child_ptr[k + 1] = (int)L.child_list.size();
   }
   L.nslots = off;
-  if (L.k_small == n) L.big_slots = off;
 }
 
 // Copies host tables to the device: packed (256-byte aligned) into the pinned
@@ -390,13 +372,6 @@ __global__ void __launch_bounds__(BLOCK) eval_sum_kernel(const hbt_subhalo *subs
   if (threadIdx.x == 0) *out = tot;
 }
 
-// HBT_MOMENTS_PER_DEPTH=1: node moments with one launch per tree depth (the
-// former schedule, kept for comparison); default: two launches per pass
-static bool moments_per_depth() {
-  static const bool v = getenv("HBT_MOMENTS_PER_DEPTH") && atoi(getenv("HBT_MOMENTS_PER_DEPTH"));
-  return v;
-}
-
 struct Runner {
   hbt_ctx *c;
   Work W;
@@ -411,7 +386,6 @@ struct Runner {
   const int *ft_cur = nullptr;
   int nt_cur = 0;
   std::vector<int> cur_cap, cur_first;
-  int64_t n_big = 0;           // visited slots that belong to large segments (per-depth moments)
   int nlisted = 0;             // segments in the current lists
 
   void list_all() {
@@ -421,7 +395,6 @@ struct Runner {
     cur_cap.resize(L->seg.size());
     for (size_t k = 0; k < L->seg.size(); k++) cur_cap[k] = L->seg[k].cap;
     cur_first = L->first_tile;
-    n_big = L->big_slots;
     nlisted = (int)L->seg.size();
     W.act_chunk = nullptr;
     W.nact = L->nslots;
@@ -446,7 +419,6 @@ struct Runner {
     std::vector<Tile> tl;
     cur_cap.clear();
     cur_first.clear();
-    n_big = 0;
     nlisted = 0;
     for (int k = 0; k < nseg; k++) {
       if (!act[k]) continue;
@@ -457,7 +429,6 @@ struct Runner {
       for (int t = 0; t < L->ntile[k]; t++) tl.push_back(L->tiles[L->first_tile[k] + t]);
       const int64_t c0 = L->ibase[k] / WAVE, c1 = (L->ibase[k] + L->capal[k]) / WAVE;
       for (int64_t q = c0; q < c1; q++) chunks.push_back((int)q);
-      if (k < L->k_small) n_big += (c1 - c0) * WAVE;
     }
     const size_t b_ch = (chunks.size() * 4 + 255) & ~(size_t)255, b_tl = (tl.size() * sizeof(Tile) + 255) & ~(size_t)255;
     RETCHK(c->actbuf.ensure(b_ch + b_tl + nseg * 4 + 256));
@@ -474,12 +445,10 @@ struct Runner {
     return HBT_OK;
   }
 
-  // Bounds for the pass from the previous pass of the level: merge levels for
+  // Bound for the pass from the previous pass of the level: merge levels for
   // segments of up to max_n elements (no count of a pass exceeds the Nlast of
-  // the pass before), node depths up to max_depth (the tree depth of a pass is
-  // at most that of the latest full pass of its segment: every later source
-  // set lies inside that pass's sources).
-  int max_n = 0, max_depth = MAX_DEPTH;
+  // the pass before).
+  int max_n = 0;
 
   // tiles of the segments with capacity above w (a prefix of the tile table)
   int tiles_above(int w) const {
@@ -528,18 +497,8 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    if (moments_per_depth()) {
-      const int nsmall = (int)L->seg.size() - L->k_small;
-      if (nsmall > 0)
-        moments_small_kernel<<<std::min(nsmall, c->cus * 8), BLOCK, 0, c->stream>>>(W, L->k_small);
-      if (n_big > 0) {
-        const int gb = grid_for(n_big, c->cus);
-        for (int v = max_depth; v >= 0; v--) moments_kernel<<<gb, BLOCK, 0, c->stream>>>(W, v, n_big);
-      }
-    } else {
-      moments_tile_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
-      moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
-    }
+    moments_tile_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
     return HBT_OK;
   }
 
@@ -826,7 +785,6 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
     }
     // the next pass is bounded by this one
     R.max_n = c->h_active[1];
-    R.max_depth = std::min(MAX_DEPTH, std::max(0, c->h_active[2]));
     // once a quarter of the listed segments have finished, the next passes
     // visit only the slots and tiles of the active ones
     if (c->h_active[0] > 0 && 4 * c->h_active[0] <= 3 * R.nlisted && compact_passes()) RETCHK(R.list_active());
