@@ -104,6 +104,22 @@ __device__ __forceinline__ int64_t slot_at(const Work &W, int64_t i) {
 #define SEG_LOOP(k, W) \
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < (W).nseg; k += gridDim.x * blockDim.x)
 
+// One wave per segment (k is wave-uniform).
+#define WAVE_SEG_LOOP(k, W)                                                     \
+  for (int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE); k < (W).nseg; \
+       k += (int)(gridDim.x * blockDim.x / WAVE))
+
+// Sums of C values over the tiles t0 .. t0+nt-1 of part: the wave's lanes take
+// strided tiles, then a fixed shuffle tree; the sums are valid in lane 0.
+template <int C>
+__device__ __forceinline__ void wave_sum_tiles(const double *__restrict__ part, int t0, int nt, double v[C]) {
+#pragma clang fp contract(off)
+  for (int c = 0; c < C; c++) v[c] = 0;
+  for (int t = t0 + (int)(threadIdx.x & (WAVE - 1)); t < t0 + nt; t += WAVE)
+    for (int c = 0; c < C; c++) v[c] += part[(int64_t)t * C + c];
+  for (int c = 0; c < C; c++) v[c] = wave_reduce_sum(v[c]);
+}
+
 // ---------------------------------------------------------------------------
 // Level initialisation
 
@@ -117,9 +133,9 @@ __global__ void fill_seg_of_slot_kernel(int nseg, const int64_t *ibase, const in
   }
 }
 
-// Builds the working list of every segment: own particles in source order,
-// then the unbound tails of nested subhalos (read from their output lists,
-// src/subhalo_unbind.cpp:437-443), and gathers the local particle columns.
+// Builds the working list of every segment: own particles in source order
+// (their positions are written by gather_kernel), then the unbound tails of
+// nested subhalos (read from their output lists, src/subhalo_unbind.cpp:437-443).
 // child_ptr/child_list: CSR list of the subhalos nested in each segment.
 __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *child_list) {
   for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
@@ -127,8 +143,6 @@ __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *chil
     const hbt_subhalo &r = W.subs[S.sub];
     const int64_t b = S.ibase;
     int pos = (int)r.count;
-    for (int i = threadIdx.x; i < (int)r.count; i += blockDim.x)
-      W.gidx[b + i] = W.g_index ? (int)W.g_index[r.offset + i] : (int)(r.offset + i);
     for (int c = child_ptr[k]; c < child_ptr[k + 1]; c++) {
       const hbt_subhalo &q = W.subs[child_list[c]];
       const int tail = (int)(q.list_len - q.nbound);
@@ -141,13 +155,21 @@ __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *chil
   }
 }
 
-// Local copies of the listed particles' columns (slot-parallel).
+// Column positions of the segments' own particles, and local copies of the
+// listed particles' columns (slot-parallel).
 __global__ void __launch_bounds__(BLOCK) gather_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0 || l >= W.seg[k].len) continue;
-    const int g = W.gidx[s];
+    int g;
+    if (l < W.seg[k].count) {
+      const int64_t o = W.subs[W.seg[k].sub].offset + l;
+      g = W.g_index ? (int)W.g_index[o] : (int)o;
+      W.gidx[s] = g;
+    } else {
+      g = W.gidx[s];
+    }
     W.px[s] = W.g_x[g]; W.py[s] = W.g_y[g]; W.pz[s] = W.g_z[g];
     W.vx[s] = W.g_vx[g]; W.vy[s] = W.g_vy[g]; W.vz[s] = W.g_vz[g];
     W.pm[s] = W.g_m[g];
@@ -1327,6 +1349,9 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   W.ncount[NC_SORT * W.nseg + (int)(&S - W.seg)] = S.nsort;
 }
 
+// One thread per segment: the wave's updates of the active count and of the
+// largest Nlast (frame_update) then combine into one atomic each.  (One wave
+// per segment, as output_combine_kernel does, measured slower here.)
 __global__ void frames_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part) {
 #pragma clang fp contract(off)
   SEG_LOOP(k, W) {
@@ -1551,16 +1576,15 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
   }
 }
 
-__global__ void output_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part) {
+__global__ void __launch_bounds__(BLOCK) output_combine_kernel(Work W, DevParams P, const int *first_tile,
+                                                               const double *part) {
 #pragma clang fp contract(off)
-  SEG_LOOP(k, W) {
+  WAVE_SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
-    if (!(S.flags & F_FINAL)) continue;
+    if (!(S.flags & F_FINAL)) continue;  // uniform across the wave
     double v[REC_SUMS];
-    for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
-    const int nt_ = (S.nbound + TILE - 1) / TILE;
-    for (int t = first_tile[k]; t < first_tile[k] + nt_; t++)
-      for (int c = 0; c < REC_SUMS; c++) v[c] += part[(int64_t)t * REC_SUMS + c];
+    wave_sum_tiles<REC_SUMS>(part, first_tile[k], (S.nbound + TILE - 1) / TILE, v);
+    if ((threadIdx.x & (WAVE - 1)) != 0) continue;
     int nt[HBT_TYPE_MAX];
     float mts[HBT_TYPE_MAX];
     for (int q = 0; q < HBT_TYPE_MAX; q++) { nt[q] = (int)v[6 + q]; mts[q] = (float)v[12 + q]; }

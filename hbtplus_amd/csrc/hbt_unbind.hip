@@ -378,6 +378,7 @@ struct Runner {
   DevParams P;
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
+  int gwseg = 1;               // segment-wave grid
   int gw;                      // walk grid: every slot block once
   // the tile table a pass uses: the level's, or that of its active segments
   // (then W.act_chunk lists their slots); caps and first tiles of the listed
@@ -569,7 +570,7 @@ struct Runner {
       output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
     } else {
       output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
-      output_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
+      output_combine_kernel<<<gwseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
     }
     HIPCHK(hipGetLastError());
     return HBT_OK;
@@ -676,6 +677,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   R.max_n = L.maxcap;
   R.list_all();
   R.gseg = grid_for(nseg, c->cus);
+  R.gwseg = (int)std::min<int64_t>(((int64_t)nseg * WAVE + BLOCK - 1) / BLOCK, (int64_t)c->cus * 8);
   const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
   R.gblk = gsb;
   fill_seg_of_slot_kernel<<<gsb, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, c->capal.as<int>(),
