@@ -30,6 +30,13 @@ struct DevParams {
   int periodic, min_part, snap, replay, kahan;
 };
 
+// What the host fixes per segment of a level (the rest of Seg starts at zero
+// and is set on the device).
+struct SegIn {
+  int64_t ibase, out_off;
+  int sub, cap, count, wgroup;
+};
+
 // Per-subhalo state for one level of the batch.
 struct Seg {
   int64_t ibase;       // first working slot (aligned to WAVE)

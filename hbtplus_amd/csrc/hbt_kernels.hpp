@@ -133,6 +133,21 @@ __global__ void fill_seg_of_slot_kernel(int nseg, const int64_t *ibase, const in
   }
 }
 
+// The level's segment records: zero, then the host's fixed fields.
+__global__ void __launch_bounds__(BLOCK) seg_expand_kernel(Seg *seg, const SegIn *in, int n) {
+  for (int k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += gridDim.x * BLOCK) {
+    const SegIn a = in[k];
+    Seg S{};
+    S.ibase = a.ibase;
+    S.out_off = a.out_off;
+    S.sub = a.sub;
+    S.cap = a.cap;
+    S.count = a.count;
+    S.wgroup = a.wgroup;
+    seg[k] = S;
+  }
+}
+
 // Builds the working list of every segment: own particles in source order
 // (their positions are written by gather_kernel), then the unbound tails of
 // nested subhalos (read from their output lists, src/subhalo_unbind.cpp:437-443).

@@ -109,7 +109,8 @@ struct hbt_ctx {
   DBuf lvlbuf;
   void *h_stage = nullptr;
   size_t h_stage_bytes = 0;
-  DView seg, ibase, tiles, first_tile, ntile, child_ptr, child_list, capal;
+  DView segin, ibase, tiles, first_tile, ntile, child_ptr, child_list, capal;
+  DBuf seg;             // per-segment state, expanded on the device from segin
   DBuf ncount, tsum, tpart;
   DBuf segact, actbuf;  // segment activity after a pass; lists of the active segments
   DBuf gaps;            // (subhalo, capacity) of the hosts whose output regions have unused slots
@@ -133,7 +134,7 @@ struct hbt_ctx {
   ~hbt_ctx() {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
-                    &isb, &seg_of_slot, &cross, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
+                    &isb, &seg_of_slot, &cross, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
                     &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
@@ -158,7 +159,7 @@ bool compact_passes() {
 // Level description built on the host.
 struct Level {
   std::vector<int> subs;        // subhalo indices
-  std::vector<Seg> seg;
+  std::vector<SegIn> seg;      // what the host fixes per segment; the rest starts at zero
   std::vector<int64_t> ibase;
   std::vector<int> capal, first_tile, ntile, child_ptr, child_list;
   std::vector<Tile> tiles;
@@ -181,7 +182,7 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
   std::vector<int> order(members.size());
   for (size_t i = 0; i < members.size(); i++) order[i] = members[key[i].second];
   L.subs = order;
-  L.seg.assign(n, Seg{});
+  L.seg.resize(n);
   L.ibase.resize(n);
   L.capal.resize(n);
   L.first_tile.resize(n);
@@ -195,13 +196,7 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
     const int s = order[k];
     const int c = (int)cap[s];
     const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
-    Seg &S = L.seg[k];
-    S.ibase = off;
-    S.out_off = subs[s].out_offset;
-    S.sub = s;
-    S.cap = c;
-    S.count = (int)subs[s].count;
-    S.wgroup = walk_group_for(tune, c);
+    L.seg[k] = SegIn{off, subs[s].out_offset, s, c, (int)subs[s].count, walk_group_for(tune, c)};
     L.ibase[k] = off;
     L.capal[k] = ca;
     L.first_tile[k] = (int)L.tiles.size();
@@ -258,7 +253,8 @@ int stage_blobs(hbt_ctx *c, std::initializer_list<Blob> blobs, void **dev) {
 int stage_level(hbt_ctx *c, const Level &L) {
   const size_t n = L.seg.size();
   void *d[8];
-  RETCHK(stage_blobs(c, {{L.seg.data(), n * sizeof(Seg)},
+  RETCHK(c->seg.ensure(std::max<size_t>(n, 1) * sizeof(Seg)));
+  RETCHK(stage_blobs(c, {{L.seg.data(), n * sizeof(SegIn)},
                          {L.ibase.data(), n * 8},
                          {L.capal.data(), n * 4},
                          {L.tiles.data(), L.tiles.size() * sizeof(Tile)},
@@ -267,7 +263,7 @@ int stage_level(hbt_ctx *c, const Level &L) {
                          {L.child_ptr.data(), L.child_ptr.size() * 4},
                          {L.child_list.data(), L.child_list.size() * 4}},
                      d));
-  c->seg.p = d[0]; c->ibase.p = d[1]; c->capal.p = d[2]; c->tiles.p = d[3];
+  c->segin.p = d[0]; c->ibase.p = d[1]; c->capal.p = d[2]; c->tiles.p = d[3];
   c->first_tile.p = d[4]; c->ntile.p = d[5]; c->child_ptr.p = d[6]; c->child_list.p = d[7];
   return HBT_OK;
 }
@@ -680,6 +676,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   R.gwseg = (int)std::min<int64_t>(((int64_t)nseg * WAVE + BLOCK - 1) / BLOCK, (int64_t)c->cus * 8);
   const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
   R.gblk = gsb;
+  seg_expand_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W.seg, c->segin.as<SegIn>(), (int)nseg);
   fill_seg_of_slot_kernel<<<gsb, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, c->capal.as<int>(),
                                                         c->seg_of_slot.as<int>());
   build_lists_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, c->child_ptr.as<int>(), c->child_list.as<int>());
