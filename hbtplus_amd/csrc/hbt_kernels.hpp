@@ -110,14 +110,14 @@ __device__ __forceinline__ int64_t slot_at(const Work &W, int64_t i) {
        k += (int)(gridDim.x * blockDim.x / WAVE))
 
 // Sums of C values over the tiles t0 .. t0+nt-1 of part: the wave's lanes take
-// strided tiles, then a fixed shuffle tree; the sums are valid in lane 0.
+// strided tiles, then wave_total; the sums are valid in every lane.
 template <int C>
 __device__ __forceinline__ void wave_sum_tiles(const double *__restrict__ part, int t0, int nt, double v[C]) {
 #pragma clang fp contract(off)
   for (int c = 0; c < C; c++) v[c] = 0;
   for (int t = t0 + (int)(threadIdx.x & (WAVE - 1)); t < t0 + nt; t += WAVE)
     for (int c = 0; c < C; c++) v[c] += part[(int64_t)t * C + c];
-  for (int c = 0; c < C; c++) v[c] = wave_reduce_sum(v[c]);
+  for (int c = 0; c < C; c++) v[c] = wave_total(v[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1307,7 +1307,7 @@ frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
       }
     }
     for (int c = 0; c < FRAME_SUMS; c++) {
-      const double r = wave_reduce_sum(v[c]);
+      const double r = wave_total(v[c]);
       if ((threadIdx.x & (WAVE - 1)) == 0) sh[c][threadIdx.x / WAVE] = r;
     }
     __syncthreads();
@@ -1578,7 +1578,7 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
         if (q == ty) { v[6 + q] += 1.0; v[12 + q] += W.pm[b + li]; }
     }
     for (int c = 0; c < REC_SUMS; c++) {
-      const double r = wave_reduce_sum(v[c]);
+      const double r = wave_total(v[c]);
       if ((threadIdx.x & (WAVE - 1)) == 0) sh[c][threadIdx.x / WAVE] = r;
     }
     __syncthreads();

@@ -49,6 +49,32 @@ __device__ __forceinline__ T wave_inclusive_scan(T x) {
   return x;
 }
 
+// A double moved across lanes by a DPP control (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_double(double x, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                          __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+
+// Sum of x over the wave (every lane active), returned in every lane: DPP
+// butterflies within each row of 16 lanes (quad permutes, half-row and row
+// mirrors), then the four row sums read from lanes 0, 16, 32 and 48.  The
+// order of the additions is fixed, so the result is deterministic.
+__device__ __forceinline__ double wave_total(double x) {
+#pragma clang fp contract(off)
+  x += dpp_double<0xb1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_double<0x4e>(x);   // quad_perm [2,3,0,1]
+  x += dpp_double<0x141>(x);  // row_half_mirror
+  x += dpp_double<0x140>(x);  // row_mirror
+  return (readlane_double(x, 0) + readlane_double(x, 16)) + (readlane_double(x, 32) + readlane_double(x, 48));
+}
+
 template <class T>
 __device__ __forceinline__ T wave_reduce_sum(T x) {
 #pragma unroll
