@@ -65,7 +65,9 @@ struct Work {
   signed char *dl;
   u32 *bits;
   int *pre, *nlo, *nhi, *npos;
-  int *cross;                // per segment: pairs of the nodes whose leaves span tiles
+  int *cross;                // per super-tile: pairs of the nodes whose leaves span tiles in it
+  int *cross2;               // per segment: pairs of the nodes whose leaves span super-tiles
+  int *tcount;               // per tile of the pass's table: list length of the super-tile it starts
   Item *items;               // [2*nslots] tree items in pre-order
   u64 *scan64;
   int *pk, *gk;
@@ -130,6 +132,22 @@ __global__ void fill_seg_of_slot_kernel(int nseg, const int64_t *ibase, const in
     const int64_t b = ibase[k];
     const int c = cap_aligned[k];
     for (int i = threadIdx.x; i < c; i += blockDim.x) seg_of_slot[b + i] = k;
+  }
+}
+
+// The same, slot-parallel, for a level whose segments tile the slots in order
+// (ibase ascending, each segment's slots followed by the next one's): a slot's
+// segment is the last one starting at or before it.  A giant segment is then
+// not left to one workgroup.
+__global__ void __launch_bounds__(BLOCK) seg_of_slot_kernel(int nseg, const int64_t *__restrict__ ibase,
+                                                            int64_t nslots, int *seg_of_slot) {
+  for (int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * BLOCK) {
+    int lo = 0, hi = nseg - 1;  // ibase[lo] <= s
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ibase[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    seg_of_slot[s] = lo;
   }
 }
 
@@ -277,6 +295,7 @@ __global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W, const Tile *tiles, 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     Seg &S = W.seg[tl.seg];
+    if (threadIdx.x == 0) W.tcount[t] = 0;  // node lists of the pass's moments
     const int cnt = min(TILE, S.n_src - tl.start);
     if (cnt <= 0) continue;  // uniform
     const int64_t b = S.ibase;
@@ -525,17 +544,35 @@ __device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
                                g, (int)(ib + W.items[g].ax.y));
 }
 
-// Node moments in two launches.  moments_tile_kernel: one workgroup per tile
-// stages the tile's items in LDS (its leaves, the headers of the nodes whose
-// leftmost leaf it holds: fewer than 2 TILE + MAX_DEPTH), computes the nodes
-// whose leaves all lie in the tile deepest first (their children do too),
-// depth by depth with a workgroup barrier between depths, and lists in W.cross
-// the pairs of the nodes whose leaves cross a tile edge (at most one per depth
-// per edge).  moments_cross_kernel: one workgroup per segment computes those,
-// deepest first (their children are either inside a tile, done by the first
-// launch, or listed and deeper).  Each node sums its children in son order
-// (sum_children), so the results are those of the per-depth launches.
+// Node moments in two or three launches.  moments_tile_kernel: one workgroup
+// per tile stages the tile's items in LDS (its leaves, the headers of the
+// nodes whose leftmost leaf it holds: fewer than 2 TILE + MAX_DEPTH), computes
+// the nodes whose leaves all lie in the tile deepest first (their children do
+// too), depth by depth with a workgroup barrier between depths, and lists the
+// pairs of the nodes whose leaves cross a tile edge (at most one per depth per
+// edge): in W.cross, by super-tile (SUPER_TILES tiles), when the node lies in
+// one, else in W.cross2 by segment.  moments_super_kernel: one workgroup per
+// super-tile computes its listed nodes, deepest first; moments_cross_kernel
+// (only for segments of more than one super-tile): one workgroup per segment
+// computes the rest, deepest first.  A node's children are inside a tile (done
+// by the first launch) or listed at the same or a lower level and deeper.
+// Each node sums its children in son order (sum_children), so the results are
+// those of the per-depth launches.
 constexpr int TILE_ITEMS = 2 * TILE + MAX_DEPTH + 2;
+constexpr int SUPER_TILES = 64;
+constexpr int SUPER = SUPER_TILES * TILE;
+
+// Lists the pair l (in tile t) of a node with leaves lo..hi that crosses a tile edge.
+__device__ __forceinline__ void list_crossing(const Work &W, int t, const Tile &tl, int64_t b, int l, int lo,
+                                              int hi, int *ncross) {
+  if (lo / SUPER == hi / SUPER) {
+    const int st = tl.start / SUPER * SUPER;  // first leaf of the super-tile
+    const int ts = t - (tl.start - st) / TILE;  // its first tile in the table
+    W.cross[b + st + atomicAdd(&W.tcount[ts], 1)] = l;
+  } else {
+    W.cross2[b + atomicAdd(ncross, 1)] = l;
+  }
+}
 
 __global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile *__restrict__ tiles, int ntiles) {
   __shared__ float4 spm[TILE_ITEMS];
@@ -565,7 +602,7 @@ __global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile 
           const int64_t s = b + tl.start + i;
           if (W.dl[s] != v || W.nlo[s] < 0) continue;
           if (W.nlo[s] >= t_lo && W.nhi[s] <= t_hi) node_moments(W, ib, W.npos[s]);
-          else W.cross[b + atomicAdd(ncross, 1)] = tl.start + i;
+          else list_crossing(W, t, tl, b, tl.start + i, W.nlo[s], W.nhi[s], ncross);
         }
         __syncthreads();
       }
@@ -582,7 +619,7 @@ __global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile 
       const int64_t s = b + tl.start + i;
       if (W.nlo[s] < 0) continue;
       if (W.nlo[s] >= t_lo && W.nhi[s] <= t_hi) atomicAdd(&sdepth[W.dl[s]], 1);
-      else W.cross[b + atomicAdd(ncross, 1)] = tl.start + i;
+      else list_crossing(W, t, tl, b, tl.start + i, W.nlo[s], W.nhi[s], ncross);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -615,22 +652,37 @@ __global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile 
   }
 }
 
+// The nodes of a list (pairs at list[0..ng)) of segment S, deepest first.
+__device__ __forceinline__ void moments_of_list(const Work &W, const Seg &S, const int *list, int ng) {
+  const int64_t b = S.ibase, ib = 2 * b;
+  for (int v = min(max(S.D, S.Dfull), MAX_DEPTH); v >= 0; v--) {
+    for (int i = threadIdx.x; i < ng; i += BLOCK) {
+      const int64_t s = b + list[i];
+      if (W.dl[s] == v) node_moments(W, ib, W.npos[s]);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) moments_super_kernel(Work W, const Tile *__restrict__ tiles, int ntiles) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    if (tl.start % SUPER != 0) continue;  // a super-tile's first tile stands for it
+    const int ng = W.tcount[t];
+    if (ng == 0) continue;  // uniform
+    const Seg &S = W.seg[tl.seg];
+    moments_of_list(W, S, W.cross + S.ibase + tl.start, ng);
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK) moments_cross_kernel(Work W, const Tile *__restrict__ tiles, int ntiles) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     if (tl.start != 0) continue;  // the segment's first tile stands for it
-    const int k = tl.seg;
-    const int ng = W.ncount[NC_CROSS * W.nseg + k];
-    if (ng == 0) continue;
-    const Seg &S = W.seg[k];
-    const int64_t b = S.ibase, ib = 2 * b;
-    for (int v = min(max(S.D, S.Dfull), MAX_DEPTH); v >= 0; v--) {
-      for (int i = threadIdx.x; i < ng; i += BLOCK) {
-        const int64_t s = b + W.cross[b + i];
-        if (W.dl[s] == v) node_moments(W, ib, W.npos[s]);
-      }
-      __syncthreads();
-    }
+    const int ng = W.ncount[NC_CROSS * W.nseg + tl.seg];
+    if (ng == 0) continue;  // uniform
+    const Seg &S = W.seg[tl.seg];
+    moments_of_list(W, S, W.cross2 + S.ibase, ng);
   }
 }
 
@@ -1364,18 +1416,45 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   W.ncount[NC_SORT * W.nseg + (int)(&S - W.seg)] = S.nsort;
 }
 
+// Segments whose bound block spans more than SUPER_TILES tiles: one thread per
+// run of SUPER_TILES tiles sums the run's partials in tile order into gpart
+// (at the run's first tile), so that frames_combine_kernel adds runs.
+__global__ void __launch_bounds__(BLOCK) frames_runs_kernel(Work W, const Tile *__restrict__ tiles, int ntiles,
+                                                            const double *part, double *gpart) {
+#pragma clang fp contract(off)
+  for (int t = blockIdx.x * BLOCK + threadIdx.x; t < ntiles; t += gridDim.x * BLOCK) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const int nt = (S.flags & F_ACTIVE) ? (S.nbound + TILE - 1) / TILE : 0;
+    const int r = tl.start / TILE;  // tile number within the segment
+    if (nt <= SUPER_TILES || r >= nt || r % SUPER_TILES != 0) continue;
+    double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
+    for (int u = t; u < t + min(SUPER_TILES, nt - r); u++)
+      for (int c = 0; c < FRAME_SUMS; c++) v[c] += part[(int64_t)u * FRAME_SUMS + c];
+    for (int c = 0; c < FRAME_SUMS; c++) gpart[(int64_t)t * FRAME_SUMS + c] = v[c];
+  }
+}
+
 // One thread per segment: the wave's updates of the active count and of the
 // largest Nlast (frame_update) then combine into one atomic each.  (One wave
-// per segment, as output_combine_kernel does, measured slower here.)
-__global__ void frames_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part) {
+// per segment, as output_combine_kernel does, measured slower here.)  A bound
+// block of more than SUPER_TILES tiles is summed over the runs of
+// frames_runs_kernel.
+__global__ void frames_combine_kernel(Work W, DevParams P, const int *first_tile, const double *part,
+                                      const double *gpart) {
 #pragma clang fp contract(off)
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     if (!(S.flags & F_ACTIVE)) continue;
     double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
     const int nt = (S.nbound + TILE - 1) / TILE;
-    for (int t = first_tile[k]; t < first_tile[k] + nt; t++)
-      for (int c = 0; c < FRAME_SUMS; c++) v[c] += part[(int64_t)t * FRAME_SUMS + c];
+    if (nt <= SUPER_TILES) {
+      for (int t = first_tile[k]; t < first_tile[k] + nt; t++)
+        for (int c = 0; c < FRAME_SUMS; c++) v[c] += part[(int64_t)t * FRAME_SUMS + c];
+    } else {
+      for (int t = first_tile[k]; t < first_tile[k] + nt; t += SUPER_TILES)
+        for (int c = 0; c < FRAME_SUMS; c++) v[c] += gpart[(int64_t)t * FRAME_SUMS + c];
+    }
     frame_update(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
   }
 }

@@ -63,7 +63,8 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
   R.gblk = 1;
   hipStream_t st = ctx->stream;
   seg_expand_kernel<<<1, BLOCK, 0, st>>>(R.W.seg, ctx->segin.as<SegIn>(), 1);
-  fill_seg_of_slot_kernel<<<1, BLOCK, 0, st>>>(1, R.W.ibase, ctx->capal.as<int>(), ctx->seg_of_slot.as<int>());
+  seg_of_slot_kernel<<<grid_for(L.nslots, ctx->cus), BLOCK, 0, st>>>(1, R.W.ibase, L.nslots,
+                                                                      ctx->seg_of_slot.as<int>());
   tree_only_setup_kernel<<<1, 1, 0, st>>>(R.W, (int)n_src);
   tree_sources_kernel<<<grid_for(n_src, ctx->cus), BLOCK, 0, st>>>(R.W, (int)n_src, sx, sy, sz, smass);
   RETCHK(R.build_tree(false));

@@ -104,7 +104,7 @@ struct hbt_ctx {
   int cus = 256;
   // per-slot work arrays
   DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
-      npos, items, scan64, pk, gk, isb, seg_of_slot, cross;
+      npos, items, scan64, pk, gk, isb, seg_of_slot, cross, cross2, tcount;
   // per-level tables: staged through one pinned host buffer and one copy
   DBuf lvlbuf;
   void *h_stage = nullptr;
@@ -134,7 +134,7 @@ struct hbt_ctx {
   ~hbt_ctx() {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
-                    &isb, &seg_of_slot, &cross, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
+                    &isb, &seg_of_slot, &cross, &cross2, &tcount, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
                     &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
@@ -495,7 +495,8 @@ struct Runner {
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     moments_tile_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
-    moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    moments_super_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    if (max_n > SUPER) moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
     return HBT_OK;
   }
 
@@ -555,7 +556,12 @@ struct Runner {
       frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     } else {
       frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
-      frames_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
+      // the run sums go after the tile partials in the same buffer
+      double *gpart = c->tpart.as<double>() + (int64_t)(nt_cur + 1) * FRAME_SUMS;
+      if (max_n > SUPER)
+        frames_runs_kernel<<<grid_for(nt_cur, c->cus), BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur, c->tpart.as<double>(),
+                                                                             gpart);
+      frames_combine_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>(), gpart);
     }
     // energy sort of the removed block, and of the bound block on convergence
     energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
@@ -599,6 +605,8 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->items.ensure((s * 2 + 2) * sizeof(Item)));
   RETCHK(c->scan64.ensure(s * 8));
   RETCHK(c->cross.ensure(s * 4));
+  RETCHK(c->cross2.ensure(s * 4));
+  RETCHK(c->tcount.ensure((size_t)(ntiles + 1) * 4));
   RETCHK(c->pk.ensure((s + 1) * 4));
   RETCHK(c->gk.ensure((s + 1) * 4));
   RETCHK(c->isb.ensure(s));
@@ -645,6 +653,8 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.npos = c->npos.as<int>();
   W.items = c->items.as<Item>();
   W.cross = c->cross.as<int>();
+  W.cross2 = c->cross2.as<int>();
+  W.tcount = c->tcount.as<int>();
   W.scan64 = c->scan64.as<u64>();
   W.pk = c->pk.as<int>(); W.gk = c->gk.as<int>();
   W.isb = c->isb.as<unsigned char>();
@@ -677,8 +687,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
   R.gblk = gsb;
   seg_expand_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W.seg, c->segin.as<SegIn>(), (int)nseg);
-  fill_seg_of_slot_kernel<<<gsb, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, c->capal.as<int>(),
-                                                        c->seg_of_slot.as<int>());
+  seg_of_slot_kernel<<<R.gs, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, L.nslots, c->seg_of_slot.as<int>());
   build_lists_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, c->child_ptr.as<int>(), c->child_list.as<int>());
   gather_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
   seg_init_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P);
