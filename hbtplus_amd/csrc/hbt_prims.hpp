@@ -174,21 +174,43 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
   }
 }
 
-// Merge level `level` (runs of width TILE << level merged pairwise), one block
-// per BLOCK-element part of a tile, one element per thread.  A segment that
-// needs m levels takes part in levels 0..m-1 and ping-pongs between the two
-// buffers so that level m-1 writes (k0, v0); segments with m <= level are
-// done and are not touched.
+// Merge level `level` (runs of width TILE << level merged pairwise).  A
+// segment that needs m levels takes part in levels 0..m-1 and ping-pongs
+// between the two buffers so that level m-1 writes (k0, v0); segments with
+// m <= level are done and are not touched.  Merge path: one block per tile of
+// output; two threads find by binary search how many of the tile's outputs come
+// from the left run at its two ends, the block stages those parts of the two
+// runs in LDS, and each thread finds its own split in LDS and merges
+// MERGE_PER outputs.  Keys with their values are distinct, so the merged order
+// is unique.
+constexpr int MERGE_PER = TILE / BLOCK;
+
+// Outputs among the first d of the merge of A (la) and B (lb) that come from A.
+template <class GetA, class GetB>
+__device__ __forceinline__ int merge_split(GetA a, GetB b, int la, int lb, int d) {
+  int lo = d > lb ? d - lb : 0, hi = d < la ? d : la;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    u64 ka, kb;
+    u32 va, vb;
+    a(mid, ka, va);
+    b(d - 1 - mid, kb, vb);
+    if (pair_less(ka, va, kb, vb)) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
 __global__ void __launch_bounds__(BLOCK)
 merge_pass_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__restrict__ base,
                   const int *__restrict__ n, int level, u64 *k0, u32 *v0, u64 *k1, u32 *v1) {
+  __shared__ u64 sk[TILE];
+  __shared__ u32 sv[TILE];
+  __shared__ int ssplit[2];
   const int width = TILE << level;
-  constexpr int PARTS = TILE / BLOCK;
-  for (int vb = blockIdx.x; vb < ntiles * PARTS; vb += gridDim.x) {
-    const Tile tl = tiles[vb / PARTS];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
     const int cnt = n[tl.seg];
-    const int start = tl.start + (vb % PARTS) * BLOCK;
-    if (cnt <= width || start >= cnt) continue;  // uniform across the block
+    if (cnt <= width || tl.start >= cnt) continue;  // uniform across the block
     const int m = merge_levels(cnt);
     // buffer holding the input of this level: the tile sort wrote parity m&1
     const bool in1 = ((m + level) & 1) != 0;
@@ -196,35 +218,38 @@ merge_pass_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__r
     const u32 *vin = in1 ? v1 : v0;
     u64 *kout = in1 ? k0 : k1;
     u32 *vout = in1 ? v0 : v1;
-    const int64_t b = base[tl.seg];
-    {
-      const int l = start + (int)threadIdx.x;
-      if (l >= cnt) continue;
-      const u64 key = kin[b + l];
-      const u32 val = vin[b + l];
-      const int run = l / width;
-      const int a0 = (run & ~1) * width;  // start of the merged pair
-      int pos;
-      if ((run & 1) == 0) {
-        // count elements of the right run that precede (key,val)
-        int lo = a0 + width, hi = min(a0 + 2 * width, cnt);
-        const int b0 = lo;
-        while (lo < hi) {
-          int mid = (lo + hi) >> 1;
-          if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
-        }
-        pos = (l - a0) + (lo - b0);
-      } else {
-        int lo = a0, hi = a0 + width;
-        while (lo < hi) {
-          int mid = (lo + hi) >> 1;
-          if (pair_less(kin[b + mid], vin[b + mid], key, val)) lo = mid + 1; else hi = mid;
-        }
-        pos = (l - a0 - width) + (lo - a0);
+    const int64_t bs = base[tl.seg];
+    const int a0 = tl.start / (2 * width) * (2 * width);  // start of the merged pair
+    const int la = min(width, cnt - a0);
+    const int lb = max(0, min(width, cnt - a0 - width));
+    const u64 *ka = kin + bs + a0, *kb = ka + width;
+    const u32 *va = vin + bs + a0, *vb = va + width;
+    const int o0 = tl.start - a0, o1 = min(tl.start + TILE, cnt) - a0;  // outputs of this tile
+    auto ga = [&](int i, u64 &k, u32 &v) { k = ka[i]; v = va[i]; };
+    auto gb = [&](int i, u64 &k, u32 &v) { k = kb[i]; v = vb[i]; };
+    if (threadIdx.x < 2) ssplit[threadIdx.x] = merge_split(ga, gb, la, lb, threadIdx.x == 0 ? o0 : o1);
+    __syncthreads();
+    const int i0 = ssplit[0], i1 = ssplit[1];
+    const int j0 = o0 - i0, j1 = o1 - i1;
+    const int na = i1 - i0, nb = j1 - j0;  // na + nb = o1 - o0 <= TILE
+    for (int i = threadIdx.x; i < na; i += BLOCK) { sk[i] = ka[i0 + i]; sv[i] = va[i0 + i]; }
+    for (int i = threadIdx.x; i < nb; i += BLOCK) { sk[na + i] = kb[j0 + i]; sv[na + i] = vb[j0 + i]; }
+    __syncthreads();
+    const int d = (int)threadIdx.x * MERGE_PER;
+    if (d < na + nb) {
+      auto la_ = [&](int i, u64 &k, u32 &v) { k = sk[i]; v = sv[i]; };
+      auto lb_ = [&](int i, u64 &k, u32 &v) { k = sk[na + i]; v = sv[na + i]; };
+      int i = merge_split(la_, lb_, na, nb, d), j = d - i;
+      const int e = min(d + MERGE_PER, na + nb);
+      u64 *ko = kout + bs + a0 + o0;
+      u32 *vo = vout + bs + a0 + o0;
+      for (int o = d; o < e; o++) {
+        const bool take_a = j >= nb || (i < na && pair_less(sk[i], sv[i], sk[na + j], sv[na + j]));
+        if (take_a) { ko[o] = sk[i]; vo[o] = sv[i]; i++; }
+        else { ko[o] = sk[na + j]; vo[o] = sv[na + j]; j++; }
       }
-      kout[b + a0 + pos] = key;
-      vout[b + a0 + pos] = val;
     }
+    __syncthreads();
   }
 }
 

@@ -314,8 +314,7 @@ extern "C" int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particl
   const int gk = grid_for(nseg, ctx->cus);
   double *prof_part = ctx->tpart.as<double>();
   double *shape_part = prof_part + (size_t)(ntiles + 1) * PROF_PART;
-  fill_seg_of_slot_kernel<<<std::min(nseg, ctx->cus * 8), BLOCK, 0, st>>>(nseg, A.ibase, (const int *)d[2],
-                                                                          ctx->seg_of_slot.as<int>());
+  seg_of_slot_kernel<<<gs, BLOCK, 0, st>>>(nseg, A.ibase, off, ctx->seg_of_slot.as<int>());
   props_gather_kernel<<<gs, BLOCK, 0, st>>>(A, P);
   // sort by distance
   tile_sort_kernel<<<gt, BLOCK, 0, st>>>(dt, ntiles, A.ibase, A.n, A.k0, A.v0, A.k1, A.v1);

@@ -113,7 +113,7 @@ struct hbt_ctx {
   DBuf seg;             // per-segment state, expanded on the device from segin
   DBuf ncount, tsum, tpart;
   DBuf segact, actbuf;  // segment activity after a pass; lists of the active segments
-  DBuf gaps;            // (subhalo, capacity) of the hosts whose output regions have unused slots
+  DBuf gaps;            // (subhalo, capacity, first flat index) of the hosts whose output regions have unused slots
   DBuf active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
   int prof_mode = 0;
@@ -345,16 +345,22 @@ __global__ void partition_counts_kernel(Work W, const int *first_tile, const u64
 
 // A host's output region holds its own list and every nested particle (the
 // planned capacity); only its subhalos' unbound tails arrive, so the slots
-// past its final list are set to -1 and NaN.  gap: (subhalo, capacity) pairs.
+// past its final list are set to -1 and NaN.  gap: (subhalo, capacity, first
+// flat index) triples; the flat index runs over the hosts' regions in turn, so
+// the grid spreads over them whatever their sizes.
 __global__ void __launch_bounds__(BLOCK) fill_gaps_kernel(const hbt_subhalo *subs, const int64_t *gap, int ngap,
-                                                          int64_t *out_index, float *out_energy) {
-  for (int h = blockIdx.x; h < ngap; h += gridDim.x) {
-    const int64_t s = gap[2 * h], cap = gap[2 * h + 1];
-    const int64_t o = subs[s].out_offset, len = subs[s].list_len;
-    for (int64_t i = (len > 0 ? len : 0) + threadIdx.x; i < cap; i += BLOCK) {
-      out_index[o + i] = -1;
-      out_energy[o + i] = __builtin_nanf("");
+                                                          int64_t total, int64_t *out_index, float *out_energy) {
+  for (int64_t f = (int64_t)blockIdx.x * BLOCK + threadIdx.x; f < total; f += (int64_t)gridDim.x * BLOCK) {
+    int lo = 0, hi = ngap - 1;  // last host whose first flat index is <= f
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (gap[3 * mid + 2] <= f) lo = mid; else hi = mid - 1;
     }
+    const int64_t s = gap[3 * lo], i = f - gap[3 * lo + 2];
+    if (i < subs[s].list_len) continue;
+    const int64_t o = subs[s].out_offset + i;
+    out_index[o] = -1;
+    out_energy[o] = __builtin_nanf("");
   }
 }
 
@@ -468,7 +474,7 @@ struct Runner {
     for (int w = TILE; w < max_n; w *= 2, level++) {
       const int ntw = tiles_above(w);
       if (ntw == 0) break;
-      const int gmerge = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ntw * (TILE / BLOCK), (int64_t)c->cus * 16));
+      const int gmerge = (int)std::max<int64_t>(1, std::min<int64_t>(ntw, (int64_t)c->cus * 16));
       merge_pass_kernel<<<gmerge, BLOCK, 0, c->stream>>>(tl, ntw, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
     }
     return HBT_OK;
@@ -1130,15 +1136,21 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   }
   static thread_local std::vector<int64_t> gap;
   gap.clear();
+  int64_t gap_total = 0;
   for (int64_t s = 0; s < nsub; s++)
-    if (!children[s].empty()) { gap.push_back(s); gap.push_back(cap[s]); }
+    if (!children[s].empty()) {
+      gap.push_back(s);
+      gap.push_back(cap[s]);
+      gap.push_back(gap_total);
+      gap_total += cap[s];
+    }
   if (!gap.empty()) {
     RETCHK(ctx->gaps.ensure(gap.size() * sizeof(int64_t)));
     HIPCHK(hipMemcpyAsync(ctx->gaps.p, gap.data(), gap.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                           ctx->stream));
-    const int ngap = (int)(gap.size() / 2);
-    fill_gaps_kernel<<<std::min(ngap, 4 * ctx->cus), BLOCK, 0, ctx->stream>>>(
-        subs, ctx->gaps.as<int64_t>(), ngap, out_index, out_energy);
+    const int ngap = (int)(gap.size() / 3);
+    fill_gaps_kernel<<<grid_for(gap_total, ctx->cus), BLOCK, 0, ctx->stream>>>(
+        subs, ctx->gaps.as<int64_t>(), ngap, gap_total, out_index, out_energy);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
