@@ -54,6 +54,7 @@ struct Work {
   const float *g_x, *g_y, *g_z, *g_vx, *g_vy, *g_vz, *g_m;
   const int32_t *g_type;
   const int64_t *g_index;    // optional: list entry -> particle column position
+  const float4 *g_packed;    // optional: the columns interleaved, 2 float4 per particle
   // working arrays [nslots]
   int *gidx;
   float *px, *py, *pz, *vx, *vy, *vz, *pm;
@@ -188,6 +189,16 @@ __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *chil
   }
 }
 
+// The snapshot's columns interleaved per particle (x, y, z, mass | vx, vy, vz,
+// type): a gather in list order, which is random in the snapshot, then reads
+// one 32-byte record per particle instead of eight scattered words.
+__global__ void __launch_bounds__(BLOCK) pack_particles_kernel(hbt_particles P, float4 *out) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * BLOCK) {
+    out[2 * i] = make_float4(P.x[i], P.y[i], P.z[i], P.mass[i]);
+    out[2 * i + 1] = make_float4(P.vx[i], P.vy[i], P.vz[i], __int_as_float(P.type ? P.type[i] : 1));
+  }
+}
+
 // Column positions of the segments' own particles, and local copies of the
 // listed particles' columns (slot-parallel).
 __global__ void __launch_bounds__(BLOCK) gather_kernel(Work W) {
@@ -203,10 +214,16 @@ __global__ void __launch_bounds__(BLOCK) gather_kernel(Work W) {
     } else {
       g = W.gidx[s];
     }
-    W.px[s] = W.g_x[g]; W.py[s] = W.g_y[g]; W.pz[s] = W.g_z[g];
-    W.vx[s] = W.g_vx[g]; W.vy[s] = W.g_vy[g]; W.vz[s] = W.g_vz[g];
-    W.pm[s] = W.g_m[g];
-    W.ptype[s] = W.g_type ? W.g_type[g] : 1;
+    if (W.g_packed) {
+      const float4 a = W.g_packed[2 * (int64_t)g], c = W.g_packed[2 * (int64_t)g + 1];
+      W.px[s] = a.x; W.py[s] = a.y; W.pz[s] = a.z; W.pm[s] = a.w;
+      W.vx[s] = c.x; W.vy[s] = c.y; W.vz[s] = c.z; W.ptype[s] = __float_as_int(c.w);
+    } else {
+      W.px[s] = W.g_x[g]; W.py[s] = W.g_y[g]; W.pz[s] = W.g_z[g];
+      W.vx[s] = W.g_vx[g]; W.vy[s] = W.g_vy[g]; W.vz[s] = W.g_vz[g];
+      W.pm[s] = W.g_m[g];
+      W.ptype[s] = W.g_type ? W.g_type[g] : 1;
+    }
   }
 }
 

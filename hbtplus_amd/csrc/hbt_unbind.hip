@@ -128,6 +128,8 @@ struct hbt_ctx {
   DBuf st_cols, st_subs, st_out_i, st_out_e, st_phi;
   int *h_active = nullptr;  // pinned
   const int64_t *list_index = nullptr;  // set for the duration of hbt_unbind_batch_indexed
+  const float4 *packed = nullptr;       // likewise, when the snapshot is gathered from packed records
+  DBuf pack;                            // the packed snapshot
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   int64_t last_evals = 0, last_passes = 0;
@@ -135,7 +137,7 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &cross, &cross2, &tcount, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
-                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps})
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -639,6 +641,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.ibase = c->ibase.as<int64_t>();
   W.ncount = c->ncount.as<int>();
   W.g_index = c->list_index;
+  W.g_packed = c->packed;
   if (parts) {
     W.g_id = parts->id; W.g_x = parts->x; W.g_y = parts->y; W.g_z = parts->z;
     W.g_vx = parts->vx; W.g_vy = parts->vy; W.g_vz = parts->vz; W.g_m = parts->mass;
@@ -1084,9 +1087,20 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (bad) return fail(HBT_ERR_INVALID_ARG, std::to_string(bad) + " list entries are not snapshot positions");
   }
+  // lists that cover a good part of the snapshot are gathered from packed
+  // records (one streaming pass over the snapshot, then one 32-byte read per
+  // listed particle); sparse lists read the columns directly
+  if (n_list > 0 && snapshot->n <= 4 * n_list) {
+    RETCHK(ctx->pack.ensure((size_t)snapshot->n * 2 * sizeof(float4)));
+    pack_particles_kernel<<<grid_for(snapshot->n, ctx->cus), BLOCK, 0, ctx->stream>>>(*snapshot,
+                                                                                      ctx->pack.as<float4>());
+    HIPCHK(hipGetLastError());
+    ctx->packed = ctx->pack.as<float4>();
+  }
   ctx->list_index = list_index;
   int rc = unbind_batch_impl(ctx, snapshot, n_list, subs, subs_host, nsub, out_index, out_energy, plain);
   ctx->list_index = nullptr;
+  ctx->packed = nullptr;
   return rc;
 }
 
