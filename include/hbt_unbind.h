@@ -194,7 +194,11 @@ int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles,
  *               its entries are snapshot positions (negative or out-of-range
  *               positions are rejected with HBT_ERR_INVALID_ARG)
  *   out_index : snapshot positions, layout as in hbt_unbind_batch
- * Other arguments as in hbt_unbind_batch. */
+ * Other arguments as in hbt_unbind_batch.  When n_list is at least a quarter
+ * of snapshot->n the context first interleaves the snapshot's columns into
+ * 32-byte records (a device buffer of 32 * snapshot->n bytes that the context
+ * keeps for later calls), so the gather in list order reads one record per
+ * particle. */
 int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot,
                              const int64_t *list_index, int64_t n_list, hbt_subhalo *subs,
                              const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
