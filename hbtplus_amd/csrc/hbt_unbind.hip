@@ -130,6 +130,7 @@ struct hbt_ctx {
   const int64_t *list_index = nullptr;  // set for the duration of hbt_unbind_batch_indexed
   const float4 *packed = nullptr;       // likewise, when the snapshot is gathered from packed records
   DBuf pack;                            // the packed snapshot
+  DBuf idtab;                           // hbt_id_to_index's table
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   int64_t last_evals = 0, last_passes = 0;
@@ -137,7 +138,7 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &cross, &cross2, &tcount, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
-                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack})
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack, &idtab})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (h_stage) (void)hipHostFree(h_stage);

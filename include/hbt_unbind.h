@@ -339,7 +339,9 @@ int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_
 
 /* ID -> index resolution: builds a device hash table of the n_table IDs
  * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs
- * and negative IDs give -1 (src/hash.tpp:52).  Device pointers. */
+ * and negative IDs give -1 (src/hash.tpp:52).  Device pointers.  The table
+ * (16 bytes per slot, the next power of two at or above 2 * n_table slots)
+ * stays with the context for later calls. */
 int hbt_id_to_index(hbt_ctx *ctx, int64_t n_table, const int64_t *table_ids,
                     int64_t n_query, const int64_t *query_ids, int64_t *out_index);
 
