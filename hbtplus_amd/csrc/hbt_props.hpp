@@ -194,7 +194,7 @@ props_shape_kernel(PropsArgs A, DevParams P, const Tile *tiles, int ntiles, doub
   }
 }
 
-__global__ void props_combine_kernel(PropsArgs A, DevParams P, const int *first_tile,
+__global__ void __launch_bounds__(BLOCK) props_combine_kernel(PropsArgs A, DevParams P, const int *first_tile,
                                      const double *prof_part, const double *shape_part) {
 #pragma clang fp contract(off)
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A.nseg; k += gridDim.x * blockDim.x) {
@@ -206,9 +206,31 @@ __global__ void props_combine_kernel(PropsArgs A, DevParams P, const int *first_
     int lso = -1;
     double sums[SHAPE_SUMS];
     for (int q = 0; q < SHAPE_SUMS; q++) sums[q] = 0.;
-    for (int t = t0; t < t0 + nt; t++) {
+    // tiles in order (the first maximum stays; the sums keep their order);
+    // PREF tiles are loaded ahead of their use, so a subhalo of thousands of
+    // tiles does not wait for each load in turn
+    constexpr int PREF = 4;
+    int t = t0;
+    for (; t + PREF <= t0 + nt; t += PREF) {
+      double pv[PREF], pr[PREF], pl[PREF], pm[PREF], sp[PREF][SHAPE_SUMS];
+#pragma unroll
+      for (int u = 0; u < PREF; u++) {
+        const double *p = prof_part + (int64_t)(t + u) * PROF_PART;
+        pv[u] = p[0]; pr[u] = p[2]; pl[u] = p[3]; pm[u] = p[4];
+#pragma unroll
+        for (int q = 0; q < SHAPE_SUMS; q++) sp[u][q] = shape_part[(int64_t)(t + u) * SHAPE_SUMS + q];
+      }
+#pragma unroll
+      for (int u = 0; u < PREF; u++) {
+        if (pv[u] > v) { v = pv[u]; rr = pr[u]; }
+        if ((int)pl[u] > lso) { lso = (int)pl[u]; mso = pm[u]; }
+#pragma unroll
+        for (int q = 0; q < SHAPE_SUMS; q++) sums[q] += sp[u][q];
+      }
+    }
+    for (; t < t0 + nt; t++) {
       const double *p = prof_part + (int64_t)t * PROF_PART;
-      if (p[0] > v) { v = p[0]; rr = p[2]; }  // tiles in order: the first maximum stays
+      if (p[0] > v) { v = p[0]; rr = p[2]; }
       if ((int)p[3] > lso) { lso = (int)p[3]; mso = p[4]; }
       for (int q = 0; q < SHAPE_SUMS; q++) sums[q] += shape_part[(int64_t)t * SHAPE_SUMS + q];
     }
