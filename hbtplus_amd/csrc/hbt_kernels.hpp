@@ -1433,6 +1433,30 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   W.ncount[NC_SORT * W.nseg + (int)(&S - W.seg)] = S.nsort;
 }
 
+// v[c] += part[(t0 + i * step) * C + c] for i = 0 .. cnt-1, in that order, with
+// the values of the next few tiles loaded ahead of their use.
+template <int C>
+__device__ __forceinline__ void sum_tiles_in_order(const double *__restrict__ part, int t0, int cnt, int step,
+                                                   double v[C]) {
+#pragma clang fp contract(off)
+  constexpr int PREF = 4;
+  int i = 0;
+  for (; i + PREF <= cnt; i += PREF) {
+    double x[PREF][C];
+#pragma unroll
+    for (int u = 0; u < PREF; u++)
+#pragma unroll
+      for (int c = 0; c < C; c++) x[u][c] = part[(int64_t)(t0 + (i + u) * step) * C + c];
+#pragma unroll
+    for (int u = 0; u < PREF; u++)
+#pragma unroll
+      for (int c = 0; c < C; c++) v[c] += x[u][c];
+  }
+  for (; i < cnt; i++)
+#pragma unroll
+    for (int c = 0; c < C; c++) v[c] += part[(int64_t)(t0 + i * step) * C + c];
+}
+
 // Segments whose bound block spans more than SUPER_TILES tiles: one thread per
 // run of SUPER_TILES tiles sums the run's partials in tile order into gpart
 // (at the run's first tile), so that frames_combine_kernel adds runs.
@@ -1446,8 +1470,7 @@ __global__ void __launch_bounds__(BLOCK) frames_runs_kernel(Work W, const Tile *
     const int r = tl.start / TILE;  // tile number within the segment
     if (nt <= SUPER_TILES || r >= nt || r % SUPER_TILES != 0) continue;
     double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
-    for (int u = t; u < t + min(SUPER_TILES, nt - r); u++)
-      for (int c = 0; c < FRAME_SUMS; c++) v[c] += part[(int64_t)u * FRAME_SUMS + c];
+    sum_tiles_in_order<FRAME_SUMS>(part, t, min(SUPER_TILES, nt - r), 1, v);
     for (int c = 0; c < FRAME_SUMS; c++) gpart[(int64_t)t * FRAME_SUMS + c] = v[c];
   }
 }
@@ -1465,13 +1488,10 @@ __global__ void frames_combine_kernel(Work W, DevParams P, const int *first_tile
     if (!(S.flags & F_ACTIVE)) continue;
     double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
     const int nt = (S.nbound + TILE - 1) / TILE;
-    if (nt <= SUPER_TILES) {
-      for (int t = first_tile[k]; t < first_tile[k] + nt; t++)
-        for (int c = 0; c < FRAME_SUMS; c++) v[c] += part[(int64_t)t * FRAME_SUMS + c];
-    } else {
-      for (int t = first_tile[k]; t < first_tile[k] + nt; t += SUPER_TILES)
-        for (int c = 0; c < FRAME_SUMS; c++) v[c] += gpart[(int64_t)t * FRAME_SUMS + c];
-    }
+    if (nt <= SUPER_TILES)
+      sum_tiles_in_order<FRAME_SUMS>(part, first_tile[k], nt, 1, v);
+    else
+      sum_tiles_in_order<FRAME_SUMS>(gpart, first_tile[k], (nt + SUPER_TILES - 1) / SUPER_TILES, SUPER_TILES, v);
     frame_update(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
   }
 }
