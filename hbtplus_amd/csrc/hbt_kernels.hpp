@@ -573,8 +573,8 @@ __device__ __forceinline__ void node_moments(const Work &W, int64_t ib, int g) {
 // (only for segments of more than one super-tile): one workgroup per segment
 // computes the rest, deepest first.  A node's children are inside a tile (done
 // by the first launch) or listed at the same or a lower level and deeper.
-// Each node sums its children in son order (sum_children), so the results are
-// those of the per-depth launches.
+// Each node sums its children in son order (sum_children), so the results do
+// not depend on which launch computes a node.
 constexpr int TILE_ITEMS = 2 * TILE + MAX_DEPTH + 2;
 constexpr int SUPER_TILES = 64;
 constexpr int SUPER = SUPER_TILES * TILE;
