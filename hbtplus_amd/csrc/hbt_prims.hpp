@@ -157,7 +157,8 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
     for (int k = 2; k <= P; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
         for (int p = threadIdx.x; p < (P >> 1); p += BLOCK) {
-          const int i = 2 * j * (p / j) + (p % j);
+          // 2 j (p / j) + p % j, with j a power of two
+          const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
           const int q = i + j;
           const bool up = (i & k) == 0;
           u64 ka = sk[i], kb = sk[q];
