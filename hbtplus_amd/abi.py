@@ -32,7 +32,8 @@ PARAMS_DTYPE = np.dtype([
     ("G", "<f4"), ("softening", "<f4"), ("open_angle", "<f4"), ("box_size", "<f4"),
     ("periodic", "<i4"), ("bound_mass_precision", "<f4"), ("min_num_part_of_sub", "<i4"),
     ("source_sub_relax_factor", "<f4"), ("max_sample_size", "<i4"), ("scale_factor", "<f4"),
-    ("hz", "<f4"), ("snapshot_index", "<i4"), ("precision", "<i4"), ("reserved", "<i4", (3,)),
+    ("hz", "<f4"), ("snapshot_index", "<i4"), ("precision", "<i4"),
+    ("refine_mostbound_particle", "<i4"), ("reserved", "<i4", (2,)),
 ], align=True)
 
 SUB_DTYPE = np.dtype([
@@ -83,12 +84,14 @@ PARTICLE_COLUMNS = ("id", "x", "y", "z", "vx", "vy", "vz", "mass", "type")
 
 def make_params(**kw) -> np.ndarray:
     """An ``hbt_params`` record; defaults are HBT's (src/config_parser.h:68-110)
-    in kpc/h, km/s, 1e10 Msun/h units, with sampling off."""
+    in kpc/h, km/s, 1e10 Msun/h units, with sampling off (the reference's
+    default MaxSampleSizeOfPotentialEstimate is 1000; RefineMostboundParticle
+    keeps its default, on)."""
     p = np.zeros(1, PARAMS_DTYPE)
     defaults = dict(G=43007.1, softening=0.5, open_angle=0.45, box_size=1.0e5, periodic=0,
                     bound_mass_precision=0.995, min_num_part_of_sub=20,
                     source_sub_relax_factor=3.0, max_sample_size=0, scale_factor=1.0, hz=0.0,
-                    snapshot_index=0, precision=HBT_PREC_FAST)
+                    snapshot_index=0, precision=HBT_PREC_FAST, refine_mostbound_particle=1)
     defaults.update(kw)
     for k, v in defaults.items():
         p[k] = v

@@ -97,12 +97,13 @@ typedef struct hbt_params {
   float   bound_mass_precision;     /* BoundMassPrecision */
   int32_t min_num_part_of_sub;      /* MinNumPartOfSub */
   float   source_sub_relax_factor;  /* SourceSubRelaxFactor */
-  int32_t max_sample_size;          /* MaxSampleSizeOfPotentialEstimate; only 0 (no sampling) is supported */
+  int32_t max_sample_size;          /* MaxSampleSizeOfPotentialEstimate (0: no sampling) */
   float   scale_factor;             /* Cosmology.ScaleFactor */
   float   hz;                       /* Cosmology.Hz */
   int32_t snapshot_index;           /* epoch.GetSnapshotIndex() */
   int32_t precision;                /* HBT_PREC_FAST or HBT_PREC_REPLAY */
-  int32_t reserved[3];
+  int32_t refine_mostbound_particle;/* RefineMostboundParticle (read when max_sample_size > 0) */
+  int32_t reserved[2];
 } hbt_params;
 
 /* Particle columns (device or host pointers, see each entry point). */

@@ -36,6 +36,9 @@ def lib():
         L = ctypes.CDLL(LIB)
         vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
         L.oracle_unbind_batch.argtypes = [vp, vp, vp, i64, vp, vp, i32]
+        L.oracle_unbind_batch_rs.argtypes = [vp, vp, vp, i64, vp, vp, i32, ctypes.c_uint32, vp]
+        L.oracle_glibc_rand.argtypes = [ctypes.c_uint32, i64, i64, vp]
+        L.oracle_random_shuffle.argtypes = [ctypes.c_uint32, i64, i64, vp, vp]
         L.oracle_tree_potential.argtypes = [vp, i64, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]
         L.oracle_tree_ncell.argtypes = [vp, i64, vp, vp, vp, vp]
         L.oracle_tree_ncell.restype = i64
@@ -59,8 +62,11 @@ def _f32(a):
     return np.ascontiguousarray(a, np.float32)
 
 
-def unbind_batch(cat, nthreads: int = 1, params=None):
-    """Run the restatement on a catalogue.  Returns (subs_out, out_index, out_energy)."""
+def unbind_batch(cat, nthreads: int = 1, params=None, rand_state=None):
+    """Run the restatement on a catalogue.  Returns (subs_out, out_index, out_energy).
+    rand_state: {"seed": s, "draws": d}, the rand() stream the sampling
+    shuffles draw from (srand(s) then d calls; default srand(1), no calls);
+    "draws" is updated in place."""
     params = cat.params if params is None else params
     cols = {k: np.ascontiguousarray(v) for k, v in cat.cols.items()}
     subs = cat.subs.copy()
@@ -68,11 +74,33 @@ def unbind_batch(cat, nthreads: int = 1, params=None):
     out_index = np.full(total, -1, np.int64)
     out_energy = np.full(total, np.nan, np.float32)
     ps = particles_struct(cols)
-    rc = lib().oracle_unbind_batch(_p(params), ctypes.addressof(ps), _p(subs), len(subs),
-                                   _p(out_index), _p(out_energy), nthreads)
+    rs = rand_state if rand_state is not None else {"seed": 1, "draws": 0}
+    draws = ctypes.c_int64(int(rs["draws"]))
+    rc = lib().oracle_unbind_batch_rs(_p(params), ctypes.addressof(ps), _p(subs), len(subs),
+                                      _p(out_index), _p(out_energy), nthreads,
+                                      ctypes.c_uint32(int(rs["seed"])), ctypes.byref(draws))
     if rc != 0:
         raise RuntimeError(f"oracle_unbind_batch failed: {rc}")
+    rs["draws"] = draws.value
     return subs, out_index, out_energy
+
+
+def glibc_rand(n, seed=1, skip=0):
+    """n values of the restated glibc rand() after srand(seed) and skip calls."""
+    out = np.zeros(max(n, 1), np.int32)
+    if lib().oracle_glibc_rand(seed, skip, n, _p(out)) != 0:
+        raise RuntimeError("oracle_glibc_rand failed")
+    return out[:n]
+
+
+def random_shuffle(lengths, seed=1, skip=0):
+    """The restated std::random_shuffle of each range [0, len) in turn
+    (concatenated), drawing from srand(seed) after skip calls."""
+    ln = np.ascontiguousarray(lengths, np.int64)
+    out = np.zeros(max(int(ln.sum()), 1), np.int64)
+    if lib().oracle_random_shuffle(seed, skip, len(ln), _p(ln), _p(out)) != 0:
+        raise RuntimeError("oracle_random_shuffle failed")
+    return out[:int(ln.sum())]
 
 
 def tree_potential(params, src, tgt, tmass=None):
@@ -260,7 +288,7 @@ def ref_time_unbind(cat, nthreads, params=None):
         return float(np.fromfile(fout, np.float64, 1)[0])
 
 
-def ref_unbind_batch(cat, params=None, props_in=None):
+def ref_unbind_batch(cat, params=None, props_in=None, levels=False):
     """Run the reference's own RecursiveUnbind/Unbind/TruncateSource on the
     catalogue through oracle/_ref/hbt_ref_harness (single thread).  With
     props_in (an array of PROPS_DTYPE): also CalculateProfileProperties and
@@ -274,7 +302,7 @@ def ref_unbind_batch(cat, params=None, props_in=None):
             with open(fin, "ab") as f:
                 f.write(np.ascontiguousarray(props_in, PROPS_DTYPE).tobytes())
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        mode = "unbind" if props_in is None else "props"
+        mode = "levels" if levels else ("unbind" if props_in is None else "props")
         subprocess.run([REF_HARNESS, mode, fin, fout], check=True, env=env)
         raw = open(fout, "rb").read()
     nsub = int(np.frombuffer(raw, np.int64, 1)[0])

@@ -21,10 +21,23 @@
 //   RecursiveUnbind / TruncateSource src/subhalo_unbind.cpp:432-458
 //   CountParticleTypes/CountParticles src/subhalo.cpp:400-444,479-499
 //   derived parameters               src/config_parser.cpp:95-103
+//   sampling: random_shuffle, SetMassUnit, RefineBindingEnergyOrder
+//                                    src/subhalo_unbind.cpp:74-79,234-262,299-302,
+//                                    335-339,385-388,406-407
 // Third-party behaviour restated: POSIX drand48 (48-bit LCG, a=0x5DEECE66D,
 // c=0xB; glibc starts an unseeded generator at state 0) used at
-// src/oct_tree.tpp:118, and
-// libstdc++ std::sort (called directly, same library as the reference build).
+// src/oct_tree.tpp:118; glibc rand() (random(3)'s TYPE_3 additive feedback
+// generator, an unseeded generator being srand(1)) and libstdc++
+// std::random_shuffle (GCC 11 bits/stl_algo.h: for i = 1..n-1, swap a[i] with
+// a[rand() % (i+1)]) used at src/subhalo_unbind.cpp:302; libstdc++ std::sort
+// (called directly, same library as the reference build).
+//
+// Order of the rand() draws in a batch with sampling on: subhalos are unbound
+// level by level (a level: the subhalos whose nested subhalos are all done,
+// i.e. by nest height), in index order within a level.  Without nesting this
+// is the index order of the reference's single-threaded RefineParticles loop;
+// the reference harness (oracle/ref_harness.cpp, mode "levels") drives the
+// reference's own Unbind in this order for the sampling fixtures.
 //
 // Build: see oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp, no -march).
 
@@ -51,6 +64,8 @@ struct Params {
   float G, eps, theta2, res, res_half, box, box_half, bmp, relax, a, hz;
   bool periodic;
   int min_part, snap;
+  Int max_sample;      // MaxSampleSizeOfPotentialEstimate
+  bool refine;         // RefineMostboundParticle, only with sampling (:264-265)
   explicit Params(const hbt_params &p) {
     G = p.G;
     eps = p.softening;
@@ -66,6 +81,8 @@ struct Params {
     periodic = p.periodic != 0;
     min_part = p.min_num_part_of_sub;
     snap = p.snapshot_index;
+    max_sample = p.max_sample_size;
+    refine = max_sample > 0 && p.refine_mostbound_particle != 0;
   }
   // NEAREST macro, config_parser.h:142, in float and in double
   inline float nearest(float x) const {
@@ -87,6 +104,57 @@ struct Drand48 {
     return (double)x / 281474976710656.0;
   }
 };
+
+// ---- glibc rand() ----------------------------------------------------------
+// random(3)'s TYPE_3 generator: srand(seed) sets r[0] = seed (1 when 0) and
+// r[i] = 16807 r[i-1] mod (2^31 - 1) for i = 1..30 (Schrage's method in
+// 32-bit words), r[i] = r[i-31] for i = 31..33; then r[i] = r[i-31] + r[i-3]
+// mod 2^32.  The values r[34..343] are discarded and the k-th rand() call
+// returns r[344+k] >> 1.  A process that never calls srand behaves as srand(1).
+struct GlibcRand {
+  uint32_t w[31];  // w[i % 31] = r[i] for the last 31 indices
+  int64_t m = 0;   // next index of r
+  int64_t draws = 0;
+  explicit GlibcRand(uint32_t seed = 1) {
+    int32_t r[34];
+    int32_t word = (int32_t)(seed == 0 ? 1 : seed);
+    r[0] = word;
+    for (int i = 1; i < 31; i++) {
+      const long hi = word / 127773, lo = word % 127773;
+      word = (int32_t)(16807 * lo - 2836 * hi);
+      if (word < 0) word += 2147483647;
+      r[i] = word;
+    }
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    for (int i = 3; i < 34; i++) w[i % 31] = (uint32_t)r[i];
+    m = 34;
+    for (int i = 0; i < 310; i++) step();
+  }
+  uint32_t step() {
+    const int u = (int)(m % 31);
+    w[u] += w[(u + 28) % 31];
+    m++;
+    return w[u];
+  }
+  int32_t next() {
+    draws++;
+    return (int32_t)(step() >> 1);
+  }
+  void skip(int64_t n) {
+    for (int64_t i = 0; i < n; i++) next();
+  }
+};
+
+// std::random_shuffle(first, last) of libstdc++ (GCC 11, bits/stl_algo.h):
+// draws[i-1] is the rand() value of step i.
+template <class T>
+void random_shuffle_with(std::vector<T> &a, const int32_t *draws) {
+  const Int n = (Int)a.size();
+  for (Int i = 1; i < n; i++) {
+    const Int j = (Int)draws[i - 1] % (i + 1);
+    if (i != j) std::swap(a[i], a[j]);
+  }
+}
 
 struct Vec3f { float v[3]; };
 
@@ -306,7 +374,11 @@ void count_types(const PartCols &C, Sub &S, bool skip_null) {
   for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = nb[t]; r.mbound_type[t] = mb[t]; }
 }
 
-void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree) {
+// draws: the rand() values of the list's shuffle (read only when the list is
+// longer than MaxSampleSizeOfPotentialEstimate > 0: one per element after the
+// first).
+void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree,
+            const int32_t *draws = nullptr) {
   hbt_subhalo &r = *S.rec;
   const Int N = (Int)S.list.size();
   auto alive = [&]() { return r.snapshot_index_of_death == -1; };
@@ -328,20 +400,33 @@ void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree
   float OldRefPos[3] = {0, 0, 0}, OldRefVel[3] = {0, 0, 0};
   const Int old_mb_id = C.p->id[S.list[0]];
   Int Nbound = N, Nlast = 0;
+  const Int MS = P.max_sample;
+  // shuffle for the sampled trees (:302)
+  if (MS > 0 && N > MS) random_shuffle_with(S.list, draws);
   std::vector<PE> El(N);
   for (Int i = 0; i < N; i++) { El[i].pid = i; El[i].E = 0.f; }
   auto gpos = [&](Int i, float o[3]) { Int g = S.list[El[i].pid]; for (int j = 0; j < 3; j++) o[j] = C.x(g, j); };
   auto gvel = [&](Int i, float o[3]) { Int g = S.list[El[i].pid]; for (int j = 0; j < 3; j++) o[j] = C.v(g, j); };
   auto gmass = [&](Int i) { return C.m(S.list[El[i].pid]); };
-  auto load_tree = [&](Int b, Int e) {
+  // tree over Elist[b, e) with the masses times mf (EnergySnapshot_t::GetMass
+  // with MassFactor, :74-79)
+  auto load_tree = [&](Int b, Int e, float mf) {
     Int n = e - b;
     tree.np = n;
     tree.px.resize(n); tree.py.resize(n); tree.pz.resize(n); tree.pm.resize(n);
     for (Int i = 0; i < n; i++) {
       float x[3]; gpos(b + i, x);
-      tree.px[i] = x[0]; tree.py[i] = x[1]; tree.pz[i] = x[2]; tree.pm[i] = gmass(b + i);
+      tree.px[i] = x[0]; tree.py[i] = x[1]; tree.pz[i] = x[2]; tree.pm[i] = gmass(b + i) * mf;
     }
     tree.build(P, rng);
+  };
+  // BindingEnergy (src/gravity_tree.cpp:166-175) of Elist entry i in the frame
+  auto binding_energy = [&](Int i, float tmass) {
+    float x[3], v[3], dv[3];
+    gpos(i, x); gvel(i, v);
+    double pot = tree.potential(P, x[0], x[1], x[2], tmass);
+    rel_vel(P, x, v, RefPos, RefVel, dv);
+    return (float)((double)vnorm(dv) * 0.5 + pot);
   };
   bool correction = false;
   while (true) {
@@ -350,7 +435,7 @@ void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree
       float RefVelDiff[3];
       rel_vel(P, OldRefPos, OldRefVel, RefPos, RefVel, RefVelDiff);
       float dK = (float)(0.5 * (double)vnorm(RefVelDiff));
-      load_tree(Nbound, Nlast);
+      load_tree(Nbound, Nlast, 1.f);
       for (Int i = 0; i < Nbound; i++) {
         float x[3], v[3], ov[3];
         gpos(i, x); gvel(i, v);
@@ -363,14 +448,17 @@ void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree
       Nlast = Nbound;
     } else {
       Nlast = Nbound;
-      load_tree(0, Nlast);
-      for (Int i = 0; i < Nlast; i++) {
-        float x[3], v[3], dv[3];
-        gpos(i, x); gvel(i, v);
-        double pot = tree.potential(P, x[0], x[1], x[2], gmass(i));
-        rel_vel(P, x, v, RefPos, RefVel, dv);
-        El[i].E = (float)((double)vnorm(dv) * 0.5 + pot);
+      // sampled tree over the first MaxSampleSize entries with their masses
+      // scaled by Nlast / MaxSampleSize (:332-340); targets outside the
+      // sample carry no self term
+      Int np_tree = Nlast;
+      float mf = 1.f;
+      if (MS > 0 && Nlast > MS) {
+        np_tree = MS;
+        mf = (float)Nlast / (float)MS;
       }
+      load_tree(0, np_tree, mf);
+      for (Int i = 0; i < Nlast; i++) El[i].E = binding_energy(i, i < np_tree ? gmass(i) * mf : 0.f);
       r.n_evals += Nlast;
     }
     Nbound = partition_energy(El, Nlast);
@@ -387,7 +475,7 @@ void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree
     }
     std::sort(El.begin() + Nbound, El.begin() + Nlast, comp_energy);
     Int Ndiff = Nlast - Nbound;
-    if (Ndiff < Nbound) {
+    if (Ndiff < Nbound && (MS <= 0 || Ndiff < MS)) {  // :383-389
       correction = true;
       for (int j = 0; j < 3; j++) { OldRefPos[j] = RefPos[j]; OldRefVel[j] = RefVel[j]; }
     }
@@ -434,6 +522,17 @@ void unbind(const Params &P, const PartCols &C, Sub &S, Drand48 &rng, Tree &tree
       if (!alive()) r.snapshot_index_of_death = -1;
       if (r.sink_track_id != -1) { r.snapshot_index_of_sink = -1; r.sink_track_id = -1; }
       std::sort(El.begin(), El.begin() + Nbound, comp_energy);
+      if (P.refine && Nbound > MS) {
+        // RefineBindingEnergyOrder (:234-262): the first MaxSampleSize entries
+        // re-ordered by their energy in a tree over themselves (full masses),
+        // keeping their unbinding energies
+        load_tree(0, MS, 1.f);
+        std::vector<PE> Ein(MS);
+        for (Int i = 0; i < MS; i++) { Ein[i].pid = i; Ein[i].E = binding_energy(i, gmass(i)); }
+        std::sort(Ein.begin(), Ein.end(), comp_energy);
+        for (Int i = 0; i < MS; i++) Ein[i] = El[Ein[i].pid];
+        for (Int i = 0; i < MS; i++) El[i] = Ein[i];
+      }
       std::vector<Int> nl(N);
       for (Int i = 0; i < N; i++) { nl[i] = S.list[El[i].pid]; S.energy[i] = El[i].E; El[i].pid = i; }
       S.list.swap(nl);
@@ -657,6 +756,55 @@ void merge_core(const Params &P, const PartCols &C, const Int *list, Int nb, Mer
     sv2[j] -= sv[j] * sv[j];
   }
   h.sig_v = (float)std::sqrt(sv2[0] + sv2[1] + sv2[2]);
+}
+
+// With sampling on (MaxSampleSizeOfPotentialEstimate > 0): the subhalos of
+// each level in index order, each with RecursiveUnbind's handling of its
+// nested subhalos' tails (:432-447) and its shuffle's draws taken from the
+// rand() stream in that order.  The draws of a level are generated before the
+// level is unbound, so several threads give the same result as one.
+void unbind_levels(Batch &B, const std::vector<std::vector<Int>> &levels, int nthreads, GlibcRand &rnd) {
+  const Params &P = B.P;
+  Drand48 drand;
+  Tree tree;
+  for (const auto &lev : levels) {
+    const Int nl = (Int)lev.size();
+    std::vector<std::vector<Int>> backup(nl);
+    std::vector<Int> off(nl + 1, 0);
+    for (Int q = 0; q < nl; q++) {
+      Sub &S = B.subs[lev[q]];
+      const bool orphan = S.rec->nbound_prev <= 1;
+      if (orphan) backup[q] = S.list;
+      for (Int c : B.children[lev[q]]) {
+        Sub &K = B.subs[c];
+        S.list.insert(S.list.end(), K.list.begin() + K.nbound, K.list.end());
+      }
+      if (orphan) S.list.swap(backup[q]);
+      const Int ulen = (Int)S.list.size();
+      off[q + 1] = off[q] + ((P.max_sample > 0 && ulen > P.max_sample) ? ulen - 1 : 0);
+    }
+    std::vector<int32_t> dr(std::max<Int>(off[nl], 1));
+    for (Int i = 0; i < off[nl]; i++) dr[i] = rnd.next();
+    auto one = [&](Int q, Drand48 &rg, Tree &tr) {
+      Sub &S = B.subs[lev[q]];
+      unbind(P, B.C, S, rg, tr, dr.data() + off[q]);
+      if (S.rec->nbound_prev <= 1) {
+        S.list.swap(backup[q]);
+        S.energy.assign(S.list.size(), std::numeric_limits<float>::quiet_NaN());
+      }
+    };
+    if (nthreads <= 1) {
+      for (Int q = 0; q < nl; q++) one(q, drand, tree);
+    } else {
+#pragma omp parallel num_threads(nthreads)
+      {
+        Drand48 rg;
+        Tree tr;
+#pragma omp for schedule(dynamic, 1)
+        for (Int q = 0; q < nl; q++) one(q, rg, tr);
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -935,11 +1083,14 @@ int oracle_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
 // subhalo (parent < 0) in index order through RecursiveUnbind, then
 // TruncateSource on all.  nthreads>1 runs top-level nests in parallel
 // (schedule(dynamic,1), src/subhalo_unbind.cpp:479) with one drand48 stream
-// per thread; nthreads==1 matches a single-threaded reference run.
-int oracle_unbind_batch(const hbt_params *hp, const hbt_particles *parts, hbt_subhalo *subs,
-                        int64_t nsub, int64_t *out_index, float *out_energy, int nthreads) {
-  if (!hp || !parts || (nsub > 0 && !subs)) return HBT_ERR_INVALID_ARG;
-  if (hp->max_sample_size > 0) return HBT_ERR_UNSUPPORTED;
+// per thread; nthreads==1 matches a single-threaded reference run.  With
+// sampling on, subhalos go level by level (unbind_levels).
+// rand_seed / rand_draws: the rand() stream the shuffles draw from: srand(seed)
+// followed by *rand_draws calls; on return *rand_draws counts the calls made.
+int oracle_unbind_batch_rs(const hbt_params *hp, const hbt_particles *parts, hbt_subhalo *subs,
+                           int64_t nsub, int64_t *out_index, float *out_energy, int nthreads,
+                           uint32_t rand_seed, int64_t *rand_draws) {
+  if (!hp || !parts || (nsub > 0 && !subs) || !rand_draws || *rand_draws < 0) return HBT_ERR_INVALID_ARG;
   Params P(*hp);
   PartCols C{parts};
   std::vector<Sub> S(nsub);
@@ -954,7 +1105,23 @@ int oracle_unbind_batch(const hbt_params *hp, const hbt_particles *parts, hbt_su
     else roots.push_back(s);
   }
   Batch B{P, C, S, children};
-  if (nthreads <= 1) {
+  if (P.max_sample > 0) {
+    // levels: nest heights, subhalos in index order within each
+    std::vector<int> height(nsub, 0);
+    std::function<int(Int)> hgt = [&](Int s) {
+      int h = 0;
+      for (Int c : children[s]) h = std::max(h, hgt(c) + 1);
+      return height[s] = h;
+    };
+    int hmax = 0;
+    for (Int s : roots) hmax = std::max(hmax, hgt(s));
+    std::vector<std::vector<Int>> levels(hmax + 1);
+    for (Int s = 0; s < nsub; s++) levels[height[s]].push_back(s);
+    GlibcRand rnd(rand_seed);
+    rnd.skip(*rand_draws);
+    unbind_levels(B, levels, nthreads, rnd);
+    *rand_draws = rnd.draws;
+  } else if (nthreads <= 1) {
     Drand48 rng;
     Tree tree;
     for (Int s : roots) recursive_unbind(B, s, rng, tree);
@@ -985,6 +1152,40 @@ int oracle_unbind_batch(const hbt_params *hp, const hbt_particles *parts, hbt_su
     if (out_energy)
       for (Int i = 0; i < r.list_len; i++)
         out_energy[r.out_offset + i] = i < (Int)X.energy.size() ? X.energy[i] : std::numeric_limits<float>::quiet_NaN();
+  }
+  return HBT_OK;
+}
+
+int oracle_unbind_batch(const hbt_params *hp, const hbt_particles *parts, hbt_subhalo *subs,
+                        int64_t nsub, int64_t *out_index, float *out_energy, int nthreads) {
+  int64_t draws = 0;
+  return oracle_unbind_batch_rs(hp, parts, subs, nsub, out_index, out_energy, nthreads, 1, &draws);
+}
+
+// n rand() values after srand(seed) and `skip` calls.
+int oracle_glibc_rand(uint32_t seed, int64_t skip, int64_t n, int32_t *out) {
+  if (skip < 0 || n < 0 || (n > 0 && !out)) return HBT_ERR_INVALID_ARG;
+  GlibcRand r(seed);
+  r.skip(skip);
+  for (int64_t i = 0; i < n; i++) out[i] = r.next();
+  return HBT_OK;
+}
+
+// std::random_shuffle of each index range [0, len[k]) in turn, drawing from
+// srand(seed) after `skip` calls: perm holds the ranges one after another.
+int oracle_random_shuffle(uint32_t seed, int64_t skip, int64_t nseg, const int64_t *len, int64_t *perm) {
+  if (skip < 0 || nseg < 0) return HBT_ERR_INVALID_ARG;
+  GlibcRand r(seed);
+  r.skip(skip);
+  int64_t o = 0;
+  for (int64_t k = 0; k < nseg; k++) {
+    std::vector<int64_t> a(len[k]);
+    std::iota(a.begin(), a.end(), 0);
+    std::vector<int32_t> d(std::max<int64_t>(len[k] - 1, 1));
+    for (int64_t i = 0; i + 1 < len[k]; i++) d[i] = r.next();
+    random_shuffle_with(a, d.data());
+    std::copy(a.begin(), a.end(), perm + o);
+    o += len[k];
   }
   return HBT_OK;
 }

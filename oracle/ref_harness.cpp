@@ -7,19 +7,25 @@
 // It pins the CPU restatement (oracle/hbt_oracle.cpp) and writes the golden
 // fixtures under tests/golden/.  No MPI call is made on this path.
 //
-// Usage:  hbt_ref_harness unbind <in.bin> <out.bin>
-//         hbt_ref_harness tree   <in.bin> <out.bin>
+// Usage:  hbt_ref_harness unbind|time|props|levels|tree <in.bin> <out.bin>
 // Binary formats are produced/consumed by oracle/refrun.py.
 //
 // `unbind` mirrors SubhaloSnapshot_t::RefineParticles (src/subhalo_unbind.cpp:478-515):
 // every top-level subhalo in index order through Subhalo_t::RecursiveUnbind,
 // then TruncateSource on every subhalo.  Run with OMP_NUM_THREADS=1.
+// `levels` unbinds the same subhalos level by level (by nest height, index
+// order within a level), each with the steps RecursiveUnbind takes for it
+// (src/subhalo_unbind.cpp:432-447: its nested subhalos' unbound tails
+// appended, orphans unbound on their own list): the order in which the
+// library and the oracle consume the process's rand() stream when sampling
+// (MaxSampleSizeOfPotentialEstimate > 0, src/subhalo_unbind.cpp:302).
 // `tree` builds GravityTree_t over the sources and calls EvaluatePotential.
 
 #include <omp.h>
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -57,7 +63,7 @@ void configure(const hbt_params &p, Snapshot_t &epoch) {
   HBTConfig.SourceSubRelaxFactor = p.source_sub_relax_factor;
   HBTConfig.TreeNodeOpenAngle = p.open_angle;
   HBTConfig.MaxSampleSizeOfPotentialEstimate = p.max_sample_size;
-  HBTConfig.RefineMostboundParticle = true;
+  HBTConfig.RefineMostboundParticle = p.refine_mostbound_particle != 0;
   // derived parameters exactly as Parameter_t::ParseConfigFile computes them
   HBTConfig.BoxHalf = HBTConfig.BoxSize / 2.;
   HBTConfig.TreeNodeResolution = HBTConfig.SofteningHalo * 0.1;
@@ -86,7 +92,8 @@ void wr(FILE *f, const T *p, size_t n) {
 
 // mode 0: unbind and write the outputs; 1: time the nest-parallel unbinding;
 // 2: as 0, then CalculateProfileProperties + CalculateShape of every subhalo
-// (src/subhalo_tracking.cpp:904-905), with hbt_properties in and out.
+// (src/subhalo_tracking.cpp:904-905), with hbt_properties in and out;
+// 3: as 0 with the subhalos unbound level by level.
 int run_unbind(FILE *in, FILE *out, int mode) {
   const bool timed = mode == 1;
   hbt_params p;
@@ -152,7 +159,43 @@ int run_unbind(FILE *in, FILE *out, int mode) {
     wr(out, &secs, 1);
     return 0;
   }
-  for (int64_t s : roots) subs[s].RecursiveUnbind(subs, epoch);
+  if (mode == 3) {
+    std::vector<int> height(nsub, -1);
+    std::vector<int64_t> stack;
+    int hmax = 0;
+    for (int64_t r : roots) {  // heights by a post-order walk
+      stack.push_back(r);
+      while (!stack.empty()) {
+        const int64_t s = stack.back();
+        bool ready = true;
+        for (HBTInt c : subs[s].NestedSubhalos)
+          if (height[c] < 0) { stack.push_back(c); ready = false; }
+        if (!ready) continue;
+        int h = 0;
+        for (HBTInt c : subs[s].NestedSubhalos) h = std::max(h, height[c] + 1);
+        height[s] = h;
+        hmax = std::max(hmax, h);
+        stack.pop_back();
+      }
+    }
+    for (int h = 0; h <= hmax; h++)
+      for (int64_t s = 0; s < nsub; s++) {
+        if (height[s] != h) continue;
+        Subhalo_t &S = subs[s];
+        const bool orphan = S.Nbound <= 1;
+        Subhalo_t::ParticleList_t own;
+        if (orphan) own = S.Particles;
+        for (HBTInt c : S.NestedSubhalos) {
+          const Subhalo_t &K = subs[c];
+          S.Particles.insert(S.Particles.end(), K.Particles.begin() + K.Nbound, K.Particles.end());
+        }
+        if (orphan) S.Particles.swap(own);
+        S.Unbind(epoch);
+        if (orphan) S.Particles.swap(own);
+      }
+  } else {
+    for (int64_t s : roots) subs[s].RecursiveUnbind(subs, epoch);
+  }
   for (size_t s = 0; s < props.size(); s++) {
     Subhalo_t &S = subs[s];
     hbt_properties &q = props[s];
@@ -246,7 +289,7 @@ int run_tree(FILE *in, FILE *out) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|props|tree in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|props|levels|tree in.bin out.bin\n", argv[0]); return 1; }
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
   if (!in || !out) { fprintf(stderr, "cannot open files\n"); return 1; }
@@ -254,6 +297,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "unbind")) rc = run_unbind(in, out, 0);
   else if (!strcmp(argv[1], "time")) rc = run_unbind(in, out, 1);
   else if (!strcmp(argv[1], "props")) rc = run_unbind(in, out, 2);
+  else if (!strcmp(argv[1], "levels")) rc = run_unbind(in, out, 3);
   else if (!strcmp(argv[1], "tree")) rc = run_tree(in, out);
   fclose(in);
   fclose(out);

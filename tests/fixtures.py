@@ -10,6 +10,8 @@ from hbtplus_amd.synth import Catalogue
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 UNBIND_CASES = ["c1_small", "c1_small_bmp05", "c2_small", "c3_small", "c4_small", "hubble", "edge",
                 "floor"]
+# sampling on (MaxSampleSizeOfPotentialEstimate > 0), the reference driven level by level
+SAMPLING_CASES = ["samp_c1", "samp_c2", "samp_c4", "samp_edge"]
 TREE_CASES = ["tree_c1", "tree_straddle", "tree_floor"]
 PROPS_CASES = ["c2_small", "c3_small", "c4_small", "hubble", "edge"]
 MASK_CASES = ["mask_a", "mask_b"]

@@ -14,6 +14,15 @@ the given targets.  props_<name>.npz: for the catalogue of the unbind fixture
 what CalculateProfileProperties + CalculateShape produced after unbinding.
 Runs are single-threaded with sampling off.
 
+samp_<name>.npz: as the unbind fixtures, with sampling on
+(MaxSampleSizeOfPotentialEstimate > 0: shuffled lists, sampled trees with
+scaled masses, RefineBindingEnergyOrder where RefineMostboundParticle is on),
+the reference's own Unbind driven level by level (harness mode "levels", the
+order in which the library draws from rand()) in a fresh process, so that
+its rand() stream starts at srand(1).
+
+    python tests/golden/make_golden.py --samp-only    # only the samp_ files
+
     python tests/golden/make_golden.py --props-only   # only the props_ files
     python tests/golden/make_golden.py --mask-only    # only the mask_ files
     python tests/golden/make_golden.py --merge-only   # only the merge_ files
@@ -52,7 +61,7 @@ from hbtplus_amd import synth  # noqa: E402
 from hbtplus_amd.abi import PROPS_DTYPE, SUB_DTYPE, make_params  # noqa: E402
 
 
-def edge_catalogue():
+def edge_catalogue(sampling_orphan=False):
     """Degenerate and special cases of Unbind/RecursiveUnbind: empty list,
     a single particle, fewer than MinNumPartOfSub, an all-unbound list, an
     orphan (Nbound<=1) with a nested child, a trapped subhalo (SinkTrackId set)
@@ -76,6 +85,11 @@ def edge_catalogue():
     # a trapped subhalo that converges (sink cleared) and a dead one revived
     groups.append(nfw(500, sig_scale=0.4)); parents.append(-1); nbprev.append(500); sink.append(77); death.append(-1)
     groups.append(nfw(500, sig_scale=0.4)); parents.append(-1); nbprev.append(500); sink.append(-1); death.append(3)
+    if sampling_orphan:
+        # an orphan with a long own list and a nested child
+        groups.append(nfw(300, centre=(2e4, 2e4, 2e4), sig_scale=0.3)); parents.append(-1); nbprev.append(1); sink.append(-1); death.append(-1)
+        groups.append(nfw(250, centre=(2e4 + 3, 2e4, 2e4), sig_scale=4.0)); parents.append(len(groups) - 2)
+        nbprev.append(250); sink.append(-1); death.append(-1)
     params = make_params(softening=0.5, open_angle=0.45, periodic=1, box_size=1e5, snapshot_index=9)
     cat = synth._assemble(rng, groups, 1e5, True, params, "edge cases", parent=np.array(parents))
     cat.subs["nbound_prev"] = nbprev
@@ -118,8 +132,37 @@ CASES = {
 }
 
 
-def save_unbind(name, cat):
-    subs, idx, en = oracle.ref_unbind_batch(cat)
+def sampling_edge_catalogue():
+    """The edge cases with sampling at 100 particles, plus an orphan whose own
+    list (300) is shuffled for its unbinding while its output list keeps the
+    original order, with a nested child whose tail it receives."""
+    cat = edge_catalogue(sampling_orphan=True)
+    cat.name = "edge cases with sampling at 100"
+    cat.params["max_sample_size"] = 100
+    cat.params["refine_mostbound_particle"] = 1
+    return cat
+
+
+def _sampled(cat, max_sample, refine, name):
+    cat.params["max_sample_size"] = max_sample
+    cat.params["refine_mostbound_particle"] = 1 if refine else 0
+    cat.name = name
+    return cat
+
+
+SAMPLING_CASES = {
+    "samp_c1": lambda: _sampled(synth.config1(n_nfw=4000, n_cont=200, bound_mass_precision=0.995), 1000, True,
+                                "C1 4.2k NFW, sampling 1000, refine"),
+    "samp_c2": lambda: _sampled(synth.config2(seed=36, nsub=40, hi=1500), 200, False,
+                                "C2 40 subhalos, sampling 200, no refine"),
+    "samp_c4": lambda: _sampled(synth.config4(n_host=8000, n_sat=6, sat_lo=200, sat_hi=1500), 1000, True,
+                                "C4 nested host, sampling 1000, refine"),
+    "samp_edge": sampling_edge_catalogue,
+}
+
+
+def save_unbind(name, cat, levels=False):
+    subs, idx, en = oracle.ref_unbind_batch(cat, levels=levels)
     np.savez_compressed(
         os.path.join(HERE, name + ".npz"),
         kind="unbind", name=cat.name, params=cat.params.view(np.uint8), subs_in=cat.subs.view(np.uint8),
@@ -253,12 +296,18 @@ def main():
         for name, kw in MERGE_CASES.items():
             save_merge(name, kw)
         return
+    if "--samp-only" in sys.argv:
+        for name, make in SAMPLING_CASES.items():
+            save_unbind(name, make(), levels=True)
+        return
     if "--props-only" in sys.argv:
         for name in PROPS_CASES:
             save_props(name)
         return
     for name, make in CASES.items():
         save_unbind(name, make())
+    for name, make in SAMPLING_CASES.items():
+        save_unbind(name, make(), levels=True)
     save_tree("tree_c1", synth.config1(n_nfw=3000, n_cont=600))
     save_tree("tree_straddle", straddle_catalogue())
     save_tree("tree_floor", floor_catalogue())

@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import (MASK_CASES, MERGE_CASES, NEST_CASES, PROPS_CASES, TREE_CASES, UNBIND_CASES, load_mask, load_merge,
-                      load_nest, load_props, load_tree, load_unbind)
+from fixtures import (MASK_CASES, MERGE_CASES, NEST_CASES, PROPS_CASES, SAMPLING_CASES, TREE_CASES, UNBIND_CASES,
+                      load_mask, load_merge, load_nest, load_props, load_tree, load_unbind)
 from hbtplus_amd.abi import HBT_STATUS_CONVERGED, PROPS_FIELDS
 from helpers import assert_exact, assert_merge, check_unbind_invariants
 
@@ -17,6 +17,27 @@ def test_oracle_matches_reference_unbinding(name):
     cat, ref = load_unbind(name)
     got = oracle.unbind_batch(cat)
     assert_exact(ref, got)
+
+
+@pytest.mark.parametrize("name", SAMPLING_CASES)
+def test_oracle_matches_reference_sampling(name):
+    """Sampling on: the shuffle's rand() draws, the sampled trees with their
+    scaled masses, the correction switch's sample bound and
+    RefineBindingEnergyOrder, against the reference's own Unbind driven level
+    by level from a fresh rand() stream; the same with four threads (each
+    level's draws are taken before it is unbound)."""
+    cat, ref = load_unbind(name)
+    assert cat.params["max_sample_size"][0] > 0
+    rs = {"seed": 1, "draws": 0}
+    got = oracle.unbind_batch(cat, rand_state=rs)
+    assert_exact(ref, got)
+    # one draw per element after the first of every unbound list longer than
+    # the sample (an orphan unbinds its own list, the others their extended one)
+    ms = int(cat.params["max_sample_size"][0])
+    rsub = ref[0]
+    ulen = np.where(cat.subs["nbound_prev"] <= 1, cat.subs["count"], rsub["list_len"])
+    assert rs["draws"] == int(np.sum(np.where(ulen > ms, ulen - 1, 0))) > 0
+    assert_exact(got, oracle.unbind_batch(cat, nthreads=4))
 
 
 @pytest.mark.parametrize("name", UNBIND_CASES)

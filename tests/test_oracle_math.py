@@ -284,3 +284,40 @@ def test_id_to_index_against_a_dictionary():
     where = {int(v): i for i, v in enumerate(table.tolist())}
     want = np.array([where.get(int(v), -1) if v >= 0 else -1 for v in q.tolist()], np.int64)
     assert np.array_equal(oracle.id_to_index(table, q), want)
+
+
+def test_glibc_rand_restated():
+    """The restated rand() equals glibc's own (srand, then rand calls) for
+    the unseeded default (srand(1)), seed 0 (taken as 1), a seed above 2^31
+    and after skipped calls."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    for seed, skip in [(1, 0), (0, 0), (12345, 777), (2**31 + 5, 100)]:
+        libc.srand(ctypes.c_uint(seed))
+        for _ in range(skip):
+            libc.rand()
+        want = np.array([libc.rand() for _ in range(3000)], np.int32)
+        assert np.array_equal(oracle.glibc_rand(3000, seed=seed, skip=skip), want), (seed, skip)
+
+
+def test_random_shuffle_restated(tmp_path):
+    """The restated std::random_shuffle equals libstdc++'s own, compiled here
+    with g++ (C++11, where it still exists), over ranges shuffled one after
+    another from one rand() stream."""
+    import subprocess
+    src = tmp_path / "shuf.cpp"
+    src.write_text(
+        "#include <algorithm>\n#include <cstdio>\n#include <cstdlib>\n#include <numeric>\n#include <vector>\n"
+        "int main(int c, char **v) {\n"
+        "  srand((unsigned)atol(v[1])); for (long i = 0; i < atol(v[2]); i++) rand();\n"
+        "  for (int k = 3; k < c; k++) { std::vector<long> a(atol(v[k])); std::iota(a.begin(), a.end(), 0L);\n"
+        "    std::random_shuffle(a.begin(), a.end()); for (long x : a) printf(\"%ld\\n\", x); }\n"
+        "  return 0; }\n")
+    exe = tmp_path / "shuf"
+    subprocess.run(["g++", "-std=c++11", "-O1", "-w", str(src), "-o", str(exe)], check=True)
+    lens = [1, 2, 7, 1001, 0, 5000, 64]
+    for seed, skip in [(1, 0), (9, 31)]:
+        out = subprocess.run([str(exe), str(seed), str(skip)] + [str(n) for n in lens], check=True,
+                             capture_output=True, text=True).stdout.split()
+        want = np.array(out, np.int64)
+        assert np.array_equal(oracle.random_shuffle(lens, seed=seed, skip=skip), want)
