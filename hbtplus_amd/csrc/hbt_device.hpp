@@ -21,6 +21,10 @@ enum : int {
   F_ORPHAN = 32,      // orphan: unbinds its own list only, output keeps the extended list
   F_FINAL = 64,       // converged or disrupted in this pass, outputs pending
   F_DEEP = 128,       // tree keys truncated at MAX_DEPTH above the resolution floor
+  F_SAMP = 256,       // this full pass's tree holds only the first MaxSampleSize entries
+  F_SHUF = 512,       // the unbound list was shuffled (longer than MaxSampleSize)
+  F_REFPEND = 1024,   // converged; RefineBindingEnergyOrder and the outputs are pending
+  F_REFINE = 2048,    // the refinement pass: energies go to Er, not E
 };
 
 // Unbinding parameters with the reference's derived values
@@ -28,13 +32,19 @@ enum : int {
 struct DevParams {
   float G, eps, theta2, res, box, box_half, bmp, relax, a, hz;
   int periodic, min_part, snap, replay, kahan;
+  int max_sample;     // MaxSampleSizeOfPotentialEstimate (0: no sampling)
+  int refine;         // RefineMostboundParticle, with sampling only
 };
 
 // What the host fixes per segment of a level (the rest of Seg starts at zero
 // and is set on the device).
 struct SegIn {
   int64_t ibase, out_off;
+  int64_t list_off;  // first entry of the own list in the list positions
+  int64_t nbprev;    // Nbound before the pass (RecursiveUnbind's orphan test)
   int sub, cap, count, wgroup;
+  int draw_off;      // first rand() value of the list's shuffle in the level's draws, -1: none
+  int shuf_len;      // the list length the host counted those draws for
 };
 
 // Per-subhalo state for one level of the batch.
@@ -42,6 +52,8 @@ struct Seg {
   int64_t ibase;       // first working slot (aligned to WAVE)
   int64_t out_off;     // caller's out_offset
   int sub;             // index into the caller's hbt_subhalo array
+  int64_t list_off;    // first entry of the own list in the list positions
+  int64_t nbprev;      // Nbound before the pass (orphan test)
   int cap;             // working capacity (own list + nested tails)
   int count;           // own source list length
   int len;             // working list length (own + appended tails)
@@ -59,6 +71,12 @@ struct Seg {
   int sfull;                  // straddle of the latest full pass, whose sources hold
                               // every later pass's sources and targets
   int wgroup;                 // lanes per traversal group in the walk
+  int draw_off;               // shuffle: first draw in the level's draws (-1: not shuffled)
+  int shuf_len;               // the unbound list length the host counted the draws for
+  int sall;                   // straddle over the whole unbound list (shuffled lists)
+  int swapq;                  // disruption of a shuffled list: first position of the old most-bound ID
+  float mf;                   // mass factor of the sampled tree (Nlast / MaxSampleSize)
+  long long old_mb_id;        // ID of the list's first particle before the shuffle
   float ref_pos[3], ref_vel[3], old_pos[3], old_vel[3];
   float rvd[3], dK;
   float mbound;

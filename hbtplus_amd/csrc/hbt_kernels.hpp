@@ -60,6 +60,7 @@ struct Work {
   float *px, *py, *pz, *vx, *vy, *vz, *pm;
   int *ptype;
   float *E;
+  float *Er;                 // energies of the refinement pass (RefineBindingEnergyOrder)
   int *el, *el2, *sp;
   u64 *k0, *k1;
   u32 *v0, *v1;
@@ -162,7 +163,13 @@ __global__ void __launch_bounds__(BLOCK) seg_expand_kernel(Seg *seg, const SegIn
     S.sub = a.sub;
     S.cap = a.cap;
     S.count = a.count;
+    S.list_off = a.list_off;
+    S.nbprev = a.nbprev;
     S.wgroup = a.wgroup;
+    S.draw_off = a.draw_off;
+    S.shuf_len = a.shuf_len;
+    S.mf = 1.f;
+    S.swapq = INT_MAX;
     seg[k] = S;
   }
 }
@@ -174,9 +181,8 @@ __global__ void __launch_bounds__(BLOCK) seg_expand_kernel(Seg *seg, const SegIn
 __global__ void build_lists_kernel(Work W, const int *child_ptr, const int *child_list) {
   for (int k = blockIdx.x; k < W.nseg; k += gridDim.x) {
     Seg &S = W.seg[k];
-    const hbt_subhalo &r = W.subs[S.sub];
     const int64_t b = S.ibase;
-    int pos = (int)r.count;
+    int pos = S.count;
     for (int c = child_ptr[k]; c < child_ptr[k + 1]; c++) {
       const hbt_subhalo &q = W.subs[child_list[c]];
       const int tail = (int)(q.list_len - q.nbound);
@@ -199,21 +205,28 @@ __global__ void __launch_bounds__(BLOCK) pack_particles_kernel(hbt_particles P, 
   }
 }
 
-// Column positions of the segments' own particles, and local copies of the
-// listed particles' columns (slot-parallel).
-__global__ void __launch_bounds__(BLOCK) gather_kernel(Work W) {
+// Column position of list entry l of a segment's own list.
+__device__ __forceinline__ int own_entry(const Work &W, const Seg &S, int l) {
+  const int64_t o = S.list_off + l;
+  return W.g_index ? (int)W.g_index[o] : (int)o;
+}
+
+// Column positions of the segments' own particles (phase bit 1), and local
+// copies of the listed particles' columns (phase bit 2), slot-parallel.  A
+// shuffled list is permuted between the two phases.
+__global__ void __launch_bounds__(BLOCK) gather_kernel(Work W, int phase) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0 || l >= W.seg[k].len) continue;
     int g;
-    if (l < W.seg[k].count) {
-      const int64_t o = W.subs[W.seg[k].sub].offset + l;
-      g = W.g_index ? (int)W.g_index[o] : (int)o;
+    if ((phase & 1) && l < W.seg[k].count) {
+      g = own_entry(W, W.seg[k], l);
       W.gidx[s] = g;
     } else {
       g = W.gidx[s];
     }
+    if (!(phase & 2)) continue;
     if (W.g_packed) {
       const float4 a = W.g_packed[2 * (int64_t)g], c = W.g_packed[2 * (int64_t)g + 1];
       W.px[s] = a.x; W.py[s] = a.y; W.pz[s] = a.z; W.pm[s] = a.w;
@@ -227,13 +240,15 @@ __global__ void __launch_bounds__(BLOCK) gather_kernel(Work W) {
   }
 }
 
-// Unbind entry (src/subhalo_unbind.cpp:263-309): early exits, initial Elist.
+// Unbind entry (src/subhalo_unbind.cpp:263-309): early exits, initial Elist;
+// the shuffle's sort count (steps 1..ulen-1) of a shuffled list.
 __global__ void seg_init_kernel(Work W, DevParams P) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.active[3] = 0;  // refinements pending in the level
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     hbt_subhalo &r = W.subs[S.sub];
-    const bool orphan = r.nbound_prev <= 1;
-    S.ulen = orphan ? (int)r.count : S.len;
+    const bool orphan = S.nbprev <= 1;
+    S.ulen = orphan ? S.count : S.len;
     S.flags = orphan ? F_ORPHAN : 0;
     S.iter = 0;
     S.evals = 0;
@@ -248,6 +263,127 @@ __global__ void seg_init_kernel(Work W, DevParams P) {
       S.nbound = S.ulen;
       S.nlast = S.ulen;
     }
+    // the host shuffles exactly the lists longer than the sample (:302), with
+    // draws counted for the same length (a list of another length would read
+    // other segments' draws: it is not shuffled, and parity tests show it)
+    if (S.draw_off >= 0 && S.ulen > P.max_sample && P.max_sample > 0 && S.ulen == S.shuf_len) S.flags |= F_SHUF;
+    else S.draw_off = -1;
+    W.ncount[NC_SRC * W.nseg + k] = S.draw_off >= 0 ? S.ulen - 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shuffle (std::random_shuffle at src/subhalo_unbind.cpp:302; libstdc++'s
+// form: step i = 1..n-1 swaps positions i and j_i = rand() % (i+1)), replayed
+// in parallel.  Before step i position i still holds element i, so the last
+// step that moves an element into position p from above (the largest i > p
+// with j_i = p) leaves element i there for good.  A position that no later
+// step reaches keeps what step p left: element p if j_p = p, else what
+// position q = j_p held just before step p -- the element of the latest step
+// i in (q, p) with j_i = q, or, if none, what step q left there, and so on
+// down the chain p -> j_p -> ...  Sorting the pairs (j_i, i) groups each
+// position's movers in step order; pre = j_i, nlo = the previous mover into
+// j_i (-1: none), nhi = the last mover into a position (-1: none), npos =
+// the list before the shuffle.
+
+// Keys (j_i << 32 | i) of steps i = 1..ulen-1 at sort positions i-1.
+__global__ void __launch_bounds__(BLOCK) shuffle_keys_kernel(Work W, const int *__restrict__ draws) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    Seg &S = W.seg[k];
+    if (S.draw_off < 0 || l >= S.ulen) continue;
+    const int64_t b = S.ibase;
+    W.npos[s] = W.gidx[s];
+    W.nlo[s] = -1;
+    W.nhi[s] = -1;
+    if (l == 0) {
+      W.pre[s] = 0;
+      S.old_mb_id = W.g_id[W.gidx[s]];  // OldMostboundParticle is taken before the shuffle (:297)
+      continue;
+    }
+    const int j = (int)((unsigned)draws[S.draw_off + l - 1] % (unsigned)(l + 1));
+    W.pre[s] = j;
+    W.k0[b + l - 1] = ((u64)(unsigned)j << 32) | (u64)(unsigned)l;
+    W.v0[b + l - 1] = 0u;
+  }
+}
+
+// Previous mover into the same position, and the last mover into each position.
+__global__ void __launch_bounds__(BLOCK) shuffle_links_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int t = seg_lane_local(W, s, k);
+    if (t < 0) continue;
+    const Seg &S = W.seg[k];
+    const int n = S.ulen - 1;  // sorted steps
+    if (S.draw_off < 0 || t >= n) continue;
+    const int64_t b = S.ibase;
+    const u64 e = W.k0[s];
+    const int q = (int)(e >> 32), i = (int)(e & 0xffffffffu);
+    if (i == q) continue;  // a step that swaps a position with itself moves nothing
+    int prev = -1;
+    if (t > 0) {
+      const u64 f = W.k0[s - 1];
+      if ((int)(f >> 32) == q && (int)(f & 0xffffffffu) > q) prev = (int)(f & 0xffffffffu);
+    }
+    W.nlo[b + i] = prev;
+    if (t + 1 == n || (int)(W.k0[s + 1] >> 32) != q) W.nhi[b + q] = i;
+  }
+}
+
+// The shuffled list: position p takes the element the chain above names.
+__global__ void __launch_bounds__(BLOCK) shuffle_apply_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int p = seg_lane_local(W, s, k);
+    if (p < 0) continue;
+    const Seg &S = W.seg[k];
+    if (S.draw_off < 0 || p >= S.ulen) continue;
+    const int64_t b = S.ibase;
+    int src = W.nhi[s];
+    if (src < 0) {
+      int cur = p;  // strictly decreases along the chain
+      while (true) {
+        if (cur == 0) { src = 0; break; }
+        const int q = W.pre[b + cur];
+        if (q == cur) { src = cur; break; }
+        const int w = W.nlo[b + cur];
+        if (w >= 0) { src = w; break; }
+        cur = q;
+      }
+    }
+    W.gidx[s] = W.npos[b + src];
+  }
+}
+
+// Spatial target order of a shuffled list (its sampled trees hold only the
+// first MaxSampleSize entries): a sort of every particle's octant digits in
+// the cube of the whole list.  Setup: the whole unbound list as the sources.
+__global__ void prelude_setup_kernel(Work W) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    const bool on = S.draw_off >= 0 && (S.flags & F_ACTIVE);
+    S.n_src = on ? S.ulen : 0;
+    S.src_lo = 0;
+    S.n_tgt = 0;
+    for (int j = 0; j < 3; j++) { S.bmin[j] = 0xffffffffu; S.bmax[j] = 0u; }
+    W.ncount[NC_SRC * W.nseg + k] = S.n_src;
+  }
+}
+
+// After the digit sort: the order becomes the target list, and the whole
+// list's periodic straddle bounds every later pass of the segment.
+__global__ void __launch_bounds__(BLOCK) prelude_sp_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    Seg &S = W.seg[k];
+    if (S.draw_off < 0 || !(S.flags & F_ACTIVE) || l >= S.ulen) continue;
+    W.sp[s] = (int)W.v0[s];
+    if (l == 0) S.sall = S.straddle;
   }
 }
 
@@ -268,7 +404,8 @@ __global__ void pass_setup_kernel(Work W, DevParams P) {
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
     W.segact[k] = 0;
-    S.flags &= ~F_FINAL;
+    S.flags &= ~(F_FINAL | F_SAMP);
+    S.mf = 1.f;
     S.n_src = 0; S.n_tgt = 0; S.src_lo = 0; S.nsort = 0; S.sort_lo = 0; S.n_items = 0;
     S.K = 0; S.nP = 0; S.nG = 0;
     if (!(S.flags & F_ACTIVE)) continue;
@@ -287,6 +424,13 @@ __global__ void pass_setup_kernel(Work W, DevParams P) {
       S.src_lo = 0;
       S.n_src = S.nlast;
       S.n_tgt = S.nlast;
+      if (P.max_sample > 0 && S.nlast > P.max_sample) {
+        // sampled tree: the first MaxSampleSize entries with their masses
+        // scaled by Nlast / MaxSampleSize (float / long -> float, :334-339)
+        S.n_src = P.max_sample;
+        S.mf = (float)S.nlast / (float)P.max_sample;
+        S.flags |= F_SAMP;
+      }
     }
     S.evals += S.n_tgt;
     S.nsort = S.n_src;
@@ -376,7 +520,9 @@ __global__ void tree_params_kernel(Work W, DevParams P) {
     // hold both sets
     const int st = (P.periodic && Len > 0.5 * (double)P.box_half) ? 1 : 0;
     if (!(S.flags & F_CORR)) S.sfull = st;
-    S.straddle = st | S.sfull;
+    // a shuffled list's sampled sources may span less than its targets: the
+    // straddle of the whole list (prelude) bounds every pass
+    S.straddle = st | S.sfull | S.sall;
   }
 }
 
@@ -470,7 +616,7 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Work W, int full_copy_sp) {
     const Seg &S = W.seg[k];
     const int n = S.n_src;
     if (l >= n) continue;
-    if (full_copy_sp && !(S.flags & F_CORR)) W.sp[s] = (int)W.v0[s];
+    if (full_copy_sp && !(S.flags & (F_CORR | F_SAMP | F_REFINE))) W.sp[s] = (int)W.v0[s];
     W.nlo[s] = -1;
     const u64 *key = W.k0 + S.ibase;
     const int v = (l + 1 < n) ? key_delta(key[l], key[l + 1], S.D) : -1;
@@ -505,7 +651,9 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
         const u32 bl = W.bits[s];
         const int pos = l + W.pre[s] + __popc(bl);
         const int li = (int)W.v0[s];
-        W.items[ib + pos].pm = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], W.pm[b + li]);
+        // a sampled tree's leaves carry mass * MassFactor (EnergySnapshot_t::GetMass, :79)
+        const float m = (S.flags & F_SAMP) ? W.pm[b + li] * S.mf : W.pm[b + li];
+        W.items[ib + pos].pm = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], m);
         W.items[ib + pos].ax = make_int2(0, pos + 1);
         if (l + 1 == S.n_src) {
           W.seg[k].n_items = pos + 1;
@@ -1066,6 +1214,16 @@ __device__ __forceinline__ double walk_binned(const Item *__restrict__ it, const
   }
 }
 
+// The target mass that removes a target's own softened self term: none in a
+// correction pass (its tree holds the removed block only); in a sampled pass
+// the scaled mass of the sampled entries and none for the rest (:344-349);
+// otherwise the particle's mass.  isb flags the sampled entries.
+__device__ __forceinline__ float target_mass(const Work &W, const Seg &S, int64_t b, int li) {
+  if (S.flags & F_CORR) return 0.f;
+  if (S.flags & F_SAMP) return W.isb[b + li] ? W.pm[b + li] * S.mf : 0.f;
+  return W.pm[b + li];
+}
+
 // A target's binding energy from its potential.
 // Full pass: E = 0.5|dv|^2 + phi (BindingEnergy, src/subhalo_unbind.cpp:341-354).
 // Correction pass: E += OldVel.RefVelDiff + dK - phi_removed (:319-328).
@@ -1078,11 +1236,13 @@ __device__ __forceinline__ void finish_target(const Work &W, const DevParams &P,
   if (!(S.flags & F_CORR)) {
     float dv[3];
     rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
+    // the refinement pass keeps the unbinding energies (:252-259)
+    float *E = (S.flags & F_REFINE) ? W.Er : W.E;
     if (REPLAY) {
-      W.E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
+      E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
     } else {
       const double kin = 0.5 * ((double)dv[0] * dv[0] + (double)dv[1] * dv[1] + (double)dv[2] * dv[2]);
-      W.E[b + li] = (float)(kin + pot);
+      E[b + li] = (float)(kin + pot);
     }
   } else {
     float ov[3];
@@ -1107,7 +1267,7 @@ __device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, co
   if (COUNT && live) nt++;
   const int li = live ? W.sp[s] : 0;
   const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
-  const float tm = (S.flags & F_CORR) ? 0.f : W.pm[b + li];
+  const float tm = target_mass(W, S, b, li);
   const Item *seg_it = it + 2 * b;  // item positions and skips are segment-local
   const int end = S.n_items;
   double pot;
@@ -1131,6 +1291,14 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
   if (COUNT) flush_counts(W, ni, no, nt, ns);
 }
 
+// Instrumented passes: the items of the trees the walk read (counters[5]).
+__global__ void tree_items_kernel(Work W) {
+  SEG_LOOP(k, W) {
+    const Seg &S = W.seg[k];
+    if (S.n_tgt > 0 && S.n_items > 0) atomicAdd(&W.counters[5], (unsigned long long)S.n_items);
+  }
+}
+
 // Binding energies of the pass's targets, in spatial order.
 // Full pass: E = 0.5|dv|^2 + phi (BindingEnergy, :341-354).
 // Correction pass: E += OldVel.RefVelDiff + dK - phi_removed (:319-328).
@@ -1150,14 +1318,15 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(Work W, DevParams P) {
     const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
     const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
     if (!(S.flags & F_CORR)) {
-      const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, W.pm[b + li], ni, no);
+      const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, target_mass(W, S, b, li), ni, no);
       float dv[3];
       rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
+      float *E = (S.flags & F_REFINE) ? W.Er : W.E;
       if (P.replay) {
-        W.E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
+        E[b + li] = (float)((double)vnorm_f(dv) * 0.5 + pot);
       } else {
         const double kin = 0.5 * ((double)dv[0] * dv[0] + (double)dv[1] * dv[1] + (double)dv[2] * dv[2]);
-        W.E[b + li] = (float)(kin + pot);
+        E[b + li] = (float)(kin + pot);
       }
     } else {
       const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, 0.f, ni, no);
@@ -1273,6 +1442,47 @@ __global__ void partition_done_kernel(Work W, DevParams P) {
     }
     S.nsort = 0;  // the energy sort follows the frames
     W.ncount[NC_BOUND * W.nseg + k] = (S.flags & F_ACTIVE) ? S.nlast : 0;
+  }
+}
+
+// A shuffled list that is disrupted puts its old most-bound particle (the
+// first list entry before the shuffle) back in front: the first particle
+// with its ID swaps with position 0 (:363-370).  Find, then swap.
+__global__ void __launch_bounds__(BLOCK) disrupt_find_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_SHUF) || (S.flags & (F_DISR | F_FINAL)) != (F_DISR | F_FINAL) || l >= S.ulen) continue;
+    if (W.g_id[W.gidx[s]] == S.old_mb_id) atomicMin(&S.swapq, l);
+  }
+}
+
+__global__ void disrupt_swap_kernel(Work W) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_SHUF) || (S.flags & (F_DISR | F_FINAL)) != (F_DISR | F_FINAL)) continue;
+    const int q = S.swapq;
+    S.swapq = INT_MAX;
+    if (q == INT_MAX || q == 0) continue;
+    const int64_t a = S.ibase, c = S.ibase + q;
+    auto sw = [](auto *v, int64_t i, int64_t j) { const auto t = v[i]; v[i] = v[j]; v[j] = t; };
+    sw(W.gidx, a, c); sw(W.px, a, c); sw(W.py, a, c); sw(W.pz, a, c);
+    sw(W.vx, a, c); sw(W.vy, a, c); sw(W.vz, a, c); sw(W.pm, a, c); sw(W.ptype, a, c);
+  }
+}
+
+// Sampled pass: isb flags the sampled entries Elist[0, MaxSampleSize) (read
+// by the walk for the self term; the partition then reuses isb).
+__global__ void __launch_bounds__(BLOCK) sample_flags_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (!(S.flags & F_SAMP) || l >= S.nlast) continue;
+    W.isb[S.ibase + W.el[s]] = l < S.n_src ? 1 : 0;
   }
 }
 
@@ -1398,7 +1608,9 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   const int nb = S.nbound;
   const int l0 = W.el[b];
   const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
-  if (S.nlast - nb < nb) {  // correction switch uses the frame of the pass just done
+  // correction switch (uses the frame of the pass just done); with sampling
+  // only when fewer than MaxSampleSize were removed (:383-389)
+  if (S.nlast - nb < nb && (P.max_sample <= 0 || S.nlast - nb < P.max_sample)) {
     S.flags |= F_CORR;
     for (int j = 0; j < 3; j++) { S.old_pos[j] = S.ref_pos[j]; S.old_vel[j] = S.ref_vel[j]; }
   }
@@ -1420,7 +1632,14 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   // energy sort of the removed block Elist[nb, nlast) (:359-360) and, on
   // convergence, of the bound block Elist[0, nb) too (:401)
   if ((float)nb >= (float)S.nlast * P.bmp) {
-    S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_FINAL;
+    // RefineBindingEnergyOrder (:406-407) runs after the level's passes; the
+    // outputs wait for it
+    if (P.refine && nb > P.max_sample) {
+      S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_REFPEND;
+      atomicAdd(&W.active[3], 1);
+    } else {
+      S.flags = (S.flags & ~F_ACTIVE) | F_CONV | F_FINAL;
+    }
     S.sort_lo = 0;
     S.nsort = S.nlast;
   } else {
@@ -1546,8 +1765,11 @@ __global__ void __launch_bounds__(BLOCK) output_list_kernel(Work W, int early) {
       W.out_index[o] = W.gidx[S.ibase + li];
       W.out_energy[o] = l < S.ulen ? W.E[S.ibase + li] : __builtin_nanf("");
     } else {
-      // disrupted, early, orphan: the list keeps its order
-      W.out_index[o] = W.gidx[s];
+      // disrupted, early, orphan: the list keeps its order -- an orphan's
+      // own list as it came in, even when a shuffled copy of it was unbound
+      // (:434-446)
+      const bool orig = (S.flags & F_ORPHAN) && (S.flags & F_SHUF) && l < S.count;
+      W.out_index[o] = orig ? own_entry(W, S, l) : W.gidx[s];
       W.out_energy[o] = __builtin_nanf("");
     }
   }
@@ -1720,6 +1942,68 @@ __global__ void __launch_bounds__(BLOCK) output_combine_kernel(Work W, DevParams
     float mts[HBT_TYPE_MAX];
     for (int q = 0; q < HBT_TYPE_MAX; q++) { nt[q] = (int)v[6 + q]; mts[q] = (float)v[12 + q]; }
     write_record(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], nt, mts);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RefineBindingEnergyOrder (src/subhalo_unbind.cpp:234-262), for the segments
+// that converged with more bound particles than MaxSampleSize: a tree over the
+// first MaxSampleSize entries of the energy-sorted list (full masses), each of
+// them a target with its own mass (self term removed) in the final frame,
+// then those entries re-ordered by that energy; they keep their unbinding
+// energies.  Runs once per level after its passes.
+
+__global__ void refine_setup_kernel(Work W, DevParams P) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    const bool on = (S.flags & F_REFPEND) != 0;
+    S.flags &= ~(F_CORR | F_SAMP | F_FINAL);
+    if (on) S.flags |= F_REFINE;
+    S.mf = 1.f;
+    S.src_lo = 0;
+    S.n_src = on ? P.max_sample : 0;
+    S.n_tgt = S.n_src;
+    S.nsort = S.n_src;
+    S.sort_lo = 0;
+    S.n_items = 0;
+    for (int j = 0; j < 3; j++) { S.bmin[j] = 0xffffffffu; S.bmax[j] = 0u; }
+    W.ncount[NC_SRC * W.nseg + k] = S.n_src;
+    W.ncount[NC_SORT * W.nseg + k] = S.n_src;
+  }
+}
+
+// targets: the entries themselves (their order does not change any result)
+__global__ void __launch_bounds__(BLOCK) refine_targets_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (!(S.flags & F_REFINE) || l >= S.n_tgt) continue;
+    W.sp[s] = W.el[s];
+  }
+}
+
+// sort keys (refinement energy, position): ties keep the energy order
+__global__ void __launch_bounds__(BLOCK) refine_keys_kernel(Work W) {
+  SLOT_LOOP(s, W) {
+    int k;
+    int l = seg_lane_local(W, s, k);
+    if (l < 0) continue;
+    const Seg &S = W.seg[k];
+    if (!(S.flags & F_REFINE) || l >= S.nsort) continue;
+    const int li = W.el[s];
+    W.k0[s] = ((u64)f2ord(W.Er[S.ibase + li]) << 31) | (u64)(unsigned)l;
+    W.v0[s] = (u32)li;
+  }
+}
+
+__global__ void refine_done_kernel(Work W) {
+  SEG_LOOP(k, W) {
+    Seg &S = W.seg[k];
+    if (!(S.flags & F_REFINE)) continue;
+    S.flags = (S.flags & ~(F_REFINE | F_REFPEND)) | F_FINAL;
+    S.n_src = S.n_tgt = 0;
   }
 }
 

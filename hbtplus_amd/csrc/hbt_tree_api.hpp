@@ -49,7 +49,7 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
   hbt_subhalo h{};
   h.count = n_src;
   Level L;
-  build_level(&h, cap, children, std::vector<int>{0}, ctx->tune, L);
+  build_level(&h, cap, children, std::vector<int>{0}, ctx->tune, nullptr, 1, L);
   RETCHK(ensure_slots(ctx, L.nslots, 1, (int64_t)L.tiles.size()));
   RETCHK(stage_level(ctx, L));
   Runner R;

@@ -24,6 +24,8 @@
 
 using namespace hbt;
 
+#include "hbt_rand.hpp"
+
 namespace {
 
 thread_local std::string g_err;
@@ -63,6 +65,8 @@ DevParams make_dev_params(const hbt_params &p) {
   d.snap = p.snapshot_index;
   d.replay = p.precision == HBT_PREC_REPLAY;
   d.kahan = p.precision != HBT_PREC_FP32;
+  d.max_sample = p.max_sample_size > 0 ? p.max_sample_size : 0;
+  d.refine = d.max_sample > 0 && p.refine_mostbound_particle != 0;
   return d;
 }
 
@@ -131,6 +135,12 @@ struct hbt_ctx {
   const float4 *packed = nullptr;       // likewise, when the snapshot is gathered from packed records
   DBuf pack;                            // the packed snapshot
   DBuf idtab;                           // hbt_id_to_index's table
+  // the rand() stream of the sampling shuffles: srand(rand_seed), rand_pos calls made
+  uint32_t rand_seed = 1;
+  int64_t rand_pos = 0;
+  DBuf Er, draws, randtab;              // refinement energies; a level's draws; chunk jump matrices
+  DBuf kick, kickin, kickcnt;           // lists compacted by KickNullParticles, their ranges and counts
+  DView rwin;                           // the level's window of the stream (staged)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   int64_t last_evals = 0, last_passes = 0;
@@ -138,7 +148,8 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &cross, &cross2, &tcount, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
-                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack, &idtab})
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack, &idtab,
+                    &Er, &draws, &randtab, &kick, &kickin, &kickcnt})
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -168,15 +179,20 @@ struct Level {
   std::vector<Tile> tiles;
   int64_t nslots = 0;
   int maxcap = 0;
+  int64_t ndraws = 0;                 // rand() values the level's shuffles take
+  uint32_t win[31] = {};              // the stream's window before the first of them
 };
 
 int walk_group_for(const hbt_tuning &t, int64_t len) {
   return len < t.bin_edge[0] ? t.walk_group[0] : (len < t.bin_edge[1] ? t.walk_group[1] : t.walk_group[2]);
 }
 
+// draw_off: per subhalo, the first of its shuffle's draws in the level's
+// draws (-1: not shuffled) and the list length they were counted for
+// (draw_off[nsub + s]); null when nothing is shuffled.
 void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
                  const std::vector<std::vector<int>> &children, const std::vector<int> &members,
-                 const hbt_tuning &tune, Level &L) {
+                 const hbt_tuning &tune, const int64_t *draw_off, int64_t nsub, Level &L) {
   const int n = (int)members.size();
   // largest first: the merge levels of a sort then cover a prefix
   std::vector<std::pair<int64_t, int>> key(members.size());
@@ -199,7 +215,9 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
     const int s = order[k];
     const int c = (int)cap[s];
     const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
-    L.seg[k] = SegIn{off, subs[s].out_offset, s, c, (int)subs[s].count, walk_group_for(tune, c)};
+    L.seg[k] = SegIn{off, subs[s].out_offset, subs[s].offset, subs[s].nbound_prev, s, c, (int)subs[s].count,
+                     walk_group_for(tune, c), draw_off ? (int)draw_off[s] : -1,
+                     draw_off ? (int)draw_off[nsub + s] : 0};
     L.ibase[k] = off;
     L.capal[k] = ca;
     L.first_tile[k] = (int)L.tiles.size();
@@ -255,7 +273,7 @@ int stage_blobs(hbt_ctx *c, std::initializer_list<Blob> blobs, void **dev) {
 // table, first tile and tile count per segment, nest children (CSR).
 int stage_level(hbt_ctx *c, const Level &L) {
   const size_t n = L.seg.size();
-  void *d[8];
+  void *d[9];
   RETCHK(c->seg.ensure(std::max<size_t>(n, 1) * sizeof(Seg)));
   RETCHK(stage_blobs(c, {{L.seg.data(), n * sizeof(SegIn)},
                          {L.ibase.data(), n * 8},
@@ -264,10 +282,12 @@ int stage_level(hbt_ctx *c, const Level &L) {
                          {L.first_tile.data(), n * 4},
                          {L.ntile.data(), n * 4},
                          {L.child_ptr.data(), L.child_ptr.size() * 4},
-                         {L.child_list.data(), L.child_list.size() * 4}},
+                         {L.child_list.data(), L.child_list.size() * 4},
+                         {L.win, sizeof L.win}},
                      d));
   c->segin.p = d[0]; c->ibase.p = d[1]; c->capal.p = d[2]; c->tiles.p = d[3];
   c->first_tile.p = d[4]; c->ntile.p = d[5]; c->child_ptr.p = d[6]; c->child_list.p = d[7];
+  c->rwin.p = d[8];
   return HBT_OK;
 }
 
@@ -375,6 +395,67 @@ __global__ void __launch_bounds__(BLOCK) eval_sum_kernel(const hbt_subhalo *subs
   unsigned long long tot;
   (void)block_exclusive_scan(v, &tot);
   if (threadIdx.x == 0) *out = tot;
+}
+
+// List entries that are not positions in [0, n): cnt[0] counts those >= n
+// (rejected), cnt[1] the negative ones (null particles: IDs the snapshot did
+// not resolve, dropped by KickNullParticles).
+__global__ void __launch_bounds__(BLOCK) list_check_kernel(const int64_t *idx, int64_t n_list, int64_t n, int *cnt) {
+  int big = 0, null = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_list; i += (int64_t)gridDim.x * BLOCK) {
+    big += idx[i] >= n ? 1 : 0;
+    null += idx[i] < 0 ? 1 : 0;
+  }
+  big = (int)wave_reduce_sum((unsigned)big);
+  null = (int)wave_reduce_sum((unsigned)null);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (big) atomicAdd(&cnt[0], big);
+    if (null) atomicAdd(&cnt[1], null);
+  }
+}
+
+// Subhalo_t::KickNullParticles (src/subhalo.cpp:446-477) on the device: a
+// list keeps its entries that are not null, in order, and the Nbound it
+// carries (nbound_prev, RecursiveUnbind's orphan test) counts the kept ones
+// among its first Nbound entries.  One workgroup per subhalo.
+struct KickIn {
+  int64_t off, cnt, front, dst;  // list range, entries counted by Nbound, first kept entry's place
+};
+
+__global__ void __launch_bounds__(BLOCK) kick_count_kernel(const int64_t *__restrict__ idx, const KickIn *in,
+                                                           int64_t nsub, int64_t *keep) {
+  for (int64_t s = blockIdx.x; s < nsub; s += gridDim.x) {
+    const KickIn k = in[s];
+    unsigned long long a = 0, f = 0;
+    for (int64_t i = threadIdx.x; i < k.cnt; i += BLOCK) {
+      const bool live = idx[k.off + i] >= 0;
+      a += live ? 1 : 0;
+      f += (live && i < k.front) ? 1 : 0;
+    }
+    unsigned long long ta, tf;
+    (void)block_exclusive_scan(a, &ta);
+    (void)block_exclusive_scan(f, &tf);
+    if (threadIdx.x == 0) {
+      keep[2 * s] = (int64_t)ta;
+      keep[2 * s + 1] = (int64_t)tf;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) kick_compact_kernel(const int64_t *__restrict__ idx, const KickIn *in,
+                                                             int64_t nsub, int64_t *out) {
+  for (int64_t s = blockIdx.x; s < nsub; s += gridDim.x) {
+    const KickIn k = in[s];
+    int64_t run = 0;
+    for (int64_t i0 = 0; i0 < k.cnt; i0 += BLOCK) {  // uniform across the block
+      const int64_t i = i0 + threadIdx.x;
+      const int64_t v = i < k.cnt ? idx[k.off + i] : -1;
+      unsigned long long tot;
+      const unsigned long long ex = block_exclusive_scan((unsigned long long)(v >= 0 ? 1 : 0), &tot);
+      if (v >= 0) out[k.dst + run + (int64_t)ex] = v;
+      run += (int64_t)tot;
+    }
+  }
 }
 
 struct Runner {
@@ -540,9 +621,11 @@ struct Runner {
 
   int pass() {
     pass_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    if (P.max_sample > 0) sample_flags_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(build_tree(true));
     if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evw0, c->stream));
     launch_walk(c->prof_mode == 2);
+    if (c->prof_mode == 2) tree_items_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
     if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evw1, c->stream));
     // partition
     RETCHK(seg_scan(PartLoad{W.ibase, W.el, W.E}, S64Store{W.ibase, W.scan64},
@@ -554,6 +637,10 @@ struct Runner {
     partition_move_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_done_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    if (P.max_sample > 0) {
+      disrupt_find_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+      disrupt_swap_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
+    }
     // spatial target list keeps the bound particles
     bound_flags_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(seg_scan(BoundLoad{W.ibase, W.sp, W.isb}, PreStore{W.ibase, W.pre},
@@ -576,6 +663,13 @@ struct Runner {
     energy_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(sort_pairs(count_row(NC_SORT)));
     energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    outputs();
+    HIPCHK(hipGetLastError());
+    return HBT_OK;
+  }
+
+  // The outputs of the record and its list for the segments flagged F_FINAL.
+  void outputs() {
     output_list_kernel<<<gs, BLOCK, 0, c->stream>>>(W, 0);
     if (P.replay) {
       output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
@@ -583,6 +677,55 @@ struct Runner {
       output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
       output_combine_kernel<<<gwseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
     }
+  }
+
+  // RefineBindingEnergyOrder for the segments that converged with more than
+  // MaxSampleSize bound particles, then their outputs (once per level, after
+  // its passes; those segments finished in earlier passes, so every slot and
+  // tile is listed again).
+  int refine() {
+    list_all();
+    max_n = std::min(L->maxcap, P.max_sample);
+    refine_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    refine_targets_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    RETCHK(build_tree(false));
+    launch_walk(false);
+    refine_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    RETCHK(sort_pairs(count_row(NC_SORT)));
+    energy_writeback_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    refine_done_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
+    outputs();
+    HIPCHK(hipGetLastError());
+    return HBT_OK;
+  }
+
+  // The level's shuffles (std::random_shuffle, src/subhalo_unbind.cpp:302):
+  // the level's rand() values, then the parallel replay of each list's swaps
+  // on its column positions.  The columns are gathered afterwards.
+  int shuffle() {
+    if (L->ndraws <= 0) return HBT_OK;
+    RETCHK(c->draws.ensure((size_t)L->ndraws * 4));
+    const int64_t nchunk = (L->ndraws + RCHUNK - 1) / RCHUNK;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nchunk * WAVE + BLOCK - 1) / BLOCK, (int64_t)c->cus * 16));
+    rand_gen_kernel<<<g, BLOCK, 0, c->stream>>>(c->randtab.as<uint32_t>(), c->rwin.as<uint32_t>(), L->ndraws,
+                                                  c->draws.as<int>());
+    shuffle_keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W, c->draws.as<int>());
+    RETCHK(sort_pairs(count_row(NC_SRC)));
+    shuffle_links_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    shuffle_apply_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    HIPCHK(hipGetLastError());
+    return HBT_OK;
+  }
+
+  // Spatial target order of the shuffled lists: their octant digits sorted.
+  int prelude() {
+    if (L->ndraws <= 0) return HBT_OK;
+    prelude_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
+    bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
+    keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    RETCHK(sort_pairs(count_row(NC_SRC)));
+    prelude_sp_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     HIPCHK(hipGetLastError());
     return HBT_OK;
   }
@@ -596,6 +739,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->pm.ensure(s * 4));
   RETCHK(c->ptype.ensure(s * 4));
   RETCHK(c->E.ensure(s * 4));
+  RETCHK(c->Er.ensure(s * 4));
   RETCHK(c->el.ensure(s * 4));
   RETCHK(c->el2.ensure(s * 4));
   RETCHK(c->sp.ensure(s * 4));
@@ -655,6 +799,7 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.pm = c->pm.as<float>();
   W.ptype = c->ptype.as<int>();
   W.E = c->E.as<float>();
+  W.Er = c->Er.as<float>();
   W.el = c->el.as<int>(); W.el2 = c->el2.as<int>(); W.sp = c->sp.as<int>();
   W.k0 = c->k0.as<u64>(); W.k1 = c->k1.as<u64>(); W.v0 = c->v0.as<u32>(); W.v1 = c->v1.as<u32>();
   W.dl = c->dl.as<signed char>();
@@ -674,6 +819,17 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   W.out_index = out_index;
   W.out_energy = out_energy;
   return W;
+}
+
+// After a level's passes: the refinements that wait for them.
+int finish_level(hbt_ctx *c, Runner &R) {
+  if (!R.P.refine) return HBT_OK;
+  HIPCHK(hipMemcpyAsync(c->h_active + 3, R.W.active + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->h_active[3] == 0) return HBT_OK;
+  RETCHK(R.refine());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return HBT_OK;
 }
 
 // Runs one level: builds the working lists, iterates passes until every
@@ -699,9 +855,17 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   seg_expand_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W.seg, c->segin.as<SegIn>(), (int)nseg);
   seg_of_slot_kernel<<<R.gs, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, L.nslots, c->seg_of_slot.as<int>());
   build_lists_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, c->child_ptr.as<int>(), c->child_list.as<int>());
-  gather_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
   seg_init_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P);
+  if (L.ndraws > 0) {
+    // shuffled lists: positions, the shuffle, then the columns in shuffled order
+    gather_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 1);
+    RETCHK(R.shuffle());
+    gather_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 2);
+  } else {
+    gather_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 3);
+  }
   el_init_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W);
+  RETCHK(R.prelude());
   output_list_kernel<<<R.gs, BLOCK, 0, c->stream>>>(R.W, 1);
   output_record_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P, 1);
   HIPCHK(hipGetLastError());
@@ -753,7 +917,8 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
       if (*c->h_active == 0) break;
     }
     release();
-    return HBT_OK;
+    if (*c->h_active != 0) return fail(HBT_ERR_DEVICE, "unbinding passes did not converge within the pass cap");
+    return finish_level(c, R);
   }
   // HBT_PASS_TIMES=1: per pass, the GPU time between its first and last
   // launch and the host time, printed to stderr (diagnostics)
@@ -792,7 +957,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
       for (size_t i = 0; i < el.size(); i++) fprintf(stderr, " %d", el[i]);
       fprintf(stderr, "\n");
     }
-    HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_active, R.W.active, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (pass_times) {
       float ms = 0;
@@ -816,7 +981,10 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   }
   if (pe0) (void)hipEventDestroy(pe0);
   if (pe1) (void)hipEventDestroy(pe1);
-  return HBT_OK;
+  // every pass that does not converge removes a particle, so with a valid
+  // BoundMassPrecision the loop ends before the cap
+  if (*c->h_active != 0) return fail(HBT_ERR_DEVICE, "unbinding passes did not converge within the pass cap");
+  return finish_level(c, R);
 }
 
 // Nest structure on the host: capacities, heights, children in index order.
@@ -877,10 +1045,21 @@ int hbt_version(int *major, int *minor) {
   return HBT_OK;
 }
 
+// Parameters the unbinding cannot run with: a BoundMassPrecision outside
+// (0, 1] never converges (or converges at once), a negative sample size.
+static int check_params(const hbt_params &p) {
+  if (!(p.bound_mass_precision > 0.f && p.bound_mass_precision <= 1.f))
+    return fail(HBT_ERR_INVALID_ARG, "bound_mass_precision must lie in (0, 1]");
+  if (p.max_sample_size < 0) return fail(HBT_ERR_INVALID_ARG, "max_sample_size must be >= 0");
+  if (!(p.softening > 0.f)) return fail(HBT_ERR_INVALID_ARG, "softening must be > 0");
+  if (p.precision != HBT_PREC_FAST && p.precision != HBT_PREC_REPLAY && p.precision != HBT_PREC_FP32)
+    return fail(HBT_ERR_INVALID_ARG, "unknown precision");
+  return HBT_OK;
+}
+
 int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   if (!params || !out) return fail(HBT_ERR_INVALID_ARG, "null argument");
-  if (params->max_sample_size > 0)
-    return fail(HBT_ERR_UNSUPPORTED, "MaxSampleSizeOfPotentialEstimate > 0 is not supported");
+  RETCHK(check_params(*params));
   hbt_ctx *c = new (std::nothrow) hbt_ctx();
   if (!c) return fail(HBT_ERR_OUT_OF_MEMORY, "context allocation");
   c->device = device;
@@ -925,9 +1104,22 @@ int hbt_ctx_destroy(hbt_ctx *ctx) {
 
 int hbt_ctx_set_params(hbt_ctx *ctx, const hbt_params *params) {
   if (!ctx || !params) return fail(HBT_ERR_INVALID_ARG, "null argument");
-  if (params->max_sample_size > 0)
-    return fail(HBT_ERR_UNSUPPORTED, "MaxSampleSizeOfPotentialEstimate > 0 is not supported");
+  RETCHK(check_params(*params));
   ctx->params = *params;
+  return HBT_OK;
+}
+
+int hbt_set_rand_state(hbt_ctx *ctx, uint32_t seed, int64_t draws) {
+  if (!ctx || draws < 0) return fail(HBT_ERR_INVALID_ARG, "bad argument");
+  ctx->rand_seed = seed;
+  ctx->rand_pos = draws;
+  return HBT_OK;
+}
+
+int hbt_get_rand_state(hbt_ctx *ctx, uint32_t *seed, int64_t *draws) {
+  if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  if (seed) *seed = ctx->rand_seed;
+  if (draws) *draws = ctx->rand_pos;
   return HBT_OK;
 }
 
@@ -954,7 +1146,7 @@ int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
 
 static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64_t n_list,
                              hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
-                             int64_t *out_index, float *out_energy, int plain);
+                             int64_t *out_index, float *out_energy, int plain, const int64_t *out_cap);
 
 int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs,
                      const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
@@ -962,7 +1154,7 @@ int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *
   if (!ctx || !particles) return fail(HBT_ERR_INVALID_ARG, "null argument");
   ctx->list_index = nullptr;
   return unbind_batch_impl(ctx, particles, particles->n, subs, subs_host, nsub, out_index, out_energy,
-                           plain);
+                           plain, nullptr);
 }
 
 static bool tuning_valid(const hbt_tuning &t) {
@@ -998,12 +1190,16 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
   const hbt_tuning start = ctx->tune;
   hbt_tuning best = start;
   double best_ms = 0;
+  // every run draws its shuffles from the same point of the rand() stream;
+  // the stream is left as after one run
+  const int64_t pos0 = ctx->rand_pos;
   // device time of the batch under t, best of reps; t replaces best when it is
   // faster by more than 1% (earlier, simpler settings win ties)
   auto trial = [&](const hbt_tuning &t, bool first) -> int {
     ctx->tune = t;
     double ms = 1e300;
     for (int r = 0; r < reps; r++) {
+      ctx->rand_pos = pos0;
       const int rc = hbt_unbind_batch(ctx, particles, subs, subs_host, nsub, out_index, out_energy, 0);
       if (rc != HBT_OK) return rc;
       ms = std::min(ms, ctx->last_ms);
@@ -1077,16 +1273,65 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
     return fail(HBT_ERR_INVALID_ARG, "null argument");
   if (n_list >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 list entries");
   HIPCHK(hipSetDevice(ctx->device));
+  if (nsub > 0 && !subs_host) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  int cnt[2] = {0, 0};
   if (n_list > 0) {
     RETCHK(ctx->active.ensure(16));
-    int zero = 0;
-    HIPCHK(hipMemcpyAsync(ctx->active.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    check_index_kernel<<<grid_for(n_list, ctx->cus), BLOCK, 0, ctx->stream>>>(list_index, n_list, snapshot->n,
-                                                                               ctx->active.as<int>());
-    int bad = 0;
-    HIPCHK(hipMemcpyAsync(&bad, ctx->active.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->active.p, cnt, sizeof cnt, hipMemcpyHostToDevice, ctx->stream));
+    list_check_kernel<<<grid_for(n_list, ctx->cus), BLOCK, 0, ctx->stream>>>(list_index, n_list, snapshot->n,
+                                                                              ctx->active.as<int>());
+    HIPCHK(hipMemcpyAsync(cnt, ctx->active.p, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (bad) return fail(HBT_ERR_INVALID_ARG, std::to_string(bad) + " list entries are not snapshot positions");
+    if (cnt[0]) return fail(HBT_ERR_INVALID_ARG, std::to_string(cnt[0]) + " list entries lie past the snapshot");
+  }
+  // null entries: the lists are compacted first (KickNullParticles); the
+  // records keep their input fields, the unbinding sees the compacted lists
+  // and the adjusted Nbound, and output slots the shorter lists leave free
+  // are set to -1 / NaN
+  std::vector<hbt_subhalo> kicked;
+  std::vector<int64_t> out_cap;
+  if (cnt[1] > 0) {
+    for (int64_t s = 0; s < nsub; s++)
+      if (subs_host[s].offset < 0 || subs_host[s].count < 0 || subs_host[s].offset + subs_host[s].count > n_list)
+        return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the list positions");
+    {
+      std::vector<int64_t> cap;
+      std::vector<int> height, order;
+      std::vector<std::vector<int>> children;
+      RETCHK(plan_nests(subs_host, nsub, cap, height, children, order));
+      out_cap = cap;
+    }
+    std::vector<KickIn> kin(nsub);
+    for (int64_t s = 0; s < nsub; s++)
+      kin[s] = KickIn{subs_host[s].offset, subs_host[s].count,
+                      std::max<int64_t>(0, std::min(subs_host[s].nbound_prev, subs_host[s].count)), 0};
+    RETCHK(ctx->kickin.ensure(nsub * sizeof(KickIn)));
+    RETCHK(ctx->kickcnt.ensure(nsub * 2 * sizeof(int64_t)));
+    HIPCHK(hipMemcpyAsync(ctx->kickin.p, kin.data(), nsub * sizeof(KickIn), hipMemcpyHostToDevice, ctx->stream));
+    const int gk = (int)std::max<int64_t>(1, std::min<int64_t>(nsub, (int64_t)ctx->cus * 16));
+    kick_count_kernel<<<gk, BLOCK, 0, ctx->stream>>>(list_index, ctx->kickin.as<KickIn>(), nsub,
+                                                     ctx->kickcnt.as<int64_t>());
+    std::vector<int64_t> keep(2 * nsub);
+    HIPCHK(hipMemcpyAsync(keep.data(), ctx->kickcnt.p, keep.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    kicked.assign(subs_host, subs_host + nsub);
+    int64_t dst = 0;
+    for (int64_t s = 0; s < nsub; s++) {
+      kin[s].dst = dst;
+      kicked[s].offset = dst;
+      kicked[s].count = keep[2 * s];
+      kicked[s].nbound_prev = keep[2 * s + 1];
+      dst += keep[2 * s];
+    }
+    RETCHK(ctx->kick.ensure((size_t)std::max<int64_t>(dst, 1) * 8));
+    HIPCHK(hipMemcpyAsync(ctx->kickin.p, kin.data(), nsub * sizeof(KickIn), hipMemcpyHostToDevice, ctx->stream));
+    kick_compact_kernel<<<gk, BLOCK, 0, ctx->stream>>>(list_index, ctx->kickin.as<KickIn>(), nsub,
+                                                       ctx->kick.as<int64_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    list_index = ctx->kick.as<int64_t>();
+    n_list = dst;
+    subs_host = kicked.data();
   }
   // lists that cover a good part of the snapshot are gathered from packed
   // records (one streaming pass over the snapshot, then one 32-byte read per
@@ -1099,15 +1344,18 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
     ctx->packed = ctx->pack.as<float4>();
   }
   ctx->list_index = list_index;
-  int rc = unbind_batch_impl(ctx, snapshot, n_list, subs, subs_host, nsub, out_index, out_energy, plain);
+  int rc = unbind_batch_impl(ctx, snapshot, n_list, subs, subs_host, nsub, out_index, out_energy, plain,
+                             out_cap.empty() ? nullptr : out_cap.data());
   ctx->list_index = nullptr;
   ctx->packed = nullptr;
   return rc;
 }
 
+// out_cap: the output capacity the caller planned per subhalo when it exceeds
+// what the lists now need (lists compacted by KickNullParticles), or null.
 static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64_t n_list,
                              hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
-                             int64_t *out_index, float *out_energy, int plain) {
+                             int64_t *out_index, float *out_energy, int plain, const int64_t *out_cap) {
   if (nsub > 0 && (!subs || !subs_host || !out_index || !out_energy))
     return fail(HBT_ERR_INVALID_ARG, "null argument");
   if (nsub == 0) return HBT_OK;
@@ -1127,38 +1375,98 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   }
   hbt_particles P = *particles;
   int64_t passes = 0;
+  const int64_t MS = ctx->params.max_sample_size > 0 ? ctx->params.max_sample_size : 0;
+  if (MS > 0 && !particles->id) return fail(HBT_ERR_INVALID_ARG, "sampling needs the particle ID column");
+  if (MS > 0) {
+    if (ctx->rand_pos > ((int64_t)1 << 46)) return fail(HBT_ERR_UNSUPPORTED, "rand() stream position above 2^46");
+    if (!ctx->randtab.p) {
+      const RandJumps &J = rand_jumps();
+      RETCHK(ctx->randtab.ensure(J.chunk.size() * 4));
+      HIPCHK(hipMemcpy(ctx->randtab.p, J.chunk.data(), J.chunk.size() * 4, hipMemcpyHostToDevice));
+    }
+  }
+  // levels: nest heights; within a level the subhalos in index order (the
+  // order in which sampled lists draw their shuffles from rand()); the
+  // batched form runs a level at once, the plain form one subhalo at a time
+  int hmax = 0;
+  for (int64_t s = 0; s < nsub; s++) hmax = std::max(hmax, height[s]);
+  std::vector<std::vector<int>> by_h(hmax + 1);
+  for (int64_t s = 0; s < nsub; s++) by_h[height[s]].push_back((int)s);
+  static thread_local std::vector<int64_t> draw_off;
+  static thread_local std::vector<hbt_subhalo> recs;
+  if (MS > 0) draw_off.assign(2 * nsub, -1);
+  // The shuffles of a level: a list longer than MaxSampleSize takes one
+  // rand() value per element after its first, in level order.  A nested
+  // host's list holds its children's unbound tails, known once their level ran.
+  auto plan_draws = [&](const std::vector<int> &members, Level &Lv) -> int {
+    Lv.ndraws = 0;
+    if (MS <= 0) return HBT_OK;
+    bool nested = false;
+    for (int s : members) nested = nested || !children[s].empty();
+    if (nested) {
+      recs.resize(nsub);
+      HIPCHK(hipMemcpyAsync(recs.data(), subs, nsub * sizeof(hbt_subhalo), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    for (int s : members) {
+      int64_t len = subs_host[s].count;
+      if (subs_host[s].nbound_prev > 1)
+        for (int ch : children[s]) len += recs[ch].list_len - recs[ch].nbound;
+      draw_off[s] = -1;
+      draw_off[nsub + s] = len;
+      if (len > MS) {
+        draw_off[s] = Lv.ndraws;
+        Lv.ndraws += len - 1;
+      }
+    }
+    if (Lv.ndraws >= (int64_t)RCHUNK << RJUMPS) return fail(HBT_ERR_UNSUPPORTED, "too many shuffle draws in one level");
+    rand_window(ctx->rand_seed, ctx->rand_pos, Lv.win);
+    ctx->rand_pos += Lv.ndraws;
+    return HBT_OK;
+  };
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  if (plain) {
-    for (int s : order) {
-      build_level(subs_host, cap, children, std::vector<int>{s}, ctx->tune, L);
-      RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
+  static const bool pass_times = getenv("HBT_PASS_TIMES") && atoi(getenv("HBT_PASS_TIMES"));
+  for (int h = 0; h <= hmax; h++) {
+    if (plain) {
+      for (int s : by_h[h]) {
+        const std::vector<int> one{s};
+        RETCHK(plan_draws(one, L));
+        const int64_t nd = L.ndraws;
+        uint32_t win[31];
+        memcpy(win, L.win, sizeof win);
+        build_level(subs_host, cap, children, one, ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
+        L.ndraws = nd;
+        memcpy(L.win, win, sizeof win);
+        RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
+      }
+      continue;
     }
-  } else {
-    int hmax = 0;
-    for (int64_t s = 0; s < nsub; s++) hmax = std::max(hmax, height[s]);
-    std::vector<std::vector<int>> by_h(hmax + 1);
+    const auto t0 = std::chrono::steady_clock::now();
+    RETCHK(plan_draws(by_h[h], L));
+    const int64_t nd = L.ndraws;
+    uint32_t win[31];
+    memcpy(win, L.win, sizeof win);
     // build_level lays each level out largest first
-    for (int s : order) by_h[height[s]].push_back(s);
-    static const bool pass_times = getenv("HBT_PASS_TIMES") && atoi(getenv("HBT_PASS_TIMES"));
-    for (int h = 0; h <= hmax; h++) {
-      const auto t0 = std::chrono::steady_clock::now();
-      build_level(subs_host, cap, children, by_h[h], ctx->tune, L);
-      if (pass_times)
-        fprintf(stderr, "[hbt] level %d of %zu segments built on the host in %.3f ms\n", h, L.seg.size(),
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-      RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
-    }
+    build_level(subs_host, cap, children, by_h[h], ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
+    L.ndraws = nd;
+    memcpy(L.win, win, sizeof win);
+    if (pass_times)
+      fprintf(stderr, "[hbt] level %d of %zu segments built on the host in %.3f ms\n", h, L.seg.size(),
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
   }
   static thread_local std::vector<int64_t> gap;
   gap.clear();
   int64_t gap_total = 0;
-  for (int64_t s = 0; s < nsub; s++)
-    if (!children[s].empty()) {
+  for (int64_t s = 0; s < nsub; s++) {
+    const int64_t room = out_cap ? out_cap[s] : cap[s];
+    if (!children[s].empty() || room > cap[s]) {
       gap.push_back(s);
-      gap.push_back(cap[s]);
+      gap.push_back(room);
       gap.push_back(gap_total);
-      gap_total += cap[s];
+      gap_total += room;
     }
+  }
   if (!gap.empty()) {
     RETCHK(ctx->gaps.ensure(gap.size() * sizeof(int64_t)));
     HIPCHK(hipMemcpyAsync(ctx->gaps.p, gap.data(), gap.size() * sizeof(int64_t), hipMemcpyHostToDevice,
@@ -1205,10 +1513,10 @@ int hbt_profile(hbt_ctx *ctx, int mode) {
 }
 
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int64_t *interactions,
-                     int64_t *opened, int64_t *targets, int64_t *lane_slots) {
+                     int64_t *opened, int64_t *targets, int64_t *lane_slots, int64_t *tree_items) {
   if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(ctx->device));
-  unsigned long long h[4] = {0, 0, 0, 0};
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
   if (ctx->counters.p) HIPCHK(hipMemcpy(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost));
   if (walk_ms) *walk_ms = ctx->walk_ms;
   if (walk_launches) *walk_launches = ctx->walk_launches;
@@ -1216,6 +1524,7 @@ int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int6
   if (opened) *opened = (int64_t)h[1];
   if (targets) *targets = (int64_t)h[2];
   if (lane_slots) *lane_slots = (int64_t)h[3];
+  if (tree_items) *tree_items = (int64_t)h[5];
   return HBT_OK;
 }
 

@@ -57,6 +57,18 @@ class GpuUnbinder:
         self.params = np.ascontiguousarray(params, PARAMS_DTYPE).copy()
         check(self.lib.hbt_ctx_set_params(self.ctx, self.params.ctypes.data), "hbt_ctx_set_params")
 
+    def set_rand_state(self, seed=1, draws=0):
+        """The rand() stream the sampling shuffles draw from: srand(seed) then
+        `draws` calls (a new context: seed 1, no calls)."""
+        check(self.lib.hbt_set_rand_state(self.ctx, ctypes.c_uint32(seed), ctypes.c_int64(draws)),
+              "hbt_set_rand_state")
+
+    def rand_state(self):
+        seed, draws = ctypes.c_uint32(), ctypes.c_int64()
+        check(self.lib.hbt_get_rand_state(self.ctx, ctypes.byref(seed), ctypes.byref(draws)),
+              "hbt_get_rand_state")
+        return {"seed": seed.value, "draws": draws.value}
+
     def _dev(self, a):
         return self.torch.from_numpy(np.ascontiguousarray(a)).to(f"cuda:{self.device}")
 
@@ -219,6 +231,16 @@ class GpuUnbinder:
                     total=total, subs0=self._dev(subs.view(np.uint8)),
                     ps=particles_struct(cols, ptr=lambda t: t.data_ptr()))
 
+    def subset_handle(self, h, sel):
+        """A handle over the records sel of h (same device columns and lists;
+        the output layout is planned for the subset alone)."""
+        subs = h["subs_host"][np.asarray(sel)].copy()
+        subs["parent"] = -1
+        total = plan_batch(subs)
+        out = dict(h)
+        out.update(subs_host=subs, total=total, subs0=self._dev(subs.view(np.uint8)))
+        return out
+
     def resolve_ids(self, h):
         """ID -> snapshot position for every list entry (hbt_id_to_index)."""
         t = self.torch
@@ -242,6 +264,27 @@ class GpuUnbinder:
               "hbt_unbind_batch_indexed")
         return subs_dev, out_i, out_e
 
+    def unbind_positions(self, cols, positions, subs, plain=False):
+        """hbt_unbind_batch_indexed on host inputs: snapshot columns, the lists
+        as snapshot positions (negative: null particles, which the call drops
+        as KickNullParticles does) and records whose offset/count index them.
+        Returns (records, out_index, out_energy) as numpy arrays."""
+        t = self.torch
+        c = {k: self._dev(np.ascontiguousarray(cols[k])) for k in PARTICLE_COLUMNS}
+        ps = particles_struct(c, ptr=lambda x: x.data_ptr())
+        li = self._dev(np.ascontiguousarray(positions, np.int64))
+        subs = np.ascontiguousarray(subs, SUB_DTYPE).copy()
+        total = plan_batch(subs)
+        sd = self._dev(subs.view(np.uint8))
+        out_i = t.empty(max(total, 1), dtype=t.int64, device=f"cuda:{self.device}")
+        out_e = t.empty(max(total, 1), dtype=t.float32, device=f"cuda:{self.device}")
+        t.cuda.synchronize(self.device)
+        check(self.lib.hbt_unbind_batch_indexed(self.ctx, ctypes.byref(ps), li.data_ptr(), len(positions),
+                                                sd.data_ptr(), subs.ctypes.data, len(subs), out_i.data_ptr(),
+                                                out_e.data_ptr(), 1 if plain else 0), "hbt_unbind_batch_indexed")
+        return (sd.cpu().numpy().view(SUB_DTYPE).copy(), out_i.cpu().numpy()[:total].copy(),
+                out_e.cpu().numpy()[:total].copy())
+
     def last_stats(self):
         ms = ctypes.c_double()
         ev = ctypes.c_int64()
@@ -255,11 +298,12 @@ class GpuUnbinder:
 
     def profile_read(self):
         ms = ctypes.c_double()
-        vals = [ctypes.c_int64() for _ in range(5)]
+        vals = [ctypes.c_int64() for _ in range(6)]
         check(self.lib.hbt_profile_read(self.ctx, ctypes.byref(ms), *[ctypes.byref(v) for v in vals]),
               "hbt_profile_read")
         return dict(walk_ms=ms.value, walk_launches=vals[0].value, interactions=vals[1].value,
-                    opened=vals[2].value, targets=vals[3].value, lane_slots=vals[4].value)
+                    opened=vals[2].value, targets=vals[3].value, lane_slots=vals[4].value,
+                    tree_items=vals[5].value)
 
     # per-subhalo form with host buffers --------------------------------------
     def unbind_one(self, cols: dict, sub: np.ndarray):

@@ -17,7 +17,8 @@
  *                                                        src/gravity_tree.h:15
  *   hbt_direct_potential  exact pair sum with the same softened kernel as
  *                       GravityTree_t::EvaluatePotential (src/gravity_tree.cpp:144-160);
- *                       used for small subhalos and as the precision-study path
+ *                       the FP64 check of the tree potentials and the precision study
+ *                       (the unbinding itself always walks the tree, as the reference does)
  *   hbt_id_to_index     ParticleSnapshot_t::GetIndices / MappedIndexTable_t::GetIndices
  *                                                        src/hash_remote.tpp:58-109,
  *                                                        src/hash.tpp:18-56
@@ -159,6 +160,18 @@ int hbt_ctx_set_params(hbt_ctx *ctx, const hbt_params *params);
 /* The HIP stream (hipStream_t) the context enqueues on. */
 int hbt_ctx_stream(hbt_ctx *ctx, void **stream);
 
+/* The rand() stream the sampling shuffles draw from (MaxSampleSizeOfPotentialEstimate
+ * > 0: std::random_shuffle at src/subhalo_unbind.cpp:302 on glibc's rand()):
+ * the stream of srand(seed) after `draws` calls.  A new context holds seed 1,
+ * no calls (a process that never calls srand); every unbinding call advances
+ * it by the values its shuffles take.  Within a call the subhalos draw level
+ * by level (a level: the subhalos whose nested subhalos are all done), in
+ * index order within a level -- the order of the reference's single-threaded
+ * RefineParticles loop for a batch without nesting.  No counterpart in the
+ * reference's interface (its stream is the process's global rand() state). */
+int hbt_set_rand_state(hbt_ctx *ctx, uint32_t seed, int64_t draws);
+int hbt_get_rand_state(hbt_ctx *ctx, uint32_t *seed, int64_t *draws);
+
 /* Output capacity needed by hbt_unbind_batch for a given nest layout:
  * for every subhalo, its own count plus the counts of everything nested
  * below it (unbound tails are appended to the parent, src/subhalo_unbind.cpp:442).
@@ -192,8 +205,16 @@ int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles,
  * src/particle_exchanger.h:193-208; on the device: hbt_id_to_index).
  *   snapshot  : device particle columns of the snapshot
  *   list_index: DEVICE int64[n_list]; subs[].offset/count are ranges of it and
- *               its entries are snapshot positions (negative or out-of-range
- *               positions are rejected with HBT_ERR_INVALID_ARG)
+ *               its entries are snapshot positions.  A negative entry is a null
+ *               particle (an ID the snapshot does not hold, -1 from
+ *               hbt_id_to_index): before unbinding, every list drops its null
+ *               entries keeping the others in order, and the Nbound the orphan
+ *               test reads becomes the number of kept entries among the first
+ *               nbound_prev (Subhalo_t::KickNullParticles, src/subhalo.cpp:446-477,
+ *               after the particle query, src/particle_exchanger.h:239-240).  The
+ *               records' input fields are not changed; the output slots that
+ *               the shorter lists leave free are set to -1 and NaN.  Positions
+ *               at or past snapshot->n are rejected with HBT_ERR_INVALID_ARG
  *   out_index : snapshot positions, layout as in hbt_unbind_batch
  * Other arguments as in hbt_unbind_batch.  When n_list is at least a quarter
  * of snapshot->n the context first interleaves the snapshot's columns into
@@ -391,14 +412,14 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
  * reference; used by the benchmark for the roofline).  mode 0: off;
  * 1: time every walk launch with HIP events on the context stream;
  * 2: count, in every walk launch, the summed sources (leaves and accepted
- * nodes), the opened nodes, the targets and the lane slots spent by the lane
+ * nodes), the opened nodes, the targets, the lane slots spent by the lane
  * groups (items a group visited times its lanes, whether or not a lane
- * needed the item).
+ * needed the item) and the items of the trees walked.
  * Resets the accumulators. */
 int hbt_profile(hbt_ctx *ctx, int mode);
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches,
                      int64_t *interactions, int64_t *opened, int64_t *targets,
-                     int64_t *lane_slots);
+                     int64_t *lane_slots, int64_t *tree_items);
 
 #ifdef __cplusplus
 }

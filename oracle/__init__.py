@@ -288,7 +288,29 @@ def ref_time_unbind(cat, nthreads, params=None):
         return float(np.fromfile(fout, np.float64, 1)[0])
 
 
-def ref_unbind_batch(cat, params=None, props_in=None, levels=False):
+def kick_null_particles(cat):
+    """Subhalo_t::KickNullParticles (src/subhalo.cpp:446-477) on every list of
+    a catalogue: entries whose ID is the null ID (-1) are dropped, the rest
+    keep their order, and the Nbound the orphan rule reads (nbound_prev) loses
+    the null entries among its first nbound_prev.  Returns (a catalogue with
+    the lists compacted, rows: for each kept entry its row in cat.cols)."""
+    from hbtplus_amd.synth import Catalogue
+    subs = cat.subs.copy()
+    cnt = subs["count"].astype(np.int64)
+    seg = np.repeat(np.arange(len(subs)), cnt)
+    pos = np.arange(len(seg)) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    rows = np.repeat(subs["offset"], cnt) + pos
+    keep = cat.cols["id"][rows] != -1
+    front = pos < np.repeat(np.minimum(subs["nbound_prev"], cnt), cnt)
+    subs["nbound_prev"] = np.bincount(seg[keep & front], minlength=len(subs))
+    subs["count"] = np.bincount(seg[keep], minlength=len(subs))
+    subs["offset"] = np.concatenate([[0], np.cumsum(subs["count"])[:-1]]) if len(subs) else subs["offset"]
+    kept = rows[keep]
+    cols = {k: v[kept] for k, v in cat.cols.items()}
+    return Catalogue(cols, subs, cat.params.copy(), cat.name + " (null particles kicked)"), kept
+
+
+def ref_unbind_batch(cat, params=None, props_in=None, levels=False, kick=False):
     """Run the reference's own RecursiveUnbind/Unbind/TruncateSource on the
     catalogue through oracle/_ref/hbt_ref_harness (single thread).  With
     props_in (an array of PROPS_DTYPE): also CalculateProfileProperties and
@@ -302,7 +324,7 @@ def ref_unbind_batch(cat, params=None, props_in=None, levels=False):
             with open(fin, "ab") as f:
                 f.write(np.ascontiguousarray(props_in, PROPS_DTYPE).tobytes())
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        mode = "levels" if levels else ("unbind" if props_in is None else "props")
+        mode = "kick" if kick else ("levels" if levels else ("unbind" if props_in is None else "props"))
         subprocess.run([REF_HARNESS, mode, fin, fout], check=True, env=env)
         raw = open(fout, "rb").read()
     nsub = int(np.frombuffer(raw, np.int64, 1)[0])

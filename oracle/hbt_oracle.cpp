@@ -1005,10 +1005,18 @@ int oracle_merge_subhalos(const hbt_params *hp, const hbt_particles *parts, cons
     for (auto &g : heads) merge_rec(g.second[0]);
     Drand48 rng;
     Tree tree;
+    // the merged hosts are unbound in index order (src/subhalo_merge.cpp:207-210);
+    // with sampling their shuffles draw from a fresh rand() stream in that order
+    GlibcRand rnd(1);
+    std::vector<int32_t> dr;
     for (Int s = 0; s < nsub; s++) {
       if (!H[s].merged) continue;
       subs[s].nbound_prev = INT64_MAX;
-      unbind(P, C, S[s], rng, tree);
+      const Int n = (Int)S[s].list.size();
+      dr.assign(std::max<Int>(n - 1, 1), 0);
+      if (P.max_sample > 0 && n > P.max_sample)
+        for (Int i = 0; i + 1 < n; i++) dr[i] = rnd.next();
+      unbind(P, C, S[s], rng, tree, dr.data());
       Int ns = S[s].nbound <= 1 ? S[s].nbound : (Int)((float)S[s].nbound * P.relax);
       if (ns > (Int)S[s].list.size()) ns = (Int)S[s].list.size();
       S[s].list.resize(ns);

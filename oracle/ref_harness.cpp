@@ -7,7 +7,7 @@
 // It pins the CPU restatement (oracle/hbt_oracle.cpp) and writes the golden
 // fixtures under tests/golden/.  No MPI call is made on this path.
 //
-// Usage:  hbt_ref_harness unbind|time|props|levels|tree <in.bin> <out.bin>
+// Usage:  hbt_ref_harness unbind|time|props|levels|kick|tree <in.bin> <out.bin>
 // Binary formats are produced/consumed by oracle/refrun.py.
 //
 // `unbind` mirrors SubhaloSnapshot_t::RefineParticles (src/subhalo_unbind.cpp:478-515):
@@ -19,6 +19,10 @@
 // appended, orphans unbound on their own list): the order in which the
 // library and the oracle consume the process's rand() stream when sampling
 // (MaxSampleSizeOfPotentialEstimate > 0, src/subhalo_unbind.cpp:302).
+// `kick` first drops null particles (Id -1, an ID the snapshot did not
+// resolve) from every list with Subhalo_t::KickNullParticles (src/subhalo.cpp:446-477,
+// as ParticleExchanger_t::RestoreParticles does, src/particle_exchanger.h:239-240),
+// the record's nbound_prev as the Nbound it adjusts, then runs `unbind`.
 // `tree` builds GravityTree_t over the sources and calls EvaluatePotential.
 
 #include <omp.h>
@@ -93,7 +97,8 @@ void wr(FILE *f, const T *p, size_t n) {
 // mode 0: unbind and write the outputs; 1: time the nest-parallel unbinding;
 // 2: as 0, then CalculateProfileProperties + CalculateShape of every subhalo
 // (src/subhalo_tracking.cpp:904-905), with hbt_properties in and out;
-// 3: as 0 with the subhalos unbound level by level.
+// 3: as 0 with the subhalos unbound level by level; 4: as 0 after
+// KickNullParticles on every subhalo.
 int run_unbind(FILE *in, FILE *out, int mode) {
   const bool timed = mode == 1;
   hbt_params p;
@@ -159,6 +164,8 @@ int run_unbind(FILE *in, FILE *out, int mode) {
     wr(out, &secs, 1);
     return 0;
   }
+  if (mode == 4)
+    for (int64_t s = 0; s < nsub; s++) subs[s].KickNullParticles();
   if (mode == 3) {
     std::vector<int> height(nsub, -1);
     std::vector<int64_t> stack;
@@ -289,7 +296,7 @@ int run_tree(FILE *in, FILE *out) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|props|levels|tree in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s unbind|time|props|levels|kick|tree in.bin out.bin\n", argv[0]); return 1; }
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
   if (!in || !out) { fprintf(stderr, "cannot open files\n"); return 1; }
@@ -298,6 +305,7 @@ int main(int argc, char **argv) {
   else if (!strcmp(argv[1], "time")) rc = run_unbind(in, out, 1);
   else if (!strcmp(argv[1], "props")) rc = run_unbind(in, out, 2);
   else if (!strcmp(argv[1], "levels")) rc = run_unbind(in, out, 3);
+  else if (!strcmp(argv[1], "kick")) rc = run_unbind(in, out, 4);
   else if (!strcmp(argv[1], "tree")) rc = run_tree(in, out);
   fclose(in);
   fclose(out);

@@ -23,6 +23,11 @@ its rand() stream starts at srand(1).
 
     python tests/golden/make_golden.py --samp-only    # only the samp_ files
 
+kick_c2.npz: lists holding null particles (ID -1) and what the reference's
+KickNullParticles followed by its unbinding produced (harness mode "kick").
+
+    python tests/golden/make_golden.py --kick-only    # only kick_c2
+
     python tests/golden/make_golden.py --props-only   # only the props_ files
     python tests/golden/make_golden.py --mask-only    # only the mask_ files
     python tests/golden/make_golden.py --merge-only   # only the merge_ files
@@ -161,8 +166,34 @@ SAMPLING_CASES = {
 }
 
 
-def save_unbind(name, cat, levels=False):
-    subs, idx, en = oracle.ref_unbind_batch(cat, levels=levels)
+def kick_catalogue():
+    """Lists holding null particles (ID -1: IDs the snapshot did not resolve),
+    as ParticleExchanger_t leaves them before KickNullParticles: 5% of the
+    entries at random, a subhalo whose first two entries are null with a
+    previous Nbound of 2 (it becomes an orphan), a subhalo with every entry
+    null, previous Nbound values anywhere in [0, count], and a host with two
+    nested subhalos."""
+    cat = synth.config2(seed=37, nsub=40, hi=1500)
+    rng = np.random.Generator(np.random.Philox(key=4646))
+    ids = cat.cols["id"].copy()
+    ids[rng.random(len(ids)) < 0.05] = -1
+    cnt = cat.subs["count"]
+    off = cat.subs["offset"]
+    nbp = rng.integers(0, cnt + 1)
+    nbp[0] = cnt[0]
+    ids[off[1]:off[1] + 2] = -1
+    nbp[1] = 2
+    ids[off[2]:off[2] + cnt[2]] = -1
+    cat.cols["id"] = ids
+    cat.subs["nbound_prev"] = nbp
+    cat.subs["parent"][[5, 6]] = 4
+    cat.subs["nbound_prev"][4] = cnt[4]
+    cat.name = "C2 40 subhalos with null particles"
+    return cat
+
+
+def save_unbind(name, cat, levels=False, kick=False):
+    subs, idx, en = oracle.ref_unbind_batch(cat, levels=levels, kick=kick)
     np.savez_compressed(
         os.path.join(HERE, name + ".npz"),
         kind="unbind", name=cat.name, params=cat.params.view(np.uint8), subs_in=cat.subs.view(np.uint8),
@@ -296,6 +327,9 @@ def main():
         for name, kw in MERGE_CASES.items():
             save_merge(name, kw)
         return
+    if "--kick-only" in sys.argv:
+        save_unbind("kick_c2", kick_catalogue(), kick=True)
+        return
     if "--samp-only" in sys.argv:
         for name, make in SAMPLING_CASES.items():
             save_unbind(name, make(), levels=True)
@@ -308,6 +342,7 @@ def main():
         save_unbind(name, make())
     for name, make in SAMPLING_CASES.items():
         save_unbind(name, make(), levels=True)
+    save_unbind("kick_c2", kick_catalogue(), kick=True)
     save_tree("tree_c1", synth.config1(n_nfw=3000, n_cont=600))
     save_tree("tree_straddle", straddle_catalogue())
     save_tree("tree_floor", floor_catalogue())

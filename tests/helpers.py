@@ -76,12 +76,16 @@ def assert_exact(ref, got, fields=OUTPUT_FIELDS):
         assert np.array_equal(a_i, b_i), f"particle list differs in subhalo {s}"
 
 
-def assert_tolerant(ref, got, band_rel=1e-5):
+def assert_tolerant(ref, got, band_rel=1e-5, refined=0):
     """FAST-mode parity: bound sets identical except particles whose oracle
     binding energy lies within band_rel*|E_min| of zero; bound counts differ
     by at most the number of such particles; most-bound IDs agree unless the
     two most bound oracle energies are within the band; the bound block is in
-    oracle-energy order up to the band."""
+    oracle-energy order up to the band.  refined > 0: a bound block longer
+    than that (RefineBindingEnergyOrder with MaxSampleSize = refined) has its
+    first `refined` entries re-ordered by another energy: they must be the
+    same set, the most-bound IDs must agree, and the order is checked after
+    them."""
     rs, ri, re_ = ref
     gs, gi, ge = got
     stats = dict(n=0, band=0, set_diff=0, mb_mismatch=0)
@@ -106,11 +110,16 @@ def assert_tolerant(ref, got, band_rel=1e-5):
         stats["set_diff"] += len(diff)
         assert diff <= near, f"subhalo {s}: {len(diff - near)} bound-set differences outside the band"
         assert abs(na - nb) <= len(near), f"subhalo {s}: Nbound {na} vs {nb}"
-        if a_i[0] != b_i[0]:
+        skip = refined if (refined and na > refined) else 0
+        if skip:
+            assert set(a_i[:skip].tolist()) ^ set(b_i[:skip].tolist()) <= near, \
+                f"subhalo {s}: refined prefix differs"
+            assert a_i[0] == b_i[0], f"subhalo {s}: most-bound ID after the refinement differs"
+        elif a_i[0] != b_i[0]:
             top = np.sort(a_e[:na])[:2]
             assert len(top) > 1 and top[1] - top[0] <= band, f"subhalo {s}: most-bound ID differs"
             stats["mb_mismatch"] += 1
-        eg = np.array([e_of.get(p, np.nan) for p in b_i[:nb].tolist()])
+        eg = np.array([e_of.get(p, np.nan) for p in b_i[skip:nb].tolist()])
         eg = eg[~np.isnan(eg)]
         if len(eg):
             run = np.maximum.accumulate(eg)
@@ -152,7 +161,7 @@ def assert_merge(ref, got, ties=False):
         assert np.array_equal(a, b), f"list of subhalo {s} differs"
 
 
-def check_unbind_invariants(cat, out, conv=None):
+def check_unbind_invariants(cat, out, conv=None, refined=0):
     """Properties of an unbinding result that hold at any size, checked with
     array operations (no per-subhalo loop): every output list is a
     permutation of its subhalo's source particles plus the unbound tails of
@@ -162,7 +171,9 @@ def check_unbind_invariants(cat, out, conv=None):
     first particle is the most-bound particle of the record; the per-type
     bound counts add up to the bound particles of each type; mbound is their
     mass sum.  conv: which subhalos converged (default: the records'
-    status).  Returns their number."""
+    status).  refined > 0: the first `refined` entries of a bound block
+    longer than that are re-ordered by RefineBindingEnergyOrder; the energy
+    order is checked from there on.  Returns their number."""
     subs, idx, en = out
     src = cat.subs
     n = len(subs)
@@ -191,6 +202,8 @@ def check_unbind_invariants(cat, out, conv=None):
     e = en[np.repeat(subs["out_offset"], L) + pos]
     assert np.all(e[bound] < 0), "a bound particle has E >= 0"
     step = bound[1:] & bound[:-1] & (seg[1:] == seg[:-1])
+    if refined:
+        step &= ~((nb[seg[1:]] > refined) & (pos[1:] < refined))
     assert np.all(e[1:][step] >= e[:-1][step]), "bound block is not in ascending energy"
     first = subs["out_offset"][conv]
     for k, c in enumerate("xyz"):

@@ -288,13 +288,15 @@ def test_indexed_batch_matches_direct(gpu):
     assert np.array_equal(e1.cpu().numpy()[: len(e2)].view(np.uint32), e2.view(np.uint32))
 
 
-def test_indexed_batch_rejects_unresolved(gpu):
+def test_indexed_batch_rejects_positions_past_the_snapshot(gpu):
+    """negative positions are null particles (dropped); a position at or past
+    the snapshot's end is an error"""
     snap, ids, cat = synth.config5(seed=6, nsub=20)
     U = gpu(cat.params)
     h = U.upload_snapshot(snap, ids, cat.subs)
     idx = U.resolve_ids(h)
-    idx[3] = -1
-    with pytest.raises(RuntimeError, match="not snapshot positions"):
+    idx[3] = len(snap["x"])
+    with pytest.raises(RuntimeError, match="past the snapshot"):
         U.unbind_indexed(h, idx)
 
 

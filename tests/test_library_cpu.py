@@ -55,6 +55,9 @@ def test_null_arguments_are_errors_not_crashes():
     assert lib.hbt_unbind_batch(None, None, None, None, 1, None, None, 0) == -1
     assert lib.hbt_tree_potential(None, 0, None, None, None, None, 0, None, None, None, None, None) == -1
     assert lib.hbt_ctx_create(None, 0, None) == -1
-    p = abi.make_params(max_sample_size=1000)
     ctx = ctypes.c_void_p()
-    assert lib.hbt_ctx_create(p.ctypes.data, 0, ctypes.byref(ctx)) == -4  # sampling unsupported
+    # parameters the unbinding cannot run with are rejected before any device call
+    for bad in (dict(bound_mass_precision=1.5), dict(bound_mass_precision=0.0), dict(max_sample_size=-1),
+                dict(softening=0.0), dict(precision=7)):
+        p = abi.make_params(**bad)
+        assert lib.hbt_ctx_create(p.ctypes.data, 0, ctypes.byref(ctx)) == -1, bad
