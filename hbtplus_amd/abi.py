@@ -139,6 +139,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_unbind_one": ([vp, vp, vp, vp, vp], i32),
         "hbt_tree_potential": ([vp, i64, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp], i32),
         "hbt_direct_potential": ([vp, i64, vp, vp, vp, vp, vp, i32, vp], i32),
+        "hbt_direct_potential_noalloc": ([vp, i64, vp, vp, vp, vp, vp, i32, vp], i32),
         "hbt_id_to_index": ([vp, i64, vp, i64, vp, vp], i32),
         "hbt_last_batch_stats": ([vp, vp, vp, vp], i32),
         "hbt_profile": ([vp, i32], i32),

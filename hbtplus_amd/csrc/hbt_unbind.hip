@@ -1045,6 +1045,11 @@ int hbt_version(int *major, int *minor) {
   return HBT_OK;
 }
 
+namespace {
+// the scratch address of hbt_direct_potential_noalloc on a device (hbt_direct.hpp)
+int noalloc_scratch_setup(int device);
+}  // namespace
+
 // Parameters the unbinding cannot run with: a BoundMassPrecision outside
 // (0, 1] never converges (or converges at once), a negative sample size.
 static int check_params(const hbt_params &p) {
@@ -1089,6 +1094,11 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   if (e != hipSuccess) {
     delete c;
     return fail(HBT_ERR_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  // the device's scratch address for the entry point that allocates nothing
+  if (const int rc = noalloc_scratch_setup(device); rc != HBT_OK) {
+    delete c;
+    return rc;
   }
   *out = c;
   return HBT_OK;

@@ -252,6 +252,19 @@ int hbt_direct_potential(hbt_ctx *ctx, int64_t nseg, const int64_t *seg_offset,
                          const float *x, const float *y, const float *z,
                          const float *mass, int mode, double *out_phi);
 
+/* hbt_direct_potential without allocating anything (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and results as hbt_direct_potential; at most 65536 groups (larger
+ * nseg: HBT_ERR_CAPACITY).  Its tile table lives in a file-scope device array
+ * of the library (about 512 KiB, held by each GPU the library runs on until
+ * the process ends) that every call on a GPU shares: calls that overlap on one
+ * GPU race.  Offsets that are not ascending give HBT_ERR_INVALID_ARG after the
+ * call (such groups get no targets). */
+int hbt_direct_potential_noalloc(hbt_ctx *ctx, int64_t nseg, const int64_t *seg_offset,
+                                 const float *x, const float *y, const float *z,
+                                 const float *mass, int mode, double *out_phi);
+
 /* Post-unbind properties of one subhalo: the Subhalo_t fields set by
  * CalculateProfileProperties and CalculateShape (src/subhalo.h:48-77,
  * src/subhalo.cpp:242-398).  Fields marked in/out keep their input value
