@@ -147,8 +147,8 @@ def load_library() -> ctypes.CDLL:
         "hbt_set_rand_state": ([vp, ctypes.c_uint32, i64], i32),
         "hbt_get_rand_state": ([vp, vp, vp], i32),
         "hbt_subhalo_properties": ([vp, vp, vp, i64, vp, vp], i32),
-        "hbt_mask_subhalos": ([vp, vp, vp, vp, i64, vp, vp], i32),
-        "hbt_merge_subhalos": ([vp, vp, vp, i64, vp, vp, i64, i32, vp, i64, vp], i32),
+        "hbt_mask_subhalos": ([vp, vp, vp, vp, vp, i64, vp, vp], i32),
+        "hbt_merge_subhalos": ([vp, vp, vp, i64, vp, vp, vp, i64, i32, vp, i64, vp], i32),
         "hbt_nest_subhalos": ([vp, i64, vp, vp, vp, vp], i32),
         "hbt_get_tuning": ([vp, vp], i32),
         "hbt_set_tuning": ([vp, vp], i32),

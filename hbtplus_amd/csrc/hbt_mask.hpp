@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(BLOCK) mask_write_kernel(MaskArgs A) {
 }  // namespace
 
 extern "C" int hbt_mask_subhalos(hbt_ctx *ctx, const hbt_particles *particles, const hbt_subhalo *subs_host,
-                                 const int64_t *host_group, int64_t nsub, int64_t *out_list,
+                                 const int64_t *host_group, const int64_t *visit_rank, int64_t nsub, int64_t *out_list,
                                  int64_t *new_count) {
   if (!ctx || !particles || nsub < 0 || (nsub > 0 && (!subs_host || !host_group || !out_list || !new_count)))
     return fail(HBT_ERR_INVALID_ARG, "bad argument");
@@ -141,6 +141,13 @@ extern "C" int hbt_mask_subhalos(hbt_ctx *ctx, const hbt_particles *particles, c
         heads[it->second].push_back((int)s);
       }
     }
+  }
+  // siblings in the caller's visit order (the reference's SubGroups order for
+  // a group's heads, its central first; NestedSubhalos order for nested ones)
+  if (visit_rank) {
+    auto by_rank = [&](int a, int b) { return visit_rank[a] != visit_rank[b] ? visit_rank[a] < visit_rank[b] : a < b; };
+    for (auto &v : nested) std::sort(v.begin(), v.end(), by_rank);
+    for (auto &v : heads) std::sort(v.begin(), v.end(), by_rank);
   }
   // visit order (post-order from the central; the other heads nested in the
   // central after its own nested subhalos), claim positions per group

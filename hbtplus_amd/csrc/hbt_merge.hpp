@@ -241,8 +241,8 @@ __global__ void merge_first_kernel(const int64_t *__restrict__ list, const int64
 
 extern "C" int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
                                   int64_t n_list, hbt_subhalo *subs_io, const int64_t *host_group,
-                                  int64_t nsub, int merge_trapped, int64_t *out_list, int64_t out_capacity,
-                                  int64_t *out_total) {
+                                  const int64_t *visit_rank, int64_t nsub, int merge_trapped, int64_t *out_list,
+                                  int64_t out_capacity, int64_t *out_total) {
   if (!ctx || !snapshot || n_list < 0 || nsub < 0 || out_capacity < 0 ||
       (nsub > 0 && (!subs_io || !host_group || !list_index || !out_list)))
     return fail(HBT_ERR_INVALID_ARG, "bad argument");
@@ -288,6 +288,13 @@ extern "C" int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, c
         heads[it->second].push_back((int)s);
       }
     }
+  }
+  // siblings in the caller's visit order (the reference's SubGroups order for
+  // a group's heads, its central first; NestedSubhalos order for nested ones)
+  if (visit_rank) {
+    auto by_rank = [&](int a, int b) { return visit_rank[a] != visit_rank[b] ? visit_rank[a] < visit_rank[b] : a < b; };
+    for (auto &v : nested) std::sort(v.begin(), v.end(), by_rank);
+    for (auto &v : heads) std::sort(v.begin(), v.end(), by_rank);
   }
   std::vector<int64_t> host(nsub, -1);
   for (int64_t s = 0; s < nsub; s++) host[s] = subs_host[s].parent;

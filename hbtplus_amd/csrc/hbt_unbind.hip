@@ -397,6 +397,15 @@ __global__ void __launch_bounds__(BLOCK) eval_sum_kernel(const hbt_subhalo *subs
   if (threadIdx.x == 0) *out = tot;
 }
 
+// Particle types outside [0, HBT_TYPE_MAX) (the per-type counts index by type).
+__global__ void __launch_bounds__(BLOCK) type_check_kernel(const int32_t *type, int64_t n, int *bad) {
+  int cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    cnt += (type[i] < 0 || type[i] >= HBT_TYPE_MAX) ? 1 : 0;
+  cnt = (int)wave_reduce_sum((unsigned)cnt);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && cnt) atomicAdd(bad, cnt);
+}
+
 // List entries that are not positions in [0, n): cnt[0] counts those >= n
 // (rejected), cnt[1] the negative ones (null particles: IDs the snapshot did
 // not resolve, dropped by KickNullParticles).
@@ -520,9 +529,14 @@ struct Runner {
     const size_t b_ch = (chunks.size() * 4 + 255) & ~(size_t)255, b_tl = (tl.size() * sizeof(Tile) + 255) & ~(size_t)255;
     RETCHK(c->actbuf.ensure(b_ch + b_tl + nseg * 4 + 256));
     char *d = (char *)c->actbuf.p;
-    if (!chunks.empty()) HIPCHK(hipMemcpy(d, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
-    if (!tl.empty()) HIPCHK(hipMemcpy(d + b_ch, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d + b_ch + b_tl, ft.data(), nseg * 4, hipMemcpyHostToDevice));
+    // ordered on the context's stream, which the next pass's launches follow;
+    // the host vectors live until the synchronisation below
+    if (!chunks.empty())
+      HIPCHK(hipMemcpyAsync(d, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!tl.empty())
+      HIPCHK(hipMemcpyAsync(d + b_ch, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d + b_ch + b_tl, ft.data(), nseg * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     W.act_chunk = (const int *)d;
     W.nact = (int64_t)chunks.size() * WAVE;
     tl_cur = (const Tile *)(d + b_ch);
@@ -943,8 +957,9 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
       std::vector<float> e(std::min(8, S.len));
       std::vector<int> el(std::min(8, S.len));
       if (!e.empty()) {
-        HIPCHK(hipMemcpy(e.data(), R.W.E + S.ibase, e.size() * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(el.data(), R.W.el + S.ibase, el.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(e.data(), R.W.E + S.ibase, e.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(el.data(), R.W.el + S.ibase, el.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
       }
       fprintf(stderr,
               "[hbt] pass %d seg0: flags %d len %d nbound %d nlast %d n_src %d n_tgt %d items %d "
@@ -1385,6 +1400,16 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   }
   hbt_particles P = *particles;
   int64_t passes = 0;
+  if (particles->type && particles->n > 0) {
+    RETCHK(ctx->active.ensure(16));
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(ctx->active.p, &bad, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    type_check_kernel<<<grid_for(particles->n, ctx->cus), BLOCK, 0, ctx->stream>>>(particles->type, particles->n,
+                                                                                  ctx->active.as<int>());
+    HIPCHK(hipMemcpyAsync(&bad, ctx->active.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (bad) return fail(HBT_ERR_INVALID_ARG, std::to_string(bad) + " particle types outside [0, 6)");
+  }
   const int64_t MS = ctx->params.max_sample_size > 0 ? ctx->params.max_sample_size : 0;
   if (MS > 0 && !particles->id) return fail(HBT_ERR_INVALID_ARG, "sampling needs the particle ID column");
   if (MS > 0) {
@@ -1392,7 +1417,8 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     if (!ctx->randtab.p) {
       const RandJumps &J = rand_jumps();
       RETCHK(ctx->randtab.ensure(J.chunk.size() * 4));
-      HIPCHK(hipMemcpy(ctx->randtab.p, J.chunk.data(), J.chunk.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpyAsync(ctx->randtab.p, J.chunk.data(), J.chunk.size() * 4, hipMemcpyHostToDevice,
+                            ctx->stream));
     }
   }
   // levels: nest heights; within a level the subhalos in index order (the
@@ -1527,7 +1553,10 @@ int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int6
   if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(ctx->device));
   unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
-  if (ctx->counters.p) HIPCHK(hipMemcpy(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost));
+  if (ctx->counters.p) {
+    HIPCHK(hipMemcpyAsync(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   if (walk_ms) *walk_ms = ctx->walk_ms;
   if (walk_launches) *walk_launches = ctx->walk_launches;
   if (interactions) *interactions = (int64_t)h[0];

@@ -132,9 +132,10 @@ class GpuUnbinder:
         return subs, out[1].cpu().numpy()[:total].copy(), out[2].cpu().numpy()[:total].copy(), props
 
     # exclusive ownership inside host groups ----------------------------------
-    def mask_subhalos(self, cols, subs, host_group):
+    def mask_subhalos(self, cols, subs, host_group, visit_rank=None):
         """MaskSubhalos on host columns/records: returns (out_list laid out
-        like the columns, new counts), numpy."""
+        like the columns, new counts), numpy.  visit_rank: sibling order
+        (None: index order)."""
         t = self.torch
         c = {k: self._dev(np.ascontiguousarray(cols[k])) for k in PARTICLE_COLUMNS}
         ps = particles_struct(c, ptr=lambda x: x.data_ptr())
@@ -144,8 +145,10 @@ class GpuUnbinder:
         out = t.full((max(n, 1),), -1, dtype=t.int64, device=f"cuda:{self.device}")
         cnt = np.zeros(len(subs), np.int64)
         t.cuda.synchronize(self.device)
+        vr = None if visit_rank is None else np.ascontiguousarray(visit_rank, np.int64)
         rc = self.lib.hbt_mask_subhalos(self.ctx, ctypes.byref(ps), subs.ctypes.data, grp.ctypes.data,
-                                        len(subs), out.data_ptr(), cnt.ctypes.data)
+                                        None if vr is None else vr.ctypes.data, len(subs), out.data_ptr(),
+                                        cnt.ctypes.data)
         check(rc, "hbt_mask_subhalos")
         return out.cpu().numpy()[:n].copy(), cnt
 
@@ -168,7 +171,7 @@ class GpuUnbinder:
         return dict(cols=c, ps=particles_struct(c, ptr=lambda x: x.data_ptr()),
                     li=self._dev(np.ascontiguousarray(list_index, np.int64)), n_list=len(list_index))
 
-    def merge_device(self, hm, subs, host_group, merge_trapped=True, out=None):
+    def merge_device(self, hm, subs, host_group, merge_trapped=True, out=None, visit_rank=None):
         """hbt_merge_subhalos on uploaded columns and lists.  Returns (records
         after merging, the device output tensor, entries used)."""
         t = self.torch
@@ -179,20 +182,22 @@ class GpuUnbinder:
             out = t.full((max(cap, 1),), -1, dtype=t.int64, device=f"cuda:{self.device}")
         used = ctypes.c_int64()
         t.cuda.synchronize(self.device)
+        vr = None if visit_rank is None else np.ascontiguousarray(visit_rank, np.int64)
         rc = self.lib.hbt_merge_subhalos(self.ctx, ctypes.byref(hm["ps"]), hm["li"].data_ptr(), hm["n_list"],
-                                         subs.ctypes.data, grp.ctypes.data, len(subs),
+                                         subs.ctypes.data, grp.ctypes.data, None if vr is None else vr.ctypes.data,
+                                         len(subs),
                                          1 if merge_trapped else 0, out.data_ptr(), out.numel(),
                                          ctypes.byref(used))
         check(rc, "hbt_merge_subhalos")
         return subs, out, used.value
 
-    def merge_subhalos(self, cols, list_index, subs, host_group, merge_trapped=True):
+    def merge_subhalos(self, cols, list_index, subs, host_group, merge_trapped=True, visit_rank=None):
         """MergeSubhalos after unbinding.  cols: host particle columns;
         list_index: the lists as column positions (subs offset/count index
         it); subs: post-unbinding records.  Returns (records after merging,
         their lists as column positions laid out by the records), numpy."""
         hm = self.upload_merge(cols, list_index)
-        subs, out, used = self.merge_device(hm, subs, host_group, merge_trapped)
+        subs, out, used = self.merge_device(hm, subs, host_group, merge_trapped, visit_rank=visit_rank)
         return subs, out.cpu().numpy()[:used].copy()
 
     # execution tuning (results do not depend on it) -----------------------

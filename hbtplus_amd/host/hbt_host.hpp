@@ -155,14 +155,23 @@ class Unbinder {
   // SubhaloSnapshot_t::MergeSubhalos after RefineParticles: trap detection
   // along the (glued) nests, MergeTo in post-order from every group's central,
   // re-unbinding and TruncateSource of the hosts that received subhalos.
-  // HostHaloId: the subhalos' host halos (-1 for field subhalos); the central
-  // of a group is its first subhalo in index order that no other nests, and
-  // a nested subhalo shares its host's halo.
+  // HostHaloId: the subhalos' host halos (-1 for field subhalos); a nested
+  // subhalo shares its host's halo.  GroupRank: each subhalo's place in
+  // MemberTable.SubGroups of its halo (most massive first, the central at 0);
+  // empty: index order.  Nested subhalos go in NestedSubhalos order.
   void MergeSubhalos(std::vector<Subhalo_t> &subs, const std::vector<int64_t> &HostHaloId,
-                     bool MergeTrappedSubhalos = true) {
+                     bool MergeTrappedSubhalos = true, const std::vector<int64_t> &GroupRank = {}) {
     const int64_t nsub = (int64_t)subs.size();
     if ((int64_t)HostHaloId.size() != nsub) throw Error(HBT_ERR_INVALID_ARG, "one host halo per subhalo");
-    std::vector<int64_t> parent(nsub, -1);
+    if (!GroupRank.empty() && (int64_t)GroupRank.size() != nsub)
+      throw Error(HBT_ERR_INVALID_ARG, "one group rank per subhalo");
+    std::vector<int64_t> parent(nsub, -1), VisitRank(nsub);
+    for (int64_t i = 0; i < nsub; i++) VisitRank[i] = GroupRank.empty() ? i : GroupRank[i];
+    for (int64_t i = 0; i < nsub; i++)
+      for (size_t k = 0; k < subs[i].NestedSubhalos.size(); k++) {
+        const int64_t c = subs[i].NestedSubhalos[k];
+        if (c >= 0 && c < nsub) VisitRank[c] = (int64_t)k;
+      }
     for (int64_t i = 0; i < nsub; i++)
       for (int64_t c : subs[i].NestedSubhalos) {
         if (c < 0 || c >= nsub) throw Error(HBT_ERR_INVALID_ARG, "bad nested index");
@@ -222,8 +231,9 @@ class Unbinder {
     hbt_particles P{n, d_id.p, d_col[0].p, d_col[1].p, d_col[2].p, d_col[3].p, d_col[4].p, d_col[5].p,
                     d_col[6].p, d_type.p};
     int64_t used = 0;
-    check(hbt_merge_subhalos(ctx_, &P, d_lst.p, n, rec.data(), HostHaloId.data(), nsub,
-                             MergeTrappedSubhalos ? 1 : 0, d_out.p, cap, &used),
+    check(hbt_merge_subhalos(ctx_, &P, d_lst.p, n, rec.data(), HostHaloId.data(),
+                             VisitRank.empty() ? nullptr : VisitRank.data(), nsub, MergeTrappedSubhalos ? 1 : 0,
+                             d_out.p, cap, &used),
           "hbt_merge_subhalos");
     std::vector<int64_t> out(used);
     d_out.get(out);

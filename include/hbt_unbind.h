@@ -292,10 +292,16 @@ int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particles, const h
 
 /* Exclusive particle ownership inside each host halo (SubhaloMasker_t and
  * SubhaloSnapshot_t::MaskSubhalos, src/subhalo_tracking.cpp:793-841).
- * A host group's subhalos are visited from its central -- the first head
- * (parent < 0) of the group in index order -- in post-order: a subhalo's nested
- * subhalos (index order) before it, the group's other heads counting as nested
- * in the central after its own nested subhalos.  A visited subhalo with
+ * A host group's subhalos are visited from its central -- the first of the
+ * group's heads (parent < 0) -- in post-order: a subhalo's nested subhalos
+ * before it, the group's other heads counting as nested in the central after
+ * its own nested subhalos.  Siblings are taken in ascending visit_rank (HOST
+ * int64[nsub], ties and NULL: index order): the reference's order is a group's
+ * members as MemberTable.SubGroups[g] holds them (most massive first, the
+ * central at [0]; SubGroupsOfHeads keeps that order) and a subhalo's nested
+ * subhalos in NestedSubhalos order (src/subhalo_tracking.cpp:96-104,676-686),
+ * so visit_rank = a head's place in SubGroups[g] and a nested subhalo's place
+ * in its parent's NestedSubhalos.  A visited subhalo with
  * nbound_prev > 1 keeps the list entries whose particle ID no earlier entry of
  * the group has kept (list order); orphans (nbound_prev <= 1) and field
  * subhalos (host_group < 0) keep their lists and claim nothing.
@@ -307,8 +313,8 @@ int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particles, const h
  *                (so out_list is laid out like the input columns)
  *   new_count  : HOST int64[nsub], output */
 int hbt_mask_subhalos(hbt_ctx *ctx, const hbt_particles *particles, const hbt_subhalo *subs_host,
-                      const int64_t *host_group, int64_t nsub, int64_t *out_list,
-                      int64_t *new_count);
+                      const int64_t *host_group, const int64_t *visit_rank, int64_t nsub,
+                      int64_t *out_list, int64_t *new_count);
 
 /* Nest bookkeeping before masking (SubhaloSnapshot_t::NestSubhalos ->
  * LevelUpDetachedSubhalos, Subhalo_t::LevelUpDetachedMembers,
@@ -333,7 +339,7 @@ int hbt_nest_subhalos(hbt_ctx *ctx, int64_t nsub, const int64_t *host_group,
  * Subhalo_t::MergeTo, src/subhalo_merge.cpp:136-262).
  *   1. DetectTraps: every subhalo not yet trapped looks up its chain of hosts
  *      (its parent; a group's other heads count as nested in the group's
- *      central -- the first head in index order, as in hbt_mask_subhalos) for
+ *      central -- its first head in visit_rank order, as in hbt_mask_subhalos) for
  *      the first host with nbound > 1 whose core (mass-weighted mean position
  *      and velocity of its min(nbound, 20) most bound particles) lies within
  *      phase-space distance 2 of the subhalo's most bound particle, in units
@@ -363,14 +369,16 @@ int hbt_nest_subhalos(hbt_ctx *ctx, int64_t nsub, const int64_t *host_group,
  *                that stays bound and marks the others dead).
  *   host_group : HOST int64[nsub], -1 for field subhalos; a nested subhalo
  *                shares its parent's group
+ *   visit_rank : HOST int64[nsub] or NULL, the sibling order of
+ *                hbt_mask_subhalos (MergeRecursive visits in post-order)
  *   out_list   : DEVICE int64[out_capacity]; the lists after merging,
  *                compacted in subhalo order.  Sum of counts + nsub always
  *                suffices; *out_total (may be NULL) receives the length used
  *                (HBT_ERR_CAPACITY, with *out_total set, when it does not fit). */
 int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
                        int64_t n_list, hbt_subhalo *subs_host, const int64_t *host_group,
-                       int64_t nsub, int merge_trapped, int64_t *out_list, int64_t out_capacity,
-                       int64_t *out_total);
+                       const int64_t *visit_rank, int64_t nsub, int merge_trapped, int64_t *out_list,
+                       int64_t out_capacity, int64_t *out_total);
 
 /* ID -> index resolution: builds a device hash table of the n_table IDs
  * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs

@@ -45,8 +45,8 @@ def lib():
         L.oracle_direct_potential.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, i32]
         L.oracle_id_to_index.argtypes = [i64, vp, i64, vp, vp]
         L.oracle_subhalo_properties.argtypes = [vp, vp, vp, i64, vp, vp, i32]
-        L.oracle_mask_subhalos.argtypes = [vp, vp, vp, i64, vp, vp]
-        L.oracle_merge_subhalos.argtypes = [vp, vp, vp, i64, vp, vp, i64, i32, vp, vp, i64, vp]
+        L.oracle_mask_subhalos.argtypes = [vp, vp, vp, vp, i64, vp, vp]
+        L.oracle_merge_subhalos.argtypes = [vp, vp, vp, i64, vp, vp, vp, i64, i32, vp, vp, i64, vp]
         L.oracle_nest_subhalos.argtypes = [i64, vp, vp, vp, vp]
         L.oracle_merge_cores.argtypes = [vp, vp, vp, vp, i64, vp]
         L.oracle_abi_sizes.argtypes = [vp, vp, vp]
@@ -147,15 +147,37 @@ def subhalo_properties(params, cols, subs, out_index, props_in=None, nthreads=1)
     return props
 
 
-def mask_subhalos(cols, subs, host_group):
-    """MaskSubhalos restated: (out_list laid out like the columns, new counts)."""
+def _rank(visit_rank, n):
+    return None if visit_rank is None else np.ascontiguousarray(visit_rank, np.int64)
+
+
+def mass_visit_rank(subs, host_group):
+    """The reference's visit order as ranks: a group's members by Mbound,
+    most massive first (MemberShipTable_t, src/subhalo_tracking.cpp:96-104;
+    the records' mbound must not tie within a group), nested subhalos in
+    index order (NestedSubhalos as the catalogue lists them)."""
+    grp = np.asarray(host_group, np.int64)
+    order = np.lexsort((-np.asarray(subs["mbound"], np.float64), grp))
+    rank = np.empty(len(subs), np.int64)
+    start = np.searchsorted(grp[order], grp[order], side="left")
+    rank[order] = np.arange(len(order)) - start
+    nested = np.asarray(subs["parent"]) >= 0
+    rank[nested] = np.nonzero(nested)[0]
+    return rank
+
+
+def mask_subhalos(cols, subs, host_group, visit_rank=None):
+    """MaskSubhalos restated: (out_list laid out like the columns, new counts).
+    visit_rank: sibling order (None: index order)."""
     c = {k: np.ascontiguousarray(v) for k, v in cols.items()}
     ps = particles_struct(c)
     subs = np.ascontiguousarray(subs, SUB_DTYPE)
     grp = np.ascontiguousarray(host_group, np.int64)
     out = np.full(len(c["id"]), -1, np.int64)
     cnt = np.zeros(len(subs), np.int64)
-    rc = lib().oracle_mask_subhalos(ctypes.byref(ps), _p(subs), _p(grp), len(subs), _p(out), _p(cnt))
+    vr = _rank(visit_rank, len(subs))
+    rc = lib().oracle_mask_subhalos(ctypes.byref(ps), _p(subs), _p(grp), None if vr is None else _p(vr),
+                                    len(subs), _p(out), _p(cnt))
     if rc != 0:
         raise RuntimeError("oracle_mask_subhalos failed")
     return out, cnt
@@ -221,7 +243,7 @@ def ref_nest_subhalos(host_group, is_central, parent, rank):
     return np.frombuffer(raw, np.int64, n, 0).copy(), np.frombuffer(raw, np.int32, n, 8 * n).copy()
 
 
-def merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True):
+def merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True, visit_rank=None):
     """MergeSubhalos restated (one thread): (records after merging, their
     lists as column positions, packed in subhalo order, and the binding
     energies along the lists of the subhalos unbound again, NaN elsewhere)."""
@@ -234,9 +256,10 @@ def merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=Tru
     out = np.full(max(cap, 1), -1, np.int64)
     en = np.full(max(cap, 1), np.nan, np.float32)
     used = ctypes.c_int64()
+    vr = _rank(visit_rank, len(subs))
     rc = lib().oracle_merge_subhalos(_p(params), ctypes.byref(ps), _p(li), len(li), _p(subs), _p(grp),
-                                     len(subs), 1 if merge_trapped else 0, _p(out), _p(en), cap,
-                                     ctypes.byref(used))
+                                     None if vr is None else _p(vr), len(subs), 1 if merge_trapped else 0,
+                                     _p(out), _p(en), cap, ctypes.byref(used))
     if rc != 0:
         raise RuntimeError(f"oracle_merge_subhalos failed: {rc}")
     return subs, out[:used.value].copy(), en[:used.value].copy()
@@ -351,10 +374,12 @@ def ref_unbind_batch(cat, params=None, props_in=None, levels=False, kick=False):
     return subs, out_index, out_energy
 
 
-def ref_mask_subhalos(cols, subs, host_group):
+def ref_mask_subhalos(cols, subs, host_group, by_mass=False):
     """The reference's own SubhaloSnapshot_t::MaskSubhalos on the catalogue
     (oracle/_ref/hbt_ref_tracking): (out_list laid out like the columns, new
-    counts), as oracle.mask_subhalos returns them."""
+    counts), as oracle.mask_subhalos returns them.  by_mass: the member table
+    ordered by the records' mbound (the reference's own order); otherwise by
+    index."""
     ids = np.ascontiguousarray(cols["id"], np.int64)
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -363,7 +388,7 @@ def ref_mask_subhalos(cols, subs, host_group):
             f.write(ids.tobytes())
             f.write(np.ascontiguousarray(subs, SUB_DTYPE).tobytes())
             f.write(np.ascontiguousarray(host_group, np.int64).tobytes())
-        subprocess.run([REF_TRACKING, "mask", fin, fout], check=True,
+        subprocess.run([REF_TRACKING, "mask_mass" if by_mass else "mask", fin, fout], check=True,
                        env=dict(os.environ, OMP_NUM_THREADS="1"))
         raw = np.fromfile(fout, np.int64)
     return raw[:len(ids)].copy(), raw[len(ids):].copy()
@@ -386,7 +411,8 @@ def merge_input(cat, subs_out, out_index):
     return lst, subs
 
 
-def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True, time_threads=0):
+def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped=True, time_threads=0,
+                       by_mass=False):
     """The reference's own SubhaloSnapshot_t::MergeSubhalos
     (oracle/_ref/hbt_ref_tracking, single thread) on the state given by the
     columns, the lists (column positions) and the records.  Returns (records
@@ -408,7 +434,8 @@ def ref_merge_subhalos(params, cols, list_index, subs, host_group, merge_trapped
             f.write(np.ascontiguousarray(subs, SUB_DTYPE).tobytes())
             f.write(np.ascontiguousarray(host_group, np.int64).tobytes())
         nt = str(time_threads) if time_threads > 0 else "1"
-        subprocess.run([REF_TRACKING, "merge_time" if time_threads > 0 else "merge", fin, fout], check=True,
+        mode = "merge_time" if time_threads > 0 else ("merge_mass" if by_mass else "merge")
+        subprocess.run([REF_TRACKING, mode, fin, fout], check=True,
                        env=dict(os.environ, OMP_NUM_THREADS=nt))
         raw = open(fout, "rb").read()
     if time_threads > 0:

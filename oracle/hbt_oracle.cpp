@@ -807,6 +807,17 @@ void unbind_levels(Batch &B, const std::vector<std::vector<Int>> &levels, int nt
   }
 }
 
+// The visit order of a group's members: siblings (a subhalo's nested
+// subhalos; a group's heads) in ascending visit_rank, index order for ties or
+// without ranks.
+void sort_siblings(const int64_t *visit_rank, std::vector<std::vector<Int>> &nested,
+                   std::map<Int, std::vector<Int>> &heads) {
+  if (!visit_rank) return;
+  auto by_rank = [&](Int a, Int b) { return visit_rank[a] != visit_rank[b] ? visit_rank[a] < visit_rank[b] : a < b; };
+  for (auto &v : nested) std::sort(v.begin(), v.end(), by_rank);
+  for (auto &g : heads) std::sort(g.second.begin(), g.second.end(), by_rank);
+}
+
 }  // namespace
 
 extern "C" {
@@ -831,8 +842,11 @@ int oracle_subhalo_properties(const hbt_params *hp, const hbt_particles *parts, 
 // (the central's list temporarily extended by the group's other heads,
 // :828-834), orphans skipped (:808), first claim of an ID kept (:811-819).
 // subs: offset/count/parent/nbound_prev; out_list laid out like the columns.
+// visit_rank (or null: index order): siblings in ascending rank -- a group's
+// heads in MemberTable.SubGroups order (the central first, :826-834, heads
+// collected in that order at :676-686), nested subhalos in NestedSubhalos order.
 int oracle_mask_subhalos(const hbt_particles *parts, const hbt_subhalo *subs, const int64_t *group,
-                         int64_t nsub, int64_t *out_list, int64_t *new_count) {
+                         const int64_t *visit_rank, int64_t nsub, int64_t *out_list, int64_t *new_count) {
   if (!parts || (nsub > 0 && (!subs || !group || !out_list || !new_count))) return HBT_ERR_INVALID_ARG;
   std::vector<std::vector<Int>> nested(nsub);
   std::map<Int, std::vector<Int>> heads;
@@ -845,6 +859,7 @@ int oracle_mask_subhalos(const hbt_particles *parts, const hbt_subhalo *subs, co
       heads[group[s]].push_back(s);
     }
   }
+  sort_siblings(visit_rank, nested, heads);
   for (Int s = 0; s < nsub; s++) {
     new_count[s] = subs[s].count;
     for (Int i = 0; i < subs[s].count; i++) out_list[subs[s].offset + i] = subs[s].offset + i;
@@ -927,9 +942,9 @@ int oracle_nest_subhalos(int64_t nsub, const int64_t *group, const int32_t *is_c
 // out_energy (optional): the re-unbound subhalos' binding energies along
 // their lists, NaN elsewhere.
 int oracle_merge_subhalos(const hbt_params *hp, const hbt_particles *parts, const int64_t *list_index,
-                          int64_t n_list, hbt_subhalo *subs, const int64_t *group, int64_t nsub,
-                          int merge_trapped, int64_t *out_list, float *out_energy, int64_t out_capacity,
-                          int64_t *out_total) {
+                          int64_t n_list, hbt_subhalo *subs, const int64_t *group, const int64_t *visit_rank,
+                          int64_t nsub, int merge_trapped, int64_t *out_list, float *out_energy,
+                          int64_t out_capacity, int64_t *out_total) {
   if (!hp || !parts || (nsub > 0 && (!subs || !group || !list_index || !out_list))) return HBT_ERR_INVALID_ARG;
   Params P(*hp);
   PartCols C{parts};
@@ -945,6 +960,7 @@ int oracle_merge_subhalos(const hbt_params *hp, const hbt_particles *parts, cons
       heads[group[s]].push_back(s);
     }
   }
+  sort_siblings(visit_rank, nested, heads);
   for (auto &g : heads)  // GlueHeadNests
     nested[g.second[0]].insert(nested[g.second[0]].end(), g.second.begin() + 1, g.second.end());
   std::vector<Int> host(nsub, -1);  // FillHostTrackIds

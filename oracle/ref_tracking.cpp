@@ -7,14 +7,14 @@
 // target `ref`); a singleton MPI process, since SubhaloSnapshot_t builds its
 // MPI datatypes on construction.  Run with OMP_NUM_THREADS=1.
 //
-//   ref_tracking mask in.bin out.bin
+//   ref_tracking mask|mask_mass in.bin out.bin
 //     in : int64 nsub, int64 ncol; int64 id[ncol];
-//          hbt_subhalo subs[nsub] (offset, count, parent, nbound_prev read);
+//          hbt_subhalo subs[nsub] (offset, count, parent, nbound_prev, mbound read);
 //          int64 host_group[nsub]
 //     out: int64 out_list[ncol] (column positions, laid out like the columns;
 //          -1 where unused); int64 new_count[nsub]
 //
-//   ref_tracking merge in.bin out.bin
+//   ref_tracking merge|merge_mass in.bin out.bin
 //     in : hbt_params; int64 merge_trapped; int64 nsub, ncol, nlist;
 //          columns id[ncol] (int64), x y z vx vy vz mass [ncol] (float),
 //          type[ncol] (int32); int64 list_index[nlist];
@@ -43,9 +43,13 @@
 //     threads, the reference's own parallel loops)
 //
 // The member table is built by the reference (MemberShipTable_t::Build, which
-// orders each group by Mbound): every subhalo gets an Mbound that decreases
-// with its index and the group's first head the largest, so the group's
-// central and the order of its heads are those of hbt_mask_subhalos.
+// orders each group by Mbound).  mask / merge: every subhalo gets an Mbound
+// that decreases with its index and the group's first head the largest, so the
+// group's central and the order of its heads are index order (the library's
+// order without visit ranks).  mask_mass / merge_mass: the subhalos' own
+// Mbound (the records' mbound) orders each group, the heads are collected in
+// that order as NestSubhalos does, and the library is given that order as
+// visit ranks.
 #include <mpi.h>
 #include <omp.h>
 
@@ -119,7 +123,23 @@ void build_members(SubhaloSnapshot_t &snap, const std::vector<int64_t> &grp,
   for (int64_t s = 0; s < nsub; s++) snap.Subhalos[s].Mbound = keep[s];
 }
 
-int run_mask(FILE *in, FILE *out) {
+// The member table as the reference has it: MemberShipTable_t::Build orders
+// each group by the subhalos' own Mbound (CompareMass_t, src/subhalo_tracking.cpp:96-104)
+// and the heads are collected from that order (the Rank 0 members,
+// :676-686; Rank 0 here: no nesting host), so the central is the most massive
+// member.  The library and the oracle are given the same order as visit ranks.
+void build_members_by_mass(SubhaloSnapshot_t &snap, const std::vector<int64_t> &grp,
+                           const std::vector<int64_t> &parent, int64_t nhalos) {
+  const int64_t nsub = (int64_t)snap.Subhalos.size();
+  for (int64_t s = 0; s < nsub; s++) snap.Subhalos[s].HostHaloId = grp[s];
+  snap.MemberTable.Build((HBTInt)nhalos, snap.Subhalos, true);
+  snap.MemberTable.SubGroupsOfHeads.assign(nhalos, std::vector<HBTInt>());
+  for (int64_t g = 0; g < nhalos; g++)
+    for (HBTInt s : snap.MemberTable.SubGroups[g])
+      if (parent[s] < 0) snap.MemberTable.SubGroupsOfHeads[g].push_back(s);
+}
+
+int run_mask(FILE *in, FILE *out, bool by_mass) {
   int64_t nsub, ncol;
   rd(in, &nsub, 1); rd(in, &ncol, 1);
   std::vector<int64_t> id(ncol), grp(nsub);
@@ -137,10 +157,17 @@ int run_mask(FILE *in, FILE *out) {
     S.Particles.resize(rec[s].count);
     for (int64_t i = 0; i < rec[s].count; i++) S.Particles[i].Id = id[rec[s].offset + i];
     S.Nbound = rec[s].nbound_prev;
+    S.Mbound = rec[s].mbound;
     if (rec[s].parent >= 0) snap.Subhalos[rec[s].parent].NestedSubhalos.push_back(s);
     else if (grp[s] >= 0) heads[grp[s]].push_back(s);
   }
-  build_members(snap, grp, heads);
+  if (by_mass) {
+    std::vector<int64_t> parent(nsub);
+    for (int64_t s = 0; s < nsub; s++) parent[s] = rec[s].parent;
+    build_members_by_mass(snap, grp, parent, nhalos);
+  } else {
+    build_members(snap, grp, heads);
+  }
   snap.MaskSubhalos();
   std::vector<int64_t> out_list(ncol, -1), cnt(nsub);
   for (int64_t s = 0; s < nsub; s++) {
@@ -267,7 +294,7 @@ int run_nest(FILE *in, FILE *out) {
   return 0;
 }
 
-int run_merge(FILE *in, FILE *out, bool timed) {
+int run_merge(FILE *in, FILE *out, bool timed, bool by_mass) {
   hbt_params p;
   int64_t flag, nsub, ncol, nlist;
   rd(in, &p, 1); rd(in, &flag, 1); rd(in, &nsub, 1); rd(in, &ncol, 1); rd(in, &nlist, 1);
@@ -325,7 +352,13 @@ int run_merge(FILE *in, FILE *out, bool timed) {
     if (r.parent >= 0) snap.Subhalos[r.parent].NestedSubhalos.push_back(s);
     else if (grp[s] >= 0) heads[grp[s]].push_back(s);
   }
-  build_members(snap, grp, heads);
+  if (by_mass) {
+    std::vector<int64_t> parent(nsub);
+    for (int64_t s = 0; s < nsub; s++) parent[s] = rec[s].parent;
+    build_members_by_mass(snap, grp, parent, nhalos);
+  } else {
+    build_members(snap, grp, heads);
+  }
   const double t0 = omp_get_wtime();
   snap.MergeSubhalos();
   const double secs = omp_get_wtime() - t0;
@@ -374,15 +407,17 @@ int run_merge(FILE *in, FILE *out, bool timed) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 4) { fprintf(stderr, "usage: %s mask|merge|merge_time|nest|cores in.bin out.bin\n", argv[0]); return 1; }
+  if (argc != 4) { fprintf(stderr, "usage: %s mask|mask_mass|merge|merge_mass|merge_time|nest|cores in.bin out.bin\n", argv[0]); return 1; }
   MPI_Init(&argc, &argv);
   FILE *in = fopen(argv[2], "rb");
   FILE *out = fopen(argv[3], "wb");
   int rc = 1;
   if (!in || !out) fprintf(stderr, "cannot open files\n");
-  else if (!strcmp(argv[1], "mask")) rc = run_mask(in, out);
-  else if (!strcmp(argv[1], "merge")) rc = run_merge(in, out, false);
-  else if (!strcmp(argv[1], "merge_time")) rc = run_merge(in, out, true);
+  else if (!strcmp(argv[1], "mask")) rc = run_mask(in, out, false);
+  else if (!strcmp(argv[1], "mask_mass")) rc = run_mask(in, out, true);
+  else if (!strcmp(argv[1], "merge")) rc = run_merge(in, out, false, false);
+  else if (!strcmp(argv[1], "merge_mass")) rc = run_merge(in, out, false, true);
+  else if (!strcmp(argv[1], "merge_time")) rc = run_merge(in, out, true, false);
   else if (!strcmp(argv[1], "nest")) rc = run_nest(in, out);
   else if (!strcmp(argv[1], "cores")) rc = run_cores(in, out);
   if (in) fclose(in);

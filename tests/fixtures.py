@@ -15,6 +15,8 @@ SAMPLING_CASES = ["samp_c1", "samp_c2", "samp_c4", "samp_edge"]
 TREE_CASES = ["tree_c1", "tree_straddle", "tree_floor"]
 PROPS_CASES = ["c2_small", "c3_small", "c4_small", "hubble", "edge"]
 MASK_CASES = ["mask_a", "mask_b"]
+# masking with the member table in the reference's Mbound order (visit ranks)
+MASK_MASS = "mask_mass"
 MERGE_CASES = ["merge_a", "merge_b"]
 NEST_CASES = ["nest_a", "nest_b"]
 
@@ -81,3 +83,13 @@ def load_nest(name):
     """(host_group, is_central, parent, rank, reference parent, reference rank)"""
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
     return tuple(z[k] for k in ("host_group", "is_central", "parent", "rank", "ref_parent", "ref_rank"))
+
+
+def load_merge_mass():
+    """Merging with the member table in Mbound order: (params, cols,
+    list_index, records, host groups, reference (records, lists, energies))."""
+    z = np.load(os.path.join(GOLDEN, "merge_mass.npz"))
+    cols = {k: z[k] for k in ("id", "x", "y", "z", "vx", "vy", "vz", "mass", "type")}
+    return (z["params"].view(PARAMS_DTYPE).copy(), cols, z["list_index"].copy(), z["subs"].view(SUB_DTYPE).copy(),
+            z["host_group"].copy(),
+            (z["out_subs"].view(SUB_DTYPE).copy(), z["out_list"].copy(), z["out_energy"].copy()))

@@ -28,6 +28,12 @@ KickNullParticles followed by its unbinding produced (harness mode "kick").
 
     python tests/golden/make_golden.py --kick-only    # only kick_c2
 
+mask_mass.npz, merge_mass.npz: masking and merging with the member table
+ordered by the subhalos' own Mbound (the reference's order; harness modes
+mask_mass / merge_mass), heads massive in an order other than their index.
+
+    python tests/golden/make_golden.py --order-only   # only these two
+
     python tests/golden/make_golden.py --props-only   # only the props_ files
     python tests/golden/make_golden.py --mask-only    # only the mask_ files
     python tests/golden/make_golden.py --merge-only   # only the merge_ files
@@ -261,6 +267,46 @@ NEST_CASES = {
 }
 
 
+def mass_order_mbound(subs, grp, seed):
+    """Distinct masses that order each group differently from index order:
+    heads (no parent) get random values above every nested subhalo's, so a
+    group's most massive member -- its central -- is a head, but often not the
+    lowest-index one."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    n = len(subs)
+    m = np.where(subs["parent"] < 0, 100.0 + rng.permutation(n), 1.0 + rng.random(n) * 50.0)
+    return m.astype(np.float32)
+
+
+def save_mask_mass():
+    cols, subs, grp = synth.masking_catalogue(seed=23, ngroups=20)
+    subs = subs.copy()
+    subs["mbound"] = mass_order_mbound(subs, grp, 2323)
+    rank = oracle.mass_visit_rank(subs, grp)
+    moved = 0
+    for g in np.unique(grp[grp >= 0]):
+        h = np.nonzero((grp == g) & (subs["parent"] < 0))[0]
+        moved += int(h[np.argmin(rank[h])] != h[0])
+    assert moved > 0
+    out, cnt = oracle.ref_mask_subhalos(cols, subs, grp, by_mass=True)
+    np.savez_compressed(os.path.join(HERE, "mask_mass.npz"), kind="mask", id=cols["id"], subs=subs.view(np.uint8),
+                        host_group=grp, ref_out=out, ref_count=cnt)
+    print(f"mask_mass: {len(subs)} subhalos, {moved} groups whose central is not their first head, kept {cnt.sum()}")
+
+
+def save_merge_mass():
+    cat, grp = synth.merging_catalogue(seed=1007, periodic=True)
+    lst, subs, _ = merge_state(cat, grp)
+    subs = subs.copy()
+    subs["mbound"] = mass_order_mbound(subs, grp, 4747)
+    on = oracle.ref_merge_subhalos(cat.params, cat.cols, lst, subs, grp, True, by_mass=True)
+    np.savez_compressed(os.path.join(HERE, "merge_mass.npz"), kind="merge_mass", params=cat.params.view(np.uint8),
+                        **{k: cat.cols[k] for k in cat.cols}, list_index=lst, subs=subs.view(np.uint8),
+                        host_group=grp, out_subs=on[0].view(np.uint8), out_list=on[1], out_energy=on[2])
+    print(f"merge_mass: {len(subs)} subhalos, {int((on[0]['sink_track_id'] != subs['sink_track_id']).sum())} "
+          f"newly trapped")
+
+
 def save_nest(name, kw):
     g, c, p, r = synth.nesting_catalogue(**kw)
     rp, rr = oracle.ref_nest_subhalos(g, c, p, r)
@@ -318,6 +364,11 @@ def main():
     if "--mask-only" in sys.argv:
         for name, kw in MASK_CASES.items():
             save_mask(name, kw)
+        save_mask_mass()
+        return
+    if "--order-only" in sys.argv:
+        save_mask_mass()
+        save_merge_mass()
         return
     if "--nest-only" in sys.argv:
         for name, kw in NEST_CASES.items():
@@ -352,6 +403,8 @@ def main():
         save_mask(name, kw)
     for name, kw in MERGE_CASES.items():
         save_merge(name, kw)
+    save_mask_mass()
+    save_merge_mass()
     for name, kw in NEST_CASES.items():
         save_nest(name, kw)
 

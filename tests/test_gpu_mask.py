@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import MASK_CASES, load_mask
+from fixtures import MASK_CASES, MASK_MASS, load_mask
 from hbtplus_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -42,6 +42,16 @@ def test_mask_matches_reference(gpu, name):
     """against the reference's own MaskSubhalos (golden fixtures)"""
     cols, subs, grp, ref_out, ref_cnt = load_mask(name)
     out, cnt = _run(gpu, cols, subs, grp)
+    assert np.array_equal(cnt, ref_cnt)
+    for o, c in zip(subs["offset"], cnt):
+        assert np.array_equal(out[o:o + c], ref_out[o:o + c])
+
+
+def test_mask_in_mass_order_matches_reference(gpu):
+    """the member table in the reference's Mbound order, given as visit ranks"""
+    cols, subs, grp, ref_out, ref_cnt = load_mask(MASK_MASS)
+    from hbtplus_amd.abi import make_params
+    out, cnt = gpu(make_params()).mask_subhalos(cols, subs, grp, visit_rank=oracle.mass_visit_rank(subs, grp))
     assert np.array_equal(cnt, ref_cnt)
     for o, c in zip(subs["offset"], cnt):
         assert np.array_equal(out[o:o + c], ref_out[o:o + c])

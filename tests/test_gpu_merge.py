@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import MERGE_CASES, load_merge
+from fixtures import MERGE_CASES, load_merge, load_merge_mass
 from hbtplus_amd import abi, synth
 from hbtplus_amd.abi import HBT_ERR_CAPACITY, SUB_DTYPE, particles_struct
 from helpers import assert_merge
@@ -34,6 +34,15 @@ def test_merge_matches_reference(gpu, name, merge_trapped):
     U = gpu(_with(F.params, precision=abi.HBT_PREC_REPLAY))
     got = U.merge_subhalos(F.cols, F.list_index, F.subs, F.host_group, merge_trapped)
     assert_merge(F.out if merge_trapped else F.off, got, ties=True)
+
+
+def test_merge_in_mass_order_matches_reference(gpu):
+    """the member table in the reference's Mbound order, given as visit ranks
+    (REPLAY re-unbinding)"""
+    params, cols, lst, subs, grp, ref = load_merge_mass()
+    U = gpu(_with(params, precision=abi.HBT_PREC_REPLAY))
+    got = U.merge_subhalos(cols, lst, subs, grp, True, visit_rank=oracle.mass_visit_rank(subs, grp))
+    assert_merge(ref, got, ties=True)
 
 
 @pytest.mark.parametrize("precision", [abi.HBT_PREC_REPLAY, abi.HBT_PREC_FAST])
@@ -89,7 +98,7 @@ def _merge_raw(U, cols, lst, subs, grp, cap):
     used = ctypes.c_int64()
     t.cuda.synchronize()
     rc = U.lib.hbt_merge_subhalos(U.ctx, ctypes.byref(ps), li.data_ptr(), len(lst), subs.ctypes.data,
-                                  grp.ctypes.data, len(subs), 1, out.data_ptr(), cap, ctypes.byref(used))
+                                  grp.ctypes.data, None, len(subs), 1, out.data_ptr(), cap, ctypes.byref(used))
     return rc, used.value
 
 

@@ -449,3 +449,13 @@ def test_indexed_batch_c5_tenth_size(gpu):
     rs, ri, re_ = oracle.unbind_batch(part, nthreads=16)
     stats = assert_tolerant((rs, as_ids(part.cols["id"], ri), re_), (got[0][sel], as_ids(cat.cols["id"], got[1]), got[2]))
     print(f"C5/10: {nconv} of {len(cat.subs)} converged, sample {stats}")
+
+
+def test_particle_type_outside_the_table_is_an_error(gpu):
+    """the per-type bound counts index by type: a type outside [0, 6) is
+    rejected before any unbinding"""
+    cat = synth.config2(seed=64, nsub=20, hi=2000)
+    cat.cols["type"] = cat.cols["type"].copy()
+    cat.cols["type"][5] = 6
+    with pytest.raises(RuntimeError, match="particle types outside"):
+        gpu(cat.params).unbind_batch(cat)

@@ -6,8 +6,9 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import (MASK_CASES, MERGE_CASES, NEST_CASES, PROPS_CASES, SAMPLING_CASES, TREE_CASES, UNBIND_CASES,
-                      load_mask, load_merge, load_nest, load_props, load_tree, load_unbind)
+from fixtures import (MASK_CASES, MASK_MASS, MERGE_CASES, NEST_CASES, PROPS_CASES, SAMPLING_CASES, TREE_CASES,
+                      UNBIND_CASES, load_mask, load_merge, load_merge_mass, load_nest, load_props, load_tree,
+                      load_unbind)
 from hbtplus_amd.abi import HBT_STATUS_CONVERGED, PROPS_FIELDS
 from helpers import assert_exact, assert_merge, check_unbind_invariants
 
@@ -147,6 +148,32 @@ def test_oracle_mask_matches_reference(name):
     assert np.array_equal(cnt, ref_cnt)
     for o, c in zip(subs["offset"], cnt):
         assert np.array_equal(out[o:o + c], ref_out[o:o + c])
+
+
+def _same_kept(out, cnt, ref_out, ref_cnt, subs):
+    return np.array_equal(cnt, ref_cnt) and all(
+        np.array_equal(out[o:o + c], ref_out[o:o + c]) for o, c in zip(subs["offset"], cnt))
+
+
+def test_oracle_mask_in_mass_order():
+    """The reference's member table ordered by the subhalos' own Mbound (the
+    central a head other than the group's first in index order): the
+    restatement given that order as visit ranks keeps what the reference
+    keeps, and in index order it would not"""
+    cols, subs, grp, ref_out, ref_cnt = load_mask(MASK_MASS)
+    rank = oracle.mass_visit_rank(subs, grp)
+    assert _same_kept(*oracle.mask_subhalos(cols, subs, grp, visit_rank=rank), ref_out, ref_cnt, subs)
+    assert not _same_kept(*oracle.mask_subhalos(cols, subs, grp), ref_out, ref_cnt, subs)
+
+
+def test_oracle_merge_in_mass_order():
+    params, cols, lst, subs, grp, ref = load_merge_mass()
+    rank = oracle.mass_visit_rank(subs, grp)
+    got = oracle.merge_subhalos(params, cols, lst, subs, grp, True, visit_rank=rank)
+    assert_merge(ref, (got[0], got[1]), ties=True)
+    plain = oracle.merge_subhalos(params, cols, lst, subs, grp, True)
+    with pytest.raises(AssertionError):
+        assert_merge(ref, (plain[0], plain[1]), ties=True)
 
 
 @pytest.mark.parametrize("name", MERGE_CASES)
