@@ -155,6 +155,9 @@ def walk_roofline(U, run, ms_per_step, config):
     U.profile(0)
     I, O, T = prof_c["interactions"], prof_c["opened"], prof_c["targets"]
     lane_eff = (I + O) / max(prof_c["lane_slots"], 1)
+    # a wave's groups step together until the slowest is done: wave steps
+    # against the lane slots its groups needed (64 lanes per wave step)
+    wave_eff = prof_c["lane_slots"] / 64.0 / max(prof_c["wave_steps"], 1)
     # op-mix ceiling: FP32 VALU issue slots (one per lane per instruction,
     # 3.93e13 lane-slots/s on 256 CUs x 4 SIMD16 at 2.4 GHz; an FMA counts 2
     # FLOPs, so 78.6 TFLOP/s is the non-packed FMA peak) for the algorithmic
@@ -181,6 +184,8 @@ def walk_roofline(U, run, ms_per_step, config):
                 walk_ms_per_step=round(prof_t["walk_ms"], 3),
                 flops_per_step=flops, interactions_per_eval=round(I / max(T, 1), 1),
                 opened_per_eval=round(O / max(T, 1), 1), lane_efficiency=round(lane_eff, 3),
+                wave_efficiency=round(wave_eff, 3), wave_steps_per_step=prof_c["wave_steps"],
+                ns_per_wave_step=round(prof_t["walk_ms"] * 1e6 / max(prof_c["wave_steps"], 1), 3),
                 walk_share_of_step=round(prof_t["walk_ms"] / ms_per_step, 3)), flops
 
 

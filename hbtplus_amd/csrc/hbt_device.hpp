@@ -31,7 +31,7 @@ enum : int {
 // (src/config_parser.cpp:95-103).
 struct DevParams {
   float G, eps, theta2, res, box, box_half, bmp, relax, a, hz;
-  int periodic, min_part, snap, replay, kahan;
+  int periodic, min_part, snap, replay, wide_sum;
   int max_sample;     // MaxSampleSizeOfPotentialEstimate (0: no sampling)
   int refine;         // RefineMostboundParticle, with sampling only
 };

@@ -33,7 +33,7 @@ enum { NC_SRC = 0, NC_PART = 1, NC_BOUND = 2, NC_SORT = 3, NC_CROSS = 4, NC_ROWS
 struct __align__(32) Item {
   float4 pm;   // x, y, z, mass (a node: centre of mass and total mass)
   int2 ax;     // float bits of len^2 (0 for a leaf), skip (first item after the subtree)
-  int2 pad;
+  int2 sb;     // skip as a byte offset from the segment's first item (32 * skip), unused
 };
 
 struct Work {
@@ -653,14 +653,14 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
         const int li = (int)W.v0[s];
         // a sampled tree's leaves carry mass * MassFactor (EnergySnapshot_t::GetMass, :79)
         const float m = (S.flags & F_SAMP) ? W.pm[b + li] * S.mf : W.pm[b + li];
-        W.items[ib + pos].pm = make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], m);
-        W.items[ib + pos].ax = make_int2(0, pos + 1);
+        W.items[ib + pos] = Item{make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], m),
+                                 make_int2(0, pos + 1), make_int2(32 * (pos + 1), 0)};
         if (l + 1 == S.n_src) {
           W.seg[k].n_items = pos + 1;
           // sentinel after the last item: massless, never opened, its skip is
           // itself, so a group walk that has finished reads it and stays put
-          W.items[ib + pos + 1].pm = make_float4(0.f, 0.f, 0.f, 0.f);
-          W.items[ib + pos + 1].ax = make_int2(0, pos + 1);
+          W.items[ib + pos + 1] = Item{make_float4(0.f, 0.f, 0.f, 0.f), make_int2(0, pos + 1),
+                                       make_int2(32 * (pos + 1), 0)};
         }
         const int lo = W.nlo[s];
         if (lo >= 0) {
@@ -670,8 +670,8 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
           const int hpos = hi + W.pre[b + hi] + __popc(W.bits[b + hi]);
           const float len = node_len(S, v);
           const float lsq = len * len;
-          W.items[ib + npos].pm = make_float4(0.f, 0.f, 0.f, 0.f);
-          W.items[ib + npos].ax = make_int2(__float_as_int(lsq), hpos + 1);
+          W.items[ib + npos] = Item{make_float4(0.f, 0.f, 0.f, 0.f), make_int2(__float_as_int(lsq), hpos + 1),
+                                    make_int2(32 * (hpos + 1), 0)};
           W.npos[s] = (int)(ib + npos);
         }
       }
@@ -883,15 +883,17 @@ __device__ double walk_replay(const Work &W, const DevParams &P, int64_t ib, int
   return pot * (double)P.G / (double)P.a;
 }
 
-// FP32 walk with Kahan-compensated accumulation.  Straddling segments (a
-// periodic subhalo wider than a quarter box) form the separation in double.
+// FP32 walk, the terms summed in FP64 (or in FP32 when !P.wide_sum), as the
+// group walk's FastTerm sums them.  Straddling segments (a periodic subhalo
+// wider than a quarter box) form the separation in double.
 template <bool STRADDLE, bool COUNT>
 __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n_items, float tx,
                             float ty, float tz, float tmass, unsigned &ni, unsigned &no) {
 #pragma clang fp contract(off)
   const float h = 2.8f * P.eps, h2 = h * h, hinv = 1.0f / h;
-  Kahan<true> acc;
-  acc.init(tmass / P.eps);
+  const float self = tmass / P.eps;
+  double accd = self;
+  float accf = self;
   const int end = (int)(ib + n_items);
   int j = (int)ib;
   while (j < end) {
@@ -910,9 +912,11 @@ __device__ double walk_fast(const Work &W, const DevParams &P, int64_t ib, int n
     const int nj = (int)(ib + a.y);
     j = nj > j ? nj : end;  // skips always advance
     if (COUNT) ni++;
-    acc.add(kernel_term_f(q.w, r2, h2, hinv));
+    const float t = kernel_term_f(q.w, r2, h2, hinv);
+    if (P.wide_sum) accd += t;
+    else accf += t;
   }
-  return (double)acc.s * (double)P.G / (double)P.a;
+  return (P.wide_sum ? accd : (double)accf) * (double)P.G / (double)P.a;
 }
 
 template <bool COUNT>
@@ -926,14 +930,23 @@ __device__ __forceinline__ double walk_dispatch(const Work &W, const DevParams &
 }
 
 __device__ __forceinline__ void flush_counts(const Work &W, unsigned ni, unsigned no, unsigned nt,
-                                             unsigned ns = 0) {
-  unsigned long long a = ni, b = no, c = nt, d = ns;
+                                             unsigned ns = 0, unsigned nw = 0) {
+  unsigned long long a = ni, b = no, c = nt, d = ns, e = nw;
   a = wave_reduce_sum(a); b = wave_reduce_sum(b); c = wave_reduce_sum(c); d = wave_reduce_sum(d);
+  e = wave_reduce_sum(e);
   if ((threadIdx.x & (WAVE - 1)) == 0) {
     atomicAdd(&W.counters[0], a); atomicAdd(&W.counters[1], b); atomicAdd(&W.counters[2], c);
-    atomicAdd(&W.counters[3], d);
+    atomicAdd(&W.counters[3], d); atomicAdd(&W.counters[6], e);
   }
 }
+
+// Counts of an instrumented group walk, per lane: summed sources, opened
+// nodes, targets, lane slots spent by the groups (leader lanes), wave steps
+// (lane 0 of each wave).
+struct WalkCount {
+  unsigned ni = 0, no = 0, nt = 0, ns = 0, nw = 0;
+  __device__ void flush(const Work &W) const { flush_counts(W, ni, no, nt, ns, nw); }
+};
 
 // Group-shared traversal.  The wave's 64 targets (consecutive in spatial
 // order, one segment) are split into groups of GROUP lanes; each group walks
@@ -1002,21 +1015,22 @@ __device__ __forceinline__ bool group_any_mask(u64 b) {
   return __builtin_amdgcn_inverse_ballot_w64((b & first) * fill);
 }
 
-// Per-target arithmetic of one item, FP32 (Kahan-compensated when KAHAN):
-// sep2 = squared separation, opens = the opening test, add = the softened
-// monopole term.  Straddling segments (a periodic subhalo wider than a
-// quarter box) form the separation in double.
-template <bool STRADDLE, bool KAHAN>
+// Per-target arithmetic of one item in FP32: sep2 = squared separation,
+// opens = the opening test, add = the softened monopole term, the terms summed
+// in FP64 (WIDE: the FAST precision; one conversion and one FP64 add per term,
+// exact for a few thousand terms of one sign) or in FP32.  Straddling segments
+// (a periodic subhalo wider than a quarter box) form the separation in double.
+template <bool STRADDLE, bool WIDE>
 struct FastTerm {
   using real = float;
   float h2, hinv;
-  Kahan<KAHAN> acc;
+  typename std::conditional<WIDE, double, float>::type acc;
   __device__ __forceinline__ void init(const DevParams &P, bool live, float tmass) {
 #pragma clang fp contract(off)
     const float h = 2.8f * P.eps;
     h2 = h * h;
     hinv = 1.0f / h;
-    acc.init(live ? tmass / P.eps : 0.f);
+    acc = live ? tmass / P.eps : 0.f;
   }
   __device__ __forceinline__ float sep2(const DevParams &P, const float4 &q, float tx, float ty,
                                         float tz) const {
@@ -1040,8 +1054,21 @@ struct FastTerm {
 #pragma clang fp contract(off)
     return __builtin_amdgcn_fcmpf(lsq, r2 * P.theta2, 2 /* ordered > */);
   }
-  __device__ __forceinline__ void add(float m, float r2) { acc.add(kernel_term_f(m, r2, h2, hinv)); }
-  __device__ __forceinline__ double result() const { return (double)acc.s; }
+  __device__ __forceinline__ void add(float m, float r2) {
+#pragma clang fp contract(off)
+    acc += kernel_term_f(m, r2, h2, hinv);
+  }
+  // add() for the lanes in `accept` without branching on it: kernel_term_f's
+  // Newtonian form is computed for every lane, the spline only for accepting
+  // lanes within the softening length (rare), and the others add zero
+  __device__ __forceinline__ void add_masked(bool accept, u64, float m, float r2) {
+#pragma clang fp contract(off)
+    float term = -m * __builtin_amdgcn_rsqf(r2);
+    if (accept && !(r2 >= h2)) term = kernel_term_f(m, r2, h2, hinv);
+    const float added = accept ? term : 0.f;
+    acc += added;
+  }
+  __device__ __forceinline__ double result() const { return (double)acc; }
 };
 
 // Reference arithmetic (src/gravity_tree.cpp:79-164): FP64 over float data.
@@ -1073,6 +1100,9 @@ struct ReplayTerm {
 #pragma clang fp contract(off)
     pot += kernel_term_d((double)m, sqrt(r2), h, hinv);
   }
+  __device__ __forceinline__ void add_masked(bool accept, u64, float m, double r2) {
+    if (accept) add(m, r2);
+  }
   __device__ __forceinline__ double result() const { return pot; }
 };
 
@@ -1092,8 +1122,7 @@ struct ReplayTerm {
 // its skips always advance (max with j+1).
 template <int GROUP, bool COUNT, class Term>
 __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, int ib, int end,
-                             bool live, float tx, float ty, float tz, float tmass, unsigned &ni,
-                             unsigned &no, unsigned &ns) {
+                             bool live, float tx, float ty, float tz, float tmass, WalkCount &wc) {
   Term t;
   t.init(P, live, tmass);
   const u64 lanes = group_lanes<GROUP>();
@@ -1112,16 +1141,17 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
       const float4 q1 = it[c1].pm, qs = it[cs].pm;
       const int2 a1 = it[c1].ax, as = it[cs].ax;
       bool opened = false;
-      if (COUNT && leader) ns += GROUP;
+      if (COUNT && leader) wc.ns += GROUP;
+      if (COUNT && (threadIdx.x & (WAVE - 1)) == 0) wc.nw++;
       if (until <= j) {
         const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
         opened = t.opens(P, __int_as_float(a.x), r2);
         if (opened) {
-          if (COUNT) no++;
+          if (COUNT) wc.no++;
         } else {
           t.add(q.w, r2);
           until = a.y;
-          if (COUNT) ni++;
+          if (COUNT) wc.ni++;
         }
       }
       const bool go = __any(opened);
@@ -1130,41 +1160,53 @@ __device__ double walk_group(const Item *__restrict__ it, const DevParams &P, in
       a = go ? a1 : as;
     }
   } else {
-    // lane r of every 8 fetches dword r of its group's item (r = 6, 7 read
-    // the padding) and the group takes the six fields from those lanes through
-    // the LDS crossbar: 4 bytes per lane through the vector memory pipe
+    // lane r of every 8 fetches dword r of its group's item (r = 5, 7 are
+    // not used) and the group takes the six fields it needs from those lanes
+    // through the LDS crossbar: 4 bytes per lane through the vector memory
+    // pipe.  Positions are byte offsets from the segment's first item: J =
+    // 32 j + the lane's dword offset (< 32), so J < 32 end iff j < end, and a
+    // skip kept as 32 skip (the item's sb field) is <= J iff skip <= j.
     const char *base = (const char *)it;
     const unsigned dw = 4u * (threadIdx.x & 7u);
-    int v = item_dword(base, j, dw);
-    while (__any(j < end)) {
-      const bool on = j < end;
+    // the group's lanes in a wave mask: the vote takes the group's field of a
+    // lane mask with one 64-bit shift per lane (vector unit) instead of
+    // folding the mask on the scalar unit
+    const unsigned gbase = threadIdx.x & (WAVE - GROUP);
+    constexpr u64 GFIELD = GROUP >= 64 ? ~0ull : ((1ull << (GROUP & 63)) - 1ull);
+    const unsigned endb = 32u * (unsigned)end;
+    unsigned J = 32u * (unsigned)j + dw;
+    unsigned untilb = live ? 32u * (unsigned)ib : UINT_MAX;
+    int v = *(const int *)(base + J);
+    while (__any(J < endb)) {
+      const bool on = J < endb;
       const float4 q = make_float4(__int_as_float(group_field<GROUP, 0>(v)), __int_as_float(group_field<GROUP, 1>(v)),
                                    __int_as_float(group_field<GROUP, 2>(v)), __int_as_float(group_field<GROUP, 3>(v)));
-      const int2 a = make_int2(group_field<GROUP, 4>(v), group_field<GROUP, 5>(v));
-      // j <= end throughout: every item's skip lies in (j, end], the sentinel's is end
-      const int j1 = j + 1, js = a.y;
+      const float lsq = __int_as_float(group_field<GROUP, 4>(v));
+      const unsigned sb = (unsigned)group_field<GROUP, 6>(v);
+      // J <= 32 end + dw throughout: every item's skip lies in (j, end], the sentinel's is end
+      const unsigned J1 = J + 32u, JS = sb + dw;
       // both possible successors are fetched now, under the work on item j
-      // (j1 = end + 1 reads the next region or the buffer's two spare items)
-      const int v1 = item_dword(base, j1, dw), vs = item_dword(base, js, dw);
+      // (j + 1 = end + 1 reads the next region or the buffer's two spare items)
+      const int v1 = *(const int *)(base + J1), vs = *(const int *)(base + JS);
       // lane masks straight from the compares (every lane is active here):
       // the group's vote runs on them without a round trip through VGPRs
-      const u64 pmask = __builtin_amdgcn_sicmp(j, end, 40 /* signed < */) &
-                        __builtin_amdgcn_sicmp(until, j, 41 /* signed <= */);
+      const u64 pmask = __builtin_amdgcn_uicmp(J, endb, 36 /* unsigned < */) &
+                        __builtin_amdgcn_uicmp(untilb, J, 37 /* unsigned <= */);
       const typename Term::real r2 = t.sep2(P, q, tx, ty, tz);
-      const u64 omask = pmask & t.opens_mask(P, __int_as_float(a.x), r2);
-      const bool accept = __builtin_amdgcn_inverse_ballot_w64(pmask & ~omask);
+      const u64 omask = pmask & t.opens_mask(P, lsq, r2);
+      const u64 amask = pmask & ~omask;
+      const bool accept = __builtin_amdgcn_inverse_ballot_w64(amask);
       if (COUNT) {
-        if (on && leader) ns += GROUP;
-        no += __builtin_amdgcn_inverse_ballot_w64(omask) ? 1u : 0u;
-        ni += accept ? 1u : 0u;
+        if (on && leader) wc.ns += GROUP;
+        if ((threadIdx.x & (WAVE - 1)) == 0) wc.nw++;
+        wc.no += __builtin_amdgcn_inverse_ballot_w64(omask) ? 1u : 0u;
+        wc.ni += accept ? 1u : 0u;
       }
-      const bool go = group_any_mask<GROUP>(omask);
-      j = go ? j1 : js;
+      const bool go = ((omask >> gbase) & GFIELD) != 0ull;
+      J = go ? J1 : JS;
       v = go ? v1 : vs;
-      if (accept) {
-        t.add(q.w, r2);
-        until = a.y;
-      }
+      t.add_masked(accept, amask, q.w, r2);
+      untilb = accept ? sb : untilb;
     }
   }
   return t.result();
@@ -1174,18 +1216,17 @@ template <int GROUP, bool COUNT, bool REPLAY>
 __device__ __forceinline__ double walk_group_dispatch(const Item *__restrict__ it,
                                                       const DevParams &P, const Seg &S, int ib,
                                                       int end, bool live, float x, float y, float z,
-                                                      float tm, unsigned &ni, unsigned &no,
-                                                      unsigned &ns) {
+                                                      float tm, WalkCount &wc) {
   if (REPLAY)
-    return walk_group<GROUP, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-  if (!P.kahan) {
+    return walk_group<GROUP, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, wc);
+  if (!P.wide_sum) {
     if (S.straddle)
-      return walk_group<GROUP, COUNT, FastTerm<true, false>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    return walk_group<GROUP, COUNT, FastTerm<false, false>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      return walk_group<GROUP, COUNT, FastTerm<true, false>>(it, P, ib, end, live, x, y, z, tm, wc);
+    return walk_group<GROUP, COUNT, FastTerm<false, false>>(it, P, ib, end, live, x, y, z, tm, wc);
   }
   if (S.straddle)
-    return walk_group<GROUP, COUNT, FastTerm<true, true>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-  return walk_group<GROUP, COUNT, FastTerm<false, true>>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    return walk_group<GROUP, COUNT, FastTerm<true, true>>(it, P, ib, end, live, x, y, z, tm, wc);
+  return walk_group<GROUP, COUNT, FastTerm<false, true>>(it, P, ib, end, live, x, y, z, tm, wc);
 }
 
 // Lanes per group chosen per segment (S.wgroup, set from its size bin).  The
@@ -1193,24 +1234,23 @@ __device__ __forceinline__ double walk_group_dispatch(const Item *__restrict__ i
 template <bool COUNT, bool REPLAY>
 __device__ __forceinline__ double walk_binned(const Item *__restrict__ it, const DevParams &P,
                                               const Seg &S, int ib, int end, bool live, float x,
-                                              float y, float z, float tm, unsigned &ni,
-                                              unsigned &no, unsigned &ns) {
+                                              float y, float z, float tm, WalkCount &wc) {
   if (REPLAY) {
     switch (S.wgroup) {
-      case 8: return walk_group<8, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-      case 16: return walk_group<16, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-      case 32: return walk_group<32, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-      default: return walk_group<64, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+      case 8: return walk_group<8, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, wc);
+      case 16: return walk_group<16, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, wc);
+      case 32: return walk_group<32, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, wc);
+      default: return walk_group<64, COUNT, ReplayTerm>(it, P, ib, end, live, x, y, z, tm, wc);
     }
   }
-  if (!P.kahan || S.straddle)
-    return walk_group_dispatch<64, COUNT, false>(it, P, S, ib, end, live, x, y, z, tm, ni, no, ns);
+  if (!P.wide_sum || S.straddle)
+    return walk_group_dispatch<64, COUNT, false>(it, P, S, ib, end, live, x, y, z, tm, wc);
   using T = FastTerm<false, true>;
   switch (S.wgroup) {
-    case 8: return walk_group<8, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    case 16: return walk_group<16, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    case 32: return walk_group<32, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
-    default: return walk_group<64, COUNT, T>(it, P, ib, end, live, x, y, z, tm, ni, no, ns);
+    case 8: return walk_group<8, COUNT, T>(it, P, ib, end, live, x, y, z, tm, wc);
+    case 16: return walk_group<16, COUNT, T>(it, P, ib, end, live, x, y, z, tm, wc);
+    case 32: return walk_group<32, COUNT, T>(it, P, ib, end, live, x, y, z, tm, wc);
+    default: return walk_group<64, COUNT, T>(it, P, ib, end, live, x, y, z, tm, wc);
   }
 }
 
@@ -1255,8 +1295,7 @@ __device__ __forceinline__ void finish_target(const Work &W, const DevParams &P,
 // The targets of one wave-aligned chunk of slots, one per lane.
 template <int GROUP, bool COUNT, bool REPLAY>
 __device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, const Item *__restrict__ it,
-                                           int64_t s, unsigned &ni, unsigned &no, unsigned &nt,
-                                           unsigned &ns) {
+                                           int64_t s, WalkCount &wc) {
   // segments are wave-aligned: the whole wave belongs to one segment
   const int k = __builtin_amdgcn_readfirstlane(W.seg_of_slot[s]);
   const Seg &S = W.seg[k];
@@ -1264,7 +1303,7 @@ __device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, co
   const int l = (int)(s - b);
   const bool live = l < S.n_tgt;
   if (!__any(live)) return;
-  if (COUNT && live) nt++;
+  if (COUNT && live) wc.nt++;
   const int li = live ? W.sp[s] : 0;
   const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
   const float tm = target_mass(W, S, b, li);
@@ -1272,10 +1311,10 @@ __device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, co
   const int end = S.n_items;
   double pot;
   if (GROUP == 0)
-    pot = walk_binned<COUNT, REPLAY>(seg_it, P, S, 0, end, live, x, y, z, tm, ni, no, ns);
+    pot = walk_binned<COUNT, REPLAY>(seg_it, P, S, 0, end, live, x, y, z, tm, wc);
   else
     pot = walk_group_dispatch<GROUP == 0 ? 64 : GROUP, COUNT, REPLAY>(seg_it, P, S, 0, end, live, x, y, z,
-                                                                     tm, ni, no, ns);
+                                                                     tm, wc);
   if (live) finish_target<REPLAY>(W, P, S, b, li, x, y, z, pot);
 }
 
@@ -1284,11 +1323,11 @@ __device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, co
 template <int GROUP, bool COUNT, bool REPLAY>
 __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
                                                             const Item *__restrict__ it, int xcd_run) {
-  unsigned ni = 0, no = 0, nt = 0, ns = 0;
+  WalkCount wc;
   const int vb = xcd_block(xcd_run);
   for (int64_t i = (int64_t)vb * BLOCK + threadIdx.x; i < W.nact; i += (int64_t)gridDim.x * BLOCK)
-    walk_chunk<GROUP, COUNT, REPLAY>(W, P, it, slot_at(W, i), ni, no, nt, ns);
-  if (COUNT) flush_counts(W, ni, no, nt, ns);
+    walk_chunk<GROUP, COUNT, REPLAY>(W, P, it, slot_at(W, i), wc);
+  if (COUNT) wc.flush(W);
 }
 
 // Instrumented passes: the items of the trees the walk read (counters[5]).

@@ -44,12 +44,13 @@ extern "C" int hbt_tree_potential(hbt_ctx *ctx, int64_t n_src, const float *sx, 
     return fail(HBT_ERR_INVALID_ARG, "bad argument");
   if (n_src >= INT32_MAX / 2) return fail(HBT_ERR_UNSUPPORTED, "too many sources");
   HIPCHK(hipSetDevice(ctx->device));
-  std::vector<int64_t> cap{n_src};
-  std::vector<std::vector<int>> children(1);
   hbt_subhalo h{};
   h.count = n_src;
+  h.parent = -1;
+  Nest N;
+  RETCHK(plan_nests(&h, 1, N));
   Level L;
-  build_level(&h, cap, children, std::vector<int>{0}, ctx->tune, nullptr, 1, L);
+  build_level(N, std::vector<int>{0}, ctx->tune, nullptr, 1, L);
   RETCHK(ensure_slots(ctx, L.nslots, 1, (int64_t)L.tiles.size()));
   RETCHK(stage_level(ctx, L));
   Runner R;

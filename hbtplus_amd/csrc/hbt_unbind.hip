@@ -10,12 +10,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <chrono>
 #include <initializer_list>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -64,7 +66,7 @@ DevParams make_dev_params(const hbt_params &p) {
   d.min_part = p.min_num_part_of_sub;
   d.snap = p.snapshot_index;
   d.replay = p.precision == HBT_PREC_REPLAY;
-  d.kahan = p.precision != HBT_PREC_FP32;
+  d.wide_sum = p.precision != HBT_PREC_FP32;
   d.max_sample = p.max_sample_size > 0 ? p.max_sample_size : 0;
   d.refine = d.max_sample > 0 && p.refine_mostbound_particle != 0;
   return d;
@@ -93,6 +95,29 @@ struct DBuf {
   template <class T> T *as() const { return (T *)p; }
 };
 
+// Grow-only pinned host allocation; ensure(want, keep) keeps the first
+// `keep` bytes when it grows.
+struct HBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want, size_t keep = 0) {
+    if (want <= bytes) return HBT_OK;
+    void *q = nullptr;
+    hipError_t e = hipHostMalloc(&q, want, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(HBT_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
+    if (p && keep) memcpy(q, p, std::min(keep, bytes));
+    if (p) (void)hipHostFree(p);
+    p = q;
+    bytes = want;
+    return HBT_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 // A typed view into a device buffer owned elsewhere.
 struct DView {
   void *p = nullptr;
@@ -117,6 +142,7 @@ struct hbt_ctx {
   DBuf seg;             // per-segment state, expanded on the device from segin
   DBuf ncount, tsum, tpart;
   DBuf segact, actbuf;  // segment activity after a pass; lists of the active segments
+  HBuf h_list;          // pinned: activity flags read back, lists of the active segments
   DBuf gaps;            // (subhalo, capacity, first flat index) of the hosts whose output regions have unused slots
   DBuf active, counters;
   // instrumentation: 1 = time walk launches with HIP events, 2 = count interactions
@@ -153,6 +179,7 @@ struct hbt_ctx {
       b->release();
     if (h_active) (void)hipHostFree(h_active);
     if (h_stage) (void)hipHostFree(h_stage);
+    h_list.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (evw0) (void)hipEventDestroy(evw0);
@@ -187,20 +214,54 @@ int walk_group_for(const hbt_tuning &t, int64_t len) {
   return len < t.bin_edge[0] ? t.walk_group[0] : (len < t.bin_edge[1] ? t.walk_group[1] : t.walk_group[2]);
 }
 
+// The batch's nest structure on the host, with the record fields the
+// planning reads gathered once (a record is 240 bytes, the planning needs 40
+// of them): capacities (own list plus the nested subhalos' capacities),
+// heights, children in index order (CSR) and the post-order (RecursiveUnbind
+// order).
+struct SubPlan {
+  int64_t offset, count, nbp, out_off, cap;  // cap: own list plus the nested subhalos' capacities
+};
+struct Nest {
+  std::vector<SubPlan> f;
+  std::vector<int> height, ch_ptr, ch_idx, order;
+  int64_t n = 0;
+  int nchild(int s) const { return ch_ptr[s + 1] - ch_ptr[s]; }
+  const int *child(int s) const { return ch_idx.data() + ch_ptr[s]; }
+};
+
+// Members of a level sorted by capacity, largest first, ties in member order
+// (two 16-bit passes of a stable radix sort on 31-bit capacities).
+void sort_largest_first(const Nest &N, const std::vector<int> &members, std::vector<int> &out) {
+  const size_t n = members.size();
+  out = members;
+  if (n < 64) {
+    std::stable_sort(out.begin(), out.end(), [&](int a, int b) { return N.f[a].cap > N.f[b].cap; });
+    return;
+  }
+  static thread_local std::vector<int> tmp;
+  static thread_local std::vector<uint32_t> cnt;
+  tmp.resize(n);
+  auto key = [&](int s) { return (uint32_t)(0x7fffffff - (int32_t)N.f[s].cap); };
+  int *src = out.data(), *dst = tmp.data();
+  for (int shift = 0; shift < 32; shift += 16) {
+    cnt.assign(65537, 0u);
+    for (size_t i = 0; i < n; i++) cnt[((key(src[i]) >> shift) & 0xffffu) + 1]++;
+    for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; i++) dst[cnt[(key(src[i]) >> shift) & 0xffffu]++] = src[i];
+    std::swap(src, dst);
+  }
+  // two passes: the result is back in out
+}
+
 // draw_off: per subhalo, the first of its shuffle's draws in the level's
 // draws (-1: not shuffled) and the list length they were counted for
 // (draw_off[nsub + s]); null when nothing is shuffled.
-void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
-                 const std::vector<std::vector<int>> &children, const std::vector<int> &members,
-                 const hbt_tuning &tune, const int64_t *draw_off, int64_t nsub, Level &L) {
+void build_level(const Nest &N, const std::vector<int> &members, const hbt_tuning &tune,
+                 const int64_t *draw_off, int64_t nsub, Level &L) {
   const int n = (int)members.size();
   // largest first: the merge levels of a sort then cover a prefix
-  std::vector<std::pair<int64_t, int>> key(members.size());
-  for (size_t i = 0; i < members.size(); i++) key[i] = {-cap[members[i]], (int)i};
-  std::sort(key.begin(), key.end());  // by capacity descending, then input order
-  std::vector<int> order(members.size());
-  for (size_t i = 0; i < members.size(); i++) order[i] = members[key[i].second];
-  L.subs = order;
+  sort_largest_first(N, members, L.subs);
   L.seg.resize(n);
   L.ibase.resize(n);
   L.capal.resize(n);
@@ -208,25 +269,28 @@ void build_level(const hbt_subhalo *subs, const std::vector<int64_t> &cap,
   L.ntile.resize(n);
   L.child_ptr.assign(n + 1, 0);
   L.child_list.clear();
-  L.tiles.clear();
-  int64_t off = 0;
+  int64_t ntiles = 0;
+  for (int k = 0; k < n; k++) ntiles += (N.f[L.subs[k]].cap + TILE - 1) / TILE;
+  L.tiles.resize(ntiles);
+  int64_t off = 0, t0 = 0;
   L.maxcap = 0;
   for (int k = 0; k < n; k++) {
-    const int s = order[k];
-    const int c = (int)cap[s];
+    const int s = L.subs[k];
+    const SubPlan &F = N.f[s];
+    const int c = (int)F.cap;
     const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
-    L.seg[k] = SegIn{off, subs[s].out_offset, subs[s].offset, subs[s].nbound_prev, s, c, (int)subs[s].count,
-                     walk_group_for(tune, c), draw_off ? (int)draw_off[s] : -1,
-                     draw_off ? (int)draw_off[nsub + s] : 0};
+    L.seg[k] = SegIn{off, F.out_off, F.offset, F.nbp, s, c, (int)F.count, walk_group_for(tune, c),
+                     draw_off ? (int)draw_off[s] : -1, draw_off ? (int)draw_off[nsub + s] : 0};
     L.ibase[k] = off;
     L.capal[k] = ca;
-    L.first_tile[k] = (int)L.tiles.size();
+    L.first_tile[k] = (int)t0;
     const int nt = (c + TILE - 1) / TILE;
     L.ntile[k] = nt;
-    for (int t = 0; t < nt; t++) L.tiles.push_back(Tile{k, t * TILE});
+    for (int t = 0; t < nt; t++) L.tiles[t0 + t] = Tile{k, t * TILE};
+    t0 += nt;
     off += ca;
     L.maxcap = std::max(L.maxcap, c);
-    for (int ch : children[s]) L.child_list.push_back(ch);
+    for (int i = 0; i < N.nchild(s); i++) L.child_list.push_back(N.child(s)[i]);
     L.child_ptr[k + 1] = (int)L.child_list.size();
   }
   L.nslots = off;
@@ -508,39 +572,58 @@ struct Runner {
   // segment), caps and first tiles in level order.
   int list_active() {
     const int nseg = (int)L->seg.size();
-    std::vector<int> act(nseg);
-    HIPCHK(hipMemcpyAsync(act.data(), W.segact, nseg * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // one pinned host buffer: the activity flags come down into it and the
+    // lists are written in place and go up with one copy
+    const size_t b_act = ((size_t)nseg * 4 + 255) & ~(size_t)255;
+    RETCHK(c->h_list.ensure(b_act));
+    int *act = (int *)c->h_list.p;
+    HIPCHK(hipMemcpyAsync(act, W.segact, nseg * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<int> chunks, ft(nseg, 0);
-    std::vector<Tile> tl;
-    cur_cap.clear();
-    cur_first.clear();
+    int64_t nch = 0, ntl = 0;
     nlisted = 0;
     for (int k = 0; k < nseg; k++) {
       if (!act[k]) continue;
       nlisted++;
-      cur_cap.push_back(L->seg[k].cap);
-      cur_first.push_back((int)tl.size());
-      ft[k] = (int)tl.size();
-      for (int t = 0; t < L->ntile[k]; t++) tl.push_back(L->tiles[L->first_tile[k] + t]);
-      const int64_t c0 = L->ibase[k] / WAVE, c1 = (L->ibase[k] + L->capal[k]) / WAVE;
-      for (int64_t q = c0; q < c1; q++) chunks.push_back((int)q);
+      ntl += L->ntile[k];
+      nch += L->capal[k] / WAVE;
     }
-    const size_t b_ch = (chunks.size() * 4 + 255) & ~(size_t)255, b_tl = (tl.size() * sizeof(Tile) + 255) & ~(size_t)255;
-    RETCHK(c->actbuf.ensure(b_ch + b_tl + nseg * 4 + 256));
+    const size_t b_ch = ((size_t)nch * 4 + 255) & ~(size_t)255, b_tl = ((size_t)ntl * sizeof(Tile) + 255) & ~(size_t)255;
+    const size_t b_ft = (size_t)nseg * 4;
+    // the flags stay at the front while the lists are written behind them
+    RETCHK(c->h_list.ensure(b_act + b_ch + b_tl + b_ft, b_act));
+    act = (int *)c->h_list.p;
+    char *h = (char *)c->h_list.p + b_act;
+    int *chunks = (int *)h;
+    Tile *tl = (Tile *)(h + b_ch);
+    int *ft = (int *)(h + b_ch + b_tl);
+    cur_cap.clear();
+    cur_first.clear();
+    int64_t qc = 0, qt = 0;
+    for (int k = 0; k < nseg; k++) {
+      ft[k] = (int)qt;
+      if (!act[k]) {
+        ft[k] = 0;
+        continue;
+      }
+      cur_cap.push_back(L->seg[k].cap);
+      cur_first.push_back((int)qt);
+      memcpy(tl + qt, L->tiles.data() + L->first_tile[k], (size_t)L->ntile[k] * sizeof(Tile));
+      qt += L->ntile[k];
+      const int64_t c0 = L->ibase[k] / WAVE, c1 = (L->ibase[k] + L->capal[k]) / WAVE;
+      for (int64_t q = c0; q < c1; q++) chunks[qc++] = (int)q;
+    }
+    RETCHK(c->actbuf.ensure(b_ch + b_tl + b_ft + 256));
     char *d = (char *)c->actbuf.p;
     // ordered on the context's stream, which the next pass's launches follow;
-    // the host vectors live until the synchronisation below
-    if (!chunks.empty())
-      HIPCHK(hipMemcpyAsync(d, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice, c->stream));
-    if (!tl.empty())
-      HIPCHK(hipMemcpyAsync(d + b_ch, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d + b_ch + b_tl, ft.data(), nseg * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // the pinned buffer is written again only after the next read-back has
+    // synchronised the stream
+    HIPCHK(hipMemcpyAsync(d, h, b_ch + b_tl + b_ft, hipMemcpyHostToDevice, c->stream));
+    const int64_t nchunks = nch;
+    const int ntiles_listed = (int)ntl;
     W.act_chunk = (const int *)d;
-    W.nact = (int64_t)chunks.size() * WAVE;
+    W.nact = nchunks * WAVE;
     tl_cur = (const Tile *)(d + b_ch);
-    nt_cur = (int)tl.size();
+    nt_cur = ntiles_listed;
     ft_cur = (const int *)(d + b_ch + b_tl);
     set_grids();
     return HBT_OK;
@@ -1002,46 +1085,85 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   return finish_level(c, R);
 }
 
-// Nest structure on the host: capacities, heights, children in index order.
-int plan_nests(const hbt_subhalo *subs, int64_t nsub, std::vector<int64_t> &cap,
-               std::vector<int> &height, std::vector<std::vector<int>> &children,
-               std::vector<int> &roots) {
-  cap.assign(nsub, 0);
-  height.assign(nsub, 0);
-  children.resize(nsub);
-  for (auto &ch : children) ch.clear();  // keeps each list's memory
-  roots.clear();
-  for (int64_t s = 0; s < nsub; s++) {
-    if (subs[s].count < 0 || subs[s].parent >= nsub || subs[s].parent == s)
-      return fail(HBT_ERR_INVALID_ARG, "bad subhalo record (count/parent)");
-    cap[s] = subs[s].count;
-    if (subs[s].parent >= 0) children[subs[s].parent].push_back((int)s);
-    else roots.push_back((int)s);
+// Gathers the records' planning fields, checks them and builds the nests:
+// capacities, heights, children (CSR) and the post-order; detects cycles.
+int plan_nests(const hbt_subhalo *subs, int64_t nsub, Nest &N) {
+  N.n = nsub;
+  N.f.resize(nsub);
+  N.height.assign(nsub, 0);
+  N.ch_ptr.assign(nsub + 1, 0);
+  static thread_local std::vector<int64_t> parent;
+  parent.resize(nsub);
+  // the records are read once, by a few threads for large batches (the
+  // fields sit in two cache lines of each 240-byte record)
+  const int nth = nsub >= 65536 ? 4 : 1;
+  std::atomic<int> bad{0};
+  // (the worker threads get plain pointers: `parent` is thread-local)
+  SubPlan *fp = N.f.data();
+  int64_t *pp = parent.data();
+  auto gather = [subs, nsub, fp, pp, &bad](int64_t lo, int64_t hi) {
+    for (int64_t s = lo; s < hi; s++) {
+      const hbt_subhalo &r = subs[s];
+      if (r.count < 0 || r.parent >= nsub || r.parent == s) bad.store(1, std::memory_order_relaxed);
+      fp[s] = SubPlan{r.offset, r.count, r.nbound_prev, r.out_offset, r.count};
+      pp[s] = r.parent;
+    }
+  };
+  if (nth == 1) {
+    gather(0, nsub);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; t++) th.emplace_back(gather, nsub * t / nth, nsub * (t + 1) / nth);
+    gather(0, nsub / nth);
+    for (auto &x : th) x.join();
   }
-  // post-order from roots gives heights and capacities; detects cycles
-  std::vector<int> order;
-  order.reserve(nsub);
-  std::vector<std::pair<int, size_t>> st;
-  for (int r : roots) {
-    st.push_back({r, 0});
+  if (bad.load()) return fail(HBT_ERR_INVALID_ARG, "bad subhalo record (count/parent)");
+  bool nested = false;
+  for (int64_t s = 0; s < nsub; s++)
+    if (parent[s] >= 0) {
+      N.ch_ptr[parent[s] + 1]++;
+      nested = true;
+    }
+  for (int64_t s = 0; s < nsub; s++) N.ch_ptr[s + 1] += N.ch_ptr[s];
+  N.ch_idx.resize(N.ch_ptr[nsub]);
+  N.order.clear();
+  N.order.reserve(nsub);
+  if (!nested) {
+    for (int64_t s = 0; s < nsub; s++) N.order.push_back((int)s);
+    return HBT_OK;
+  }
+  {
+    static thread_local std::vector<int> fill;
+    fill.assign(N.ch_ptr.begin(), N.ch_ptr.end() - 1);
+    for (int64_t s = 0; s < nsub; s++)
+      if (parent[s] >= 0) N.ch_idx[fill[parent[s]]++] = (int)s;
+  }
+  // post-order from the roots (index order) gives heights and capacities
+  std::vector<std::pair<int, int>> st;
+  for (int64_t r = 0; r < nsub; r++) {
+    if (parent[r] >= 0) continue;
+    st.push_back({(int)r, 0});
     while (!st.empty()) {
       auto &t = st.back();
-      if (t.second < children[t.first].size()) {
-        int ch = children[t.first][t.second++];
+      if (t.second < N.nchild(t.first)) {
+        const int ch = N.child(t.first)[t.second++];
         st.push_back({ch, 0});
         if ((int64_t)st.size() > nsub + 1) return fail(HBT_ERR_INVALID_ARG, "nest too deep");
       } else {
-        int s = t.first;
-        order.push_back(s);
+        const int s = t.first;
+        N.order.push_back(s);
         int h = 0;
-        for (int ch : children[s]) { h = std::max(h, height[ch] + 1); cap[s] += cap[ch]; }
-        height[s] = h;
+        for (int i = 0; i < N.nchild(s); i++) {
+          const int ch = N.child(s)[i];
+          h = std::max(h, N.height[ch] + 1);
+          N.f[s].cap += N.f[ch].cap;
+        }
+        N.height[s] = h;
         st.pop_back();
       }
     }
   }
-  if ((int64_t)order.size() != nsub) return fail(HBT_ERR_INVALID_ARG, "cycle in parent links");
-  roots = order;  // post-order (RecursiveUnbind order)
+  if ((int64_t)N.order.size() != nsub) return fail(HBT_ERR_INVALID_ARG, "cycle in parent links");
   return HBT_OK;
 }
 
@@ -1156,14 +1278,12 @@ int hbt_ctx_stream(hbt_ctx *ctx, void **stream) {
 
 int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
   if (nsub < 0 || (nsub > 0 && !subs) || !total) return fail(HBT_ERR_INVALID_ARG, "null argument");
-  std::vector<int64_t> cap;
-  std::vector<int> height, order;
-  std::vector<std::vector<int>> children;
-  RETCHK(plan_nests(subs, nsub, cap, height, children, order));
+  Nest N;
+  RETCHK(plan_nests(subs, nsub, N));
   int64_t off = 0;
   for (int64_t s = 0; s < nsub; s++) {
     subs[s].out_offset = off;
-    off += cap[s];
+    off += N.f[s].cap;
   }
   *total = off;
   return HBT_OK;
@@ -1320,11 +1440,10 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
       if (subs_host[s].offset < 0 || subs_host[s].count < 0 || subs_host[s].offset + subs_host[s].count > n_list)
         return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the list positions");
     {
-      std::vector<int64_t> cap;
-      std::vector<int> height, order;
-      std::vector<std::vector<int>> children;
-      RETCHK(plan_nests(subs_host, nsub, cap, height, children, order));
-      out_cap = cap;
+      Nest N;
+      RETCHK(plan_nests(subs_host, nsub, N));
+      out_cap.resize(nsub);
+      for (int64_t s = 0; s < nsub; s++) out_cap[s] = N.f[s].cap;
     }
     std::vector<KickIn> kin(nsub);
     for (int64_t s = 0; s < nsub; s++)
@@ -1388,15 +1507,13 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   HIPCHK(hipSetDevice(ctx->device));
   // host tables kept per thread from call to call (their memory is reused,
   // so a repeated batch pays no page faults for them)
-  static thread_local std::vector<int64_t> cap;
-  static thread_local std::vector<int> height, order;
-  static thread_local std::vector<std::vector<int>> children;
+  static thread_local Nest N;
   static thread_local Level L;
-  RETCHK(plan_nests(subs_host, nsub, cap, height, children, order));
+  RETCHK(plan_nests(subs_host, nsub, N));
   for (int64_t s = 0; s < nsub; s++) {
-    if (subs_host[s].offset < 0 || subs_host[s].offset + subs_host[s].count > n_list)
+    if (N.f[s].offset < 0 || N.f[s].offset + N.f[s].count > n_list)
       return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the particle columns");
-    if (cap[s] >= (int64_t)INT32_MAX / 2) return fail(HBT_ERR_UNSUPPORTED, "subhalo list too long");
+    if (N.f[s].cap >= (int64_t)INT32_MAX / 2) return fail(HBT_ERR_UNSUPPORTED, "subhalo list too long");
   }
   hbt_particles P = *particles;
   int64_t passes = 0;
@@ -1425,9 +1542,11 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   // order in which sampled lists draw their shuffles from rand()); the
   // batched form runs a level at once, the plain form one subhalo at a time
   int hmax = 0;
-  for (int64_t s = 0; s < nsub; s++) hmax = std::max(hmax, height[s]);
-  std::vector<std::vector<int>> by_h(hmax + 1);
-  for (int64_t s = 0; s < nsub; s++) by_h[height[s]].push_back((int)s);
+  for (int64_t s = 0; s < nsub; s++) hmax = std::max(hmax, N.height[s]);
+  static thread_local std::vector<std::vector<int>> by_h;
+  by_h.resize(hmax + 1);
+  for (auto &v : by_h) v.clear();
+  for (int64_t s = 0; s < nsub; s++) by_h[N.height[s]].push_back((int)s);
   static thread_local std::vector<int64_t> draw_off;
   static thread_local std::vector<hbt_subhalo> recs;
   if (MS > 0) draw_off.assign(2 * nsub, -1);
@@ -1438,16 +1557,16 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     Lv.ndraws = 0;
     if (MS <= 0) return HBT_OK;
     bool nested = false;
-    for (int s : members) nested = nested || !children[s].empty();
+    for (int s : members) nested = nested || N.nchild(s) > 0;
     if (nested) {
       recs.resize(nsub);
       HIPCHK(hipMemcpyAsync(recs.data(), subs, nsub * sizeof(hbt_subhalo), hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     for (int s : members) {
-      int64_t len = subs_host[s].count;
-      if (subs_host[s].nbound_prev > 1)
-        for (int ch : children[s]) len += recs[ch].list_len - recs[ch].nbound;
+      int64_t len = N.f[s].count;
+      if (N.f[s].nbp > 1)
+        for (int i = 0; i < N.nchild(s); i++) len += recs[N.child(s)[i]].list_len - recs[N.child(s)[i]].nbound;
       draw_off[s] = -1;
       draw_off[nsub + s] = len;
       if (len > MS) {
@@ -1470,7 +1589,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
         const int64_t nd = L.ndraws;
         uint32_t win[31];
         memcpy(win, L.win, sizeof win);
-        build_level(subs_host, cap, children, one, ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
+        build_level(N, one, ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
         L.ndraws = nd;
         memcpy(L.win, win, sizeof win);
         RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
@@ -1483,7 +1602,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     uint32_t win[31];
     memcpy(win, L.win, sizeof win);
     // build_level lays each level out largest first
-    build_level(subs_host, cap, children, by_h[h], ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
+    build_level(N, by_h[h], ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
     L.ndraws = nd;
     memcpy(L.win, win, sizeof win);
     if (pass_times)
@@ -1495,8 +1614,8 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   gap.clear();
   int64_t gap_total = 0;
   for (int64_t s = 0; s < nsub; s++) {
-    const int64_t room = out_cap ? out_cap[s] : cap[s];
-    if (!children[s].empty() || room > cap[s]) {
+    const int64_t room = out_cap ? out_cap[s] : N.f[s].cap;
+    if (N.nchild(s) > 0 || room > N.f[s].cap) {
       gap.push_back(s);
       gap.push_back(room);
       gap.push_back(gap_total);
@@ -1549,10 +1668,11 @@ int hbt_profile(hbt_ctx *ctx, int mode) {
 }
 
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int64_t *interactions,
-                     int64_t *opened, int64_t *targets, int64_t *lane_slots, int64_t *tree_items) {
+                     int64_t *opened, int64_t *targets, int64_t *lane_slots, int64_t *tree_items,
+                     int64_t *wave_steps) {
   if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(ctx->device));
-  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long h[7] = {0, 0, 0, 0, 0, 0, 0};
   if (ctx->counters.p) {
     HIPCHK(hipMemcpyAsync(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1564,6 +1684,7 @@ int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int6
   if (targets) *targets = (int64_t)h[2];
   if (lane_slots) *lane_slots = (int64_t)h[3];
   if (tree_items) *tree_items = (int64_t)h[5];
+  if (wave_steps) *wave_steps = (int64_t)h[6];
   return HBT_OK;
 }
 

@@ -303,12 +303,12 @@ class GpuUnbinder:
 
     def profile_read(self):
         ms = ctypes.c_double()
-        vals = [ctypes.c_int64() for _ in range(6)]
+        vals = [ctypes.c_int64() for _ in range(7)]
         check(self.lib.hbt_profile_read(self.ctx, ctypes.byref(ms), *[ctypes.byref(v) for v in vals]),
               "hbt_profile_read")
         return dict(walk_ms=ms.value, walk_launches=vals[0].value, interactions=vals[1].value,
                     opened=vals[2].value, targets=vals[3].value, lane_slots=vals[4].value,
-                    tree_items=vals[5].value)
+                    tree_items=vals[5].value, wave_steps=vals[6].value)
 
     # per-subhalo form with host buffers --------------------------------------
     def unbind_one(self, cols: dict, sub: np.ndarray):

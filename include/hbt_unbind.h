@@ -65,14 +65,14 @@ extern "C" {
 #define HBT_ERR_CAPACITY       -5   /* an output buffer is too small */
 
 /* ---- precision policies -------------------------------------------------- */
-/* FAST: pair/tree arithmetic in FP32 with Kahan-compensated potential sums;
- *       frame averages, kinetic energy and the E<0 cut in FP64.
+/* FAST: pair/tree arithmetic in FP32, each target's potential terms summed
+ *       in FP64; frame averages, kinetic energy and the E<0 cut in FP64.
  * REPLAY: the reference's own arithmetic (FP64 walk over float node data,
  *       float-rounded node moments, serial-order FP64 frame sums, no FMA
  *       contraction) for bit-level certification. */
 #define HBT_PREC_FAST    0
 #define HBT_PREC_REPLAY  1
-#define HBT_PREC_FP32    2   /* FAST without the Kahan compensation (precision study) */
+#define HBT_PREC_FP32    2   /* FAST with the potential terms summed in FP32 (precision study) */
 
 /* direct-sum accumulation modes (hbt_direct_potential) */
 #define HBT_DIRECT_FP32        0   /* plain FP32 sum */
@@ -435,12 +435,13 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
  * 2: count, in every walk launch, the summed sources (leaves and accepted
  * nodes), the opened nodes, the targets, the lane slots spent by the lane
  * groups (items a group visited times its lanes, whether or not a lane
- * needed the item) and the items of the trees walked.
+ * needed the item), the items of the trees walked and the wave steps of the
+ * group walks (a wave's steps last until its slowest group is done).
  * Resets the accumulators. */
 int hbt_profile(hbt_ctx *ctx, int mode);
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches,
                      int64_t *interactions, int64_t *opened, int64_t *targets,
-                     int64_t *lane_slots, int64_t *tree_items);
+                     int64_t *lane_slots, int64_t *tree_items, int64_t *wave_steps);
 
 #ifdef __cplusplus
 }
