@@ -1927,8 +1927,15 @@ __global__ void output_record_kernel(Work W, DevParams P, int early) {
 }
 
 // FAST form, tile-parallel: partial sums per tile (6 kinematic sums, 6 type
-// counts, 6 type masses in FP64), combined per segment in tile order.
+// counts, 6 type masses in FP64), combined per segment in tile order.  A
+// converged segment's bound set is read in slot order -- the slots of the
+// list it unbound, [0, ulen), flagged in isb (every pass's partition flags the
+// slots it keeps, so the flags hold the final bound set) -- so the reads
+// stream through the columns instead of following the energy order; other
+// finished segments sum their first nbound slots.
 constexpr int REC_SUMS = 18;
+
+__device__ __forceinline__ int record_span(const Seg &S) { return (S.flags & F_CONV) ? S.ulen : S.nbound; }
 
 __global__ void __launch_bounds__(BLOCK)
 output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double *part) {
@@ -1938,7 +1945,7 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
     const Tile tl = tiles[t];
     const Seg &S = W.seg[tl.seg];
     const bool fin = (S.flags & F_FINAL) != 0;
-    const int cnt = fin ? min(TILE, S.nbound - tl.start) : 0;
+    const int cnt = fin ? min(TILE, record_span(S) - tl.start) : 0;
     if (cnt <= 0) continue;  // uniform
     const int64_t b = S.ibase;
     const bool conv = (S.flags & F_CONV) != 0;
@@ -1947,7 +1954,8 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
     double v[REC_SUMS];
     for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
     for (int i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int li = conv ? W.el[b + tl.start + i] : tl.start + i;
+      const int li = tl.start + i;
+      if (conv && !W.isb[b + li]) continue;
       if (nb > 1) kin_accum(W, P, S, li, v);
       if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
       const int ty = W.ptype[b + li];
@@ -1975,7 +1983,7 @@ __global__ void __launch_bounds__(BLOCK) output_combine_kernel(Work W, DevParams
     Seg &S = W.seg[k];
     if (!(S.flags & F_FINAL)) continue;  // uniform across the wave
     double v[REC_SUMS];
-    wave_sum_tiles<REC_SUMS>(part, first_tile[k], (S.nbound + TILE - 1) / TILE, v);
+    wave_sum_tiles<REC_SUMS>(part, first_tile[k], (record_span(S) + TILE - 1) / TILE, v);
     if ((threadIdx.x & (WAVE - 1)) != 0) continue;
     int nt[HBT_TYPE_MAX];
     float mts[HBT_TYPE_MAX];
