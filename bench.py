@@ -190,7 +190,7 @@ def walk_roofline(U, run, ms_per_step, config):
 
 
 def potential_error(params, cols, subs, nsample=64, max_count=60000, seed=0):
-    """FAST tree potential (fp32 + Kahan, theta as configured) of whole
+    """FAST tree potential (fp32 terms summed in fp64, theta as configured) of whole
     source lists against the FP64 direct pair sum and against the reference
     tree (REPLAY precision: the reference's own arithmetic, bit for bit), on
     a sample of subhalos: the 8 largest lists of at most max_count particles
@@ -334,7 +334,7 @@ def bench_c5(args):
                          "subhalos": int(len(cat.subs)), "listed_particles": int(len(ids)),
                          "snapshot_particles": int(len(snap["x"])), "evals_per_step": evals,
                          "passes_per_step": int(passes), "id_resolution_ms": round(resolve_ms, 2),
-                         "precision": "FAST (fp32+Kahan walk, fp64 frames)", "sampling": SAMPLING_NOTE,
+                         "precision": "FAST (fp32 walk terms summed in fp64, fp64 frames)", "sampling": SAMPLING_NOTE,
                          "tuning": tuning},
               "roofline": roofline}
     # post-unbind properties on the step's outputs (snapshot positions)
@@ -478,7 +478,7 @@ def bench_catalogue(args):
                          "particles": int(cat.n_particles), "evals_per_step": evals,
                          "passes_per_step": int(passes),
                          "precision": "REPLAY (fp64 walk)" if args.replay else
-                         "FAST (fp32+Kahan walk, fp64 frames)", "sampling": SAMPLING_NOTE,
+                         "FAST (fp32 walk terms summed in fp64, fp64 frames)", "sampling": SAMPLING_NOTE,
                          "device_ms_per_step": round(dev_ms / args.steps, 3),
                          "tuning": tuning},
               "roofline": roofline}

@@ -6,7 +6,7 @@ lists must be bit-identical (lists up to the order of exactly tied energies).
 The one documented exception is the sub-resolution floor, where the reference
 places coincident particles with the process-global drand48 stream; there the
 GPU keeps such particles in one bucket cell and parity is tolerant.
-FAST precision (FP32+Kahan walk, FP64 frames) meets the tolerant criteria:
+FAST precision (FP32 walk terms summed in FP64, FP64 frames) meets the tolerant criteria:
 bound sets identical outside the 1e-5*|E_min| band around E=0, bound counts
 within the band count, most-bound IDs equal unless tied within the band,
 potentials within 1e-4 relative RMS of the reference tree."""

@@ -4,8 +4,9 @@ log-uniform in [20, 4096], NFW c=10, 15% contaminants).
 Potentials: FP32 and FP32+Kahan pair sums against the FP64 pair sum (itself
 checked against the CPU oracle), per-subhalo relative RMS; the bound is 1e-4.
 Unbinding on the exact pair sum (opening angle 0 opens every node, so the
-walk is the pair sum with the reference's softened kernel): the GPU's
-FP32+Kahan (FAST) and plain FP32 forms against the oracle's FP64 unbinding
+walk is the pair sum with the reference's softened kernel): the GPU's FAST
+form (FP32 terms summed in FP64) and its plain FP32 form (terms summed in
+FP32) against the oracle's FP64 unbinding
 (the reference's arithmetic), on config 3 itself -- whose contaminants at
 1000 km/s pull most initial frames off their halo, so most subhalos disrupt
 -- and on the same size law with config 2's contaminants at three times the
@@ -70,7 +71,7 @@ def test_c3_precision_study(gpu, contaminants):
     # unbinding at opening angle 0: the oracle's FP64 unbinding is the reference
     p0 = _with(cat.params, open_angle=0.0)
     ref = oracle.unbind_batch(cat, nthreads=16, params=p0)
-    for prec, name in ((abi.HBT_PREC_FAST, "fp32_kahan"), (abi.HBT_PREC_FP32, "fp32")):
+    for prec, name in ((abi.HBT_PREC_FAST, "fast"), (abi.HBT_PREC_FP32, "fp32")):
         got = gpu(_with(p0, precision=prec)).unbind_batch(cat)
         counts = _unbind_counts(ref, got)
         try:
@@ -91,7 +92,7 @@ def test_c3_precision_study(gpu, contaminants):
     assert report["potential_rel_rms_fp32_kahan"]["median"] <= report["potential_rel_rms_fp32"]["median"]
     # unbinding: FAST meets the band rule on every subhalo, so its Nbound
     # differences and most-bound mismatches are those the band allows
-    fast = report["unbind_fp32_kahan"]
+    fast = report["unbind_fast"]
     assert fast["band_rule"] == "met", fast["band_rule"]
     assert fast["most_bound_id_mismatches"] == fast["mb_mismatches_within_band"]
     assert fast["subhalos_with_nbound_diff"] <= fast["set_differences"]
