@@ -451,6 +451,41 @@ def test_indexed_batch_c5_tenth_size(gpu):
     print(f"C5/10: {nconv} of {len(cat.subs)} converged, sample {stats}")
 
 
+def test_indexed_batch_c5_full_size(gpu):
+    """The whole one-rank share, as the bench runs it (200,000 subhalos, 64.3 M
+    listed particles in a shuffled 70.7 M-particle snapshot): IDs resolved on
+    the device and unbound from the snapshot; the size-free invariants over
+    every subhalo (lists mapped back to catalogue positions), the tolerant
+    criteria against the oracle on 300 subhalos drawn at random, and a second
+    call byte-identical"""
+    snap, ids, cat = synth.config5()
+    U = gpu(cat.params)
+    h = U.upload_snapshot(snap, ids, cat.subs)
+    idx = U.resolve_ids(h)
+    pos = idx.cpu().numpy()
+    assert np.array_equal(snap["id"][pos], ids)
+    s1, i1, e1 = U.unbind_indexed(h, idx)
+    subs = s1.cpu().numpy().view(abi.SUB_DTYPE).copy()
+    out_i = i1.cpu().numpy()
+    out_e = e1.cpu().numpy()
+    # snapshot position -> catalogue position (the catalogue holds the listed
+    # particles in list order, each once)
+    inv = np.full(len(snap["id"]), -1, np.int64)
+    inv[pos] = np.arange(len(pos))
+    cat_i = np.where(out_i >= 0, inv[np.where(out_i >= 0, out_i, 0)], -1)
+    got = (subs, cat_i, out_e)
+    nconv = check_unbind_invariants(cat, got)
+    sel = np.sort(np.random.default_rng(25).choice(len(cat.subs), 300, replace=False))
+    part = cat.subset(sel)
+    rs, ri, re_ = oracle.unbind_batch(part, nthreads=16)
+    stats = assert_tolerant((rs, as_ids(part.cols["id"], ri), re_), (subs[sel], as_ids(cat.cols["id"], cat_i), out_e))
+    s2, i2, e2 = U.unbind_indexed(h, U.resolve_ids(h))
+    assert np.array_equal(s2.cpu().numpy().view(np.uint8), s1.cpu().numpy().view(np.uint8))
+    assert np.array_equal(i2.cpu().numpy(), out_i)
+    assert np.array_equal(e2.cpu().numpy().view(np.uint32), out_e.view(np.uint32))
+    print(f"C5 full: {nconv} of {len(cat.subs)} converged, sample {stats}")
+
+
 def test_particle_type_outside_the_table_is_an_error(gpu):
     """the per-type bound counts index by type: a type outside [0, 6) is
     rejected before any unbinding"""
