@@ -149,9 +149,11 @@ def walk_roofline(U, run, ms_per_step, config):
     U.profile(1)
     run()
     prof_t = U.profile_read()
+    sort_t = U.profile_sorts()
     U.profile(2)
     run()
     prof_c = U.profile_read()
+    sort_c = U.profile_sorts()
     U.profile(0)
     I, O, T = prof_c["interactions"], prof_c["opened"], prof_c["targets"]
     lane_eff = (I + O) / max(prof_c["lane_slots"], 1)
@@ -186,7 +188,24 @@ def walk_roofline(U, run, ms_per_step, config):
                 opened_per_eval=round(O / max(T, 1), 1), lane_efficiency=round(lane_eff, 3),
                 wave_efficiency=round(wave_eff, 3), wave_steps_per_step=prof_c["wave_steps"],
                 ns_per_wave_step=round(prof_t["walk_ms"] * 1e6 / max(prof_c["wave_steps"], 1), 3),
-                walk_share_of_step=round(prof_t["walk_ms"] / ms_per_step, 3)), flops
+                walk_share_of_step=round(prof_t["walk_ms"] / ms_per_step, 3),
+                sorts=sort_roofline(sort_t, sort_c, ms_per_step)), flops
+
+
+def sort_roofline(sort_t, sort_c, ms_per_step):
+    """HBM roofline of the segmented sorts of a step (the octant keys of every
+    tree, the energy order): the tile sort reads and writes each element's
+    12-byte key and value once, and every merge level once more, so the
+    algorithmic bytes are 24 B x (elements + elements x merge levels); timed
+    with HIP events around each sort (tile sort and its merge levels)."""
+    alg = 24.0 * (sort_c["elements"] + sort_c["element_levels"])
+    s = sort_t["sort_ms"] / 1e3
+    achieved = alg / s / 1e9 if s > 0 else 0.0
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(achieved / PEAK_HBM_GBS, 4), kernels="tile_sort_kernel + merge_pass_kernel",
+                sorts_per_step=sort_t["sort_calls"], ms_per_step=round(sort_t["sort_ms"], 3),
+                elements_per_step=sort_c["elements"], element_levels_per_step=sort_c["element_levels"],
+                algorithmic_bytes_per_step=int(alg), share_of_step=round(sort_t["sort_ms"] / ms_per_step, 3))
 
 
 def potential_error(params, cols, subs, nsample=64, max_count=60000, seed=0):

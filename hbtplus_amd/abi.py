@@ -144,6 +144,7 @@ def load_library() -> ctypes.CDLL:
         "hbt_last_batch_stats": ([vp, vp, vp, vp], i32),
         "hbt_profile": ([vp, i32], i32),
         "hbt_profile_read": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+        "hbt_profile_sorts": ([vp, vp, vp, vp, vp], i32),
         "hbt_set_rand_state": ([vp, ctypes.c_uint32, i64], i32),
         "hbt_get_rand_state": ([vp, vp, vp], i32),
         "hbt_subhalo_properties": ([vp, vp, vp, i64, vp, vp], i32),

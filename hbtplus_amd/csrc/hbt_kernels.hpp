@@ -1330,6 +1330,23 @@ __global__ void __launch_bounds__(BLOCK) walk_shared_kernel(Work W, DevParams P,
   if (COUNT) wc.flush(W);
 }
 
+// Instrumented passes: the elements a segmented sort ordered (counters[8])
+// and the elements times the merge levels each needed (counters[9]).
+__global__ void sort_count_kernel(const int *n, int nseg, unsigned long long *counters) {
+  unsigned long long e = 0, el = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nseg; k += gridDim.x * blockDim.x) {
+    const int c = n[k] > 0 ? n[k] : 0;
+    e += (unsigned long long)c;
+    el += (unsigned long long)c * (unsigned long long)merge_levels(c);
+  }
+  e = wave_reduce_sum(e);
+  el = wave_reduce_sum(el);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && (e | el)) {
+    atomicAdd(&counters[8], e);
+    atomicAdd(&counters[9], el);
+  }
+}
+
 // Instrumented passes: the items of the trees the walk read (counters[5]).
 __global__ void tree_items_kernel(Work W) {
   SEG_LOOP(k, W) {

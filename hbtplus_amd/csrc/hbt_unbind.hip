@@ -154,6 +154,9 @@ struct hbt_ctx {
   hipEvent_t evw0 = nullptr, evw1 = nullptr;
   double walk_ms = 0;
   int64_t walk_launches = 0;
+  hipEvent_t evs0 = nullptr, evs1 = nullptr;  // the segmented sorts of the passes
+  double sort_ms = 0;
+  int64_t sort_calls = 0;
   // staging for hbt_unbind_one / hbt_tree_potential
   DBuf st_cols, st_subs, st_out_i, st_out_e, st_phi;
   int *h_active = nullptr;  // pinned
@@ -184,6 +187,8 @@ struct hbt_ctx {
     if (ev1) (void)hipEventDestroy(ev1);
     if (evw0) (void)hipEventDestroy(evw0);
     if (evw1) (void)hipEventDestroy(evw1);
+    if (evs0) (void)hipEventDestroy(evs0);
+    if (evs1) (void)hipEventDestroy(evs1);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -647,6 +652,7 @@ struct Runner {
   int sort_pairs(const int *n) {  // sorts (k0,v0) by the counts n[seg], result in (k0,v0)
     const Tile *tl = tl_cur;
     const int nt = nt_cur;
+    if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evs0, c->stream));
     tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
     // one launch per merge level of the largest segment of the level, over
     // the tiles of the segments whose capacity exceeds the level's width
@@ -657,6 +663,18 @@ struct Runner {
       if (ntw == 0) break;
       const int gmerge = (int)std::max<int64_t>(1, std::min<int64_t>(ntw, (int64_t)c->cus * 16));
       merge_pass_kernel<<<gmerge, BLOCK, 0, c->stream>>>(tl, ntw, W.ibase, n, level, W.k0, W.v0, W.k1, W.v1);
+    }
+    // instrumentation: the sort's device time, or the elements it sorted and
+    // their merge levels
+    if (c->prof_mode == 1) {
+      HIPCHK(hipEventRecord(c->evs1, c->stream));
+      HIPCHK(hipEventSynchronize(c->evs1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->evs0, c->evs1));
+      c->sort_ms += ms;
+      c->sort_calls++;
+    } else if (c->prof_mode == 2) {
+      sort_count_kernel<<<gseg, BLOCK, 0, c->stream>>>(n, W.nseg, W.counters);
     }
     return HBT_OK;
   }
@@ -866,7 +884,7 @@ int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   RETCHK(c->tsum.ensure((size_t)(ntiles + 1) * 8));
   RETCHK(c->tpart.ensure((size_t)(ntiles + 1) * 8 * REC_SUMS));
   RETCHK(c->active.ensure(16));
-  RETCHK(c->counters.ensure(64));
+  RETCHK(c->counters.ensure(128));
   return HBT_OK;
 }
 
@@ -1224,6 +1242,8 @@ int hbt_ctx_create(const hbt_params *params, int device, hbt_ctx **out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipEventCreate(&c->evw0);
   if (e == hipSuccess) e = hipEventCreate(&c->evw1);
+  if (e == hipSuccess) e = hipEventCreate(&c->evs0);
+  if (e == hipSuccess) e = hipEventCreate(&c->evs1);
   if (e == hipSuccess) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
@@ -1661,8 +1681,10 @@ int hbt_profile(hbt_ctx *ctx, int mode) {
   ctx->prof_mode = mode;
   ctx->walk_ms = 0;
   ctx->walk_launches = 0;
-  RETCHK(ctx->counters.ensure(64));
-  HIPCHK(hipMemsetAsync(ctx->counters.p, 0, 64, ctx->stream));
+  ctx->sort_ms = 0;
+  ctx->sort_calls = 0;
+  RETCHK(ctx->counters.ensure(128));
+  HIPCHK(hipMemsetAsync(ctx->counters.p, 0, 128, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return HBT_OK;
 }
@@ -1685,6 +1707,23 @@ int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches, int6
   if (lane_slots) *lane_slots = (int64_t)h[3];
   if (tree_items) *tree_items = (int64_t)h[5];
   if (wave_steps) *wave_steps = (int64_t)h[6];
+  return HBT_OK;
+}
+
+int hbt_profile_sorts(hbt_ctx *ctx, double *sort_ms, int64_t *sort_calls, int64_t *elements,
+                      int64_t *element_levels) {
+  if (!ctx) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned long long h[2] = {0, 0};
+  if (ctx->counters.p) {
+    HIPCHK(hipMemcpyAsync(h, ctx->counters.as<unsigned long long>() + 8, sizeof(h), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  if (sort_ms) *sort_ms = ctx->sort_ms;
+  if (sort_calls) *sort_calls = ctx->sort_calls;
+  if (elements) *elements = (int64_t)h[0];
+  if (element_levels) *element_levels = (int64_t)h[1];
   return HBT_OK;
 }
 

@@ -310,6 +310,14 @@ class GpuUnbinder:
                     opened=vals[2].value, targets=vals[3].value, lane_slots=vals[4].value,
                     tree_items=vals[5].value, wave_steps=vals[6].value)
 
+    def profile_sorts(self):
+        ms = ctypes.c_double()
+        vals = [ctypes.c_int64() for _ in range(3)]
+        check(self.lib.hbt_profile_sorts(self.ctx, ctypes.byref(ms), *[ctypes.byref(v) for v in vals]),
+              "hbt_profile_sorts")
+        return dict(sort_ms=ms.value, sort_calls=vals[0].value, elements=vals[1].value,
+                    element_levels=vals[2].value)
+
     # per-subhalo form with host buffers --------------------------------------
     def unbind_one(self, cols: dict, sub: np.ndarray):
         cols = {k: np.ascontiguousarray(v) for k, v in cols.items()}

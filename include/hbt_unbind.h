@@ -442,6 +442,13 @@ int hbt_profile(hbt_ctx *ctx, int mode);
 int hbt_profile_read(hbt_ctx *ctx, double *walk_ms, int64_t *walk_launches,
                      int64_t *interactions, int64_t *opened, int64_t *targets,
                      int64_t *lane_slots, int64_t *tree_items, int64_t *wave_steps);
+/* The segmented sorts of the unbinding passes (the octant keys of each tree,
+ * the energy order): mode 1 times every sort (tile sort and its merge levels)
+ * with HIP events, mode 2 counts the elements sorted and the elements times
+ * the merge levels each needed (every level reads and writes each element's
+ * 12-byte key and value once, the tile sort once more). */
+int hbt_profile_sorts(hbt_ctx *ctx, double *sort_ms, int64_t *sort_calls, int64_t *elements,
+                      int64_t *element_levels);
 
 #ifdef __cplusplus
 }
