@@ -486,6 +486,38 @@ def test_indexed_batch_c5_full_size(gpu):
     print(f"C5 full: {nconv} of {len(cat.subs)} converged, sample {stats}")
 
 
+def test_instrumentation_counts_and_outputs(gpu):
+    """hbt_profile modes 1 (events around the walk and the sorts) and 2 (the
+    counting walk and sort counts) leave the outputs unchanged, and their
+    counts hang together: every walk launch timed, interactions and opened
+    nodes within the lane slots the groups spent, those within the wave steps
+    (64 lanes each), the targets equal to the evaluations the records count,
+    and sorts of the pass's elements with at least one merge level for lists
+    above one tile"""
+    cat = synth.config2(seed=66, nsub=300, hi=20000)
+    U = gpu(cat.params)
+    base = U.unbind_batch(cat)
+    passes = U.last_stats()[2]
+    U.profile(1)
+    got = U.unbind_batch(cat)
+    t, srt = U.profile_read(), U.profile_sorts()
+    assert _same_bytes(base, got)
+    assert t["walk_launches"] == passes and t["walk_ms"] > 0
+    assert srt["sort_calls"] >= passes and srt["sort_ms"] > 0
+    U.profile(2)
+    got = U.unbind_batch(cat)
+    c, cs = U.profile_read(), U.profile_sorts()
+    U.profile(0)
+    assert _same_bytes(base, got)
+    assert c["targets"] == int(base[0]["n_evals"].sum())
+    assert 0 < c["interactions"] + c["opened"] <= c["lane_slots"] <= 64 * c["wave_steps"]
+    assert c["tree_items"] > 0
+    assert cs["elements"] >= int(cat.subs["count"].sum())
+    big = cat.subs["count"] > 1024
+    assert (cs["element_levels"] > 0) == bool(big.any())
+    assert _same_bytes(base, U.unbind_batch(cat))
+
+
 def test_particle_type_outside_the_table_is_an_error(gpu):
     """the per-type bound counts index by type: a type outside [0, 6) is
     rejected before any unbinding"""
