@@ -56,7 +56,11 @@ struct Cosmology_t {
 };
 
 // HBTConfig fields read by the unbinding, with the reference's defaults
-// (src/config_parser.h:68-110) except sampling (unsupported, 0).
+// (src/config_parser.h:68-110).  With MaxSampleSizeOfPotentialEstimate > 0
+// lists longer than the sample are shuffled with the rand() stream the
+// context keeps (srand(1) at creation, as a process that never seeds;
+// Unbinder::SetRandState sets it), full-pass trees use the sample, and
+// RefineMostboundParticle re-orders the most bound entries.
 struct UnbindConfig {
   float G = 43007.1f;
   float SofteningHalo = 0.5f;
@@ -66,6 +70,8 @@ struct UnbindConfig {
   float BoundMassPrecision = 0.995f;
   int MinNumPartOfSub = 20;
   float SourceSubRelaxFactor = 3.f;
+  int MaxSampleSizeOfPotentialEstimate = 1000;
+  bool RefineMostboundParticle = true;
   int precision = HBT_PREC_FAST;
   hbt_params params(const Cosmology_t &c, int snapshot_index) const {
     hbt_params p{};
@@ -77,7 +83,8 @@ struct UnbindConfig {
     p.bound_mass_precision = BoundMassPrecision;
     p.min_num_part_of_sub = MinNumPartOfSub;
     p.source_sub_relax_factor = SourceSubRelaxFactor;
-    p.max_sample_size = 0;
+    p.max_sample_size = MaxSampleSizeOfPotentialEstimate;
+    p.refine_mostbound_particle = RefineMostboundParticle ? 1 : 0;
     p.scale_factor = c.ScaleFactor;
     p.hz = c.Hz;
     p.snapshot_index = snapshot_index;
@@ -127,6 +134,10 @@ class Unbinder {
   ~Unbinder() { hbt_ctx_destroy(ctx_); }
   Unbinder(const Unbinder &) = delete;
   Unbinder &operator=(const Unbinder &) = delete;
+
+  // The rand() stream the sampling shuffles draw from: srand(seed) followed
+  // by `draws` calls (a caller that calls rand() elsewhere keeps it in step).
+  void SetRandState(uint32_t seed, int64_t draws) { check(hbt_set_rand_state(ctx_, seed, draws), "hbt_set_rand_state"); }
 
   // Subhalo_t::Unbind: one subhalo, host buffers, list permuted in place.
   void Unbind(Subhalo_t &sub) {

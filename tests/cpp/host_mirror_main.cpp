@@ -49,6 +49,8 @@ int main(int argc, char **argv) {
   cfg.BoxSize = p.box_size; cfg.PeriodicBoundaryOn = p.periodic != 0;
   cfg.BoundMassPrecision = p.bound_mass_precision; cfg.MinNumPartOfSub = p.min_num_part_of_sub;
   cfg.SourceSubRelaxFactor = p.source_sub_relax_factor; cfg.precision = p.precision;
+  cfg.MaxSampleSizeOfPotentialEstimate = p.max_sample_size;
+  cfg.RefineMostboundParticle = p.refine_mostbound_particle != 0;
   hbt::Cosmology_t cosmo;
   cosmo.ScaleFactor = p.scale_factor;
   cosmo.Hz = p.hz;

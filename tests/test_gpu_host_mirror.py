@@ -51,9 +51,14 @@ def test_host_driver_is_built():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["batch", "plain"])
-def test_host_mirror_refine_particles(mode):
-    cat, (rs, ri, re_) = load_unbind("c4_small")
+@pytest.mark.parametrize("mode,name", [("batch", "c4_small"), ("plain", "c4_small"), ("batch", "samp_c4"),
+                                       ("plain", "samp_c2")])
+def test_host_mirror_refine_particles(mode, name):
+    """RefineParticles through the mirror, REPLAY, against the reference's
+    outputs; samp_*: sampling on (MaxSampleSizeOfPotentialEstimate and
+    RefineMostboundParticle mapped from the configuration, the shuffles
+    drawn from the context's stream from srand(1))"""
+    cat, (rs, ri, re_) = load_unbind(name)
     cat.params["precision"] = abi.HBT_PREC_REPLAY
     got = run_host(cat, mode)
     ids = cat.cols["id"]
