@@ -38,8 +38,23 @@ METRIC = "subhalo-particle energy evals/s: batched mixed-prec vs plain GPU, same
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 LANE_SLOTS_PER_S = 256 * 4 * 16 * 2.4e9
-SAMPLING_NOTE = ("MaxSampleSizeOfPotentialEstimate=0 (no sampling), the configs' setting; the "
-                 "reference's default samples 1000 particles per full-pass tree")
+SAMPLE = {"max_sample_size": 0}  # --max-sample: MaxSampleSizeOfPotentialEstimate for every leg
+
+
+def sampling_note():
+    ms = SAMPLE["max_sample_size"]
+    if ms <= 0:
+        return ("MaxSampleSizeOfPotentialEstimate=0 (no sampling), the configs' setting; the "
+                "reference's default samples 1000 particles per full-pass tree")
+    return (f"MaxSampleSizeOfPotentialEstimate={ms} with RefineMostboundParticle (the reference's default "
+            f"mode): lists longer than the sample shuffled from one rand() stream, full-pass trees over "
+            f"the first {ms} entries")
+
+
+def apply_sampling(cat):
+    if SAMPLE["max_sample_size"] > 0:
+        cat.params["max_sample_size"] = SAMPLE["max_sample_size"]
+        cat.params["refine_mostbound_particle"] = 1
 
 
 def log(msg):
@@ -65,7 +80,10 @@ def walk_traffic(config):
     """HBM bytes per walk launch for a workload, from the committed counter
     summary profiles/walk_<config>_traffic.md (FETCH_SIZE and WRITE_SIZE
     passes of rocprofv3, each scaled by its reading for a copy of known
-    size; see that file), or None."""
+    size; see that file), or None.  The summaries cover the workloads without
+    sampling; a sampled run reports None."""
+    if SAMPLE["max_sample_size"] > 0:
+        return None
     path = os.path.join(ROOT, "profiles", f"walk_{config}_traffic.md")
     if not os.path.exists(path):
         return None
@@ -131,15 +149,17 @@ def cpu_baseline(cat, budget_s=20.0, threads=None, with_reference=False):
     sample = (f"{take} of {len(order)} {what} drawn at random ({sub.n_particles} particles, {evals} "
               f"evals), OpenMP over nests, schedule(dynamic,1)")
     port = dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
-                sample=f"{sample}; {elapsed:.1f} s", sampling=SAMPLING_NOTE, **host)
+                sample=f"{sample}; {elapsed:.1f} s", sampling=sampling_note(), **host)
     if not (with_reference and oracle.have_reference()):
         return port
     # the reference's own unbinding, compiled from its sources, on the same sample
     log(f"cpu baseline: reference on the same {take}")
     secs = oracle.ref_time_unbind(sub, threads)
+    same = ("same trajectory" if SAMPLE["max_sample_size"] <= 0 else
+            "the reference's threads draw rand() in scheduling order, so its shuffles and count may differ slightly")
     return dict(value=evals / secs, unit="evals/s", cores=threads, kind="reference",
-                sample=f"{sample}; {secs:.1f} s (evaluation count from the port, same trajectory)",
-                sampling=SAMPLING_NOTE, port=port, **host)
+                sample=f"{sample}; {secs:.1f} s (evaluation count from the port, {same})",
+                sampling=sampling_note(), port=port, **host)
 
 
 def walk_roofline(U, run, ms_per_step, config):
@@ -299,6 +319,7 @@ def bench_c5(args):
 
     t0 = time.perf_counter()
     snap, ids, cat = synth.config5()
+    apply_sampling(cat)
     gen_s = time.perf_counter() - t0
     params = cat.params.copy()
     params["precision"] = abi.HBT_PREC_FAST
@@ -353,7 +374,7 @@ def bench_c5(args):
                          "subhalos": int(len(cat.subs)), "listed_particles": int(len(ids)),
                          "snapshot_particles": int(len(snap["x"])), "evals_per_step": evals,
                          "passes_per_step": int(passes), "id_resolution_ms": round(resolve_ms, 2),
-                         "precision": "FAST (fp32 walk terms summed in fp64, fp64 frames)", "sampling": SAMPLING_NOTE,
+                         "precision": "FAST (fp32 walk terms summed in fp64, fp64 frames)", "sampling": sampling_note(),
                          "tuning": tuning},
               "roofline": roofline}
     # post-unbind properties on the step's outputs (snapshot positions)
@@ -456,6 +477,7 @@ def bench_catalogue(args):
     from hbtplus_amd.device import GpuUnbinder
 
     cat, desc = make_workload(args.config)
+    apply_sampling(cat)
     params = cat.params.copy()
     params["precision"] = abi.HBT_PREC_REPLAY if args.replay else abi.HBT_PREC_FAST
     U = GpuUnbinder(params, device=0)
@@ -497,7 +519,7 @@ def bench_catalogue(args):
                          "particles": int(cat.n_particles), "evals_per_step": evals,
                          "passes_per_step": int(passes),
                          "precision": "REPLAY (fp64 walk)" if args.replay else
-                         "FAST (fp32 walk terms summed in fp64, fp64 frames)", "sampling": SAMPLING_NOTE,
+                         "FAST (fp32 walk terms summed in fp64, fp64 frames)", "sampling": sampling_note(),
                          "device_ms_per_step": round(dev_ms / args.steps, 3),
                          "tuning": tuning},
               "roofline": roofline}
@@ -665,6 +687,9 @@ def main():
     ap.add_argument("--plain-chunks", type=int, default=20, help="chunks of the C5 plain run")
     ap.add_argument("--replay", action="store_true", help="REPLAY precision instead of FAST")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default execution tuning")
+    ap.add_argument("--max-sample", type=int, default=0, metavar="N",
+                    help="MaxSampleSizeOfPotentialEstimate (with RefineMostboundParticle) for the GPU and "
+                         "CPU legs; 0 (default): no sampling, the configs' setting")
     ap.add_argument("--tuning", default="", metavar="JSON",
                     help="set this execution tuning (the line's config.tuning) instead of autotuning")
     ap.add_argument("--cpu-only", type=int, default=0, metavar="THREADS",
@@ -674,6 +699,7 @@ def main():
                     help="first copy MIB MiB device-to-device three times (a kernel of known "
                          "traffic that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    SAMPLE["max_sample_size"] = max(0, args.max_sample)
     if args.cpu_only:
         if args.config == "c5":
             from hbtplus_amd import synth
@@ -681,6 +707,7 @@ def main():
             desc = "C5 share (catalogue of gathered lists)"
         else:
             cat, desc = make_workload(args.config)
+        apply_sampling(cat)
         print(json.dumps({"config": args.config, "workload": desc,
                           "cpu": cpu_baseline(cat, threads=args.cpu_only, with_reference=True)}), flush=True)
         return
