@@ -40,6 +40,39 @@ def test_plan_batch_matches_python():
     assert np.array_equal(a["out_offset"], b["out_offset"])
 
 
+def test_plan_batch_large_nested_batch():
+    """70,000 records (the planner gathers them with several threads above
+    65,536): random forests of nests up to 6 deep, children listed after and
+    before their hosts; capacities and offsets equal the plain restatement's,
+    and a bad parent index anywhere is an error"""
+    lib = abi.load_library()
+    rng = np.random.default_rng(8)
+    n = 70000
+    s = np.zeros(n, abi.SUB_DTYPE)
+    s["count"] = rng.integers(0, 500, n)
+    parent = np.full(n, -1, np.int64)
+    depth = np.zeros(n, np.int64)
+    for i in rng.permutation(n)[: n // 2]:
+        p = int(rng.integers(0, n))
+        # keep the forest acyclic and shallow: no attachment under i itself or 6 deep
+        q, d = p, 0
+        while q >= 0 and q != i and d < 6:
+            q, d = parent[q], d + 1
+        if q == i or d >= 6 or p == i:
+            continue
+        parent[i] = p
+    s["parent"] = parent
+    a, b = s.copy(), s.copy()
+    t = ctypes.c_int64()
+    assert lib.hbt_plan_batch(a.ctypes.data, n, ctypes.byref(t)) == 0
+    assert t.value == abi.plan_batch(b)
+    assert np.array_equal(a["out_offset"], b["out_offset"])
+    for bad_parent in (n + 3, n - 7):  # past the batch; the record itself
+        c = s.copy()
+        c["parent"][n - 7] = bad_parent
+        assert lib.hbt_plan_batch(c.ctypes.data, n, ctypes.byref(t)) == -1
+
+
 def test_plan_batch_rejects_cycles():
     lib = abi.load_library()
     s = np.zeros(2, abi.SUB_DTYPE)
