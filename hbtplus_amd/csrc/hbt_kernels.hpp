@@ -973,27 +973,6 @@ __device__ __forceinline__ bool group_any(bool p, u64 lanes) {
   return (__ballot(p) & lanes) != 0ull;
 }
 
-// The same vote with the whole wave active, on the scalar unit: the wave's
-// ballot is OR-folded within each aligned group of GROUP lanes, spread back
-// over the group, and read per lane as a lane mask (inverse ballot).
-template <int GROUP>
-__device__ __forceinline__ bool group_any_wave(bool p) {
-  u64 b = __builtin_amdgcn_ballot_w64(p);
-  if (GROUP == WAVE) return b != 0ull;
-#pragma unroll
-  for (int d = 1; d < GROUP; d <<= 1) b |= b >> d;
-  constexpr u64 first = GROUP == 8 ? 0x0101010101010101ull
-                                   : (GROUP == 16 ? 0x0001000100010001ull : 0x0000000100000001ull);
-  constexpr u64 fill = GROUP >= 32 ? 0xffffffffull : ((1ull << (GROUP & 31)) - 1ull);
-  b = (b & first) * fill;
-  return __builtin_amdgcn_inverse_ballot_w64(b);
-}
-
-// Dword `dw` (a byte offset) of item i, at a 32-bit offset from the scalar base.
-__device__ __forceinline__ int item_dword(const char *__restrict__ base, int i, unsigned dw) {
-  return *(const int *)(base + ((unsigned)i * (unsigned)sizeof(Item) + dw));
-}
-
 // Lane K of the caller's aligned group of GROUP (8, 16 or 32) lanes, read by
 // every lane of the group (ds_swizzle, bit mode: source lane = (lane & AND) |
 // K within each half-wave).  Every lane of the wave must be active.
@@ -1002,17 +981,6 @@ __device__ __forceinline__ int group_field(int v) {
   static_assert(GROUP >= 8 && GROUP <= 32 && K < 8, "group_field: 8 to 32 lanes");
   constexpr int AND = 0x1f & ~(GROUP - 1);
   return __builtin_amdgcn_ds_swizzle(v, AND | (K << 5));
-}
-
-// group_any_wave on a lane mask taken straight from a compare.
-template <int GROUP>
-__device__ __forceinline__ bool group_any_mask(u64 b) {
-#pragma unroll
-  for (int d = 1; d < GROUP; d <<= 1) b |= b >> d;
-  constexpr u64 first = GROUP == 8 ? 0x0101010101010101ull
-                                   : (GROUP == 16 ? 0x0001000100010001ull : 0x0000000100000001ull);
-  constexpr u64 fill = GROUP >= 32 ? 0xffffffffull : ((1ull << (GROUP & 31)) - 1ull);
-  return __builtin_amdgcn_inverse_ballot_w64((b & first) * fill);
 }
 
 // Per-target arithmetic of one item in FP32: sep2 = squared separation,
@@ -1115,9 +1083,10 @@ struct ReplayTerm {
 // from the scalar base, one dword of the item per lane, and spread the six
 // fields over the group with crossbar swizzles; every lane of the wave takes
 // part in every step (lanes off their path discard the result), the dwords of
-// both possible successors are fetched as soon as skip(j) is known, and the
-// group's vote runs on the scalar unit over compare masks; a finished group
-// sits on the sentinel item at `end`, whose skip is itself.  The 64-lane form
+// both possible successors are fetched as soon as skip(j) is known, the
+// group's vote takes the group's field of the opened-lane mask, and the
+// accepting lanes add their terms without a branch; a finished group sits on
+// the sentinel item at `end`, whose skip is itself.  The 64-lane form
 // clamps its reads to the segment's last item, a leaf whose skip is `end`, and
 // its skips always advance (max with j+1).
 template <int GROUP, bool COUNT, class Term>
