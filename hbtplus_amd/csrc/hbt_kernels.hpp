@@ -57,8 +57,8 @@ struct Work {
   const float4 *g_packed;    // optional: the columns interleaved, 2 float4 per particle
   // working arrays [nslots]
   int *gidx;
-  float *px, *py, *pz, *vx, *vy, *vz, *pm;
-  int *ptype;
+  float4 *pos;               // x, y, z, mass: one record, so a gather reads one line
+  float4 *vel;               // vx, vy, vz, type (int bits)
   float *E;
   float *Er;                 // energies of the refinement pass (RefineBindingEnergyOrder)
   int *el, *el2, *sp;
@@ -228,14 +228,11 @@ __global__ void __launch_bounds__(BLOCK) gather_kernel(Work W, int phase) {
     }
     if (!(phase & 2)) continue;
     if (W.g_packed) {
-      const float4 a = W.g_packed[2 * (int64_t)g], c = W.g_packed[2 * (int64_t)g + 1];
-      W.px[s] = a.x; W.py[s] = a.y; W.pz[s] = a.z; W.pm[s] = a.w;
-      W.vx[s] = c.x; W.vy[s] = c.y; W.vz[s] = c.z; W.ptype[s] = __float_as_int(c.w);
+      W.pos[s] = W.g_packed[2 * (int64_t)g];
+      W.vel[s] = W.g_packed[2 * (int64_t)g + 1];
     } else {
-      W.px[s] = W.g_x[g]; W.py[s] = W.g_y[g]; W.pz[s] = W.g_z[g];
-      W.vx[s] = W.g_vx[g]; W.vy[s] = W.g_vy[g]; W.vz[s] = W.g_vz[g];
-      W.pm[s] = W.g_m[g];
-      W.ptype[s] = W.g_type ? W.g_type[g] : 1;
+      W.pos[s] = make_float4(W.g_x[g], W.g_y[g], W.g_z[g], W.g_m[g]);
+      W.vel[s] = make_float4(W.g_vx[g], W.g_vy[g], W.g_vz[g], __int_as_float(W.g_type ? W.g_type[g] : 1));
     }
   }
 }
@@ -463,7 +460,8 @@ __global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W, const Tile *tiles, 
     unsigned int mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
     for (int i = threadIdx.x; i < cnt; i += BLOCK) {
       const int li = W.el[b + S.src_lo + tl.start + i];
-      const unsigned int e[3] = {f2ord(W.px[b + li]), f2ord(W.py[b + li]), f2ord(W.pz[b + li])};
+      const float4 q = W.pos[b + li];
+      const unsigned int e[3] = {f2ord(q.x), f2ord(q.y), f2ord(q.z)};
       for (int j = 0; j < 3; j++) { mn[j] = min(mn[j], e[j]); mx[j] = max(mx[j], e[j]); }
     }
     for (int d = WAVE / 2; d > 0; d >>= 1)
@@ -540,7 +538,8 @@ __global__ void __launch_bounds__(BLOCK) keys_kernel(Work W) {
     if (l >= S.n_src) continue;
     const int64_t b = S.ibase;
     const int li = W.el[b + S.src_lo + l];
-    const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
+    const float4 q = W.pos[b + li];
+    const float x = q.x, y = q.y, z = q.z;
     double c0 = S.center[0], c1 = S.center[1], c2 = S.center[2];
     double lh = 0.5 * S.Len;
     u64 key = 0;
@@ -652,8 +651,9 @@ __global__ void __launch_bounds__(BLOCK) items_kernel(Work W) {
         const int pos = l + W.pre[s] + __popc(bl);
         const int li = (int)W.v0[s];
         // a sampled tree's leaves carry mass * MassFactor (EnergySnapshot_t::GetMass, :79)
-        const float m = (S.flags & F_SAMP) ? W.pm[b + li] * S.mf : W.pm[b + li];
-        W.items[ib + pos] = Item{make_float4(W.px[b + li], W.py[b + li], W.pz[b + li], m),
+        const float4 q = W.pos[b + li];
+        const float m = (S.flags & F_SAMP) ? q.w * S.mf : q.w;
+        W.items[ib + pos] = Item{make_float4(q.x, q.y, q.z, m),
                                  make_int2(0, pos + 1), make_int2(32 * (pos + 1), 0)};
         if (l + 1 == S.n_src) {
           W.seg[k].n_items = pos + 1;
@@ -1226,11 +1226,11 @@ __device__ __forceinline__ double walk_binned(const Item *__restrict__ it, const
 // The target mass that removes a target's own softened self term: none in a
 // correction pass (its tree holds the removed block only); in a sampled pass
 // the scaled mass of the sampled entries and none for the rest (:344-349);
-// otherwise the particle's mass.  isb flags the sampled entries.
-__device__ __forceinline__ float target_mass(const Work &W, const Seg &S, int64_t b, int li) {
+// otherwise the particle's mass m.  isb flags the sampled entries.
+__device__ __forceinline__ float target_mass(const Work &W, const Seg &S, int64_t b, int li, float m) {
   if (S.flags & F_CORR) return 0.f;
-  if (S.flags & F_SAMP) return W.isb[b + li] ? W.pm[b + li] * S.mf : 0.f;
-  return W.pm[b + li];
+  if (S.flags & F_SAMP) return W.isb[b + li] ? m * S.mf : 0.f;
+  return m;
 }
 
 // A target's binding energy from its potential.
@@ -1241,7 +1241,8 @@ __device__ __forceinline__ void finish_target(const Work &W, const DevParams &P,
                                               int li, float x, float y, float z, double pot) {
 #pragma clang fp contract(off)
   pot = pot * (double)P.G / (double)P.a;
-  const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
+  const float4 v = W.vel[b + li];
+  const float vx = v.x, vy = v.y, vz = v.z;
   if (!(S.flags & F_CORR)) {
     float dv[3];
     rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
@@ -1274,8 +1275,9 @@ __device__ __forceinline__ void walk_chunk(const Work &W, const DevParams &P, co
   if (!__any(live)) return;
   if (COUNT && live) wc.nt++;
   const int li = live ? W.sp[s] : 0;
-  const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
-  const float tm = target_mass(W, S, b, li);
+  const float4 q = W.pos[b + li];
+  const float x = q.x, y = q.y, z = q.z;
+  const float tm = target_mass(W, S, b, li, q.w);
   const Item *seg_it = it + 2 * b;  // item positions and skips are segment-local
   const int end = S.n_items;
   double pot;
@@ -1340,10 +1342,11 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(Work W, DevParams P) {
     if (COUNT) nt++;
     const int64_t b = S.ibase;
     const int li = W.sp[s];
-    const float x = W.px[b + li], y = W.py[b + li], z = W.pz[b + li];
-    const float vx = W.vx[b + li], vy = W.vy[b + li], vz = W.vz[b + li];
+    const float4 q = W.pos[b + li], w = W.vel[b + li];
+    const float x = q.x, y = q.y, z = q.z;
+    const float vx = w.x, vy = w.y, vz = w.z;
     if (!(S.flags & F_CORR)) {
-      const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, target_mass(W, S, b, li), ni, no);
+      const double pot = walk_dispatch<COUNT>(W, P, S, x, y, z, target_mass(W, S, b, li, q.w), ni, no);
       float dv[3];
       rel_vel(P, x, y, z, vx, vy, vz, S.ref_pos, S.ref_vel, dv);
       float *E = (S.flags & F_REFINE) ? W.Er : W.E;
@@ -1493,8 +1496,7 @@ __global__ void disrupt_swap_kernel(Work W) {
     if (q == INT_MAX || q == 0) continue;
     const int64_t a = S.ibase, c = S.ibase + q;
     auto sw = [](auto *v, int64_t i, int64_t j) { const auto t = v[i]; v[i] = v[j]; v[j] = t; };
-    sw(W.gidx, a, c); sw(W.px, a, c); sw(W.py, a, c); sw(W.pz, a, c);
-    sw(W.vx, a, c); sw(W.vy, a, c); sw(W.vz, a, c); sw(W.pm, a, c); sw(W.ptype, a, c);
+    sw(W.gidx, a, c); sw(W.pos, a, c); sw(W.vel, a, c);
   }
 }
 
@@ -1594,20 +1596,21 @@ frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
     const int cnt = (S.flags & F_ACTIVE) ? min(TILE, S.nbound - tl.start) : 0;
     if (cnt <= 0) continue;  // uniform
     const int64_t b = S.ibase;
-    const int l0 = W.el[b];
-    const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
+    const float4 p0 = W.pos[b + W.el[b]];
+    const double o[3] = {p0.x, p0.y, p0.z};
     double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < cnt; i += BLOCK) {
       const int li = W.el[b + tl.start + i];
-      const float m = W.pm[b + li];
+      const float4 q = W.pos[b + li], w = W.vel[b + li];
+      const float m = q.w;
       v[0] += m;
-      v[1] += W.vx[b + li] * m; v[2] += W.vy[b + li] * m; v[3] += W.vz[b + li] * m;
+      v[1] += w.x * m; v[2] += w.y * m; v[3] += w.z * m;
       if (P.periodic) {
-        v[4] += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
-        v[5] += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
-        v[6] += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
+        v[4] += nearest_d((double)q.x - o[0], P) * (double)m;
+        v[5] += nearest_d((double)q.y - o[1], P) * (double)m;
+        v[6] += nearest_d((double)q.z - o[2], P) * (double)m;
       } else {
-        v[4] += W.px[b + li] * m; v[5] += W.py[b + li] * m; v[6] += W.pz[b + li] * m;
+        v[4] += q.x * m; v[5] += q.y * m; v[6] += q.z * m;
       }
     }
     for (int c = 0; c < FRAME_SUMS; c++) {
@@ -1632,7 +1635,8 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   const int64_t b = S.ibase;
   const int nb = S.nbound;
   const int l0 = W.el[b];
-  const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
+  const float4 p0 = W.pos[b + l0], v0 = W.vel[b + l0];
+  const double o[3] = {p0.x, p0.y, p0.z};
   // correction switch (uses the frame of the pass just done); with sampling
   // only when fewer than MaxSampleSize were removed (:383-389)
   if (S.nlast - nb < nb && (P.max_sample <= 0 || S.nlast - nb < P.max_sample)) {
@@ -1642,9 +1646,9 @@ __device__ void frame_update(Work &W, const DevParams &P, Seg &S, double msum, d
   if (nb == 0) {
     msum = 0.;
   } else if (nb == 1) {
-    S.ref_vel[0] = W.vx[b + l0]; S.ref_vel[1] = W.vy[b + l0]; S.ref_vel[2] = W.vz[b + l0];
-    msum = W.pm[b + l0];
-    S.ref_pos[0] = W.px[b + l0]; S.ref_pos[1] = W.py[b + l0]; S.ref_pos[2] = W.pz[b + l0];
+    S.ref_vel[0] = v0.x; S.ref_vel[1] = v0.y; S.ref_vel[2] = v0.z;
+    msum = p0.w;
+    S.ref_pos[0] = p0.x; S.ref_pos[1] = p0.y; S.ref_pos[2] = p0.z;
   } else {
     S.ref_vel[0] = (float)(s1 / msum); S.ref_vel[1] = (float)(s2 / msum); S.ref_vel[2] = (float)(s3 / msum);
     double q[3] = {q1 / msum, q2 / msum, q3 / msum};
@@ -1750,21 +1754,22 @@ __global__ void frames_serial_kernel(Work W, DevParams P) {
     if (!(S.flags & F_ACTIVE)) continue;
     const int64_t b = S.ibase;
     const int nb = S.nbound;
-    const int l0 = W.el[b];
-    const double o[3] = {W.px[b + l0], W.py[b + l0], W.pz[b + l0]};
+    const float4 p0 = W.pos[b + W.el[b]];
+    const double o[3] = {p0.x, p0.y, p0.z};
     double msum = 0, s1 = 0, s2 = 0, s3 = 0, q1 = 0, q2 = 0, q3 = 0;
     if (nb > 1) {
       for (int i = 0; i < nb; i++) {
         const int li = W.el[b + i];
-        const float m = W.pm[b + li];
+        const float4 q = W.pos[b + li], w = W.vel[b + li];
+        const float m = q.w;
         msum += m;
-        s1 += W.vx[b + li] * m; s2 += W.vy[b + li] * m; s3 += W.vz[b + li] * m;
+        s1 += w.x * m; s2 += w.y * m; s3 += w.z * m;
         if (P.periodic) {
-          q1 += nearest_d((double)W.px[b + li] - o[0], P) * (double)m;
-          q2 += nearest_d((double)W.py[b + li] - o[1], P) * (double)m;
-          q3 += nearest_d((double)W.pz[b + li] - o[2], P) * (double)m;
+          q1 += nearest_d((double)q.x - o[0], P) * (double)m;
+          q2 += nearest_d((double)q.y - o[1], P) * (double)m;
+          q3 += nearest_d((double)q.z - o[2], P) * (double)m;
         } else {
-          q1 += W.px[b + li] * m; q2 += W.py[b + li] * m; q3 += W.pz[b + li] * m;
+          q1 += q.x * m; q2 += q.y * m; q3 += q.z * m;
         }
       }
     }
@@ -1837,15 +1842,16 @@ __device__ void write_record(Work &W, const DevParams &P, Seg &S, double E, doub
       if (r.sink_track_id != -1) { r.snapshot_index_of_sink = -1; r.sink_track_id = -1; }
       for (int j = 0; j < 3; j++) { r.ref_pos[j] = S.ref_pos[j]; r.ref_vel[j] = S.ref_vel[j]; }
       const int l0 = W.el[b];
-      r.mostbound_pos[0] = W.px[b + l0]; r.mostbound_pos[1] = W.py[b + l0]; r.mostbound_pos[2] = W.pz[b + l0];
-      r.mostbound_vel[0] = W.vx[b + l0]; r.mostbound_vel[1] = W.vy[b + l0]; r.mostbound_vel[2] = W.vz[b + l0];
+      const float4 q = W.pos[b + l0], w = W.vel[b + l0];
+      r.mostbound_pos[0] = q.x; r.mostbound_pos[1] = q.y; r.mostbound_pos[2] = q.z;
+      r.mostbound_vel[0] = w.x; r.mostbound_vel[1] = w.y; r.mostbound_vel[2] = w.z;
       r.mbound = S.mbound;
     } else {
       // disruption (:361-379): frame <- previous most-bound coordinates
       r.status = HBT_STATUS_DISRUPTED;
       if (r.snapshot_index_of_death == -1) r.snapshot_index_of_death = P.snap;
       for (int j = 0; j < 3; j++) { r.ref_pos[j] = r.mostbound_pos[j]; r.ref_vel[j] = r.mostbound_vel[j]; }
-      r.mbound = W.pm[b];
+      r.mbound = W.pos[b].w;
     }
   }
   for (int t = 0; t < HBT_TYPE_MAX; t++) { r.nbound_type[t] = nt[t]; r.mbound_type[t] = mts[t]; }
@@ -1864,10 +1870,11 @@ __device__ __forceinline__ void kin_accum(const Work &W, const DevParams &P, con
                                           double v[6]) {
 #pragma clang fp contract(off)
   const int64_t b = S.ibase;
-  const float m = W.pm[b + li];
+  const float4 q = W.pos[b + li], w = W.vel[b + li];
+  const float m = q.w;
   v[0] += W.E[b + li] * m;
-  const float x[3] = {W.px[b + li], W.py[b + li], W.pz[b + li]};
-  const float vel[3] = {W.vx[b + li], W.vy[b + li], W.vz[b + li]};
+  const float x[3] = {q.x, q.y, q.z};
+  const float vel[3] = {w.x, w.y, w.z};
   double dx[3], dv[3];
   for (int j = 0; j < 3; j++) {
     dx[j] = x[j] - S.ref_pos[j];
@@ -1904,9 +1911,9 @@ __global__ void output_record_kernel(Work W, DevParams P, int early) {
     for (int i = 0; i < nb; i++) {
       const int li = conv ? W.el[b + i] : i;
       if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
-      const int t = W.ptype[b + li];
+      const int t = __float_as_int(W.vel[b + li].w);
       nt[t]++;
-      mts[t] += W.pm[b + li];
+      mts[t] += W.pos[b + li].w;
     }
     write_record(W, P, S, v[0], v[1], v[2], v[3], v[4], v[5], nt, mts);
   }
@@ -1944,9 +1951,10 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
       if (conv && !W.isb[b + li]) continue;
       if (nb > 1) kin_accum(W, P, S, li, v);
       if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
-      const int ty = W.ptype[b + li];
+      const int ty = __float_as_int(W.vel[b + li].w);
+      const float m = W.pos[b + li].w;
       for (int q = 0; q < HBT_TYPE_MAX; q++)
-        if (q == ty) { v[6 + q] += 1.0; v[12 + q] += W.pm[b + li]; }
+        if (q == ty) { v[6 + q] += 1.0; v[12 + q] += m; }
     }
     for (int c = 0; c < REC_SUMS; c++) {
       const double r = wave_total(v[c]);

@@ -327,7 +327,7 @@ extern "C" int hbt_subhalo_properties(hbt_ctx *ctx, const hbt_particles *particl
   A.k0 = ctx->k0.as<u64>(); A.k1 = ctx->k1.as<u64>();
   A.v0 = ctx->v0.as<u32>(); A.v1 = ctx->v1.as<u32>();
   A.r = ctx->E.as<float>();
-  A.m = ctx->pm.as<float>();
+  A.m = ctx->Er.as<float>();
   A.cum = (double *)ctx->scan64.as<u64>();
   const Tile *dt = (const Tile *)d[6];
   const int gs = grid_for(off, ctx->cus);

@@ -19,7 +19,7 @@ __global__ void tree_only_setup_kernel(Work W, int n) {
 __global__ void tree_sources_kernel(Work W, int n, const float *x, const float *y, const float *z,
                                     const float *m) {
   for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-    W.px[i] = x[i]; W.py[i] = y[i]; W.pz[i] = z[i]; W.pm[i] = m[i];
+    W.pos[i] = make_float4(x[i], y[i], z[i], m[i]);
     W.el[i] = i;
   }
 }

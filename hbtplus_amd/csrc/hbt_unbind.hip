@@ -132,7 +132,7 @@ struct hbt_ctx {
   hbt_params params{};
   int cus = 256;
   // per-slot work arrays
-  DBuf gidx, pxyz, vxyz, pm, ptype, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
+  DBuf gidx, pos, vel, E, el, el2, sp, k0, k1, v0, v1, dl, bits, pre, nlo, nhi,
       npos, items, scan64, pk, gk, isb, seg_of_slot, cross, cross2, tcount;
   // per-level tables: staged through one pinned host buffer and one copy
   DBuf lvlbuf;
@@ -174,7 +174,7 @@ struct hbt_ctx {
   double last_ms = 0;
   int64_t last_evals = 0, last_passes = 0;
   ~hbt_ctx() {
-    for (DBuf *b : {&gidx, &pxyz, &vxyz, &pm, &ptype, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
+    for (DBuf *b : {&gidx, &pos, &vel, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &cross, &cross2, &tcount, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
                     &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack, &idtab,
@@ -849,10 +849,8 @@ struct Runner {
 int ensure_slots(hbt_ctx *c, int64_t S, int64_t nseg, int64_t ntiles) {
   const size_t s = (size_t)S;
   RETCHK(c->gidx.ensure(s * 4));
-  RETCHK(c->pxyz.ensure(s * 12));
-  RETCHK(c->vxyz.ensure(s * 12));
-  RETCHK(c->pm.ensure(s * 4));
-  RETCHK(c->ptype.ensure(s * 4));
+  RETCHK(c->pos.ensure(s * 16));
+  RETCHK(c->vel.ensure(s * 16));
   RETCHK(c->E.ensure(s * 4));
   RETCHK(c->Er.ensure(s * 4));
   RETCHK(c->el.ensure(s * 4));
@@ -909,10 +907,8 @@ Work make_work(hbt_ctx *c, const Level &L, const hbt_particles *parts, hbt_subha
   }
   const size_t S = (size_t)L.nslots;
   W.gidx = c->gidx.as<int>();
-  W.px = c->pxyz.as<float>(); W.py = W.px + S; W.pz = W.py + S;
-  W.vx = c->vxyz.as<float>(); W.vy = W.vx + S; W.vz = W.vy + S;
-  W.pm = c->pm.as<float>();
-  W.ptype = c->ptype.as<int>();
+  W.pos = c->pos.as<float4>();
+  W.vel = c->vel.as<float4>();
   W.E = c->E.as<float>();
   W.Er = c->Er.as<float>();
   W.el = c->el.as<int>(); W.el2 = c->el2.as<int>(); W.sp = c->sp.as<int>();
