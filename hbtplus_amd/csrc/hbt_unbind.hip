@@ -164,6 +164,7 @@ struct hbt_ctx {
   const float4 *packed = nullptr;       // likewise, when the snapshot is gathered from packed records
   DBuf pack;                            // the packed snapshot
   DBuf idtab;                           // hbt_id_to_index's table
+  DBuf idpart, idcnt;                   // its partitioned (key, position) pairs and bucket counts
   // the rand() stream of the sampling shuffles: srand(rand_seed), rand_pos calls made
   uint32_t rand_seed = 1;
   int64_t rand_pos = 0;
@@ -177,7 +178,7 @@ struct hbt_ctx {
     for (DBuf *b : {&gidx, &pos, &vel, &E, &el, &el2, &sp, &k0, &k1, &v0, &v1, &dl,
                     &bits, &pre, &nlo, &nhi, &npos, &items, &scan64, &pk, &gk,
                     &isb, &seg_of_slot, &cross, &cross2, &tcount, &seg, &lvlbuf, &ncount, &tsum, &tpart, &active, &counters,
-                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack, &idtab,
+                    &st_cols, &st_subs, &st_out_i, &st_out_e, &st_phi, &segact, &actbuf, &gaps, &pack, &idtab, &idpart, &idcnt,
                     &Er, &draws, &randtab, &kick, &kickin, &kickcnt})
       b->release();
     if (h_active) (void)hipHostFree(h_active);

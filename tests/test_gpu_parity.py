@@ -228,6 +228,36 @@ def test_id_to_index(gpu):
     assert U.id_to_index(table, np.zeros(0, np.int64)).size == 0
 
 
+def test_id_to_index_two_level_and_repeats(gpu):
+    """3 M IDs take both scatter levels of the partitioned table (2048
+    partitions); repeated IDs keep their smallest position; 2 M repeats of
+    one ID overfill a partition and take the single-table build; an empty
+    table and a table of null IDs resolve nothing."""
+    rng = np.random.default_rng(21)
+    U = gpu(abi.make_params())
+    table = synth.unique_ids(np.random.Generator(np.random.Philox(key=5)), 3_000_000)
+    table[rng.integers(0, len(table), 1000)] = -rng.integers(1, 9, 1000)  # null IDs are never entered
+    q = np.concatenate([table[rng.integers(0, len(table), 400000)], rng.integers(0, 2 ** 62, 20000),
+                        -rng.integers(1, 100, 10)])
+    rng.shuffle(q)
+    assert np.array_equal(U.id_to_index(table, q), oracle.id_to_index(table, q))
+    # every ID three times, in shuffled order
+    base = table[:400000]
+    rep = np.concatenate([base, base, base])
+    rng.shuffle(rep)
+    q = np.concatenate([base[rng.integers(0, len(base), 50000)], rng.integers(0, 2 ** 62, 1000)])
+    assert np.array_equal(U.id_to_index(rep, q), oracle.id_to_index(rep, q))
+    # one ID two million times among unique ones: its partition overflows
+    many = np.concatenate([np.full(2_000_000, 77, np.int64), table[:500000]])
+    rng.shuffle(many)
+    q = np.concatenate([np.array([77, 78], np.int64), table[rng.integers(0, 500000, 20000)]])
+    got = U.id_to_index(many, q)
+    assert np.array_equal(got, oracle.id_to_index(many, q))
+    assert got[0] == int(np.flatnonzero(many == 77)[0])
+    assert np.all(U.id_to_index(np.zeros(0, np.int64), q) == -1)
+    assert np.all(U.id_to_index(-np.ones(5000, np.int64), q) == -1)
+
+
 # ---- execution variants give identical results ---------------------------------
 
 def _unbinder_with_env(params, **env):
