@@ -282,25 +282,37 @@ __global__ void __launch_bounds__(BLOCK) part_build_kernel(const u64 *ikey, cons
                                                            int np, unsigned long long *tab) {
   __shared__ u64 sk[PART_SLOTS];
   __shared__ u32 sp[PART_SLOTS];
+  constexpr int PER = (PART_MAX + BLOCK - 1) / BLOCK;
   for (int p = blockIdx.x; p < np; p += gridDim.x) {
+    // the partition's pairs in one batch of loads, issued before the table is cleared
+    const u32 lo = part_off[p], hi = part_off[p + 1];
+    const bool fits = hi - lo <= (u32)PART_MAX;  // uniform; the host never uses a table with a larger partition
+    u64 key[PER];
+    u32 pos[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const u32 e = lo + j * BLOCK + threadIdx.x;
+      if (fits && e < hi) {
+        key[j] = ikey[e];
+        pos[j] = ipos[e];
+      }
+    }
     for (int j = threadIdx.x; j < PART_SLOTS; j += BLOCK) {
       sk[j] = HASH_EMPTY;
       sp[j] = 0xffffffffu;
     }
     __syncthreads();
-    const u32 lo = part_off[p], hi = part_off[p + 1];
-    if (hi - lo <= (u32)PART_MAX) {  // uniform; the host never uses a table with a larger partition
-      for (u32 e = lo + threadIdx.x; e < hi; e += BLOCK) {
-        const u64 key = ikey[e];
-        u32 s = (u32)id_hash((long long)key) & (PART_SLOTS - 1);
-        for (int probe = 0; probe < PART_SLOTS; probe++) {
-          const u64 prev = atomicCAS(&sk[s], HASH_EMPTY, key);
-          if (prev == HASH_EMPTY || prev == key) {
-            atomicMin(&sp[s], ipos[e]);
-            break;
-          }
-          s = (s + 1) & (PART_SLOTS - 1);
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      if (!fits || lo + j * BLOCK + threadIdx.x >= hi) continue;
+      u32 s = (u32)id_hash((long long)key[j]) & (PART_SLOTS - 1);
+      for (int probe = 0; probe < PART_SLOTS; probe++) {
+        const u64 prev = atomicCAS(&sk[s], HASH_EMPTY, key[j]);
+        if (prev == HASH_EMPTY || prev == key[j]) {
+          atomicMin(&sp[s], pos[j]);
+          break;
         }
+        s = (s + 1) & (PART_SLOTS - 1);
       }
     }
     __syncthreads();

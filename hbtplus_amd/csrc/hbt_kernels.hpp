@@ -773,31 +773,63 @@ __global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile 
       }
       continue;  // uniform across the block
     }
-    for (int i = threadIdx.x; i < nit; i += BLOCK) {
-      const Item q = W.items[ib + first + i];
-      spm[i] = q.pm;
-      sskip[i] = q.ax.y - first;
+    // one batch of loads per thread: its PER pairs (4-aligned slots; the
+    // group stays inside the segment's wave-aligned slots) and its items
+    constexpr int PER = TILE / BLOCK, IPER = (TILE_ITEMS + BLOCK - 1) / BLOCK;
+    static_assert(PER == 4, "pairs are loaded four at a time");
+    const int p0 = PER * (int)threadIdx.x;
+    int plo[PER], phi[PER], ppos[PER], pdl[PER];
+    if (p0 < cnt) {
+      const int64_t s = b + tl.start + p0;
+      const int4 a = *reinterpret_cast<const int4 *>(W.nlo + s);
+      const int4 h = *reinterpret_cast<const int4 *>(W.nhi + s);
+      const int4 g = *reinterpret_cast<const int4 *>(W.npos + s);
+      const char4 d = *reinterpret_cast<const char4 *>(W.dl + s);
+      plo[0] = a.x; plo[1] = a.y; plo[2] = a.z; plo[3] = a.w;
+      phi[0] = h.x; phi[1] = h.y; phi[2] = h.z; phi[3] = h.w;
+      ppos[0] = g.x; ppos[1] = g.y; ppos[2] = g.z; ppos[3] = g.w;
+      pdl[0] = d.x; pdl[1] = d.y; pdl[2] = d.z; pdl[3] = d.w;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      if (p0 + j >= cnt) plo[j] = -1;
+    float4 qpm[IPER];
+    int qsk[IPER];
+#pragma unroll
+    for (int k = 0; k < IPER; k++) {
+      const int i = (int)threadIdx.x + k * BLOCK;
+      if (i < nit) {
+        qpm[k] = W.items[ib + first + i].pm;
+        qsk[k] = W.items[ib + first + i].ax.y - first;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < IPER; k++) {
+      const int i = (int)threadIdx.x + k * BLOCK;
+      if (i < nit) {
+        spm[i] = qpm[k];
+        sskip[i] = qsk[k];
+      }
     }
     for (int v = threadIdx.x; v <= MAX_DEPTH + 1; v += BLOCK) sdepth[v] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int64_t s = b + tl.start + i;
-      if (W.nlo[s] < 0) continue;
-      if (W.nlo[s] >= t_lo && W.nhi[s] <= t_hi) atomicAdd(&sdepth[W.dl[s]], 1);
-      else list_crossing(W, t, tl, b, tl.start + i, W.nlo[s], W.nhi[s], ncross);
+    int rank[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      if (plo[j] < 0) continue;
+      if (plo[j] >= t_lo && phi[j] <= t_hi) rank[j] = atomicAdd(&sdepth[pdl[j]], 1);
+      else list_crossing(W, t, tl, b, tl.start + p0 + j, plo[j], phi[j], ncross);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       int o = 0;
-      for (int v = MAX_DEPTH; v >= 0; v--) { sfirst[v] = o; o += sdepth[v]; sdepth[v] = sfirst[v]; }
+      for (int v = MAX_DEPTH; v >= 0; v--) { sfirst[v] = o; o += sdepth[v]; }
       sfirst[MAX_DEPTH + 1] = o;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int64_t s = b + tl.start + i;
-      if (W.nlo[s] < 0 || W.nlo[s] < t_lo || W.nhi[s] > t_hi) continue;
-      snode[atomicAdd(&sdepth[W.dl[s]], 1)] = (int)(W.npos[s] - ib) - first;
-    }
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      if (plo[j] >= t_lo && phi[j] <= t_hi) snode[sfirst[pdl[j]] + rank[j]] = (int)(ppos[j] - ib) - first;
     __syncthreads();
     // depths deepest first: places sfirst[v] .. sfirst[v] + count, in order
     for (int v = MAX_DEPTH; v >= 0; v--) {
