@@ -484,6 +484,33 @@ __global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W, const Tile *tiles, 
   }
 }
 
+// One-wave tiles: a segment's only tile, so its box is stored directly.
+__global__ void __launch_bounds__(BLOCK) bbox_wave_kernel(Work W, const Tile *tiles, int t0, int ntiles) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    Seg &S = W.seg[tl.seg];
+    if (lane == 0) W.tcount[t] = 0;
+    const int cnt = min(WAVE, S.n_src);
+    if (cnt <= 0) continue;  // uniform across the wave
+    const int64_t b = S.ibase;
+    unsigned int mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    if (lane < cnt) {
+      const float4 q = W.pos[b + W.el[b + S.src_lo + lane]];
+      mn[0] = mx[0] = f2ord(q.x);
+      mn[1] = mx[1] = f2ord(q.y);
+      mn[2] = mx[2] = f2ord(q.z);
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1)
+      for (int j = 0; j < 3; j++) {
+        mn[j] = min(mn[j], (unsigned int)__shfl_xor((int)mn[j], d, WAVE));
+        mx[j] = max(mx[j], (unsigned int)__shfl_xor((int)mx[j], d, WAVE));
+      }
+    if (lane == 0)
+      for (int j = 0; j < 3; j++) { S.bmin[j] = mn[j]; S.bmax[j] = mx[j]; }
+  }
+}
+
 // Root cube exactly as OctTree_t::Build (src/oct_tree.tpp:118-139) and the
 // depth schedule: node length halves while >= TreeNodeResolution
 // (src/gravity_tree.cpp:37-40); a node created at depth >= max(1, dlen) is
@@ -797,11 +824,10 @@ __global__ void __launch_bounds__(BLOCK) moments_tile_kernel(Work W, const Tile 
     int qsk[IPER];
 #pragma unroll
     for (int k = 0; k < IPER; k++) {
-      const int i = (int)threadIdx.x + k * BLOCK;
-      if (i < nit) {
-        qpm[k] = W.items[ib + first + i].pm;
-        qsk[k] = W.items[ib + first + i].ax.y - first;
-      }
+      // past the tile's items the last one is read again (and not stored)
+      const int i = min((int)threadIdx.x + k * BLOCK, nit - 1);
+      qpm[k] = W.items[ib + first + i].pm;
+      qsk[k] = W.items[ib + first + i].ax.y - first;
     }
 #pragma unroll
     for (int k = 0; k < IPER; k++) {
@@ -880,6 +906,64 @@ __global__ void __launch_bounds__(BLOCK) moments_cross_kernel(Work W, const Tile
     if (ng == 0) continue;  // uniform
     const Seg &S = W.seg[tl.seg];
     moments_of_list(W, S, W.cross2 + S.ibase, ng);
+  }
+}
+
+// One-wave tiles: the whole tree of a segment of at most WAVE sources (fewer
+// than 2 WAVE items) in LDS; lane l holds pair l, and the depths that hold
+// nodes are taken deepest first with a wave barrier between them.
+constexpr int WAVE_ITEMS = 2 * WAVE;
+
+__global__ void __launch_bounds__(BLOCK) moments_wave_kernel(Work W, const Tile *__restrict__ tiles, int t0,
+                                                             int ntiles) {
+  __shared__ float4 spm[BLOCK / WAVE][WAVE_ITEMS];
+  __shared__ int sskip[BLOCK / WAVE][WAVE_ITEMS];
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const int cnt = min(WAVE, S.n_src) - 1;  // pairs
+    if (cnt <= 0) continue;  // uniform across the wave
+    const int64_t b = S.ibase, ib = 2 * b;
+    const int nit = S.n_items;
+    int lo = -1, pos = 0, v = 0;
+    if (lane < cnt) {
+      lo = W.nlo[b + lane];
+      pos = W.npos[b + lane];
+      v = W.dl[b + lane];
+    }
+    unsigned int dm = lo >= 0 ? 1u << v : 0u;
+    for (int d = WAVE / 2; d > 0; d >>= 1) dm |= (unsigned int)__shfl_xor((int)dm, d, WAVE);
+    if (nit > WAVE_ITEMS) {
+      // not expected (a compressed octree has fewer nodes than leaves): the
+      // same depth order over device memory
+      while (dm) {
+        const int d = 31 - __clz(dm);
+        dm &= ~(1u << d);
+        if (lo >= 0 && v == d) node_moments(W, ib, pos);
+        __threadfence();
+        wave_sync();
+      }
+      continue;
+    }
+    for (int i = lane; i < nit; i += WAVE) {
+      const Item q = W.items[ib + i];
+      spm[wv][i] = q.pm;
+      sskip[wv][i] = q.ax.y;
+    }
+    wave_sync();
+    while (dm) {  // uniform across the wave
+      const int d = 31 - __clz(dm);
+      dm &= ~(1u << d);
+      if (lo >= 0 && v == d) {
+        const int g = pos - (int)ib;
+        spm[wv][g] = sum_children([&](int c) { return spm[wv][c]; }, [&](int c) { return sskip[wv][c]; }, g,
+                                  sskip[wv][g]);
+      }
+      wave_sync();
+    }
+    if (lo >= 0) W.items[pos].pm = spm[wv][pos - (int)ib];
+    wave_sync();  // the next tile reuses the wave's LDS
   }
 }
 
@@ -1618,6 +1702,24 @@ __global__ void __launch_bounds__(BLOCK) energy_writeback_kernel(Work W) {
 // result does not depend on scheduling.
 constexpr int FRAME_SUMS = 7;
 
+// The frame sums of bound entry li: mass, momentum, mass-weighted position
+// (relative to the most-bound entry o when periodic).
+__device__ __forceinline__ void frame_accum(const Work &W, const DevParams &P, int64_t b, int li,
+                                            const double o[3], double v[FRAME_SUMS]) {
+#pragma clang fp contract(off)
+  const float4 q = W.pos[b + li], w = W.vel[b + li];
+  const float m = q.w;
+  v[0] += m;
+  v[1] += w.x * m; v[2] += w.y * m; v[3] += w.z * m;
+  if (P.periodic) {
+    v[4] += nearest_d((double)q.x - o[0], P) * (double)m;
+    v[5] += nearest_d((double)q.y - o[1], P) * (double)m;
+    v[6] += nearest_d((double)q.z - o[2], P) * (double)m;
+  } else {
+    v[4] += q.x * m; v[5] += q.y * m; v[6] += q.z * m;
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK)
 frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double *part) {
 #pragma clang fp contract(off)
@@ -1631,20 +1733,7 @@ frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
     const float4 p0 = W.pos[b + W.el[b]];
     const double o[3] = {p0.x, p0.y, p0.z};
     double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int li = W.el[b + tl.start + i];
-      const float4 q = W.pos[b + li], w = W.vel[b + li];
-      const float m = q.w;
-      v[0] += m;
-      v[1] += w.x * m; v[2] += w.y * m; v[3] += w.z * m;
-      if (P.periodic) {
-        v[4] += nearest_d((double)q.x - o[0], P) * (double)m;
-        v[5] += nearest_d((double)q.y - o[1], P) * (double)m;
-        v[6] += nearest_d((double)q.z - o[2], P) * (double)m;
-      } else {
-        v[4] += q.x * m; v[5] += q.y * m; v[6] += q.z * m;
-      }
-    }
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) frame_accum(W, P, b, W.el[b + tl.start + i], o, v);
     for (int c = 0; c < FRAME_SUMS; c++) {
       const double r = wave_total(v[c]);
       if ((threadIdx.x & (WAVE - 1)) == 0) sh[c][threadIdx.x / WAVE] = r;
@@ -1656,6 +1745,30 @@ frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
       part[(int64_t)t * FRAME_SUMS + threadIdx.x] = r;
     }
     __syncthreads();
+  }
+}
+
+// One-wave tiles: lane l takes entry l, as the workgroup form's first wave
+// does; the total is added to zero first, as the workgroup form adds its
+// waves' totals, so both give the same bits.
+__global__ void __launch_bounds__(BLOCK)
+frames_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int ntiles, double *part) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (WAVE - 1);
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const int cnt = (S.flags & F_ACTIVE) ? min(WAVE, S.nbound) : 0;
+    if (cnt <= 0) continue;  // uniform across the wave
+    const int64_t b = S.ibase;
+    const float4 p0 = W.pos[b + W.el[b]];
+    const double o[3] = {p0.x, p0.y, p0.z};
+    double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
+    if (lane < cnt) frame_accum(W, P, b, W.el[b + lane], o, v);
+    for (int c = 0; c < FRAME_SUMS; c++) {
+      const double r = 0.0 + wave_total(v[c]);
+      if (lane == 0) part[(int64_t)t * FRAME_SUMS + c] = r;
+    }
   }
 }
 
@@ -1962,6 +2075,23 @@ constexpr int REC_SUMS = 18;
 
 __device__ __forceinline__ int record_span(const Seg &S) { return (S.flags & F_CONV) ? S.ulen : S.nbound; }
 
+// The record sums of slot li: kinematics of a bound entry, and its type
+// count and mass unless it is a null particle of a small subhalo.
+__device__ __forceinline__ void record_accum(const Work &W, const DevParams &P, const Seg &S, int64_t b, int li,
+                                             bool conv, int nb, bool skip_null, double (&v)[REC_SUMS]) {
+#pragma clang fp contract(off)
+  if (conv && !W.isb[b + li]) return;
+  if (nb > 1) kin_accum(W, P, S, li, v);
+  if (skip_null && W.g_id[W.gidx[b + li]] == -1) return;
+  const int ty = __float_as_int(W.vel[b + li].w);
+  const float m = W.pos[b + li].w;
+  for (int q = 0; q < HBT_TYPE_MAX; q++) {
+    const bool hit = q == ty;
+    v[6 + q] += hit ? 1.0 : 0.0;
+    v[12 + q] += hit ? (double)m : 0.0;
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK)
 output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double *part) {
 #pragma clang fp contract(off)
@@ -1978,16 +2108,7 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
     const bool skip_null = nb <= 100;
     double v[REC_SUMS];
     for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int li = tl.start + i;
-      if (conv && !W.isb[b + li]) continue;
-      if (nb > 1) kin_accum(W, P, S, li, v);
-      if (skip_null && W.g_id[W.gidx[b + li]] == -1) continue;
-      const int ty = __float_as_int(W.vel[b + li].w);
-      const float m = W.pos[b + li].w;
-      for (int q = 0; q < HBT_TYPE_MAX; q++)
-        if (q == ty) { v[6 + q] += 1.0; v[12 + q] += m; }
-    }
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) record_accum(W, P, S, b, tl.start + i, conv, nb, skip_null, v);
     for (int c = 0; c < REC_SUMS; c++) {
       const double r = wave_total(v[c]);
       if ((threadIdx.x & (WAVE - 1)) == 0) sh[c][threadIdx.x / WAVE] = r;
@@ -1999,6 +2120,30 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
       part[(int64_t)t * REC_SUMS + threadIdx.x] = r;
     }
     __syncthreads();
+  }
+}
+
+// One-wave tiles, the same bits as the workgroup form (see
+// frames_partial_wave_kernel).
+__global__ void __launch_bounds__(BLOCK)
+output_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int ntiles, double *part) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (WAVE - 1);
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const bool fin = (S.flags & F_FINAL) != 0;
+    const int cnt = fin ? min(WAVE, record_span(S)) : 0;
+    if (cnt <= 0) continue;  // uniform across the wave
+    const int64_t b = S.ibase;
+    const int nb = S.nbound;
+    double v[REC_SUMS];
+    for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
+    if (lane < cnt) record_accum(W, P, S, b, lane, (S.flags & F_CONV) != 0, nb, nb <= 100, v);
+    for (int c = 0; c < REC_SUMS; c++) {
+      const double r = 0.0 + wave_total(v[c]);
+      if (lane == 0) part[(int64_t)t * REC_SUMS + c] = r;
+    }
   }
 }
 

@@ -10,9 +10,10 @@
 // Primitives:
 //   segmented sort of (u64 key, u32 value) pairs by (key, value):
 //     tile_sort_kernel    : LDS bitonic sort of each tile
+//     sort_wave_kernel    : the same for segments of at most one wave, one per wave
 //     merge_pass_kernel   : one bottom-up merge level (co-rank by binary search)
 //   segmented exclusive scan (load/store functors):
-//     scan_tiles_kernel / scan_add_kernel
+//     scan_tiles_kernel / scan_add_kernel, scan_wave_kernel
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -102,6 +103,26 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T *total) {
   return off + inc - x;
 }
 
+// ---- one-wave tiles ------------------------------------------------------------
+// The tile table lists a level's segments largest first, so the tiles of the
+// segments of at most SMALL_CAP slots -- one tile each, most segments of a
+// level -- are a suffix of it.  Their kernels give each tile one wave instead
+// of one workgroup: no workgroup barriers, four tiles per workgroup, and less
+// LDS per tile in flight.
+constexpr int SMALL_CAP = WAVE;
+
+// LDS writes of the wave visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tiles t0 <= t < ntiles, one per wave (t is uniform across the wave).
+#define WAVE_TILE_LOOP(t, t0, ntiles)                                                            \
+  for (int t = (t0) + (int)(blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE); t < (ntiles); \
+       t += (int)gridDim.x * (BLOCK / WAVE))
+
 // ---- XCD-aware block order -----------------------------------------------------
 // MI355X has 8 accelerator dies (XCDs), each with its own L2; the workgroups of
 // a launch are dispatched to them round-robin (block b runs on XCD b % 8).
@@ -172,6 +193,47 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
     }
     for (int i = threadIdx.x; i < cnt; i += BLOCK) { kout[off + i] = sk[i]; vout[off + i] = sv[i]; }
     __syncthreads();
+  }
+}
+
+// One-wave tiles: lane l holds element l (padding sorts last), a bitonic
+// network over the lanes with the partners' pairs read across lanes, up to
+// the power of two that covers the segment.  Such segments need no merge
+// level, so the result stays in (k0, v0).
+__global__ void __launch_bounds__(BLOCK)
+sort_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int64_t *__restrict__ base,
+                 const int *__restrict__ n, u64 *k0, u32 *v0) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    const int cnt = min(WAVE, n[tl.seg]);
+    if (cnt <= 1) continue;  // uniform across the wave
+    const int64_t off = base[tl.seg];
+    u64 k = ~0ull;
+    u32 v = 0xffffffffu;
+    if (lane < cnt) {
+      k = k0[off + lane];
+      v = v0[off + lane];
+    }
+    int P = 2;
+    while (P < cnt) P <<= 1;
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int j = size >> 1; j > 0; j >>= 1) {
+        const u64 ko = __shfl_xor(k, j, WAVE);
+        const u32 vo = __shfl_xor(v, j, WAVE);
+        // the lower lane of a pair keeps the smaller one in an ascending run
+        const bool keep_small = ((lane & j) == 0) == ((lane & size) == 0);
+        const bool other_less = pair_less(ko, vo, k, v);
+        if (keep_small ? other_less : !other_less) {
+          k = ko;
+          v = vo;
+        }
+      }
+    }
+    if (lane < cnt) {
+      k0[off + lane] = k;
+      v0[off + lane] = v;
+    }
   }
 }
 
@@ -287,6 +349,26 @@ scan_tiles_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restr
       off += v[i];
     }
     if (threadIdx.x == 0) tile_sum[t] = total;
+  }
+}
+
+// Pass 1 for one-wave tiles (a segment's only tile: its prefixes are final).
+template <class T, class Load, class Store>
+__global__ void __launch_bounds__(BLOCK)
+scan_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int *__restrict__ n,
+                 T *__restrict__ tile_sum, Load load, Store store) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    const int cnt = min(WAVE, n[tl.seg]);
+    if (cnt <= 0) {  // uniform across the wave
+      if (lane == 0) tile_sum[t] = T(0);
+      continue;
+    }
+    const T v = lane < cnt ? load(tl.seg, lane) : T(0);
+    const T inc = wave_inclusive_scan(v);
+    if (lane < cnt) store(tl.seg, lane, inc - v);
+    if (lane == WAVE - 1) tile_sum[t] = inc;
   }
 }
 

@@ -543,6 +543,8 @@ struct Runner {
   DevParams P;
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
+  int nt_big = 0;              // tiles of segments of more than SMALL_CAP slots (a prefix)
+  int gsmall = 1;              // one-wave tile grid (four tiles per workgroup)
   int gwseg = 1;               // segment-wave grid
   int gw;                      // walk grid: every slot block once
   // the tile table a pass uses: the level's, or that of its active segments
@@ -570,8 +572,16 @@ struct Runner {
   void set_grids() {
     gs = grid_for(W.nact, c->cus);
     gw = walk_grid(W.nact, c->tune.xcd_run);
-    gtile = (int)std::max<int64_t>(1, std::min<int64_t>(nt_cur, (int64_t)c->cus * 8));
+    nt_big = tiles_above(SMALL_CAP);
+    gtile = (int)std::max<int64_t>(1, std::min<int64_t>(nt_big, (int64_t)c->cus * 8));
+    gsmall = (int)std::max<int64_t>(1, std::min<int64_t>((nt_cur - nt_big + BLOCK / WAVE - 1) / (BLOCK / WAVE),
+                                                         (int64_t)c->cus * 16));
   }
+
+  // tile kernels: the workgroup form over the large segments' tiles, the
+  // one-wave form over the rest
+  bool any_big() const { return nt_big > 0; }
+  bool any_small() const { return nt_cur > nt_big; }
 
   // Lists the segments still active after a pass: their wave-sized slot
   // chunks, their tiles (a table of its own, with a first-tile index per
@@ -654,7 +664,8 @@ struct Runner {
     const Tile *tl = tl_cur;
     const int nt = nt_cur;
     if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evs0, c->stream));
-    tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
+    if (any_big()) tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt_big, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
+    if (any_small()) sort_wave_kernel<<<gsmall, BLOCK, 0, c->stream>>>(tl, nt_big, nt, W.ibase, n, W.k0, W.v0);
     // one launch per merge level of the largest segment of the level, over
     // the tiles of the segments whose capacity exceeds the level's width
     // (segments are ordered largest first, so those tiles are a prefix)
@@ -684,15 +695,23 @@ struct Runner {
   template <class Load, class Store, class Add>
   int seg_scan(Load ld, Store st, Add ad, const int *n) {
     u64 *ts = c->tsum.as<u64>();
-    scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nt_cur, n, ts, ld, st);
-    scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nt_cur, n, ts, ad);
+    if (any_big()) scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nt_big, n, ts, ld, st);
+    if (any_small()) scan_wave_kernel<u64><<<gsmall, BLOCK, 0, c->stream>>>(tl_cur, nt_big, nt_cur, n, ts, ld, st);
+    // only segments of more than one tile have tiles after their first
+    const int nmulti = tiles_above(TILE);
+    if (nmulti > 0) scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nmulti, n, ts, ad);
     return HBT_OK;
   }
   const int *count_row(int row) const { return W.ncount + (size_t)row * W.nseg; }
 
   // tree over the pass's sources: keys, sort, structure, moments
+  void bbox() {
+    if (any_big()) bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big);
+    if (any_small()) bbox_wave_kernel<<<gsmall, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big, nt_cur);
+  }
+
   int build_tree(bool copy_sp) {
-    bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    bbox();
     tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(sort_pairs(count_row(NC_SRC)));
@@ -700,9 +719,12 @@ struct Runner {
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    moments_tile_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
-    moments_super_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
-    if (max_n > SUPER) moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    if (any_big()) {
+      moments_tile_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big);
+      moments_super_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big);
+      if (max_n > SUPER) moments_cross_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big);
+    }
+    if (any_small()) moments_wave_kernel<<<gsmall, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big, nt_cur);
     return HBT_OK;
   }
 
@@ -767,7 +789,9 @@ struct Runner {
     if (P.replay) {
       frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     } else {
-      frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
+      if (any_big()) frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, c->tpart.as<double>());
+      if (any_small())
+        frames_partial_wave_kernel<<<gsmall, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, nt_cur, c->tpart.as<double>());
       // the run sums go after the tile partials in the same buffer
       double *gpart = c->tpart.as<double>() + (int64_t)(nt_cur + 1) * FRAME_SUMS;
       if (max_n > SUPER)
@@ -790,7 +814,9 @@ struct Runner {
     if (P.replay) {
       output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
     } else {
-      output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_cur, c->tpart.as<double>());
+      if (any_big()) output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, c->tpart.as<double>());
+      if (any_small())
+        output_partial_wave_kernel<<<gsmall, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, nt_cur, c->tpart.as<double>());
       output_combine_kernel<<<gwseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
     }
   }
@@ -837,7 +863,7 @@ struct Runner {
   int prelude() {
     if (L->ndraws <= 0) return HBT_OK;
     prelude_setup_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
-    bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_cur);
+    bbox();
     tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(sort_pairs(count_row(NC_SRC)));
