@@ -484,22 +484,27 @@ __global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W, const Tile *tiles, 
   }
 }
 
-// One-wave tiles: a segment's only tile, so its box is stored directly.
+// One-wave tiles (up to R * WAVE sources, lane l taking sources l + WAVE r):
+// a segment's only tile, so its box is stored directly.
+template <int R>
 __global__ void __launch_bounds__(BLOCK) bbox_wave_kernel(Work W, const Tile *tiles, int t0, int ntiles) {
   const int lane = threadIdx.x & (WAVE - 1);
   WAVE_TILE_LOOP(t, t0, ntiles) {
     const Tile tl = tiles[t];
     Seg &S = W.seg[tl.seg];
     if (lane == 0) W.tcount[t] = 0;
-    const int cnt = min(WAVE, S.n_src);
+    const int cnt = min(R * WAVE, S.n_src);
     if (cnt <= 0) continue;  // uniform across the wave
     const int64_t b = S.ibase;
     unsigned int mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
-    if (lane < cnt) {
-      const float4 q = W.pos[b + W.el[b + S.src_lo + lane]];
-      mn[0] = mx[0] = f2ord(q.x);
-      mn[1] = mx[1] = f2ord(q.y);
-      mn[2] = mx[2] = f2ord(q.z);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int e = lane + WAVE * r;
+      if (e < cnt) {
+        const float4 q = W.pos[b + W.el[b + S.src_lo + e]];
+        const unsigned int o[3] = {f2ord(q.x), f2ord(q.y), f2ord(q.z)};
+        for (int j = 0; j < 3; j++) { mn[j] = min(mn[j], o[j]); mx[j] = max(mx[j], o[j]); }
+      }
     }
     for (int d = WAVE / 2; d > 0; d >>= 1)
       for (int j = 0; j < 3; j++) {
@@ -909,38 +914,49 @@ __global__ void __launch_bounds__(BLOCK) moments_cross_kernel(Work W, const Tile
   }
 }
 
-// One-wave tiles: the whole tree of a segment of at most WAVE sources (fewer
-// than 2 WAVE items) in LDS; lane l holds pair l, and the depths that hold
-// nodes are taken deepest first with a wave barrier between them.
-constexpr int WAVE_ITEMS = 2 * WAVE;
-
+// One-wave tiles: the whole tree of a segment of at most R * WAVE sources
+// (fewer than 2 R WAVE items) in LDS; lane l holds pairs l + WAVE r, and the
+// depths that hold nodes are taken deepest first with a wave barrier between
+// them.
+template <int R>
 __global__ void __launch_bounds__(BLOCK) moments_wave_kernel(Work W, const Tile *__restrict__ tiles, int t0,
                                                              int ntiles) {
-  __shared__ float4 spm[BLOCK / WAVE][WAVE_ITEMS];
-  __shared__ int sskip[BLOCK / WAVE][WAVE_ITEMS];
+  constexpr int NI = 2 * R * WAVE;
+  __shared__ float4 spm[BLOCK / WAVE][NI];
+  __shared__ int sskip[BLOCK / WAVE][NI];
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
   WAVE_TILE_LOOP(t, t0, ntiles) {
     const Tile tl = tiles[t];
     const Seg &S = W.seg[tl.seg];
-    const int cnt = min(WAVE, S.n_src) - 1;  // pairs
+    const int cnt = min(R * WAVE, S.n_src) - 1;  // pairs
     if (cnt <= 0) continue;  // uniform across the wave
     const int64_t b = S.ibase, ib = 2 * b;
     const int nit = S.n_items;
-    int lo = -1, pos = 0, v = 0;
-    if (lane < cnt) {
-      lo = W.nlo[b + lane];
-      pos = W.npos[b + lane];
-      v = W.dl[b + lane];
+    int lo[R], pos[R], v[R];
+    unsigned int dm = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int e = lane + WAVE * r;
+      lo[r] = -1;
+      pos[r] = 0;
+      v[r] = 0;
+      if (e < cnt) {
+        lo[r] = W.nlo[b + e];
+        pos[r] = W.npos[b + e];
+        v[r] = W.dl[b + e];
+      }
+      if (lo[r] >= 0) dm |= 1u << v[r];
     }
-    unsigned int dm = lo >= 0 ? 1u << v : 0u;
     for (int d = WAVE / 2; d > 0; d >>= 1) dm |= (unsigned int)__shfl_xor((int)dm, d, WAVE);
-    if (nit > WAVE_ITEMS) {
+    if (nit > NI) {
       // not expected (a compressed octree has fewer nodes than leaves): the
       // same depth order over device memory
       while (dm) {
         const int d = 31 - __clz(dm);
         dm &= ~(1u << d);
-        if (lo >= 0 && v == d) node_moments(W, ib, pos);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (lo[r] >= 0 && v[r] == d) node_moments(W, ib, pos[r]);
         __threadfence();
         wave_sync();
       }
@@ -955,14 +971,18 @@ __global__ void __launch_bounds__(BLOCK) moments_wave_kernel(Work W, const Tile 
     while (dm) {  // uniform across the wave
       const int d = 31 - __clz(dm);
       dm &= ~(1u << d);
-      if (lo >= 0 && v == d) {
-        const int g = pos - (int)ib;
-        spm[wv][g] = sum_children([&](int c) { return spm[wv][c]; }, [&](int c) { return sskip[wv][c]; }, g,
-                                  sskip[wv][g]);
-      }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (lo[r] >= 0 && v[r] == d) {
+          const int g = pos[r] - (int)ib;
+          spm[wv][g] = sum_children([&](int c) { return spm[wv][c]; }, [&](int c) { return sskip[wv][c]; }, g,
+                                    sskip[wv][g]);
+        }
       wave_sync();
     }
-    if (lo >= 0) W.items[pos].pm = spm[wv][pos - (int)ib];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (lo[r] >= 0) W.items[pos[r]].pm = spm[wv][pos[r] - (int)ib];
     wave_sync();  // the next tile reuses the wave's LDS
   }
 }
@@ -1748,9 +1768,12 @@ frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
   }
 }
 
-// One-wave tiles: lane l takes entry l, as the workgroup form's first wave
-// does; the total is added to zero first, as the workgroup form adds its
-// waves' totals, so both give the same bits.
+// One-wave tiles of up to R * WAVE entries: pass r of the wave takes entries
+// WAVE r .. WAVE r + WAVE - 1, lane l entry WAVE r + l, as wave r of the
+// workgroup form does, and the totals of the passes are added to zero in
+// order, as the workgroup form adds its waves' totals, so both give the same
+// bits.
+template <int R>
 __global__ void __launch_bounds__(BLOCK)
 frames_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int ntiles, double *part) {
 #pragma clang fp contract(off)
@@ -1758,17 +1781,20 @@ frames_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int n
   WAVE_TILE_LOOP(t, t0, ntiles) {
     const Tile tl = tiles[t];
     const Seg &S = W.seg[tl.seg];
-    const int cnt = (S.flags & F_ACTIVE) ? min(WAVE, S.nbound) : 0;
+    const int cnt = (S.flags & F_ACTIVE) ? min(R * WAVE, S.nbound) : 0;
     if (cnt <= 0) continue;  // uniform across the wave
     const int64_t b = S.ibase;
     const float4 p0 = W.pos[b + W.el[b]];
     const double o[3] = {p0.x, p0.y, p0.z};
-    double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
-    if (lane < cnt) frame_accum(W, P, b, W.el[b + lane], o, v);
-    for (int c = 0; c < FRAME_SUMS; c++) {
-      const double r = 0.0 + wave_total(v[c]);
-      if (lane == 0) part[(int64_t)t * FRAME_SUMS + c] = r;
+    double acc[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < R; r++) {
+      double v[FRAME_SUMS] = {0, 0, 0, 0, 0, 0, 0};
+      const int e = lane + WAVE * r;
+      if (e < cnt) frame_accum(W, P, b, W.el[b + e], o, v);
+      for (int c = 0; c < FRAME_SUMS; c++) acc[c] += wave_total(v[c]);
     }
+    if (lane == 0)
+      for (int c = 0; c < FRAME_SUMS; c++) part[(int64_t)t * FRAME_SUMS + c] = acc[c];
   }
 }
 
@@ -2123,8 +2149,9 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
   }
 }
 
-// One-wave tiles, the same bits as the workgroup form (see
-// frames_partial_wave_kernel).
+// One-wave tiles of up to R * WAVE slots, the same bits as the workgroup
+// form (see frames_partial_wave_kernel).
+template <int R>
 __global__ void __launch_bounds__(BLOCK)
 output_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int ntiles, double *part) {
 #pragma clang fp contract(off)
@@ -2133,17 +2160,21 @@ output_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int n
     const Tile tl = tiles[t];
     const Seg &S = W.seg[tl.seg];
     const bool fin = (S.flags & F_FINAL) != 0;
-    const int cnt = fin ? min(WAVE, record_span(S)) : 0;
+    const int cnt = fin ? min(R * WAVE, record_span(S)) : 0;
     if (cnt <= 0) continue;  // uniform across the wave
     const int64_t b = S.ibase;
     const int nb = S.nbound;
-    double v[REC_SUMS];
-    for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
-    if (lane < cnt) record_accum(W, P, S, b, lane, (S.flags & F_CONV) != 0, nb, nb <= 100, v);
-    for (int c = 0; c < REC_SUMS; c++) {
-      const double r = 0.0 + wave_total(v[c]);
-      if (lane == 0) part[(int64_t)t * REC_SUMS + c] = r;
+    double acc[REC_SUMS];
+    for (int c = 0; c < REC_SUMS; c++) acc[c] = 0;
+    for (int r = 0; r < R; r++) {
+      double v[REC_SUMS];
+      for (int c = 0; c < REC_SUMS; c++) v[c] = 0;
+      const int e = lane + WAVE * r;
+      if (e < cnt) record_accum(W, P, S, b, e, (S.flags & F_CONV) != 0, nb, nb <= 100, v);
+      for (int c = 0; c < REC_SUMS; c++) acc[c] += wave_total(v[c]);
     }
+    if (lane == 0)
+      for (int c = 0; c < REC_SUMS; c++) part[(int64_t)t * REC_SUMS + c] = acc[c];
   }
 }
 

@@ -10,7 +10,7 @@
 // Primitives:
 //   segmented sort of (u64 key, u32 value) pairs by (key, value):
 //     tile_sort_kernel    : LDS bitonic sort of each tile
-//     sort_wave_kernel    : the same for segments of at most one wave, one per wave
+//     sort_wave_kernel    : the same for segments of at most SMALL_CAP, one per wave
 //     merge_pass_kernel   : one bottom-up merge level (co-rank by binary search)
 //   segmented exclusive scan (load/store functors):
 //     scan_tiles_kernel / scan_add_kernel, scan_wave_kernel
@@ -109,7 +109,8 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T *total) {
 // level -- are a suffix of it.  Their kernels give each tile one wave instead
 // of one workgroup: no workgroup barriers, four tiles per workgroup, and less
 // LDS per tile in flight.
-constexpr int SMALL_CAP = WAVE;
+constexpr int SMALL_CAP = 4 * WAVE;  // one-wave tiles: 4 elements per lane ...
+constexpr int TINY_CAP = WAVE;       // ... or 1 for the segments of at most one wave
 
 // LDS writes of the wave visible to its other lanes.
 __device__ __forceinline__ void wave_sync() {
@@ -196,43 +197,81 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
   }
 }
 
-// One-wave tiles: lane l holds element l (padding sorts last), a bitonic
-// network over the lanes with the partners' pairs read across lanes, up to
-// the power of two that covers the segment.  Such segments need no merge
-// level, so the result stays in (k0, v0).
+// One-wave tiles of up to R * WAVE elements: lane l holds elements R l ..
+// R l + R - 1 (padding sorts last), a bitonic network up to the power of two
+// that covers the segment, partners less than R apart swapped in the lane and
+// the others read across lanes.  Such segments need no merge level, so the
+// result stays in (k0, v0).
+__device__ __forceinline__ void sort_keep(bool keep_small, u64 ko, u32 vo, u64 &k, u32 &v) {
+  const bool other_less = pair_less(ko, vo, k, v);
+  if (keep_small ? other_less : !other_less) {
+    k = ko;
+    v = vo;
+  }
+}
+
+// Elements a < b of one lane: the lower keeps the smaller in an ascending run.
+__device__ __forceinline__ void sort_pair(bool up, u64 &ka, u32 &va, u64 &kb, u32 &vb) {
+  if (pair_less(kb, vb, ka, va) == up) {
+    const u64 tk = ka; ka = kb; kb = tk;
+    const u32 tv = va; va = vb; vb = tv;
+  }
+}
+
+template <int R>
 __global__ void __launch_bounds__(BLOCK)
 sort_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int64_t *__restrict__ base,
                  const int *__restrict__ n, u64 *k0, u32 *v0) {
+  static_assert(R == 1 || R == 4, "one or four elements per lane");
   const int lane = threadIdx.x & (WAVE - 1);
+  const int e0 = R * lane;
   WAVE_TILE_LOOP(t, t0, ntiles) {
     const Tile tl = tiles[t];
-    const int cnt = min(WAVE, n[tl.seg]);
+    const int cnt = min(R * WAVE, n[tl.seg]);
     if (cnt <= 1) continue;  // uniform across the wave
     const int64_t off = base[tl.seg];
-    u64 k = ~0ull;
-    u32 v = 0xffffffffu;
-    if (lane < cnt) {
-      k = k0[off + lane];
-      v = v0[off + lane];
+    u64 k[R];
+    u32 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int e = e0 + r;
+      k[r] = e < cnt ? k0[off + e] : ~0ull;
+      v[r] = e < cnt ? v0[off + e] : 0xffffffffu;
     }
     int P = 2;
     while (P < cnt) P <<= 1;
     for (int size = 2; size <= P; size <<= 1) {
-      for (int j = size >> 1; j > 0; j >>= 1) {
-        const u64 ko = __shfl_xor(k, j, WAVE);
-        const u32 vo = __shfl_xor(v, j, WAVE);
-        // the lower lane of a pair keeps the smaller one in an ascending run
-        const bool keep_small = ((lane & j) == 0) == ((lane & size) == 0);
-        const bool other_less = pair_less(ko, vo, k, v);
-        if (keep_small ? other_less : !other_less) {
-          k = ko;
-          v = vo;
+      for (int j = size >> 1; j > 0; j >>= 1) {  // uniform across the wave
+        if (j >= R) {
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const int e = e0 + r;
+            const u64 ko = __shfl_xor(k[r], j / R, WAVE);
+            const u32 vo = __shfl_xor(v[r], j / R, WAVE);
+            // the lower element of a pair keeps the smaller one in an ascending run
+            sort_keep(((e & j) == 0) == ((e & size) == 0), ko, vo, k[r], v[r]);
+          }
+        } else if (R == 4) {
+          const bool up = (e0 & size) == 0;  // size > j: the same for the lane's pairs
+          if (j == 2) {
+            sort_pair(up, k[0], v[0], k[R > 2 ? 2 : 0], v[R > 2 ? 2 : 0]);
+            sort_pair(up, k[R > 1 ? 1 : 0], v[R > 1 ? 1 : 0], k[R > 3 ? 3 : 0], v[R > 3 ? 3 : 0]);
+          } else {
+            // at size 2 the lane's two pairs run in opposite directions
+            sort_pair(up, k[0], v[0], k[R > 1 ? 1 : 0], v[R > 1 ? 1 : 0]);
+            sort_pair(size == 2 ? !up : up, k[R > 2 ? 2 : 0], v[R > 2 ? 2 : 0], k[R > 3 ? 3 : 0],
+                      v[R > 3 ? 3 : 0]);
+          }
         }
       }
     }
-    if (lane < cnt) {
-      k0[off + lane] = k;
-      v0[off + lane] = v;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int e = e0 + r;
+      if (e < cnt) {
+        k0[off + e] = k[r];
+        v0[off + e] = v[r];
+      }
     }
   }
 }
@@ -352,22 +391,36 @@ scan_tiles_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restr
   }
 }
 
-// Pass 1 for one-wave tiles (a segment's only tile: its prefixes are final).
-template <class T, class Load, class Store>
+// Pass 1 for one-wave tiles of up to R * WAVE elements (a segment's only
+// tile: its prefixes are final); lane l holds elements R l .. R l + R - 1.
+template <int R, class T, class Load, class Store>
 __global__ void __launch_bounds__(BLOCK)
 scan_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int *__restrict__ n,
                  T *__restrict__ tile_sum, Load load, Store store) {
   const int lane = threadIdx.x & (WAVE - 1);
   WAVE_TILE_LOOP(t, t0, ntiles) {
     const Tile tl = tiles[t];
-    const int cnt = min(WAVE, n[tl.seg]);
+    const int cnt = min(R * WAVE, n[tl.seg]);
     if (cnt <= 0) {  // uniform across the wave
       if (lane == 0) tile_sum[t] = T(0);
       continue;
     }
-    const T v = lane < cnt ? load(tl.seg, lane) : T(0);
-    const T inc = wave_inclusive_scan(v);
-    if (lane < cnt) store(tl.seg, lane, inc - v);
+    T v[R];
+    T sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int e = R * lane + r;
+      v[r] = e < cnt ? load(tl.seg, e) : T(0);
+      sum += v[r];
+    }
+    const T inc = wave_inclusive_scan(sum);
+    T off = inc - sum;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int e = R * lane + r;
+      if (e < cnt) store(tl.seg, e, off);
+      off += v[r];
+    }
     if (lane == WAVE - 1) tile_sum[t] = inc;
   }
 }
