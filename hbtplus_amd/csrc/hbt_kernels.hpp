@@ -153,6 +153,33 @@ __global__ void __launch_bounds__(BLOCK) seg_of_slot_kernel(int nseg, const int6
   }
 }
 
+// The same for a level whose segments start at multiples of WAVE slots: one
+// search per WAVE slots.  Each wave takes WAVE such chunks, lane i searching
+// chunk i, then writes the chunks' slots one chunk per store across the lanes.
+__global__ void __launch_bounds__(BLOCK) seg_of_chunk_kernel(int nseg, const int64_t *__restrict__ ibase,
+                                                             int64_t nslots, int *seg_of_slot) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t nchunk = (nslots + WAVE - 1) / WAVE;
+  const int64_t waves = (int64_t)gridDim.x * (BLOCK / WAVE);
+  for (int64_t g = (int64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; g * WAVE < nchunk; g += waves) {
+    const int64_t c = g * WAVE + lane;
+    int lo = 0;
+    if (c < nchunk) {
+      const int64_t s = c * WAVE;
+      int hi = nseg - 1;  // ibase[lo] <= s
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ibase[mid] <= s) lo = mid; else hi = mid - 1;
+      }
+    }
+    for (int j = 0; j < WAVE; j++) {
+      const int k = __shfl(lo, j, WAVE);
+      const int64_t s = (g * WAVE + j) * WAVE + lane;
+      if (s < nslots) seg_of_slot[s] = k;
+    }
+  }
+}
+
 // The level's segment records: zero, then the host's fixed fields.
 __global__ void __launch_bounds__(BLOCK) seg_expand_kernel(Seg *seg, const SegIn *in, int n) {
   for (int k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += gridDim.x * BLOCK) {

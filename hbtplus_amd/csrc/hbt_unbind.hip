@@ -1010,7 +1010,8 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
   const int gsb = (int)std::min<int64_t>(nseg, (int64_t)c->cus * 8);
   R.gblk = gsb;
   seg_expand_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W.seg, c->segin.as<SegIn>(), (int)nseg);
-  seg_of_slot_kernel<<<R.gs, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, L.nslots, c->seg_of_slot.as<int>());
+  // segment bases are multiples of WAVE (capacities are rounded up to it)
+  seg_of_chunk_kernel<<<R.gs, BLOCK, 0, c->stream>>>((int)nseg, R.W.ibase, L.nslots, c->seg_of_slot.as<int>());
   build_lists_kernel<<<gsb, BLOCK, 0, c->stream>>>(R.W, c->child_ptr.as<int>(), c->child_list.as<int>());
   seg_init_kernel<<<R.gseg, BLOCK, 0, c->stream>>>(R.W, R.P);
   if (L.ndraws > 0) {
@@ -1353,7 +1354,12 @@ int hbt_plan_batch(hbt_subhalo *subs, int64_t nsub, int64_t *total) {
 
 static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64_t n_list,
                              hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
-                             int64_t *out_index, float *out_energy, int plain, const int64_t *out_cap);
+                             int64_t *out_index, float *out_energy, int plain, const int64_t *out_cap,
+                             bool planned);
+
+// The host plan of the last batch (memory reused from call to call, so a
+// repeated batch pays no page faults for it).
+static thread_local Nest t_nest;
 
 int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs,
                      const hbt_subhalo *subs_host, int64_t nsub, int64_t *out_index,
@@ -1361,7 +1367,7 @@ int hbt_unbind_batch(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *
   if (!ctx || !particles) return fail(HBT_ERR_INVALID_ARG, "null argument");
   ctx->list_index = nullptr;
   return unbind_batch_impl(ctx, particles, particles->n, subs, subs_host, nsub, out_index, out_energy,
-                           plain, nullptr);
+                           plain, nullptr, false);
 }
 
 static bool tuning_valid(const hbt_tuning &t) {
@@ -1473,6 +1479,21 @@ int hbt_autotune(hbt_ctx *ctx, const hbt_particles *particles, hbt_subhalo *subs
   return HBT_OK;
 }
 
+// Lists that cover a good part of the snapshot are gathered from packed
+// records (one streaming pass over the snapshot, then one 32-byte read per
+// listed particle); sparse lists read the columns directly.
+static int pack_snapshot(hbt_ctx *ctx, const hbt_particles *snapshot, int64_t n_list) {
+  ctx->packed = nullptr;
+  if (n_list > 0 && snapshot->n <= 4 * n_list) {
+    RETCHK(ctx->pack.ensure((size_t)snapshot->n * 2 * sizeof(float4)));
+    pack_particles_kernel<<<grid_for(snapshot->n, ctx->cus), BLOCK, 0, ctx->stream>>>(*snapshot,
+                                                                                      ctx->pack.as<float4>());
+    HIPCHK(hipGetLastError());
+    ctx->packed = ctx->pack.as<float4>();
+  }
+  return HBT_OK;
+}
+
 int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_t *list_index,
                              int64_t n_list, hbt_subhalo *subs, const hbt_subhalo *subs_host,
                              int64_t nsub, int64_t *out_index, float *out_energy, int plain) {
@@ -1481,14 +1502,33 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
   if (n_list >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 list entries");
   HIPCHK(hipSetDevice(ctx->device));
   if (nsub > 0 && !subs_host) return fail(HBT_ERR_INVALID_ARG, "null argument");
+  // the snapshot's packed records and the list positions are this call's only
+  struct Reset {
+    hbt_ctx *c;
+    ~Reset() { c->packed = nullptr; c->list_index = nullptr; }
+  } reset{ctx};
   int cnt[2] = {0, 0};
+  bool planned = false;
   if (n_list > 0) {
-    RETCHK(ctx->active.ensure(16));
-    HIPCHK(hipMemcpyAsync(ctx->active.p, cnt, sizeof cnt, hipMemcpyHostToDevice, ctx->stream));
+    // the check, and for dense lists the packing of the snapshot, run on the
+    // device while the host plans the nests
+    RETCHK(ctx->active.ensure(32));
+    int *dcnt = ctx->active.as<int>() + 4;
+    HIPCHK(hipMemsetAsync(dcnt, 0, 2 * sizeof(int), ctx->stream));
     list_check_kernel<<<grid_for(n_list, ctx->cus), BLOCK, 0, ctx->stream>>>(list_index, n_list, snapshot->n,
-                                                                              ctx->active.as<int>());
-    HIPCHK(hipMemcpyAsync(cnt, ctx->active.p, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+                                                                              dcnt);
+    HIPCHK(hipMemcpyAsync(ctx->h_active + 4, dcnt, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RETCHK(pack_snapshot(ctx, snapshot, n_list));
+    if (nsub > 0) {
+      for (int64_t s = 0; s < nsub; s++)
+        if (subs_host[s].offset < 0 || subs_host[s].count < 0 || subs_host[s].offset + subs_host[s].count > n_list)
+          return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the list positions");
+      RETCHK(plan_nests(subs_host, nsub, t_nest));
+      planned = true;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    cnt[0] = ctx->h_active[4];
+    cnt[1] = ctx->h_active[5];
     if (cnt[0]) return fail(HBT_ERR_INVALID_ARG, std::to_string(cnt[0]) + " list entries lie past the snapshot");
   }
   // null entries: the lists are compacted first (KickNullParticles); the
@@ -1498,15 +1538,10 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
   std::vector<hbt_subhalo> kicked;
   std::vector<int64_t> out_cap;
   if (cnt[1] > 0) {
-    for (int64_t s = 0; s < nsub; s++)
-      if (subs_host[s].offset < 0 || subs_host[s].count < 0 || subs_host[s].offset + subs_host[s].count > n_list)
-        return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the list positions");
-    {
-      Nest N;
-      RETCHK(plan_nests(subs_host, nsub, N));
-      out_cap.resize(nsub);
-      for (int64_t s = 0; s < nsub; s++) out_cap[s] = N.f[s].cap;
-    }
+    // the caller planned its outputs for the uncompacted lists
+    out_cap.resize(nsub);
+    for (int64_t s = 0; s < nsub; s++) out_cap[s] = t_nest.f[s].cap;
+    planned = false;
     std::vector<KickIn> kin(nsub);
     for (int64_t s = 0; s < nsub; s++)
       kin[s] = KickIn{subs_host[s].offset, subs_host[s].count,
@@ -1539,39 +1574,35 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
     n_list = dst;
     subs_host = kicked.data();
   }
-  // lists that cover a good part of the snapshot are gathered from packed
-  // records (one streaming pass over the snapshot, then one 32-byte read per
-  // listed particle); sparse lists read the columns directly
-  if (n_list > 0 && snapshot->n <= 4 * n_list) {
-    RETCHK(ctx->pack.ensure((size_t)snapshot->n * 2 * sizeof(float4)));
-    pack_particles_kernel<<<grid_for(snapshot->n, ctx->cus), BLOCK, 0, ctx->stream>>>(*snapshot,
-                                                                                      ctx->pack.as<float4>());
-    HIPCHK(hipGetLastError());
-    ctx->packed = ctx->pack.as<float4>();
-  }
   ctx->list_index = list_index;
-  int rc = unbind_batch_impl(ctx, snapshot, n_list, subs, subs_host, nsub, out_index, out_energy, plain,
-                             out_cap.empty() ? nullptr : out_cap.data());
-  ctx->list_index = nullptr;
-  ctx->packed = nullptr;
-  return rc;
+  return unbind_batch_impl(ctx, snapshot, n_list, subs, subs_host, nsub, out_index, out_energy, plain,
+                           out_cap.empty() ? nullptr : out_cap.data(), planned);
 }
 
 // out_cap: the output capacity the caller planned per subhalo when it exceeds
 // what the lists now need (lists compacted by KickNullParticles), or null.
 static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64_t n_list,
                              hbt_subhalo *subs, const hbt_subhalo *subs_host, int64_t nsub,
-                             int64_t *out_index, float *out_energy, int plain, const int64_t *out_cap) {
+                             int64_t *out_index, float *out_energy, int plain, const int64_t *out_cap,
+                             bool planned) {
   if (nsub > 0 && (!subs || !subs_host || !out_index || !out_energy))
     return fail(HBT_ERR_INVALID_ARG, "null argument");
   if (nsub == 0) return HBT_OK;
   if (particles->n >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 particles");
   HIPCHK(hipSetDevice(ctx->device));
-  // host tables kept per thread from call to call (their memory is reused,
-  // so a repeated batch pays no page faults for them)
-  static thread_local Nest N;
+  // the type check runs on the device while the host plans the nests
+  const bool check_types = particles->type && particles->n > 0;
+  if (check_types) {
+    RETCHK(ctx->active.ensure(32));
+    int *dbad = ctx->active.as<int>() + 6;
+    HIPCHK(hipMemsetAsync(dbad, 0, sizeof(int), ctx->stream));
+    type_check_kernel<<<grid_for(particles->n, ctx->cus), BLOCK, 0, ctx->stream>>>(particles->type, particles->n,
+                                                                                  dbad);
+    HIPCHK(hipMemcpyAsync(ctx->h_active + 6, dbad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  Nest &N = t_nest;
   static thread_local Level L;
-  RETCHK(plan_nests(subs_host, nsub, N));
+  if (!planned) RETCHK(plan_nests(subs_host, nsub, N));
   for (int64_t s = 0; s < nsub; s++) {
     if (N.f[s].offset < 0 || N.f[s].offset + N.f[s].count > n_list)
       return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the particle columns");
@@ -1579,14 +1610,9 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   }
   hbt_particles P = *particles;
   int64_t passes = 0;
-  if (particles->type && particles->n > 0) {
-    RETCHK(ctx->active.ensure(16));
-    int bad = 0;
-    HIPCHK(hipMemcpyAsync(ctx->active.p, &bad, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    type_check_kernel<<<grid_for(particles->n, ctx->cus), BLOCK, 0, ctx->stream>>>(particles->type, particles->n,
-                                                                                  ctx->active.as<int>());
-    HIPCHK(hipMemcpyAsync(&bad, ctx->active.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (check_types) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int bad = ctx->h_active[6];
     if (bad) return fail(HBT_ERR_INVALID_ARG, std::to_string(bad) + " particle types outside [0, 6)");
   }
   const int64_t MS = ctx->params.max_sample_size > 0 ? ctx->params.max_sample_size : 0;
