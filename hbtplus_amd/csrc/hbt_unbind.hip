@@ -237,27 +237,69 @@ struct Nest {
 };
 
 // Members of a level sorted by capacity, largest first, ties in member order
-// (two 16-bit passes of a stable radix sort on 31-bit capacities).
-void sort_largest_first(const Nest &N, const std::vector<int> &members, std::vector<int> &out) {
+// (a stable LSD radix sort on 31-bit keys in three 11-bit digits, the keys
+// gathered once, all three digit counts taken in one sweep and a digit that
+// is the same for every key skipped); caps receives the sorted capacities.
+void sort_largest_first(const Nest &N, const std::vector<int> &members, std::vector<int> &out,
+                        std::vector<int> &caps) {
   const size_t n = members.size();
   out = members;
+  caps.resize(n);
   if (n < 64) {
     std::stable_sort(out.begin(), out.end(), [&](int a, int b) { return N.f[a].cap > N.f[b].cap; });
+    for (size_t i = 0; i < n; i++) caps[i] = (int)N.f[out[i]].cap;
     return;
   }
+  constexpr int DB = 11, NB = 1 << DB;
   static thread_local std::vector<int> tmp;
-  static thread_local std::vector<uint32_t> cnt;
+  static thread_local std::vector<uint32_t> key, key2, cnt;
   tmp.resize(n);
-  auto key = [&](int s) { return (uint32_t)(0x7fffffff - (int32_t)N.f[s].cap); };
-  int *src = out.data(), *dst = tmp.data();
-  for (int shift = 0; shift < 32; shift += 16) {
-    cnt.assign(65537, 0u);
-    for (size_t i = 0; i < n; i++) cnt[((key(src[i]) >> shift) & 0xffffu) + 1]++;
-    for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
-    for (size_t i = 0; i < n; i++) dst[cnt[(key(src[i]) >> shift) & 0xffffu]++] = src[i];
-    std::swap(src, dst);
+  key.resize(n);
+  key2.resize(n);
+  cnt.assign(3 * NB, 0u);
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t k = (uint32_t)(0x7fffffff - (int32_t)N.f[out[i]].cap);
+    key[i] = k;
+    cnt[k & (NB - 1)]++;
+    cnt[NB + ((k >> DB) & (NB - 1))]++;
+    cnt[2 * NB + (k >> (2 * DB))]++;
   }
-  // two passes: the result is back in out
+  int *src = out.data(), *dst = tmp.data();
+  uint32_t *ks = key.data(), *kd = key2.data();
+  for (int d = 0; d < 3; d++) {
+    uint32_t *c = cnt.data() + d * NB;
+    const int shift = d * DB;
+    if (c[(ks[0] >> shift) & (NB - 1)] == n) continue;  // one value of this digit: order unchanged
+    uint32_t o = 0;
+    for (int b = 0; b < NB; b++) {
+      const uint32_t x = c[b];
+      c[b] = o;
+      o += x;
+    }
+    for (size_t i = 0; i < n; i++) {
+      const uint32_t p = c[(ks[i] >> shift) & (NB - 1)]++;
+      dst[p] = src[i];
+      kd[p] = ks[i];
+    }
+    std::swap(src, dst);
+    std::swap(ks, kd);
+  }
+  if (src != out.data()) memcpy(out.data(), src, n * sizeof(int));
+  for (size_t i = 0; i < n; i++) caps[i] = (int)(0x7fffffff - ks[i]);
+}
+
+// f(lo, hi) over [0, n), on four threads for large n.
+template <class F>
+static void parallel_ranges(int64_t n, const F &f) {
+  const int nth = n >= 65536 ? 4 : 1;
+  if (nth == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; t++) th.emplace_back(f, n * t / nth, n * (t + 1) / nth);
+  f((int64_t)0, n / nth);
+  for (auto &x : th) x.join();
 }
 
 // draw_off: per subhalo, the first of its shuffle's draws in the level's
@@ -267,39 +309,48 @@ void build_level(const Nest &N, const std::vector<int> &members, const hbt_tunin
                  const int64_t *draw_off, int64_t nsub, Level &L) {
   const int n = (int)members.size();
   // largest first: the merge levels of a sort then cover a prefix
-  sort_largest_first(N, members, L.subs);
+  static thread_local std::vector<int> caps;
+  sort_largest_first(N, members, L.subs, caps);
   L.seg.resize(n);
   L.ibase.resize(n);
   L.capal.resize(n);
   L.first_tile.resize(n);
   L.ntile.resize(n);
-  L.child_ptr.assign(n + 1, 0);
-  L.child_list.clear();
-  int64_t ntiles = 0;
-  for (int k = 0; k < n; k++) ntiles += (N.f[L.subs[k]].cap + TILE - 1) / TILE;
-  L.tiles.resize(ntiles);
+  // slot and tile offsets in level order
   int64_t off = 0, t0 = 0;
   L.maxcap = 0;
   for (int k = 0; k < n; k++) {
-    const int s = L.subs[k];
-    const SubPlan &F = N.f[s];
-    const int c = (int)F.cap;
+    const int c = caps[k];
     const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
-    L.seg[k] = SegIn{off, F.out_off, F.offset, F.nbp, s, c, (int)F.count, walk_group_for(tune, c),
-                     draw_off ? (int)draw_off[s] : -1, draw_off ? (int)draw_off[nsub + s] : 0};
+    const int nt = (c + TILE - 1) / TILE;
     L.ibase[k] = off;
     L.capal[k] = ca;
     L.first_tile[k] = (int)t0;
-    const int nt = (c + TILE - 1) / TILE;
     L.ntile[k] = nt;
-    for (int t = 0; t < nt; t++) L.tiles[t0 + t] = Tile{k, t * TILE};
-    t0 += nt;
     off += ca;
+    t0 += nt;
     L.maxcap = std::max(L.maxcap, c);
-    for (int i = 0; i < N.nchild(s); i++) L.child_list.push_back(N.child(s)[i]);
-    L.child_ptr[k + 1] = (int)L.child_list.size();
   }
+  L.tiles.resize(t0);
   L.nslots = off;
+  // nest children in level order (none in a level of top-level subhalos)
+  L.child_ptr.assign(n + 1, 0);
+  if (N.ch_ptr[N.n] > 0)
+    for (int k = 0; k < n; k++) L.child_ptr[k + 1] = L.child_ptr[k] + N.nchild(L.subs[k]);
+  L.child_list.resize(L.child_ptr[n]);
+  // caps is this thread's: the workers take its data by pointer
+  const int *cp = caps.data();
+  parallel_ranges(n, [&, cp](int64_t k0, int64_t k1) {
+    for (int64_t k = k0; k < k1; k++) {
+      const int s = L.subs[k];
+      const SubPlan &F = N.f[s];
+      const int c = cp[k];
+      L.seg[k] = SegIn{L.ibase[k], F.out_off, F.offset, F.nbp, s, c, (int)F.count, walk_group_for(tune, c),
+                       draw_off ? (int)draw_off[s] : -1, draw_off ? (int)draw_off[nsub + s] : 0};
+      for (int t = 0; t < L.ntile[k]; t++) L.tiles[L.first_tile[k] + t] = Tile{(int)k, t * TILE};
+      for (int i = 0; i < N.nchild(s); i++) L.child_list[L.child_ptr[k] + i] = N.child(s)[i];
+    }
+  });
 }
 
 // Copies host tables to the device: packed (256-byte aligned) into the pinned
@@ -1610,11 +1661,18 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
   }
   hbt_particles P = *particles;
   int64_t passes = 0;
-  if (check_types) {
+  // the type check's verdict is taken before the first level runs (the
+  // record kernels index per-type sums by type), after the host has laid the
+  // level out, so that the check and the packing overlap that work too
+  bool types_checked = !check_types;
+  auto types_ok = [&]() -> int {
+    if (types_checked) return HBT_OK;
+    types_checked = true;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int bad = ctx->h_active[6];
     if (bad) return fail(HBT_ERR_INVALID_ARG, std::to_string(bad) + " particle types outside [0, 6)");
-  }
+    return HBT_OK;
+  };
   const int64_t MS = ctx->params.max_sample_size > 0 ? ctx->params.max_sample_size : 0;
   if (MS > 0 && !particles->id) return fail(HBT_ERR_INVALID_ARG, "sampling needs the particle ID column");
   if (MS > 0) {
@@ -1680,6 +1738,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
         build_level(N, one, ctx->tune, MS > 0 ? draw_off.data() : nullptr, nsub, L);
         L.ndraws = nd;
         memcpy(L.win, win, sizeof win);
+        RETCHK(types_ok());
         RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
       }
       continue;
@@ -1696,6 +1755,7 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     if (pass_times)
       fprintf(stderr, "[hbt] level %d of %zu segments built on the host in %.3f ms\n", h, L.seg.size(),
               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    RETCHK(types_ok());
     RETCHK(run_level(ctx, L, &P, subs, out_index, out_energy, &passes));
   }
   static thread_local std::vector<int64_t> gap;
@@ -1720,17 +1780,16 @@ static int unbind_batch_impl(hbt_ctx *ctx, const hbt_particles *particles, int64
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  ctx->last_ms = ms;
-  ctx->last_passes = passes;
-  // evaluation count, summed on the device
+  // evaluation count, summed on the device after the timed work
   long long *h_ev = (long long *)(ctx->h_active + 4);
   eval_sum_kernel<<<1, BLOCK, 0, ctx->stream>>>(subs, nsub, ctx->counters.as<unsigned long long>() + 4);
   HIPCHK(hipMemcpyAsync(h_ev, ctx->counters.as<unsigned long long>() + 4, 8, hipMemcpyDeviceToHost,
                         ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  ctx->last_passes = passes;
   ctx->last_evals = *h_ev;
   return HBT_OK;
 }
