@@ -157,43 +157,58 @@ tile_sort_kernel(const Tile *__restrict__ tiles, int ntiles, const int64_t *__re
                  const int *__restrict__ n, u64 *k0, u32 *v0, u64 *k1, u32 *v1) {
   __shared__ u64 sk[TILE];
   __shared__ u32 sv[TILE];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const Tile tl = tiles[t];
-    const int nseg = n[tl.seg];
-    const int cnt = min(TILE, nseg - tl.start);
-    if (cnt <= 0) continue;  // uniform across the block
-    const int64_t off = base[tl.seg] + tl.start;
-    u64 *kout = (merge_levels(nseg) & 1) ? k1 : k0;
-    u32 *vout = (merge_levels(nseg) & 1) ? v1 : v0;
-    if (cnt == 1) {
-      if (threadIdx.x == 0) { kout[off] = k0[off]; vout[off] = v0[off]; }
-      continue;
-    }
-    int P = 1;
-    while (P < cnt) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += BLOCK) {
-      if (i < cnt) { sk[i] = k0[off + i]; sv[i] = v0[off + i]; }
-      else { sk[i] = ~0ull; sv[i] = 0xffffffffu; }
-    }
+  __shared__ int slive[BLOCK];
+  // The block's tiles (blockIdx.x + k gridDim.x) are checked BLOCK at a time,
+  // one per thread, and only the non-empty ones are sorted: a sort of a few
+  // entries per segment (the energy sort of a removed block) skips its empty
+  // tiles without a chain of header loads.
+  for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x * BLOCK) {
+    const int tc = tb + (int)threadIdx.x * (int)gridDim.x;
+    int live = 0;
+    if (tc < ntiles) live = n[tiles[tc].seg] - tiles[tc].start > 0 ? 1 : 0;
+    int nlive;
+    const int at = block_exclusive_scan(live, &nlive);
+    if (live) slive[at] = tc;
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int p = threadIdx.x; p < (P >> 1); p += BLOCK) {
-          // 2 j (p / j) + p % j, with j a power of two
-          const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-          const int q = i + j;
-          const bool up = (i & k) == 0;
-          u64 ka = sk[i], kb = sk[q];
-          u32 va = sv[i], vb = sv[q];
-          if (pair_less(kb, vb, ka, va) == up) {
-            sk[i] = kb; sk[q] = ka; sv[i] = vb; sv[q] = va;
-          }
-        }
-        __syncthreads();
+    for (int li = 0; li < nlive; li++) {
+      const int t = slive[li];
+      const Tile tl = tiles[t];
+      const int nseg = n[tl.seg];
+      const int cnt = min(TILE, nseg - tl.start);
+      const int64_t off = base[tl.seg] + tl.start;
+      u64 *kout = (merge_levels(nseg) & 1) ? k1 : k0;
+      u32 *vout = (merge_levels(nseg) & 1) ? v1 : v0;
+      if (cnt == 1) {
+        if (threadIdx.x == 0) { kout[off] = k0[off]; vout[off] = v0[off]; }
+        continue;
       }
+      int P = 1;
+      while (P < cnt) P <<= 1;
+      for (int i = threadIdx.x; i < P; i += BLOCK) {
+        if (i < cnt) { sk[i] = k0[off + i]; sv[i] = v0[off + i]; }
+        else { sk[i] = ~0ull; sv[i] = 0xffffffffu; }
+      }
+      __syncthreads();
+      for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int p = threadIdx.x; p < (P >> 1); p += BLOCK) {
+            // 2 j (p / j) + p % j, with j a power of two
+            const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+            const int q = i + j;
+            const bool up = (i & k) == 0;
+            u64 ka = sk[i], kb = sk[q];
+            u32 va = sv[i], vb = sv[q];
+            if (pair_less(kb, vb, ka, va) == up) {
+              sk[i] = kb; sk[q] = ka; sv[i] = vb; sv[q] = va;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (int i = threadIdx.x; i < cnt; i += BLOCK) { kout[off + i] = sk[i]; vout[off + i] = sv[i]; }
+      __syncthreads();
     }
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) { kout[off + i] = sk[i]; vout[off + i] = sv[i]; }
-    __syncthreads();
+    __syncthreads();  // slive is rewritten for the next batch of tiles
   }
 }
 
