@@ -514,14 +514,13 @@ __global__ void __launch_bounds__(BLOCK) bbox_kernel(Work W, const Tile *tiles, 
 // One-wave tiles (up to R * WAVE sources, lane l taking sources l + WAVE r):
 // a segment's only tile, so its box is stored directly.
 template <int R>
-__global__ void __launch_bounds__(BLOCK) bbox_wave_kernel(Work W, const Tile *tiles, int t0, int ntiles) {
+__device__ __forceinline__ void bbox_wave_tile(const Work &W, const Tile &tl, int t) {
   const int lane = threadIdx.x & (WAVE - 1);
-  WAVE_TILE_LOOP(t, t0, ntiles) {
-    const Tile tl = tiles[t];
+  {
     Seg &S = W.seg[tl.seg];
     if (lane == 0) W.tcount[t] = 0;
     const int cnt = min(R * WAVE, S.n_src);
-    if (cnt <= 0) continue;  // uniform across the wave
+    if (cnt <= 0) return;  // uniform across the wave
     const int64_t b = S.ibase;
     unsigned int mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
 #pragma unroll
@@ -540,6 +539,15 @@ __global__ void __launch_bounds__(BLOCK) bbox_wave_kernel(Work W, const Tile *ti
       }
     if (lane == 0)
       for (int j = 0; j < 3; j++) { S.bmin[j] = mn[j]; S.bmax[j] = mx[j]; }
+  }
+}
+
+// Tiles t0 <= t < tmid take four sources per lane, tmid <= t < ntiles one.
+__global__ void __launch_bounds__(BLOCK) bbox_wave_kernel(Work W, const Tile *tiles, int t0, int tmid, int ntiles) {
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    if (t < tmid) bbox_wave_tile<4>(W, tl, t);
+    else bbox_wave_tile<1>(W, tl, t);
   }
 }
 
@@ -1801,15 +1809,14 @@ frames_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
 // order, as the workgroup form adds its waves' totals, so both give the same
 // bits.
 template <int R>
-__global__ void __launch_bounds__(BLOCK)
-frames_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int ntiles, double *part) {
+__device__ __forceinline__ void frames_wave_tile(const Work &W, const DevParams &P, const Tile &tl, int t,
+                                                 double *part) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
-  WAVE_TILE_LOOP(t, t0, ntiles) {
-    const Tile tl = tiles[t];
+  {
     const Seg &S = W.seg[tl.seg];
     const int cnt = (S.flags & F_ACTIVE) ? min(R * WAVE, S.nbound) : 0;
-    if (cnt <= 0) continue;  // uniform across the wave
+    if (cnt <= 0) return;  // uniform across the wave
     const int64_t b = S.ibase;
     const float4 p0 = W.pos[b + W.el[b]];
     const double o[3] = {p0.x, p0.y, p0.z};
@@ -1822,6 +1829,16 @@ frames_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int n
     }
     if (lane == 0)
       for (int c = 0; c < FRAME_SUMS; c++) part[(int64_t)t * FRAME_SUMS + c] = acc[c];
+  }
+}
+
+// Tiles t0 <= t < tmid take four entries per lane, tmid <= t < ntiles one.
+__global__ void __launch_bounds__(BLOCK)
+frames_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int tmid, int ntiles, double *part) {
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    if (t < tmid) frames_wave_tile<4>(W, P, tl, t, part);
+    else frames_wave_tile<1>(W, P, tl, t, part);
   }
 }
 
@@ -2179,16 +2196,15 @@ output_partial_kernel(Work W, DevParams P, const Tile *tiles, int ntiles, double
 // One-wave tiles of up to R * WAVE slots, the same bits as the workgroup
 // form (see frames_partial_wave_kernel).
 template <int R>
-__global__ void __launch_bounds__(BLOCK)
-output_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int ntiles, double *part) {
+__device__ __forceinline__ void output_wave_tile(const Work &W, const DevParams &P, const Tile &tl, int t,
+                                                 double *part) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
-  WAVE_TILE_LOOP(t, t0, ntiles) {
-    const Tile tl = tiles[t];
+  {
     const Seg &S = W.seg[tl.seg];
     const bool fin = (S.flags & F_FINAL) != 0;
     const int cnt = fin ? min(R * WAVE, record_span(S)) : 0;
-    if (cnt <= 0) continue;  // uniform across the wave
+    if (cnt <= 0) return;  // uniform across the wave
     const int64_t b = S.ibase;
     const int nb = S.nbound;
     double acc[REC_SUMS];
@@ -2202,6 +2218,16 @@ output_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int n
     }
     if (lane == 0)
       for (int c = 0; c < REC_SUMS; c++) part[(int64_t)t * REC_SUMS + c] = acc[c];
+  }
+}
+
+// Tiles t0 <= t < tmid take four slots per lane, tmid <= t < ntiles one.
+__global__ void __launch_bounds__(BLOCK)
+output_partial_wave_kernel(Work W, DevParams P, const Tile *tiles, int t0, int tmid, int ntiles, double *part) {
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    if (t < tmid) output_wave_tile<4>(W, P, tl, t, part);
+    else output_wave_tile<1>(W, P, tl, t, part);
   }
 }
 

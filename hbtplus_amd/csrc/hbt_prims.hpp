@@ -234,16 +234,14 @@ __device__ __forceinline__ void sort_pair(bool up, u64 &ka, u32 &va, u64 &kb, u3
 }
 
 template <int R>
-__global__ void __launch_bounds__(BLOCK)
-sort_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int64_t *__restrict__ base,
-                 const int *__restrict__ n, u64 *k0, u32 *v0) {
+__device__ __forceinline__ void sort_wave_tile(const Tile &tl, const int64_t *__restrict__ base,
+                                               const int *__restrict__ n, u64 *k0, u32 *v0) {
   static_assert(R == 1 || R == 4, "one or four elements per lane");
   const int lane = threadIdx.x & (WAVE - 1);
   const int e0 = R * lane;
-  WAVE_TILE_LOOP(t, t0, ntiles) {
-    const Tile tl = tiles[t];
+  {
     const int cnt = min(R * WAVE, n[tl.seg]);
-    if (cnt <= 1) continue;  // uniform across the wave
+    if (cnt <= 1) return;  // uniform across the wave
     const int64_t off = base[tl.seg];
     u64 k[R];
     u32 v[R];
@@ -288,6 +286,17 @@ sort_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int64
         v0[off + e] = v[r];
       }
     }
+  }
+}
+
+// Tiles t0 <= t < tmid take four elements per lane, tmid <= t < ntiles one.
+__global__ void __launch_bounds__(BLOCK)
+sort_wave_kernel(const Tile *__restrict__ tiles, int t0, int tmid, int ntiles, const int64_t *__restrict__ base,
+                 const int *__restrict__ n, u64 *k0, u32 *v0) {
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    if (t < tmid) sort_wave_tile<4>(tl, base, n, k0, v0);
+    else sort_wave_tile<1>(tl, base, n, k0, v0);
   }
 }
 
@@ -409,16 +418,14 @@ scan_tiles_kernel(const Tile *__restrict__ tiles, int ntiles, const int *__restr
 // Pass 1 for one-wave tiles of up to R * WAVE elements (a segment's only
 // tile: its prefixes are final); lane l holds elements R l .. R l + R - 1.
 template <int R, class T, class Load, class Store>
-__global__ void __launch_bounds__(BLOCK)
-scan_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int *__restrict__ n,
-                 T *__restrict__ tile_sum, Load load, Store store) {
+__device__ __forceinline__ void scan_wave_tile(const Tile &tl, int t, const int *__restrict__ n,
+                                               T *__restrict__ tile_sum, Load load, Store store) {
   const int lane = threadIdx.x & (WAVE - 1);
-  WAVE_TILE_LOOP(t, t0, ntiles) {
-    const Tile tl = tiles[t];
+  {
     const int cnt = min(R * WAVE, n[tl.seg]);
     if (cnt <= 0) {  // uniform across the wave
       if (lane == 0) tile_sum[t] = T(0);
-      continue;
+      return;
     }
     T v[R];
     T sum = 0;
@@ -437,6 +444,18 @@ scan_wave_kernel(const Tile *__restrict__ tiles, int t0, int ntiles, const int *
       off += v[r];
     }
     if (lane == WAVE - 1) tile_sum[t] = inc;
+  }
+}
+
+// Tiles t0 <= t < tmid take four elements per lane, tmid <= t < ntiles one.
+template <class T, class Load, class Store>
+__global__ void __launch_bounds__(BLOCK)
+scan_wave_kernel(const Tile *__restrict__ tiles, int t0, int tmid, int ntiles, const int *__restrict__ n,
+                 T *__restrict__ tile_sum, Load load, Store store) {
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    if (t < tmid) scan_wave_tile<4, T>(tl, t, n, tile_sum, load, store);
+    else scan_wave_tile<1, T>(tl, t, n, tile_sum, load, store);
   }
 }
 

@@ -596,7 +596,7 @@ struct Runner {
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
   int nt_big = 0;              // tiles of segments of more than SMALL_CAP slots (a prefix)
   int nt_tiny = 0;             // then those of more than TINY_CAP; the rest are of at most TINY_CAP
-  int gsmall = 1, gtiny = 1;   // one-wave tile grids (four tiles per workgroup)
+  int gsmall = 1, gtiny = 1, gwave = 1;  // one-wave tile grids (four tiles per workgroup)
   int gwseg = 1;               // segment-wave grid
   int gw;                      // walk grid: every slot block once
   // the tile table a pass uses: the level's, or that of its active segments
@@ -629,6 +629,7 @@ struct Runner {
     gtile = (int)std::max<int64_t>(1, std::min<int64_t>(nt_big, (int64_t)c->cus * 8));
     gsmall = wave_grid(nt_tiny - nt_big);
     gtiny = wave_grid(nt_cur - nt_tiny);
+    gwave = wave_grid(nt_cur - nt_big);
   }
 
   // tile kernels: the workgroup form over the large segments' tiles, the
@@ -640,6 +641,7 @@ struct Runner {
   bool any_big() const { return nt_big > 0; }
   bool any_small() const { return nt_tiny > nt_big; }
   bool any_tiny() const { return nt_cur > nt_tiny; }
+  bool any_wave() const { return nt_cur > nt_big; }
 
   // Lists the segments still active after a pass: their wave-sized slot
   // chunks, their tiles (a table of its own, with a first-tile index per
@@ -723,8 +725,7 @@ struct Runner {
     const int nt = nt_cur;
     if (c->prof_mode == 1) HIPCHK(hipEventRecord(c->evs0, c->stream));
     if (any_big()) tile_sort_kernel<<<gtile, BLOCK, 0, c->stream>>>(tl, nt_big, W.ibase, n, W.k0, W.v0, W.k1, W.v1);
-    if (any_small()) sort_wave_kernel<4><<<gsmall, BLOCK, 0, c->stream>>>(tl, nt_big, nt_tiny, W.ibase, n, W.k0, W.v0);
-    if (any_tiny()) sort_wave_kernel<1><<<gtiny, BLOCK, 0, c->stream>>>(tl, nt_tiny, nt, W.ibase, n, W.k0, W.v0);
+    if (any_wave()) sort_wave_kernel<<<gwave, BLOCK, 0, c->stream>>>(tl, nt_big, nt_tiny, nt, W.ibase, n, W.k0, W.v0);
     // one launch per merge level of the largest segment of the level, over
     // the tiles of the segments whose capacity exceeds the level's width
     // (segments are ordered largest first, so those tiles are a prefix)
@@ -755,8 +756,7 @@ struct Runner {
   int seg_scan(Load ld, Store st, Add ad, const int *n) {
     u64 *ts = c->tsum.as<u64>();
     if (any_big()) scan_tiles_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nt_big, n, ts, ld, st);
-    if (any_small()) scan_wave_kernel<4, u64><<<gsmall, BLOCK, 0, c->stream>>>(tl_cur, nt_big, nt_tiny, n, ts, ld, st);
-    if (any_tiny()) scan_wave_kernel<1, u64><<<gtiny, BLOCK, 0, c->stream>>>(tl_cur, nt_tiny, nt_cur, n, ts, ld, st);
+    if (any_wave()) scan_wave_kernel<u64><<<gwave, BLOCK, 0, c->stream>>>(tl_cur, nt_big, nt_tiny, nt_cur, n, ts, ld, st);
     // only segments of more than one tile have tiles after their first
     const int nmulti = tiles_above(TILE);
     if (nmulti > 0) scan_add_kernel<u64><<<gtile, BLOCK, 0, c->stream>>>(tl_cur, nmulti, n, ts, ad);
@@ -767,8 +767,7 @@ struct Runner {
   // tree over the pass's sources: keys, sort, structure, moments
   void bbox() {
     if (any_big()) bbox_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big);
-    if (any_small()) bbox_wave_kernel<4><<<gsmall, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big, nt_tiny);
-    if (any_tiny()) bbox_wave_kernel<1><<<gtiny, BLOCK, 0, c->stream>>>(W, tl_cur, nt_tiny, nt_cur);
+    if (any_wave()) bbox_wave_kernel<<<gwave, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big, nt_tiny, nt_cur);
   }
 
   int build_tree(bool copy_sp) {
@@ -852,12 +851,9 @@ struct Runner {
       frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     } else {
       if (any_big()) frames_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, c->tpart.as<double>());
-      if (any_small())
-        frames_partial_wave_kernel<4><<<gsmall, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, nt_tiny,
-                                                                       c->tpart.as<double>());
-      if (any_tiny())
-        frames_partial_wave_kernel<1><<<gtiny, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_tiny, nt_cur,
-                                                                      c->tpart.as<double>());
+      if (any_wave())
+        frames_partial_wave_kernel<<<gwave, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, nt_tiny, nt_cur,
+                                                                   c->tpart.as<double>());
       // the run sums go after the tile partials in the same buffer
       double *gpart = c->tpart.as<double>() + (int64_t)(nt_cur + 1) * FRAME_SUMS;
       if (max_n > SUPER)
@@ -881,12 +877,9 @@ struct Runner {
       output_record_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P, 0);
     } else {
       if (any_big()) output_partial_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, c->tpart.as<double>());
-      if (any_small())
-        output_partial_wave_kernel<4><<<gsmall, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, nt_tiny,
-                                                                       c->tpart.as<double>());
-      if (any_tiny())
-        output_partial_wave_kernel<1><<<gtiny, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_tiny, nt_cur,
-                                                                      c->tpart.as<double>());
+      if (any_wave())
+        output_partial_wave_kernel<<<gwave, BLOCK, 0, c->stream>>>(W, P, tl_cur, nt_big, nt_tiny, nt_cur,
+                                                                   c->tpart.as<double>());
       output_combine_kernel<<<gwseg, BLOCK, 0, c->stream>>>(W, P, ft_cur, c->tpart.as<double>());
     }
   }
