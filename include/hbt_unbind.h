@@ -381,10 +381,15 @@ int hbt_merge_subhalos(hbt_ctx *ctx, const hbt_particles *snapshot, const int64_
                        int64_t out_capacity, int64_t *out_total);
 
 /* ID -> index resolution: builds a device hash table of the n_table IDs
- * (index = position in `table_ids`), then looks up n_query IDs.  Unknown IDs
- * and negative IDs give -1 (src/hash.tpp:52).  Device pointers.  The table
- * (16 bytes per slot, the next power of two at or above 2 * n_table slots)
- * stays with the context for later calls. */
+ * (index = position in `table_ids`; an ID given twice keeps its smallest
+ * position), then looks up n_query IDs.  Unknown IDs and negative IDs give -1
+ * (src/hash.tpp:52).  Device pointers.  The table is 2^pb partitions of 4,096
+ * 16-byte slots, pb chosen so that a partition holds at most 2,560 IDs on
+ * average (for example 2.1 GB for 70.7 M IDs), plus 24 bytes per ID of
+ * scatter buffers; when one partition would hold more than 3,584 IDs (many
+ * repeats of one ID) a single table of the next power of two at or above
+ * 2 * n_table slots is used instead.  Both stay with the context for later
+ * calls. */
 int hbt_id_to_index(hbt_ctx *ctx, int64_t n_table, const int64_t *table_ids,
                     int64_t n_query, const int64_t *query_ids, int64_t *out_index);
 
