@@ -167,6 +167,37 @@ def test_unbind_c2_sample_fast_and_replay(gpu):
     print("C2 sample replay:", assert_tolerant(ref, rep))
 
 
+def test_unbind_at_tile_form_boundaries(gpu):
+    """Lists of exactly 1, 2, 3, 63-65, 255-257, 511-513, 1023-1025 and 2049
+    entries: the one-wave tile kernels take the segments of at most 64 slots
+    (one entry per lane) and of at most 256 (four per lane), the workgroup
+    kernels the rest, and a segment above 1024 needs merge levels.  FAST and
+    REPLAY against the oracle, and the batched form against the plain one
+    (which runs each subhalo alone, so every kernel form meets every size)."""
+    base = synth.config2(seed=1021, nsub=3000)
+    sizes = [1, 2, 3, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025, 2049]
+    rng = np.random.default_rng(23)
+    picks, counts = [], []
+    for n in sizes:
+        cand = np.flatnonzero(base.subs["count"] >= n)
+        for s in rng.choice(cand, size=min(3, len(cand)), replace=False):
+            picks.append(int(s))
+            counts.append(n)
+    subs = base.subs[np.array(picks)].copy()
+    subs["count"] = counts  # the first n entries of each list (lists are in random order)
+    subs["nbound_prev"] = counts
+    cat = synth.Catalogue(base.cols, subs, base.params.copy(), "tile-form boundaries")
+    ref = oracle.unbind_batch(cat, nthreads=8)
+    fast = gpu(_with(cat.params, precision=abi.HBT_PREC_FAST))
+    got = fast.unbind_batch(cat)
+    print("boundaries, fast:", assert_tolerant(ref, got))
+    plain = fast.unbind_batch(cat, plain=True)
+    for x, y in zip(got, plain):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    rep = gpu(_with(cat.params, precision=abi.HBT_PREC_REPLAY)).unbind_batch(cat)
+    print("boundaries, replay:", assert_tolerant(ref, rep))
+
+
 def test_unbind_at_box_edges(gpu):
     """400 subhalos centred on the faces, edges and corners of the periodic
     box (380 wrap around it): FAST meets the tolerant criteria and the
