@@ -594,6 +594,7 @@ struct Runner {
   DevParams P;
   Level *L;
   int gs, gseg, gtile, gblk;  // slot, segment-thread, tile and segment-block grids
+  bool swap_sp = true;         // swap the spatial list with its compacted copy (direct launches)
   int nt_big = 0;              // tiles of segments of more than SMALL_CAP slots (a prefix)
   int nt_tiny = 0;             // then those of more than TINY_CAP; the rest are of at most TINY_CAP
   int gsmall = 1, gtiny = 1, gwave = 1;  // one-wave tile grids (four tiles per workgroup)
@@ -845,7 +846,12 @@ struct Runner {
     RETCHK(seg_scan(BoundLoad{W.ibase, W.sp, W.isb}, PreStore{W.ibase, W.pre},
                     PreAdd{W.ibase, W.pre}, count_row(NC_BOUND)));
     spatial_compact_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    spatial_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
+    // the compacted list (in el2) becomes the spatial list and the old one
+    // the next pass's scratch; only the active segments read either, and
+    // they are rewritten every pass.  A captured pass sequence replays fixed
+    // pointers, so it copies the list back instead.
+    if (swap_sp) std::swap(W.sp, W.el2);
+    else spatial_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     // frames, convergence (the frame sums read the bound block only, in Elist order)
     if (P.replay) {
       frames_serial_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
@@ -1092,6 +1098,7 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
       if (gx) (void)hipGraphExecDestroy(gx);
       if (g) (void)hipGraphDestroy(g);
     };
+    R.swap_sp = false;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = HBT_OK;
     for (int p = 0; p < graph_passes && rc == HBT_OK; p++) rc = R.pass();
