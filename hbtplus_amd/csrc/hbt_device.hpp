@@ -63,6 +63,7 @@ struct Seg {
   int n_src, src_lo, n_tgt;   // current pass
   int n_items;
   int nP, nG, K, hole;        // partition
+  int ncopy;                  // Elist entries the pass's partition moved (its nlast), copied back with the bound flags
   int nsort, sort_lo;         // energy sort: Elist[sort_lo, sort_lo + nsort)
   int iter;
   long long evals;

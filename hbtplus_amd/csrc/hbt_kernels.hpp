@@ -1614,24 +1614,16 @@ __global__ void __launch_bounds__(BLOCK) partition_move_kernel(Work W) {
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) partition_copy_kernel(Work W) {
-  SLOT_LOOP(s, W) {
-    int k;
-    int l = seg_lane_local(W, s, k);
-    if (l < 0) continue;
-    const Seg &S = W.seg[k];
-    if ((S.flags & F_ACTIVE) && l < S.nlast) W.el[s] = W.el2[s];
-  }
-}
-
 // Nbound after the partition; disruption test (:357-379).
 __global__ void partition_done_kernel(Work W, DevParams P) {
   SEG_LOOP(k, W) {
     Seg &S = W.seg[k];
+    S.ncopy = (S.flags & F_ACTIVE) ? S.nlast : 0;  // bound_flags_kernel copies the new Elist back
     if (!(S.flags & F_ACTIVE)) continue;
     const int64_t b = S.ibase;
-    // the original element 0 ends in the last hole (Etmp, :37-38 and :50-51)
-    int nb = S.hole + (W.E[b + W.el[b + S.hole]] < 0.f ? 1 : 0);
+    // the original element 0 ends in the last hole (Etmp, :37-38 and :50-51);
+    // the partitioned Elist is still in el2
+    int nb = S.hole + (W.E[b + W.el2[b + S.hole]] < 0.f ? 1 : 0);
     if (S.nlast == 0) nb = 0;
     S.nbound = nb;
     if (nb < P.min_part) {
@@ -1684,15 +1676,19 @@ __global__ void __launch_bounds__(BLOCK) sample_flags_kernel(Work W) {
   }
 }
 
-// spatial list maintenance: isb[l] = particle l is still bound
+// The partitioned Elist back from el2 (for every segment partitioned in the
+// pass, including those it disrupted), and for the segments still active the
+// spatial list's flags: isb[l] = particle l is still bound.
 __global__ void __launch_bounds__(BLOCK) bound_flags_kernel(Work W) {
   SLOT_LOOP(s, W) {
     int k;
     int l = seg_lane_local(W, s, k);
     if (l < 0) continue;
     const Seg &S = W.seg[k];
-    if (!(S.flags & F_ACTIVE) || l >= S.nlast) continue;
-    W.isb[S.ibase + W.el[s]] = l < S.nbound ? 1 : 0;
+    if (l >= S.ncopy) continue;
+    const int li = W.el2[s];
+    W.el[s] = li;
+    if ((S.flags & F_ACTIVE) && l < S.nlast) W.isb[S.ibase + li] = l < S.nbound ? 1 : 0;
   }
 }
 

@@ -835,7 +835,6 @@ struct Runner {
     partition_k_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_hole_kernel<<<gseg, BLOCK, 0, c->stream>>>(W);
     partition_move_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
-    partition_copy_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     partition_done_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     if (P.max_sample > 0) {
       disrupt_find_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
