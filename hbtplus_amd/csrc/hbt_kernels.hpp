@@ -651,50 +651,94 @@ __device__ __forceinline__ int prefix_lower(const u64 *key, int lo, int hi, int 
   return lo;
 }
 
-// The node split by pair l (at depth v, the pair's delta): the sources sharing
-// the first v digits with source l, found by binary search over the sorted
-// keys.  vl, vr: deltas of the pairs l-1 and l+1 (-1 where there is none).
-// Returns false when pair l is not the node's leftmost split (only the
-// leftmost split emits the node).
-__device__ __forceinline__ bool node_of_pair(const u64 *key, int l, int n, int v, int vl, int vr,
-                                             int D, int &lo, int &hi) {
-  if (vl == v) return false;               // source l-1 is in the node, in another child
-  if (vl < v) {
-    lo = l;                                // the node starts at source l
-  } else {
-    lo = prefix_lower(key, 0, l - 1, v, key_prefix(key[l], v));
-    // leftmost split iff sources lo and l are in the same child
-    if (key_delta(key[lo], key[l], D) <= v) return false;
-  }
-  if (vr < v) hi = l + 1;
-  else hi = prefix_lower(key, l + 2, n - 1, v, key_prefix(key[l], v) + 1) - 1;
-  return true;
-}
-
 // Per pair l of sorted sources: its delta, and the node of which it is the
 // leftmost split.  The neighbouring pairs' deltas come from the keys.  In a
 // full pass the sorted order also becomes the spatial target order.
-__global__ void __launch_bounds__(BLOCK) emit_kernel(Work W, int full_copy_sp) {
-  SLOT_LOOP(s, W) {
-    int k;
-    int l = seg_lane_local(W, s, k);
-    if (l < 0) continue;
-    const Seg &S = W.seg[k];
+//
+// The keys of a tile (and one on either side) are staged in LDS, so the
+// binary searches for a node's first and last sources run there; a search
+// goes to device memory only for the part of its range outside the tile (the
+// nodes that extend past it).  KeyWin is that window: sources w_lo..w_hi.
+struct KeyWin {
+  const u64 *sk;   // LDS: keys of sources w_lo..w_hi
+  const u64 *key;  // the segment's keys in device memory
+  int w_lo, w_hi;
+  __device__ __forceinline__ u64 at(int i) const { return (i >= w_lo && i <= w_hi) ? sk[i - w_lo] : key[i]; }
+  // first index in [a, h + 1] whose v-digit prefix is >= p (keys ascending)
+  __device__ __forceinline__ int lower(int a, int h, int v, u64 p) const {
+    const int wa = a > w_lo ? a : w_lo, wh = h < w_hi ? h : w_hi;
+    if (wa > wh) return prefix_lower(key, a, h, v, p);
+    if (key_prefix(sk[wa - w_lo], v) >= p) return wa == a ? a : prefix_lower(key, a, wa - 1, v, p);
+    if (key_prefix(sk[wh - w_lo], v) < p) return wh == h ? h + 1 : prefix_lower(key, wh + 1, h, v, p);
+    int lo = wa + 1, hi = wh - 1;  // the answer is in [wa + 1, wh]
+    while (lo <= hi) {
+      const int m = (lo + hi) >> 1;
+      if (key_prefix(sk[m - w_lo], v) < p) lo = m + 1;
+      else hi = m - 1;
+    }
+    return lo;
+  }
+};
+
+// One source l (slot s) of segment S: the work of emit for it.
+__device__ __forceinline__ void emit_one(const Work &W, const Seg &S, const KeyWin &K, int l, int full_copy_sp) {
+  const int n = S.n_src;
+  const int64_t s = S.ibase + l;
+  if (full_copy_sp && !(S.flags & (F_CORR | F_SAMP | F_REFINE))) W.sp[s] = (int)W.v0[s];
+  W.nlo[s] = -1;
+  const int v = (l + 1 < n) ? key_delta(K.at(l), K.at(l + 1), S.D) : -1;
+  W.dl[s] = (signed char)v;
+  if (v < 0) return;
+  const int vl = l > 0 ? key_delta(K.at(l - 1), K.at(l), S.D) : -1;
+  if (vl == v) return;  // source l-1 is in the node, in another child
+  const int vr = l + 2 < n ? key_delta(K.at(l + 1), K.at(l + 2), S.D) : -1;
+  const u64 p = key_prefix(K.at(l), v);
+  int lo = l, hi = l + 1;
+  if (vl > v) {
+    lo = K.lower(0, l - 1, v, p);
+    // leftmost split iff sources lo and l are in the same child
+    if (key_delta(K.at(lo), K.at(l), S.D) <= v) return;
+  }
+  if (vr >= v) hi = K.lower(l + 2, n - 1, v, p + 1) - 1;
+  W.nlo[s] = lo;
+  W.nhi[s] = hi;
+  atomicOr(&W.bits[S.ibase + lo], 1u << v);
+}
+
+// Workgroup form: one tile of TILE sources, window t0 - 1 .. t0 + TILE + 1.
+__global__ void __launch_bounds__(BLOCK) emit_kernel(Work W, const Tile *__restrict__ tiles, int ntiles,
+                                                     int full_copy_sp) {
+  __shared__ u64 sk[TILE + 3];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
     const int n = S.n_src;
-    if (l >= n) continue;
-    if (full_copy_sp && !(S.flags & (F_CORR | F_SAMP | F_REFINE))) W.sp[s] = (int)W.v0[s];
-    W.nlo[s] = -1;
-    const u64 *key = W.k0 + S.ibase;
-    const int v = (l + 1 < n) ? key_delta(key[l], key[l + 1], S.D) : -1;
-    W.dl[s] = (signed char)v;
-    if (v < 0) continue;
-    const int vl = l > 0 ? key_delta(key[l - 1], key[l], S.D) : -1;
-    const int vr = l + 2 < n ? key_delta(key[l + 1], key[l + 2], S.D) : -1;
-    int lo, hi;
-    if (!node_of_pair(key, l, n, v, vl, vr, S.D, lo, hi)) continue;
-    W.nlo[s] = lo;
-    W.nhi[s] = hi;
-    atomicOr(&W.bits[S.ibase + lo], 1u << v);
+    const int cnt = min(TILE, n - tl.start);
+    if (cnt <= 0) continue;  // uniform across the block
+    KeyWin K{sk, W.k0 + S.ibase, tl.start > 0 ? tl.start - 1 : 0, min(n - 1, tl.start + TILE + 1)};
+    for (int i = K.w_lo + (int)threadIdx.x; i <= K.w_hi; i += BLOCK) sk[i - K.w_lo] = K.key[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) emit_one(W, S, K, tl.start + i, full_copy_sp);
+    __syncthreads();  // the next tile restages sk
+  }
+}
+
+// One-wave form: a segment of at most SMALL_CAP sources, all of its keys in
+// the wave's LDS.
+__global__ void __launch_bounds__(BLOCK) emit_wave_kernel(Work W, const Tile *__restrict__ tiles, int t0,
+                                                          int ntiles, int full_copy_sp) {
+  __shared__ u64 sk[BLOCK / WAVE][SMALL_CAP];
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  WAVE_TILE_LOOP(t, t0, ntiles) {
+    const Tile tl = tiles[t];
+    const Seg &S = W.seg[tl.seg];
+    const int n = min(SMALL_CAP, S.n_src);
+    if (n <= 0) continue;  // uniform across the wave
+    KeyWin K{sk[wv], W.k0 + S.ibase, 0, n - 1};
+    for (int i = lane; i < n; i += WAVE) sk[wv][i] = K.key[i];
+    wave_sync();
+    for (int i = lane; i < n; i += WAVE) emit_one(W, S, K, i, full_copy_sp);
+    wave_sync();  // the next tile restages the wave's keys
   }
 }
 

@@ -776,7 +776,8 @@ struct Runner {
     tree_params_kernel<<<gseg, BLOCK, 0, c->stream>>>(W, P);
     keys_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
     RETCHK(sort_pairs(count_row(NC_SRC)));
-    emit_kernel<<<gs, BLOCK, 0, c->stream>>>(W, copy_sp ? 1 : 0);
+    if (any_big()) emit_kernel<<<gtile, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big, copy_sp ? 1 : 0);
+    if (any_wave()) emit_wave_kernel<<<gwave, BLOCK, 0, c->stream>>>(W, tl_cur, nt_big, nt_cur, copy_sp ? 1 : 0);
     RETCHK(seg_scan(PopLoad{W.ibase, W.bits}, PreStore{W.ibase, W.pre}, PreAdd{W.ibase, W.pre},
                     count_row(NC_SRC)));
     items_kernel<<<gs, BLOCK, 0, c->stream>>>(W);
