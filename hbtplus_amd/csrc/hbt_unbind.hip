@@ -1553,7 +1553,7 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
   if (n_list >= (int64_t)INT32_MAX) return fail(HBT_ERR_UNSUPPORTED, "more than 2^31 list entries");
   HIPCHK(hipSetDevice(ctx->device));
   if (nsub > 0 && !subs_host) return fail(HBT_ERR_INVALID_ARG, "null argument");
-  // the snapshot's packed records and the list positions are this call's only
+  // the packed snapshot and the list positions belong to this call: cleared on every return
   struct Reset {
     hbt_ctx *c;
     ~Reset() { c->packed = nullptr; c->list_index = nullptr; }
