@@ -90,15 +90,23 @@ __global__ void check_index_kernel(const int64_t *idx, int64_t n_list, int64_t n
   if (cnt) atomicAdd(bad, cnt);
 }
 
+// The slots a wave visits form one wave-aligned chunk, and segments start at
+// multiples of WAVE, so a wave's slots share one segment: the lookups read
+// one entry for the wave (scalar loads of a wave-uniform address).
 __device__ __forceinline__ int seg_lane_local(const Work &W, int64_t s, int &k) {
-  k = W.seg_of_slot[s];
+  const int64_t s0 = (int64_t)__builtin_amdgcn_readfirstlane((int)(s >> 6)) << 6;
+  k = W.seg_of_slot[s0];
   if (k < 0) return -1;
   return (int)(s - W.ibase[k]);
 }
 
-// i-th visited slot (chunks are wave-sized and aligned, so a wave stays in one segment)
+// i-th visited slot (the active lists hold wave-sized chunks, nact a multiple of WAVE)
 __device__ __forceinline__ int64_t slot_at(const Work &W, int64_t i) {
-  return (W.act_chunk && i < W.nact) ? (int64_t)W.act_chunk[i >> 6] * WAVE + (i & (WAVE - 1)) : i;
+  if (W.act_chunk && i < W.nact) {
+    const int c = __builtin_amdgcn_readfirstlane((int)(i >> 6));
+    return (int64_t)W.act_chunk[c] * WAVE + (i & (WAVE - 1));
+  }
+  return i;
 }
 
 #define SLOT_LOOP(s, W)                                                                        \
