@@ -236,70 +236,84 @@ struct Nest {
   const int *child(int s) const { return ch_idx.data() + ch_ptr[s]; }
 };
 
-// Members of a level sorted by capacity, largest first, ties in member order
-// (a stable LSD radix sort on 31-bit keys in three 11-bit digits, the keys
-// gathered once, all three digit counts taken in one sweep and a digit that
-// is the same for every key skipped); caps receives the sorted capacities.
-void sort_largest_first(const Nest &N, const std::vector<int> &members, std::vector<int> &out,
-                        std::vector<int> &caps) {
-  const size_t n = members.size();
-  out = members;
-  caps.resize(n);
-  if (n < 64) {
-    std::stable_sort(out.begin(), out.end(), [&](int a, int b) { return N.f[a].cap > N.f[b].cap; });
-    for (size_t i = 0; i < n; i++) caps[i] = (int)N.f[out[i]].cap;
-    return;
-  }
-  constexpr int DB = 11, NB = 1 << DB;
-  static thread_local std::vector<int> tmp;
-  static thread_local std::vector<uint32_t> key, key2, cnt;
-  tmp.resize(n);
-  key.resize(n);
-  key2.resize(n);
-  cnt.assign(3 * NB, 0u);
-  for (size_t i = 0; i < n; i++) {
-    const uint32_t k = (uint32_t)(0x7fffffff - (int32_t)N.f[out[i]].cap);
-    key[i] = k;
-    cnt[k & (NB - 1)]++;
-    cnt[NB + ((k >> DB) & (NB - 1))]++;
-    cnt[2 * NB + (k >> (2 * DB))]++;
-  }
-  int *src = out.data(), *dst = tmp.data();
-  uint32_t *ks = key.data(), *kd = key2.data();
-  for (int d = 0; d < 3; d++) {
-    uint32_t *c = cnt.data() + d * NB;
-    const int shift = d * DB;
-    if (c[(ks[0] >> shift) & (NB - 1)] == n) continue;  // one value of this digit: order unchanged
-    uint32_t o = 0;
-    for (int b = 0; b < NB; b++) {
-      const uint32_t x = c[b];
-      c[b] = o;
-      o += x;
-    }
-    for (size_t i = 0; i < n; i++) {
-      const uint32_t p = c[(ks[i] >> shift) & (NB - 1)]++;
-      dst[p] = src[i];
-      kd[p] = ks[i];
-    }
-    std::swap(src, dst);
-    std::swap(ks, kd);
-  }
-  if (src != out.data()) memcpy(out.data(), src, n * sizeof(int));
-  for (size_t i = 0; i < n; i++) caps[i] = (int)(0x7fffffff - ks[i]);
+// f(r) for r in [0, parts), each on a thread of its own (r = 0 on the caller's).
+// A thread_local the caller fills must reach f by pointer: named inside f, it
+// is the worker thread's own instance.
+template <class F>
+static void parallel_parts(int parts, const F &f) {
+  std::vector<std::thread> th;
+  for (int r = 1; r < parts; r++) th.emplace_back(f, r);
+  f(0);
+  for (auto &x : th) x.join();
 }
 
 // f(lo, hi) over [0, n), on four threads for large n.
 template <class F>
 static void parallel_ranges(int64_t n, const F &f) {
   const int nth = n >= 65536 ? 4 : 1;
-  if (nth == 1) {
-    f((int64_t)0, n);
-    return;
+  parallel_parts(nth, [&](int r) { f(n * r / nth, n * (r + 1) / nth); });
+}
+
+// Members of a level sorted by capacity, largest first, ties in member order;
+// caps receives the sorted capacities.  Most lists are short, so one stable
+// counting pass on the capacity clamped to CLAMP - 1 orders all but the
+// members at or above the clamp, which it gathers in front (in member order)
+// and a stable comparison sort then orders; that bucket holds few members.
+// Large levels count and place on four threads, each over its own quarter of
+// the members, a quarter's members of a bucket following the earlier
+// quarters' members of that bucket.
+void sort_largest_first(const Nest &N, const std::vector<int> &members, std::vector<int> &out,
+                        std::vector<int> &caps) {
+  const size_t n = members.size();
+  out.resize(n);
+  caps.resize(n);
+  constexpr int CLAMP = 4096, NT = 4;
+  static thread_local std::vector<int> key;
+  static thread_local std::vector<uint32_t> cnt;
+  key.resize(n);
+  cnt.assign((size_t)NT * CLAMP, 0u);
+  const int nth = n >= 65536 ? NT : 1;
+  int *kp = key.data(), *op = out.data(), *capp = caps.data();
+  uint32_t *cp = cnt.data();  // one histogram per quarter
+  const int *mp = members.data();
+  const Nest *np = &N;
+  parallel_parts(nth, [=](int r) {
+    uint32_t *h = cp + (size_t)r * CLAMP;
+    for (size_t i = n * r / nth; i < n * (r + 1) / nth; i++) {
+      const int c = (int)np->f[mp[i]].cap;
+      kp[i] = c;
+      h[CLAMP - 1 - std::min(c, CLAMP - 1)]++;  // bucket 0: the largest
+    }
+  });
+  uint32_t o = 0, nbig = 0;
+  for (int b = 0; b < CLAMP; b++) {
+    for (int r = 0; r < nth; r++) {
+      const uint32_t x = cp[(size_t)r * CLAMP + b];
+      cp[(size_t)r * CLAMP + b] = o;
+      o += x;
+    }
+    if (b == 0) nbig = o;
   }
-  std::vector<std::thread> th;
-  for (int t = 1; t < nth; t++) th.emplace_back(f, n * t / nth, n * (t + 1) / nth);
-  f((int64_t)0, n / nth);
-  for (auto &x : th) x.join();
+  parallel_parts(nth, [=](int r) {
+    uint32_t *h = cp + (size_t)r * CLAMP;
+    for (size_t i = n * r / nth; i < n * (r + 1) / nth; i++) {
+      const int c = kp[i];
+      const uint32_t p = h[CLAMP - 1 - std::min(c, CLAMP - 1)]++;
+      op[p] = mp[i];
+      capp[p] = c;
+    }
+  });
+  if (nbig > 1) {
+    std::vector<std::pair<int, int>> big(nbig);  // (capacity, member), in member order
+    for (uint32_t i = 0; i < nbig; i++) big[i] = {caps[i], out[i]};
+    std::stable_sort(big.begin(), big.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+      return a.first > b.first;
+    });
+    for (uint32_t i = 0; i < nbig; i++) {
+      caps[i] = big[i].first;
+      out[i] = big[i].second;
+    }
+  }
 }
 
 // draw_off: per subhalo, the first of its shuffle's draws in the level's
@@ -363,6 +377,29 @@ struct Blob {
   size_t bytes;
 };
 
+// Copies blob i to h + at[i]: the blobs' bytes taken end to end and split
+// into four equal ranges for large levels.
+void pack_blobs(char *h, const std::vector<size_t> &at, std::initializer_list<Blob> blobs) {
+  std::vector<const Blob *> bl;
+  std::vector<size_t> end;  // running total of the blobs' bytes
+  size_t total = 0;
+  for (const Blob &b : blobs) {
+    bl.push_back(&b);
+    total += b.bytes;
+    end.push_back(total);
+  }
+  const int64_t nq = (int64_t)(total >> 4);  // 16-byte units
+  parallel_ranges(nq, [&](int64_t q0, int64_t q1) {
+    const size_t lo = (size_t)q0 << 4, hi = q1 == nq ? total : (size_t)q1 << 4;
+    size_t j = std::upper_bound(end.begin(), end.end(), lo) - end.begin(), p = lo;
+    for (; p < hi && j < bl.size(); j++) {
+      const size_t b0 = end[j] - bl[j]->bytes, q = std::min(hi, end[j]);
+      if (q > p) memcpy(h + at[j] + (p - b0), (const char *)bl[j]->p + (p - b0), q - p);
+      p = q;
+    }
+  });
+}
+
 int stage_blobs(hbt_ctx *c, std::initializer_list<Blob> blobs, void **dev) {
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<size_t> at;
@@ -380,12 +417,8 @@ int stage_blobs(hbt_ctx *c, std::initializer_list<Blob> blobs, void **dev) {
   }
   RETCHK(c->lvlbuf.ensure(off));
   char *h = (char *)c->h_stage, *d = (char *)c->lvlbuf.p;
-  size_t i = 0;
-  for (const Blob &b : blobs) {
-    if (b.bytes) memcpy(h + at[i], b.p, b.bytes);
-    dev[i] = d + at[i];
-    i++;
-  }
+  for (size_t i = 0; i < blobs.size(); i++) dev[i] = d + at[i];
+  pack_blobs(h, at, blobs);
   HIPCHK(hipMemcpyAsync(c->lvlbuf.p, c->h_stage, off, hipMemcpyHostToDevice, c->stream));
   return HBT_OK;
 }
@@ -656,14 +689,39 @@ struct Runner {
     int *act = (int *)c->h_list.p;
     HIPCHK(hipMemcpyAsync(act, W.segact, nseg * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    int64_t nch = 0, ntl = 0;
-    nlisted = 0;
-    for (int k = 0; k < nseg; k++) {
-      if (!act[k]) continue;
-      nlisted++;
-      ntl += L->ntile[k];
-      nch += L->capal[k] / WAVE;
+    // listed segments, chunks and tiles per quarter of the segments (on four
+    // threads for large levels), then each quarter's lists at its offsets
+    const int nth = nseg >= 65536 ? 4 : 1;
+    int64_t qn[4] = {}, qch[4] = {}, qtl[4] = {};
+    const Level *lv = L;
+    auto quarter = [nseg, nth](int r, int &k0, int &k1) {
+      k0 = (int)((int64_t)nseg * r / nth);
+      k1 = (int)((int64_t)nseg * (r + 1) / nth);
+    };
+    parallel_parts(nth, [&, act, lv](int r) {
+      int k0, k1;
+      quarter(r, k0, k1);
+      int64_t a = 0, t = 0, ch = 0;
+      for (int k = k0; k < k1; k++) {
+        if (!act[k]) continue;
+        a++;
+        t += lv->ntile[k];
+        ch += lv->capal[k] / WAVE;
+      }
+      qn[r] = a;
+      qtl[r] = t;
+      qch[r] = ch;
+    });
+    int64_t nch = 0, ntl = 0, nl = 0, on[4], och[4], otl[4];
+    for (int r = 0; r < nth; r++) {
+      on[r] = nl;
+      och[r] = nch;
+      otl[r] = ntl;
+      nl += qn[r];
+      nch += qch[r];
+      ntl += qtl[r];
     }
+    nlisted = (int)nl;
     const size_t b_ch = ((size_t)nch * 4 + 255) & ~(size_t)255, b_tl = ((size_t)ntl * sizeof(Tile) + 255) & ~(size_t)255;
     const size_t b_ft = (size_t)nseg * 4;
     // the flags stay at the front while the lists are written behind them
@@ -673,22 +731,28 @@ struct Runner {
     int *chunks = (int *)h;
     Tile *tl = (Tile *)(h + b_ch);
     int *ft = (int *)(h + b_ch + b_tl);
-    cur_cap.clear();
-    cur_first.clear();
-    int64_t qc = 0, qt = 0;
-    for (int k = 0; k < nseg; k++) {
-      ft[k] = (int)qt;
-      if (!act[k]) {
-        ft[k] = 0;
-        continue;
+    cur_cap.resize(nl);
+    cur_first.resize(nl);
+    int *ccap = cur_cap.data(), *cfirst = cur_first.data();
+    parallel_parts(nth, [&, act, lv, chunks, tl, ft, ccap, cfirst](int r) {
+      int k0, k1;
+      quarter(r, k0, k1);
+      int64_t qc = och[r], qt = otl[r], ql = on[r];
+      for (int k = k0; k < k1; k++) {
+        if (!act[k]) {
+          ft[k] = 0;
+          continue;
+        }
+        ft[k] = (int)qt;
+        ccap[ql] = lv->seg[k].cap;
+        cfirst[ql] = (int)qt;
+        ql++;
+        memcpy(tl + qt, lv->tiles.data() + lv->first_tile[k], (size_t)lv->ntile[k] * sizeof(Tile));
+        qt += lv->ntile[k];
+        const int64_t c0 = lv->ibase[k] / WAVE, c1 = (lv->ibase[k] + lv->capal[k]) / WAVE;
+        for (int64_t q = c0; q < c1; q++) chunks[qc++] = (int)q;
       }
-      cur_cap.push_back(L->seg[k].cap);
-      cur_first.push_back((int)qt);
-      memcpy(tl + qt, L->tiles.data() + L->first_tile[k], (size_t)L->ntile[k] * sizeof(Tile));
-      qt += L->ntile[k];
-      const int64_t c0 = L->ibase[k] / WAVE, c1 = (L->ibase[k] + L->capal[k]) / WAVE;
-      for (int64_t q = c0; q < c1; q++) chunks[qc++] = (int)q;
-    }
+    });
     RETCHK(c->actbuf.ensure(b_ch + b_tl + b_ft + 256));
     char *d = (char *)c->actbuf.p;
     // ordered on the context's stream, which the next pass's launches follow;
