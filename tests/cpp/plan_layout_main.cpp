@@ -1,0 +1,192 @@
+// Host-side checks of the unbinding planner, run without a GPU (no HIP calls):
+//  - the capacity order of a level (largest first, ties in member order)
+//    against std::stable_sort, across the counting pass's clamp and its
+//    four-thread split;
+//  - the level layout build_level writes (wave-aligned slot bases, tile table,
+//    segment records, nest children) against its definition, on planned nests;
+//  - the staging copy of a level's tables (pack_blobs) against plain memcpy,
+//    padding untouched, across its four-range split.
+// Prints "ok" and exits 0, or names the first failed check and exits 1.
+#include <cstdio>
+#include <random>
+
+#include "../../hbtplus_amd/csrc/hbt_unbind.hip"
+
+#define CHECK(cond, ...)                 \
+  do {                                   \
+    if (!(cond)) {                       \
+      fprintf(stderr, "FAIL: ");         \
+      fprintf(stderr, __VA_ARGS__);      \
+      fprintf(stderr, "\n");             \
+      return 1;                          \
+    }                                    \
+  } while (0)
+
+static std::vector<int> stable_order(const Nest &N, const std::vector<int> &members) {
+  std::vector<int> r = members;
+  std::stable_sort(r.begin(), r.end(), [&](int a, int b) { return N.f[a].cap > N.f[b].cap; });
+  return r;
+}
+
+// capacities drawn from a mix: short lists, the counting pass's clamp and its
+// neighbours, long lists, or one value for all
+static int64_t draw_cap(std::mt19937 &g, int mode) {
+  switch (mode) {
+    case 0: return g() % 50;
+    case 1: return 4093 + g() % 6;
+    case 2: return g() % 4 ? (int64_t)(g() % 300) : (int64_t)(g() % 10000000);
+    default: return 777;
+  }
+}
+
+static int check_sort() {
+  std::mt19937 g(11);
+  const int sizes[] = {0, 1, 2, 5, 63, 64, 3000, 65535, 65536, 70001, 150000};
+  std::vector<int> out, caps;
+  for (int n : sizes)
+    for (int mode = 0; mode < 4; mode++) {
+      Nest M;
+      M.f.resize(n + 10);
+      for (auto &f : M.f) f.cap = draw_cap(g, mode);
+      // members: a shuffled subset (a nested level), or all in order (the top level)
+      std::vector<int> mem;
+      if (mode & 1) {
+        for (int i = 0; i < n; i++) mem.push_back(i);
+      } else {
+        for (int i = 0; i < n + 10 && (int)mem.size() < n; i++)
+          if (g() % 8) mem.push_back(i);
+        std::shuffle(mem.begin(), mem.end(), g);
+      }
+      sort_largest_first(M, mem, out, caps);
+      CHECK(out == stable_order(M, mem), "capacity order, n=%d mode=%d", n, mode);
+      CHECK(caps.size() == out.size(), "caps size, n=%d", n);
+      for (size_t i = 0; i < out.size(); i++)
+        CHECK(caps[i] == (int)M.f[out[i]].cap, "caps[%zu], n=%d mode=%d", i, n, mode);
+    }
+  return 0;
+}
+
+// random forests of nests (children listed before and after their hosts)
+static std::vector<hbt_subhalo> forest(std::mt19937 &g, int n, int64_t &total) {
+  std::vector<hbt_subhalo> s(n);
+  std::memset(s.data(), 0, n * sizeof(hbt_subhalo));
+  total = 0;
+  for (int i = 0; i < n; i++) {
+    s[i].count = g() % 16 == 0 ? 2000 + g() % 30000 : g() % 400;
+    s[i].offset = total;
+    s[i].nbound_prev = s[i].count / 2;
+    s[i].parent = -1;
+    total += s[i].count;
+  }
+  // a third attach under a random subhalo that is not their own descendant
+  // and sits less than five deep
+  for (int i = 0; i < n; i++) {
+    if (g() % 3) continue;
+    const int p = (int)(g() % n);
+    int q = p, d = 0;
+    while (q >= 0 && q != i) q = (int)s[q].parent, d++;
+    if (q != i && d < 5) s[i].parent = p;
+  }
+  int64_t o = 0;
+  for (int i = 0; i < n; i++) {
+    s[i].out_offset = o;
+    o += s[i].count + 7;
+  }
+  return s;
+}
+
+static int check_layout() {
+  std::mt19937 g(12);
+  const hbt_tuning tune = {{8, 16, 32}, {200, 3000}, 2, 0, 16};
+  for (int n : {1, 7, 900, 70000}) {
+    int64_t total = 0;
+    std::vector<hbt_subhalo> s = forest(g, n, total);
+    static Nest N;
+    CHECK(plan_nests(s.data(), n, N) == HBT_OK, "plan_nests, n=%d: %s", n, hbt_last_error());
+    // a level: every subhalo (children and all), and the heads alone
+    for (int which = 0; which < 2; which++) {
+      std::vector<int> members;
+      for (int i = 0; i < n; i++)
+        if (which == 0 || s[i].parent < 0) members.push_back(i);
+      static Level L;
+      build_level(N, members, tune, nullptr, n, L);
+      const int m = (int)members.size();
+      CHECK(L.subs == stable_order(N, members), "level order, n=%d", n);
+      int64_t off = 0, t0 = 0;
+      int maxcap = 0;
+      for (int k = 0; k < m; k++) {
+        const int sub = L.subs[k];
+        const SubPlan &F = N.f[sub];
+        const int c = (int)F.cap;
+        const int ca = std::max(WAVE, (c + WAVE - 1) / WAVE * WAVE);
+        const int nt = (c + TILE - 1) / TILE;
+        CHECK(L.ibase[k] == off && off % WAVE == 0, "ibase[%d], n=%d", k, n);
+        CHECK(L.capal[k] == ca && L.first_tile[k] == t0 && L.ntile[k] == nt, "tiles of %d, n=%d", k, n);
+        for (int t = 0; t < nt; t++)
+          CHECK(L.tiles[t0 + t].seg == k && L.tiles[t0 + t].start == t * TILE, "tile %d of %d", t, k);
+        const SegIn &S = L.seg[k];
+        CHECK(S.ibase == off && S.out_off == F.out_off && S.list_off == F.offset && S.nbprev == F.nbp &&
+                  S.sub == sub && S.cap == c && S.count == (int)F.count && S.wgroup == walk_group_for(tune, c) &&
+                  S.draw_off == -1 && S.shuf_len == 0,
+              "segment record %d, n=%d", k, n);
+        const int nc = N.nchild(sub);
+        CHECK(L.child_ptr[k + 1] - L.child_ptr[k] == nc, "children of %d", k);
+        for (int i = 0; i < nc; i++) CHECK(L.child_list[L.child_ptr[k] + i] == N.child(sub)[i], "child %d of %d", i, k);
+        off += ca;
+        t0 += nt;
+        maxcap = std::max(maxcap, c);
+      }
+      CHECK(L.nslots == off && (int64_t)L.tiles.size() == t0 && L.maxcap == maxcap, "level totals, n=%d", n);
+      // a subhalo's capacity is its own list plus its nested subhalos' capacities
+      for (int i = 0; i < n; i++) {
+        int64_t c = s[i].count;
+        for (int j = 0; j < N.nchild(i); j++) c += N.f[N.child(i)[j]].cap;
+        CHECK(N.f[i].cap == c, "capacity of %d, n=%d", i, n);
+      }
+    }
+  }
+  return 0;
+}
+
+static int check_staging() {
+  std::mt19937 g(13);
+  const size_t pick[] = {0, 1, 7, 15, 16, 17, 255, 256, 1000, 65536 * 4 + 3, 1048573, 3000001};
+  for (int rep = 0; rep < 40; rep++) {
+    const int nb = 1 + g() % 9;
+    std::vector<std::vector<unsigned char>> src(nb);
+    for (auto &v : src) {
+      v.resize(pick[g() % (sizeof pick / sizeof pick[0])]);
+      for (auto &x : v) x = (unsigned char)g();
+    }
+    std::vector<size_t> at;
+    size_t off = 0;
+    for (auto &v : src) {
+      at.push_back(off);
+      off += (std::max<size_t>(v.size(), 1) + 255) & ~(size_t)255;
+    }
+    std::vector<unsigned char> h(off + 256, 0xab);
+    const auto B = [&](int i) { return Blob{src[i].empty() ? nullptr : src[i].data(), src[i].size()}; };
+    // the entry points pass fixed lists of up to nine blobs
+    const Blob z{nullptr, 0};
+    std::vector<Blob> b(9, z);
+    for (int i = 0; i < nb; i++) b[i] = B(i);
+    std::vector<size_t> at9 = at;
+    for (int i = nb; i < 9; i++) at9.push_back(off);
+    pack_blobs((char *)h.data(), at9, {b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8]});
+    size_t p = 0;
+    for (int i = 0; i < nb; i++) {
+      for (; p < at[i]; p++) CHECK(h[p] == 0xab, "padding byte %zu overwritten (rep %d)", p, rep);
+      CHECK(src[i].empty() || std::memcmp(h.data() + at[i], src[i].data(), src[i].size()) == 0,
+            "blob %d of %zu bytes (rep %d)", i, src[i].size(), rep);
+      p = at[i] + src[i].size();
+    }
+    for (; p < h.size(); p++) CHECK(h[p] == 0xab, "trailing byte %zu overwritten (rep %d)", p, rep);
+  }
+  return 0;
+}
+
+int main() {
+  if (check_sort() || check_layout() || check_staging()) return 1;
+  printf("ok\n");
+  return 0;
+}
