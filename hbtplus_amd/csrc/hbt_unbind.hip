@@ -1263,7 +1263,9 @@ int run_level(hbt_ctx *c, Level &L, const hbt_particles *parts, hbt_subhalo *sub
 
 // Gathers the records' planning fields, checks them and builds the nests:
 // capacities, heights, children (CSR) and the post-order; detects cycles.
-int plan_nests(const hbt_subhalo *subs, int64_t nsub, Nest &N) {
+// n_list >= 0: each record's list must also lie inside [0, n_list), checked
+// during the same gather.
+int plan_nests(const hbt_subhalo *subs, int64_t nsub, Nest &N, int64_t n_list = -1) {
   N.n = nsub;
   N.f.resize(nsub);
   N.height.assign(nsub, 0);
@@ -1273,14 +1275,16 @@ int plan_nests(const hbt_subhalo *subs, int64_t nsub, Nest &N) {
   // the records are read once, by a few threads for large batches (the
   // fields sit in two cache lines of each 240-byte record)
   const int nth = nsub >= 65536 ? 4 : 1;
-  std::atomic<int> bad{0};
+  std::atomic<int> bad{0}, outside{0};
   // (the worker threads get plain pointers: `parent` is thread-local)
   SubPlan *fp = N.f.data();
   int64_t *pp = parent.data();
-  auto gather = [subs, nsub, fp, pp, &bad](int64_t lo, int64_t hi) {
+  auto gather = [subs, nsub, n_list, fp, pp, &bad, &outside](int64_t lo, int64_t hi) {
     for (int64_t s = lo; s < hi; s++) {
       const hbt_subhalo &r = subs[s];
       if (r.count < 0 || r.parent >= nsub || r.parent == s) bad.store(1, std::memory_order_relaxed);
+      if (n_list >= 0 && (r.offset < 0 || r.count < 0 || r.offset + r.count > n_list))
+        outside.store(1, std::memory_order_relaxed);
       fp[s] = SubPlan{r.offset, r.count, r.nbound_prev, r.out_offset, r.count};
       pp[s] = r.parent;
     }
@@ -1293,6 +1297,7 @@ int plan_nests(const hbt_subhalo *subs, int64_t nsub, Nest &N) {
     gather(0, nsub / nth);
     for (auto &x : th) x.join();
   }
+  if (outside.load()) return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the list positions");
   if (bad.load()) return fail(HBT_ERR_INVALID_ARG, "bad subhalo record (count/parent)");
   bool nested = false;
   for (int64_t s = 0; s < nsub; s++)
@@ -1635,10 +1640,8 @@ int hbt_unbind_batch_indexed(hbt_ctx *ctx, const hbt_particles *snapshot, const 
     HIPCHK(hipMemcpyAsync(ctx->h_active + 4, dcnt, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     RETCHK(pack_snapshot(ctx, snapshot, n_list));
     if (nsub > 0) {
-      for (int64_t s = 0; s < nsub; s++)
-        if (subs_host[s].offset < 0 || subs_host[s].count < 0 || subs_host[s].offset + subs_host[s].count > n_list)
-          return fail(HBT_ERR_INVALID_ARG, "subhalo list outside the list positions");
-      RETCHK(plan_nests(subs_host, nsub, t_nest));
+      // (the lists' range is checked while the records are gathered)
+      RETCHK(plan_nests(subs_host, nsub, t_nest, n_list));
       planned = true;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -4,11 +4,14 @@
 //    four-thread split;
 //  - the level layout build_level writes (wave-aligned slot bases, tile table,
 //    segment records, nest children) against its definition, on planned nests;
+//  - the records' checks while the planner gathers them (lists inside the
+//    list positions, counts and parents), from every gather thread's range;
 //  - the staging copy of a level's tables (pack_blobs) against plain memcpy,
 //    padding untouched, across its four-range split.
 // Prints "ok" and exits 0, or names the first failed check and exits 1.
 #include <cstdio>
 #include <random>
+#include <string>
 
 #include "../../hbtplus_amd/csrc/hbt_unbind.hip"
 
@@ -148,6 +151,48 @@ static int check_layout() {
   return 0;
 }
 
+// the records' checks: a list outside [0, n_list) or a bad count or parent is
+// an error naming which, wherever the record sits (each gather thread's range)
+static int check_record_errors() {
+  std::mt19937 g(14);
+  for (int n : {5, 70000}) {
+    int64_t total = 0;
+    const std::vector<hbt_subhalo> base = forest(g, n, total);
+    static Nest N;
+    CHECK(plan_nests(base.data(), n, N, total) == HBT_OK, "valid records rejected, n=%d: %s", n, hbt_last_error());
+    CHECK(plan_nests(base.data(), n, N) == HBT_OK, "valid records rejected without n_list, n=%d", n);
+    const std::string outside = "subhalo list outside the list positions", badrec = "bad subhalo record";
+    for (int at : {0, n / 2, n - 1}) {
+      struct Case {
+        const char *what;
+        void (*edit)(hbt_subhalo &, int64_t, int);
+        bool with_n_list;
+        const std::string *msg;
+      };
+      const Case cases[] = {
+          {"list past the end", [](hbt_subhalo &r, int64_t t, int) { r.offset = t - r.count + 1; }, true, &outside},
+          {"negative offset", [](hbt_subhalo &r, int64_t, int) { r.offset = -1; }, true, &outside},
+          {"negative count", [](hbt_subhalo &r, int64_t, int) { r.count = -3; }, true, &outside},
+          {"negative count, no n_list", [](hbt_subhalo &r, int64_t, int) { r.count = -3; }, false, &badrec},
+          {"parent past the end", [](hbt_subhalo &r, int64_t, int m) { r.parent = m; }, true, &badrec},
+          {"both", [](hbt_subhalo &r, int64_t t, int m) { r.parent = m; r.offset = t; r.count = 1; }, true, &outside},
+      };
+      for (const Case &c : cases) {
+        std::vector<hbt_subhalo> s = base;
+        c.edit(s[at], total, n);
+        const int rc = c.with_n_list ? plan_nests(s.data(), n, N, total) : plan_nests(s.data(), n, N);
+        CHECK(rc == HBT_ERR_INVALID_ARG && std::string(hbt_last_error()).rfind(*c.msg, 0) == 0,
+              "%s at %d of %d: rc %d, '%s'", c.what, at, n, rc, hbt_last_error());
+      }
+    }
+    // a list that ends exactly at n_list is inside
+    std::vector<hbt_subhalo> s = base;
+    s[n - 1].offset = total - s[n - 1].count;
+    CHECK(plan_nests(s.data(), n, N, total) == HBT_OK, "list ending at n_list rejected, n=%d", n);
+  }
+  return 0;
+}
+
 static int check_staging() {
   std::mt19937 g(13);
   const size_t pick[] = {0, 1, 7, 15, 16, 17, 255, 256, 1000, 65536 * 4 + 3, 1048573, 3000001};
@@ -186,7 +231,7 @@ static int check_staging() {
 }
 
 int main() {
-  if (check_sort() || check_layout() || check_staging()) return 1;
+  if (check_sort() || check_layout() || check_record_errors() || check_staging()) return 1;
   printf("ok\n");
   return 0;
 }
