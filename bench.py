@@ -63,7 +63,7 @@ def log(msg):
 
 def host_cpus():
     """Threads for the CPU baseline and what the host offers: the process's
-    affinity set, capped at 16 (one GPU's share of a GPU box's cores)."""
+    affinity set, capped at 16 (the host cores a one-GPU process is given)."""
     aff = len(os.sched_getaffinity(0))
     model = "unknown"
     try:

@@ -14,8 +14,8 @@ escape speed, where most converge.  FAST must meet the band rule on every
 subhalo (bound sets equal outside 1e-5 |E_min| of E = 0, Nbound differences
 bounded by the band, most-bound IDs equal unless tied within it, bound order
 up to the band); the counts of Nbound differences and most-bound mismatches
-are asserted against the band counts and reported (written to gpurun_out/
-when present)."""
+are asserted against the band counts and reported (also written as JSON to
+the folder HBT_REPORT_DIR names, when it is set)."""
 import json
 import os
 
@@ -82,8 +82,9 @@ def test_c3_precision_study(gpu, contaminants):
             counts.update(band_rule=f"not met: {e}")
         report[f"unbind_{name}"] = counts
     print(json.dumps(report, indent=1))
-    if os.path.isdir("gpurun_out"):
-        with open(f"gpurun_out/precision_c3_{contaminants}.json", "w") as f:
+    out_dir = os.environ.get("HBT_REPORT_DIR")
+    if out_dir and os.path.isdir(out_dir):
+        with open(os.path.join(out_dir, f"precision_c3_{contaminants}.json"), "w") as f:
             json.dump(report, f, indent=1)
     # potentials: per-subhalo RMS error <= 1e-4 against FP64; compensation
     # does not make the sum worse than plain FP32
