@@ -6,6 +6,8 @@
 //    segment records, nest children) against its definition, on planned nests;
 //  - the records' checks while the planner gathers them (lists inside the
 //    list positions, counts and parents), from every gather thread's range;
+//  - the rand() stream the sampled shuffles replay (the host's window for a
+//    call number and the device's chunk jumps) against glibc's own rand();
 //  - the staging copy of a level's tables (pack_blobs) against plain memcpy,
 //    padding untouched, across its four-range split.
 // Prints "ok" and exits 0, or names the first failed check and exits 1.
@@ -193,6 +195,39 @@ static int check_record_errors() {
   return 0;
 }
 
+// The rand() stream the sampled shuffles replay: the window the host computes
+// for call number pos (jumping by the matrices M^(2^k)), stepped by the
+// recurrence, gives glibc's own rand() values from that call on; and the
+// device's chunk jumps M^(RCHUNK 2^k) move a window exactly RCHUNK 2^k calls.
+static int check_rand_stream() {
+  for (uint32_t seed : {1u, 0u, 12345u})
+    for (int64_t pos : {(int64_t)0, (int64_t)1, (int64_t)30, (int64_t)31, (int64_t)344, (int64_t)1000,
+                        (int64_t)123457, (int64_t)RCHUNK * 3 + 5, (int64_t)5000011}) {
+      srand(seed);
+      for (int64_t i = 0; i < pos; i++) (void)rand();
+      uint32_t w[RWIN];
+      rand_window(seed, pos, w);
+      for (int t = 0; t < 200; t++) {
+        const uint32_t next = w[0] + w[28];
+        const int want = rand();
+        CHECK((int)(next >> 1) == want, "rand() call %lld of seed %u: %d, want %d", (long long)(pos + t), seed,
+              (int)(next >> 1), want);
+        memmove(w, w + 1, (RWIN - 1) * sizeof(uint32_t));
+        w[RWIN - 1] = next;
+      }
+    }
+  const RandJumps &J = rand_jumps();
+  for (int k = 0; k < 8; k++) {
+    const int64_t pos = 777, step = (int64_t)RCHUNK << k;
+    uint32_t a[RWIN], b[RWIN];
+    rand_window(1, pos, a);
+    rmat_apply(&J.chunk[(size_t)k * RWIN * RWIN], a, a);
+    rand_window(1, pos + step, b);
+    CHECK(memcmp(a, b, sizeof a) == 0, "chunk jump %d (%lld calls)", k, (long long)step);
+  }
+  return 0;
+}
+
 static int check_staging() {
   std::mt19937 g(13);
   const size_t pick[] = {0, 1, 7, 15, 16, 17, 255, 256, 1000, 65536 * 4 + 3, 1048573, 3000001};
@@ -231,7 +266,8 @@ static int check_staging() {
 }
 
 int main() {
-  if (check_sort() || check_layout() || check_record_errors() || check_staging()) return 1;
+  if (check_sort() || check_layout() || check_record_errors() || check_rand_stream() || check_staging())
+    return 1;
   printf("ok\n");
   return 0;
 }

@@ -1,7 +1,9 @@
-"""Host-side planning of an unbinding batch, without a GPU: the capacity order
-of a level, its slot and tile layout, and the staging copy of its tables.
-tests/cpp/plan_layout_main.cpp holds the checks; it is compiled from the
-library's own source by __graft_entry__.build()."""
+"""Host-side logic of the library, without a GPU: the capacity order of a
+level, its slot and tile layout, the records' checks while they are planned,
+the rand() stream the sampled shuffles replay (against glibc's own rand()),
+and the staging copy of a level's tables.  tests/cpp/plan_layout_main.cpp
+holds the checks; it is compiled from the library's own source by
+__graft_entry__.build()."""
 import os
 import subprocess
 
