@@ -713,7 +713,8 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the unbinding of one rank's share runs on one MI355X; "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+              "runs on several GPUs are not supported (ranks would each unbind their own share)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
