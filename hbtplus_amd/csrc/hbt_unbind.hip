@@ -267,6 +267,12 @@ void sort_largest_first(const Nest &N, const std::vector<int> &members, std::vec
   const size_t n = members.size();
   out.resize(n);
   caps.resize(n);
+  if (n < 64) {  // one-subhalo levels (the plain form, single calls) skip the histograms
+    std::copy(members.begin(), members.end(), out.begin());
+    std::stable_sort(out.begin(), out.end(), [&](int a, int b) { return N.f[a].cap > N.f[b].cap; });
+    for (size_t i = 0; i < n; i++) caps[i] = (int)N.f[out[i]].cap;
+    return;
+  }
   constexpr int CLAMP = 4096, NT = 4;
   static thread_local std::vector<int> key;
   static thread_local std::vector<uint32_t> cnt;
